@@ -1,0 +1,235 @@
+// Device-side building blocks of the FV RHS: thermodynamics (PerfectGas), flux calculators, the Navier-Stokes
+// advection / diffusion fluxes and the gradient limiters.  Templated on the arithmetic type so that the FP64
+// and the mixed-precision forms share one source.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace abl {
+
+enum : int { kRho = 0, kRhoE = 1, kRhoU = 2 };
+
+// Physics constants passed by value to every kernel.
+struct Phys {
+    double gamma, R, cv, icv, mu, k, diff, max_lim_grad;
+    int flux, recon, limiter, ns;
+    int visc;   // Navier-Stokes diffusion flux active (k or mu nonzero, or any transport active)
+    int sdiff;  // species diffusion active
+};
+
+// Geometry view (G = storage type of the face geometry: double for FP64, float for the mixed form).
+template <typename G>
+struct MeshView {
+    int64_t N, NI, F;
+    int K;
+    const int32_t* __restrict__ ell;
+    const int32_t* __restrict__ fcell;
+    const G* __restrict__ fnhat;
+    const G* __restrict__ farea;
+    const G* __restrict__ fdx;
+    const G* __restrict__ fw;
+    const G* __restrict__ fwa;
+    const double* __restrict__ inv_vol;
+};
+
+// Block index remapped so that each XCD (workgroups are dispatched round-robin over 8 XCDs) walks a contiguous
+// range of cells, keeping neighbour reuse inside one XCD's L2.
+__device__ inline int64_t xcd_cell_index() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    const uint32_t logical = x * q + (x < r ? x : r) + (b >> 3);
+    return (int64_t)logical * blockDim.x + threadIdx.x;
+}
+
+// PETSc's PetscMin/PetscMax: the comparison order decides NaN propagation.
+template <typename T>
+__device__ inline T pmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ inline T pmax(T a, T b) { return a < b ? b : a; }
+
+// Flux-calculator direction (fluxCalculator.hpp:12)
+enum : int { kNA = 0, kLeft = 1, kRight = 2 };
+
+// Liou-Steffen AUSM (ausm.cpp:2-43).
+template <typename T>
+__device__ inline int ausm(T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR, T& massFlux, T& p12) {
+    const T Mm = uR / aR;
+    T sMm, sPm;
+    if (fabs(Mm) <= T(1)) {
+        sMm = T(-0.25) * (Mm - T(1)) * (Mm - T(1));
+        sPm = -sMm * (T(2) + Mm);
+    } else {
+        sMm = T(0.5) * (Mm - fabs(Mm));
+        sPm = sMm / Mm;
+    }
+    const T Mp = uL / aL;
+    T sMp, sPp;
+    if (fabs(Mp) <= T(1)) {
+        sMp = T(0.25) * (Mp + T(1)) * (Mp + T(1));
+        sPp = sMp * (T(2) - Mp);
+    } else {
+        sMp = T(0.5) * (Mp + fabs(Mp));
+        sPp = sMp / Mp;
+    }
+    const T m = sMm + sMp;
+    int dir;
+    if (m < T(0)) {
+        massFlux = m * aR * rhoR;
+        dir = kRight;
+    } else {
+        massFlux = m * aL * rhoL;
+        dir = kLeft;
+    }
+    p12 = pR * sPm + pL * sPp;
+    return dir;
+}
+
+// Flux calculators behind FluxCalculatorFunction: Ausm, AverageFlux (averageFlux.cpp:2-10), OffFlux.
+template <typename T>
+__device__ inline int flux_calculator(int type, T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR, T& mdot, T& p12) {
+    if (type == 1) return ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, mdot, p12);
+    if (type == 2) {
+        mdot = T(0.5) * (uL * rhoL + uR * rhoR);
+        p12 = T(0.5) * (pL + pR);
+        return kNA;
+    }
+    mdot = T(0);
+    p12 = T(0);
+    return kNA;
+}
+
+// Decoded face-side state (PerfectGas T-route of AdvectionFlux, navierStokesTransport.cpp:122-140 and
+// perfectGas.cpp:67-113,165-174).
+template <typename T, int DIM>
+struct SideState {
+    T rho, irho, vel[DIM], un, e, a, p;
+};
+
+template <typename T, int DIM>
+__device__ inline void decode_side(const T* u, const T* nhat, const Phys& ph, SideState<T, DIM>& s) {
+    s.rho = u[kRho];
+    s.irho = T(1) / s.rho;
+    T ke = T(0);
+    s.un = T(0);
+    for (int d = 0; d < DIM; d++) {
+        s.vel[d] = u[kRhoU + d] * s.irho;
+        ke += s.vel[d] * s.vel[d];
+        s.un += s.vel[d] * nhat[d];
+    }
+    ke *= T(0.5);
+    const T eint = u[kRhoE] * s.irho - ke;
+    const T Tmp = eint * T(ph.icv);
+    s.e = Tmp * T(ph.cv);
+    s.p = T(ph.R) * s.rho * Tmp;
+    s.a = sqrt(T(ph.gamma) * s.p * s.irho);
+}
+
+// NavierStokesTransport::AdvectionFlux (navierStokesTransport.cpp:99-207) for euler components, plus the mass
+// flux per area and upwind side needed by SpeciesTransport::AdvectionFlux (speciesTransport.cpp:325-337: any
+// non-LEFT direction takes the right state).  flux[] receives dim+2 values.
+template <typename T, int DIM>
+__device__ inline void advection_flux(int type, const T* uL, const T* uR, const T* nhat, T area, const Phys& ph, T* flux,
+                                      T& mdotA, int& species_right, T& rhoL_out, T& rhoR_out) {
+    SideState<T, DIM> L, R;
+    decode_side<T, DIM>(uL, nhat, ph, L);
+    decode_side<T, DIM>(uR, nhat, ph, R);
+    T massFlux, p12;
+    const int dir = flux_calculator<T>(type, L.un, L.a, L.rho, L.p, R.un, R.a, R.rho, R.p, massFlux, p12);
+    const T mA = massFlux * area;
+    const T p12A = p12 * area;
+    flux[kRho] = mA;
+    if (dir == kLeft) {
+        T v2 = T(0);
+        for (int d = 0; d < DIM; d++) v2 += L.vel[d] * L.vel[d];
+        const T H = L.e + v2 / T(2) + L.p * L.irho;
+        flux[kRhoE] = H * mA;
+        for (int d = 0; d < DIM; d++) flux[kRhoU + d] = L.vel[d] * mA + p12A * nhat[d];
+    } else if (dir == kRight) {
+        T v2 = T(0);
+        for (int d = 0; d < DIM; d++) v2 += R.vel[d] * R.vel[d];
+        const T H = R.e + v2 / T(2) + R.p * R.irho;
+        flux[kRhoE] = H * mA;
+        for (int d = 0; d < DIM; d++) flux[kRhoU + d] = R.vel[d] * mA + p12A * nhat[d];
+    } else {
+        T vl2 = T(0), vr2 = T(0);
+        for (int d = 0; d < DIM; d++) {
+            vl2 += L.vel[d] * L.vel[d];
+            vr2 += R.vel[d] * R.vel[d];
+        }
+        const T HL = L.e + vl2 / T(2) + L.p * L.irho;
+        const T HR = R.e + vr2 / T(2) + R.p * R.irho;
+        flux[kRhoE] = T(0.5) * (HL + HR) * mA;
+        for (int d = 0; d < DIM; d++) flux[kRhoU + d] = T(0.5) * (L.vel[d] + R.vel[d]) * mA + p12A * nhat[d];
+    }
+    mdotA = mA;
+    species_right = dir != kLeft;
+    rhoL_out = L.rho;
+    rhoR_out = R.rho;
+}
+
+// Rusanov full-vector flux for the euler components (build-defined, absent from the reference):
+// F = A [ 1/2 (f(U_L) + f(U_R)).n - 1/2 lambda (U_R - U_L) ], lambda = max(|u_n| + a).
+template <typename T, int DIM>
+__device__ inline void rusanov_flux(const T* uL, const T* uR, const T* nhat, T area, const Phys& ph, T* flux, T& unL,
+                                    T& unR, T& lambda) {
+    SideState<T, DIM> L, R;
+    decode_side<T, DIM>(uL, nhat, ph, L);
+    decode_side<T, DIM>(uR, nhat, ph, R);
+    lambda = pmax(fabs(L.un) + L.a, fabs(R.un) + R.a);
+    unL = L.un;
+    unR = R.un;
+    const T hl = T(0.5) * lambda;
+    flux[kRho] = area * (T(0.5) * (uL[kRho] * L.un + uR[kRho] * R.un) - hl * (uR[kRho] - uL[kRho]));
+    flux[kRhoE] = area * (T(0.5) * ((uL[kRhoE] + L.p) * L.un + (uR[kRhoE] + R.p) * R.un) - hl * (uR[kRhoE] - uL[kRhoE]));
+    for (int d = 0; d < DIM; d++) {
+        const int c = kRhoU + d;
+        const T fl = uL[c] * L.un + L.p * nhat[d];
+        const T fr = uR[c] * R.un + R.p * nhat[d];
+        flux[c] = area * (T(0.5) * (fl + fr) - hl * (uR[c] - uL[c]));
+    }
+}
+
+// NavierStokesTransport::DiffusionFlux + CompressibleFlowComputeStressTensor (navierStokesTransport.cpp:454-532)
+// with face values vel[] and face gradients gT[d], gV[c][d].  normal = area-weighted normal.  Adds into flux.
+template <typename T, int DIM>
+__device__ inline void diffusion_flux_add(const T* normal, T mu, T k, const T* vel, const T* gT, const T (*gV)[DIM], T* flux) {
+    T divVel = T(0);
+    for (int c = 0; c < DIM; c++) divVel += gV[c][c];
+    T tau[DIM][DIM];
+    for (int c = 0; c < DIM; c++)
+        for (int d = 0; d < DIM; d++)
+            tau[c][d] = (c == d) ? T(2) * mu * (gV[c][d] - divVel / T(3)) : mu * (gV[c][d] + gV[d][c]);
+    for (int c = 0; c < DIM; c++) {
+        T vf = T(0);
+        for (int d = 0; d < DIM; d++) vf += -normal[d] * tau[c][d];
+        flux[kRhoU + c] += vf;
+    }
+    T e = T(0);
+    for (int d = 0; d < DIM; d++) {
+        T hf = T(0);
+        for (int c = 0; c < DIM; c++) hf += vel[c] * tau[d][c];
+        hf += k * gT[d];
+        e += -normal[d] * hf;
+    }
+    flux[kRhoE] += e;
+}
+
+// PETSc symmetric-form scalar limiters (the PetscLimiterLimit_* set used by cellInterpolant.cpp:316-324).
+template <typename T>
+__device__ inline T limiter_phi(int type, T f) {
+    switch (type) {
+        case 0: return T(1);
+        case 1: return T(0);
+        case 2: return T(2) * pmax(T(0), pmin(f, T(1) - f));
+        case 3: return pmax(T(0), T(4) * f * (T(1) - f));
+        case 4: return pmax(T(0), T(2) * f * (T(1) - f) / (f * f + (T(1) - f) * (T(1) - f)));
+        case 5: return T(2) * pmax(T(0), pmax(pmin(T(2) * f, T(1) - f), pmin(f, T(2) * (T(1) - f))));
+        case 6: return pmin(T(1), T(4) * pmax(T(0), pmin(f, T(1) - f)));
+        case 7: return sin(T(3.14159265358979323846) * pmax(pmin(f, T(1)), T(0)));
+        default: return T(1);
+    }
+}
+
+}  // namespace abl

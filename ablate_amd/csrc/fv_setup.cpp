@@ -1,0 +1,266 @@
+// Host-side construction of the device mesh layout for the FV RHS.
+//
+// Replaces the one-time work the reference does lazily on its first RHS call: the cell least-squares gradient
+// weights of CellInterpolant (BuildGradientReconstruction_Internal, cellInterpolant.cpp:641-715, which stores them
+// per face side in PetscFVFaceGeom::grad[2][3]) and the LeastSquaresAverage face stencils of FaceInterpolant
+// (stencils/leastSquaresAverage.cpp:5-155), restated as per-face-side weights of an unlimited cell gradient.
+//
+// Layout (structure of arrays; every array is indexed by face or cell so that consecutive GPU lanes touch
+// consecutive addresses):
+//   ell[k * NI + c]        k-th face of interior cell c, encoded (face << 1) | side (side 1 = c is the right cell),
+//                          faces sorted by number (the reference's accumulation order), -1 padding
+//   fcell[s * F + f]       support cells
+//   fnhat[d * F + f]       unit normal (left -> right), farea[f] = |n|
+//   fdx[(s*dim + d)*F + f] x_face - x_cell(side s) in that cell's frame
+//   fw, fwa                least-squares weights of cell `side` for the neighbour across f (cell / aux stencils)
+//   inv_vol[c]
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "fv_layout.hpp"
+
+namespace abl {
+
+namespace {
+
+// cyclic Jacobi eigen-decomposition of a symmetric n x n (n <= 3) matrix
+void sym_eigen(int n, double A[9], double V[9], double w[3]) {
+    for (int i = 0; i < 9; i++) V[i] = 0.0;
+    for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-32 * tot) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double tau = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                double t = std::copysign(1.0, tau) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+                for (int k = 0; k < n; k++) {
+                    double x = A[k * n + p], y = A[k * n + q];
+                    A[k * n + p] = cs * x - sn * y;
+                    A[k * n + q] = sn * x + cs * y;
+                }
+                for (int k = 0; k < n; k++) {
+                    double x = A[p * n + k], y = A[q * n + k];
+                    A[p * n + k] = cs * x - sn * y;
+                    A[q * n + k] = sn * x + cs * y;
+                }
+                for (int k = 0; k < n; k++) {
+                    double x = V[k * n + p], y = V[k * n + q];
+                    V[k * n + p] = cs * x - sn * y;
+                    V[k * n + q] = sn * x + cs * y;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+}
+
+}  // namespace
+
+// Minimum-norm least-squares weights (the pseudo-inverse PetscFVComputeGradient returns): full-rank stencils
+// use the normal equations with an explicit inverse, rank-deficient ones an eigen pseudo-inverse.
+void lsq_pinv_weights(int n, int dim, const double* dx, double* w) {
+    double M[9] = {0};
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++)
+            for (int b = 0; b < dim; b++) M[a * dim + b] += dx[i * dim + a] * dx[i * dim + b];
+    double Minv[9] = {0};
+    double tr = 0.0;
+    for (int a = 0; a < dim; a++) tr += M[a * dim + a];
+    bool full = false;
+    if (dim == 2) {
+        double det = M[0] * M[3] - M[1] * M[2];
+        if (det > 1e-10 * tr * tr) {
+            Minv[0] = M[3] / det;
+            Minv[1] = -M[1] / det;
+            Minv[2] = -M[2] / det;
+            Minv[3] = M[0] / det;
+            full = true;
+        }
+    } else {
+        double c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8], c02 = M[3] * M[7] - M[4] * M[6];
+        double det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+        if (det > 1e-10 * tr * tr * tr) {
+            Minv[0] = c00 / det;
+            Minv[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+            Minv[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+            Minv[3] = c01 / det;
+            Minv[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+            Minv[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+            Minv[6] = c02 / det;
+            Minv[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+            Minv[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+            full = true;
+        }
+    }
+    if (!full) {
+        double A[9], V[9], lam[3];
+        std::memcpy(A, M, sizeof(A));
+        sym_eigen(dim, A, V, lam);
+        double lmax = 0.0;
+        for (int k = 0; k < dim; k++) lmax = std::max(lmax, std::fabs(lam[k]));
+        // eigenvalues of dx^T dx are accurate to ~eps * lmax: singular values below 1e-6 s_max count as zero
+        for (int k = 0; k < dim; k++) {
+            if (!(lam[k] > 1e-12 * lmax) || lmax == 0.0) continue;
+            for (int a = 0; a < dim; a++)
+                for (int b = 0; b < dim; b++) Minv[a * dim + b] += V[a * dim + k] * V[b * dim + k] / lam[k];
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) {
+            double s = 0.0;
+            for (int b = 0; b < dim; b++) s += Minv[a * dim + b] * dx[i * dim + b];
+            w[i * dim + a] = s;
+        }
+}
+
+void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layout& L) {
+    const int dim = m.dim;
+    if (dim != 2 && dim != 3) throw std::invalid_argument("mesh dim must be 2 or 3");
+    if (m.n_interior <= 0 || m.n_interior > m.n_cells) throw std::invalid_argument("invalid n_interior / n_cells");
+    if (m.n_cells >= ((int64_t)1 << 31) || m.n_faces >= ((int64_t)1 << 30))
+        throw std::invalid_argument("mesh too large for 32-bit cell/face indices");
+    if (!m.face_cells || !m.face_normal || !m.face_centroid || !m.cell_centroid || !m.cell_volume)
+        throw std::invalid_argument("mesh descriptor has a null required array");
+    if (p.n_species < 0 || p.n_species > 64) throw std::invalid_argument("n_species must be in [0, 64]");
+    const int64_t N = m.n_cells, NI = m.n_interior, F = m.n_faces;
+    L.dim = dim;
+    L.N = N;
+    L.NI = NI;
+    L.F = F;
+    L.ns = p.n_species;
+    L.nc = dim + 2 + p.n_species;
+
+    // ---- validation + incidence
+    std::vector<int32_t> deg(N, 0);
+    for (int64_t f = 0; f < F; f++) {
+        int32_t a = m.face_cells[2 * f], b = m.face_cells[2 * f + 1];
+        if (a < 0 || a >= N || b < 0 || b >= N || a == b) throw std::invalid_argument("face_cells out of range at face " + std::to_string(f));
+        if (a >= NI) throw std::invalid_argument("the left cell of every face must be interior (face " + std::to_string(f) + ")");
+        deg[a]++;
+        deg[b]++;
+        bool bnd = b >= NI;
+        if (bnd && !m.face_bc) throw std::invalid_argument("boundary face without face_bc slots");
+        if (m.face_bc) {
+            if (bnd != (m.face_bc[f] >= 0)) throw std::invalid_argument("face_bc must be >= 0 exactly on boundary faces");
+            if (bnd && m.face_bc[f] >= kMaxBcSlots) throw std::invalid_argument("face_bc slot exceeds the supported number");
+        }
+        double nn = 0.0;
+        for (int d = 0; d < dim; d++) nn += m.face_normal[3 * f + d] * m.face_normal[3 * f + d];
+        if (!(nn > 0)) throw std::invalid_argument("zero-area face " + std::to_string(f));
+    }
+    int K = 0;
+    for (int64_t c = 0; c < NI; c++) {
+        if (deg[c] == 0) throw std::invalid_argument("isolated interior cell " + std::to_string(c));
+        K = std::max(K, deg[c]);
+    }
+    for (int64_t c = NI; c < N; c++)
+        if (deg[c] != 1) throw std::invalid_argument("ghost cell " + std::to_string(c) + " must have exactly one face");
+    if (K > kMaxFacesPerCell) throw std::invalid_argument("cells with more than " + std::to_string(kMaxFacesPerCell) + " faces are not supported");
+    for (int64_t c = 0; c < N; c++)
+        if (!(m.cell_volume[c] > 0)) throw std::invalid_argument("non-positive cell volume at cell " + std::to_string(c));
+    L.K = K;
+
+    L.ell.assign((size_t)K * NI, -1);
+    std::vector<int32_t> fill(NI, 0);
+    for (int64_t f = 0; f < F; f++)  // increasing face number per cell
+        for (int s = 0; s < 2; s++) {
+            int32_t c = m.face_cells[2 * f + s];
+            if (c >= NI) continue;
+            L.ell[(size_t)fill[c]++ * NI + c] = (int32_t)((f << 1) | s);
+        }
+
+    // ---- face geometry
+    L.fcell.resize(2 * F);
+    L.fnhat.resize((size_t)dim * F);
+    L.farea.resize(F);
+    L.fdx.resize((size_t)2 * dim * F);
+    auto shift = [&](int64_t f, int d) { return m.face_shift ? m.face_shift[3 * f + d] : 0.0; };
+    for (int64_t f = 0; f < F; f++) {
+        int32_t a = m.face_cells[2 * f], b = m.face_cells[2 * f + 1];
+        L.fcell[f] = a;
+        L.fcell[F + f] = b;
+        double mag = 0.0;
+        for (int d = 0; d < dim; d++) mag += m.face_normal[3 * f + d] * m.face_normal[3 * f + d];
+        mag = std::sqrt(mag);
+        L.farea[f] = mag;
+        for (int d = 0; d < dim; d++) {
+            L.fnhat[(size_t)d * F + f] = m.face_normal[3 * f + d] / mag;
+            L.fdx[(size_t)(0 * dim + d) * F + f] = m.face_centroid[3 * f + d] - m.cell_centroid[3 * a + d];
+            L.fdx[(size_t)(1 * dim + d) * F + f] = m.face_centroid[3 * f + d] - (m.cell_centroid[3 * b + d] + shift(f, d));
+        }
+        if (b >= NI) L.bfaces.push_back({(int32_t)f, m.face_bc[f]});
+    }
+    L.inv_vol.resize(N);
+    for (int64_t c = 0; c < N; c++) L.inv_vol[c] = 1.0 / m.cell_volume[c];
+
+    // neighbour offset of cell c across face f in c's frame: x_n - x_c
+    auto nbr_dx = [&](int64_t f, int s, double* out) {
+        int32_t a = m.face_cells[2 * f], b = m.face_cells[2 * f + 1];
+        for (int d = 0; d < dim; d++) {
+            double xa = m.cell_centroid[3 * a + d], xb = m.cell_centroid[3 * b + d] + shift(f, d);
+            out[d] = s == 0 ? xb - xa : xa - xb;
+        }
+    };
+
+    // ---- cell least-squares weights (non-boundary faces only) and aux (face-stencil) weights (all faces)
+    const bool recon = p.reconstruction == ABL_FV_RECON_LEASTSQUARES && p.flux_calculator != ABL_FV_FLUX_NONE;
+    const bool diffusion = p.k != 0.0 || p.mu != 0.0 || p.diff != 0.0;
+    const bool has_bnd = !L.bfaces.empty();
+    L.has_boundary = has_bnd;
+    if (recon || diffusion) {
+        L.fw.assign((size_t)2 * dim * F, 0.0);
+        if (has_bnd && diffusion) L.fwa.assign((size_t)2 * dim * F, 0.0);
+        std::vector<double> dx, w;
+        std::vector<int64_t> used;
+        std::vector<int> sides;
+        for (int pass = 0; pass < ((has_bnd && diffusion) ? 2 : 1); pass++) {
+            std::vector<double>& dst = pass == 0 ? L.fw : L.fwa;
+            for (int64_t c = 0; c < NI; c++) {
+                dx.clear();
+                used.clear();
+                sides.clear();
+                for (int k = 0; k < K; k++) {
+                    int32_t e = L.ell[(size_t)k * NI + c];
+                    if (e < 0) break;
+                    int64_t f = e >> 1;
+                    int s = e & 1;
+                    bool bnd = L.fcell[F + f] >= NI;
+                    if (pass == 0 && bnd) continue;  // CellInterpolant excludes boundary faces
+                    double v[3];
+                    nbr_dx(f, s, v);
+                    for (int d = 0; d < dim; d++) dx.push_back(v[d]);
+                    used.push_back(f);
+                    sides.push_back(s);
+                }
+                if (used.empty()) continue;
+                w.resize(dx.size());
+                lsq_pinv_weights((int)used.size(), dim, dx.data(), w.data());
+                for (size_t i = 0; i < used.size(); i++)
+                    for (int d = 0; d < dim; d++) dst[(size_t)(sides[i] * dim + d) * F + used[i]] = w[i * dim + d];
+            }
+            if (pass == 1) {
+                // ghost cells: one-neighbour stencil, minimum-norm solution dx / |dx|^2 (ghost is always side 1)
+                for (auto& bf : L.bfaces) {
+                    int64_t f = bf.face;
+                    double v[3];
+                    nbr_dx(f, 1, v);
+                    double wv[3];
+                    lsq_pinv_weights(1, dim, v, wv);
+                    for (int d = 0; d < dim; d++) dst[(size_t)(1 * dim + d) * F + f] = wv[d];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace abl

@@ -1,0 +1,267 @@
+"""Host-side mirror of ABLATE's finite-volume process / flux-calculator / EOS surface, backed by the HIP RHS.
+
+Class and parameter names follow the reference's YAML surface:
+  ablate::eos::PerfectGas{parameters{gamma, Rgas}, species}                 (src/eos/perfectGas.cpp:4-7,328-329)
+  ablate::eos::transport::Constant{k, mu, diff}                             (src/eos/transport/constant.cpp:74-75)
+  ablate::finiteVolume::fluxCalculator::{Ausm, AverageFlux, OffFlux}        (fluxCalculator/*.cpp)
+  ablate::finiteVolume::processes::NavierStokesTransport{parameters{cfl}, eos, fluxCalculator, transport}
+                                                                            (processes/navierStokesTransport.cpp:569-574)
+  ablate::finiteVolume::processes::SpeciesTransport{eos, fluxCalculator, transport}   (speciesTransport.cpp:474-478)
+  ablate::finiteVolume::FiniteVolumeSolver::ComputeRHSFunction(time, locX, locF)     (finiteVolumeSolver.cpp:233-291)
+Rusanov is build-defined (the reference has no Rusanov flux calculator).
+
+State and residual are CUDA (HIP) float64 tensors in structure-of-arrays layout ``[n_comp, n_cells]``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .mesh import _NativeMesh
+
+
+# ----------------------------------------------------------------------------------------------- EOS / transport
+@dataclass
+class PerfectGas:
+    parameters: dict = field(default_factory=dict)
+    species: list = field(default_factory=list)
+
+    @property
+    def gamma(self) -> float:
+        return float(self.parameters.get("gamma", 1.4))
+
+    @property
+    def Rgas(self) -> float:
+        return float(self.parameters.get("Rgas", 287.0))
+
+
+@dataclass
+class Constant:
+    k: float = 0.0
+    mu: float = 0.0
+    diff: float = 0.0
+
+
+# ----------------------------------------------------------------------------------------------- flux calculators
+class FluxCalculator:
+    kind = "none"
+
+
+class Ausm(FluxCalculator):
+    """AUSM flux splitting, Liou and Steffen 1993 (fluxCalculator/ausm.cpp:2-43)."""
+
+    kind = "ausm"
+
+
+class AverageFlux(FluxCalculator):
+    kind = "average"
+
+
+class OffFlux(FluxCalculator):
+    kind = "off"
+
+
+class Rusanov(FluxCalculator):
+    """Local Lax-Friedrichs flux (build-defined, full-vector)."""
+
+    kind = "rusanov"
+
+
+# ----------------------------------------------------------------------------------------------- processes
+@dataclass
+class NavierStokesTransport:
+    parameters: dict = field(default_factory=dict)
+    eos: PerfectGas = field(default_factory=PerfectGas)
+    fluxCalculator: FluxCalculator | None = None
+    transport: Constant | None = None
+    pgs: object = None
+
+
+@dataclass
+class SpeciesTransport:
+    eos: PerfectGas = field(default_factory=PerfectGas)
+    fluxCalculator: FluxCalculator | None = None
+    transport: Constant | None = None
+    parameters: dict = field(default_factory=dict)
+
+
+# ----------------------------------------------------------------------------------------------- boundaries
+@dataclass
+class InflowGhost:
+    """EssentialGhost with a constant (rho, velocity, T, Yi) state (boundaryConditions/essentialGhost.cpp)."""
+
+    labelIds: list
+    rho: float
+    velocity: list
+    T: float
+    Yi: list | None = None
+
+
+@dataclass
+class OutflowGhost:
+    """Zero-gradient extrapolation ghost (build-defined)."""
+
+    labelIds: list
+
+
+# ----------------------------------------------------------------------------------------------- solver
+class FiniteVolumeSolver:
+    """Drop-in for FiniteVolumeSolver::ComputeRHSFunction on one GPU.
+
+    options: ``petscfv_type`` ("upwind" | "leastsquares") and ``petsclimiter_type`` ("none", "minmod",
+    "vanleer", ..., "barthjespersen"); ``maxLimGrad`` as CompressibleFlowSolver{maxLimGrad}.
+    """
+
+    def __init__(self, mesh: _NativeMesh, processes, options: dict | None = None, boundaryConditions=(),
+                 maxLimGrad: float = 1e10, form: str = "fp64"):
+        options = options or {}
+        ns_proc = [p for p in processes if isinstance(p, NavierStokesTransport)]
+        sp_proc = [p for p in processes if isinstance(p, SpeciesTransport)]
+        if len(ns_proc) != 1:
+            raise ValueError("exactly one NavierStokesTransport process is required")
+        nst = ns_proc[0]
+        eos = nst.eos
+        n_species = len(eos.species)
+        flux = nst.fluxCalculator.kind if nst.fluxCalculator else "none"
+        transport = nst.transport
+        if sp_proc:
+            spt = sp_proc[0]
+            if (spt.fluxCalculator.kind if spt.fluxCalculator else "none") != flux:
+                raise ValueError("SpeciesTransport and NavierStokesTransport must use the same flux calculator")
+            if (spt.transport or Constant()) != (transport or Constant()):
+                raise ValueError("SpeciesTransport and NavierStokesTransport must use the same transport model")
+        elif n_species:
+            raise ValueError("species are declared in the EOS but no SpeciesTransport process was given")
+        tr = transport or Constant()
+        recon = options.get("petscfv_type", "upwind")
+        limiter = options.get("petsclimiter_type", "none").replace("-", "").replace("_", "")
+        phys = _lib.PhysicsDesc(
+            gamma=eos.gamma,
+            Rgas=eos.Rgas,
+            n_species=n_species,
+            flux_calculator=_lib.FLUX[flux],
+            reconstruction=_lib.RECON[recon],
+            limiter=_lib.LIMITER[limiter],
+            max_lim_grad=float(maxLimGrad),
+            k=float(tr.k),
+            mu=float(tr.mu),
+            diff=float(tr.diff),
+            cfl=float(nst.parameters.get("cfl", 0.5)),
+        )
+        self.mesh = mesh
+        self.phys = phys
+        self.form = form
+        self.n_species = n_species
+        self.dim = mesh.dim
+        self.n_comp = mesh.dim + 2 + n_species
+        h = ctypes.c_void_p()
+        fm = {"fp64": _lib.FORM_FP64, "mixed": _lib.FORM_MIXED}[form]
+        _lib.check(_lib.lib().abl_fv_create(ctypes.byref(mesh.desc), ctypes.byref(phys), fm, ctypes.byref(h)), None,
+                   "abl_fv_create")
+        self._h = h
+        self._theta_slots = []
+        for bc in boundaryConditions:
+            for slot in bc.labelIds:
+                self.set_boundary(slot, bc)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().abl_fv_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> _lib.Info:
+        inf = _lib.Info()
+        _lib.check(_lib.lib().abl_fv_get_info(self._h, ctypes.byref(inf)), self._h, "abl_fv_get_info")
+        return inf
+
+    def set_boundary(self, slot: int, bc) -> None:
+        desc = _lib.BcDesc()
+        yi_buf = None
+        if isinstance(bc, InflowGhost):
+            desc.kind = _lib.BC_INFLOW
+            desc.rho = float(bc.rho)
+            vel = list(bc.velocity) + [0.0] * (3 - len(bc.velocity))
+            desc.vel = (ctypes.c_double * 3)(*vel)
+            desc.T = float(bc.T)
+            if bc.Yi is not None:
+                yi_buf = (ctypes.c_double * len(bc.Yi))(*bc.Yi)
+                desc.Yi = ctypes.cast(yi_buf, ctypes.POINTER(ctypes.c_double))
+            self._theta_slots.append(slot)
+        elif isinstance(bc, OutflowGhost):
+            desc.kind = _lib.BC_OUTFLOW
+        else:
+            raise TypeError(f"unsupported boundary condition {bc!r}")
+        _lib.check(_lib.lib().abl_fv_set_boundary(self._h, int(slot), ctypes.byref(desc)), self._h,
+                   "abl_fv_set_boundary")
+
+    # ---- hot path
+    @staticmethod
+    def _stream_ptr(stream):
+        if stream is None:
+            import torch
+
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(int(stream))
+
+    def _check_tensor(self, t, name):
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise TypeError(f"{name} must be a contiguous float64 device tensor")
+        if tuple(t.shape) != (self.n_comp, self.mesh.n_cells):
+            raise ValueError(f"{name} must have shape {(self.n_comp, self.mesh.n_cells)}, got {tuple(t.shape)}")
+
+    def compute_rhs(self, time: float, U, R, accumulate: bool = False, stream=None) -> None:
+        self._check_tensor(U, "U")
+        self._check_tensor(R, "R")
+        _lib.check(
+            _lib.lib().abl_fv_rhs(self._h, float(time), ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(R.data_ptr()),
+                                  int(bool(accumulate)), self._stream_ptr(stream)),
+            self._h,
+            "abl_fv_rhs",
+        )
+
+    def ComputeRHSFunction(self, time: float, locX, locF) -> int:
+        """Reference semantics: locF is accumulated into (the caller zeroes it); returns 0 (PetscErrorCode)."""
+        self.compute_rhs(time, locX, locF, accumulate=True)
+        return 0
+
+    def rhs_vjp(self, time: float, U, Rbar, Ubar, stream=None) -> np.ndarray:
+        self._check_tensor(U, "U")
+        self._check_tensor(Rbar, "Rbar")
+        self._check_tensor(Ubar, "Ubar")
+        theta = np.zeros(3 * 16)
+        _lib.check(
+            _lib.lib().abl_fv_rhs_vjp(self._h, float(time), ctypes.c_void_p(U.data_ptr()),
+                                      ctypes.c_void_p(Rbar.data_ptr()), ctypes.c_void_p(Ubar.data_ptr()),
+                                      theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self._stream_ptr(stream)),
+            self._h,
+            "abl_fv_rhs_vjp",
+        )
+        return theta.reshape(16, 3)
+
+    def set_profiling(self, enable: bool) -> None:
+        _lib.check(_lib.lib().abl_fv_set_profiling(self._h, int(bool(enable))), self._h, "abl_fv_set_profiling")
+
+    def phase_times(self, reset: bool = True) -> dict:
+        """Device milliseconds and launch counts per RHS phase (event-timed on the launch stream)."""
+        n = len(_lib.PHASES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        _lib.check(_lib.lib().abl_fv_get_phase_times(self._h, ms, cnt, int(bool(reset))), self._h,
+                   "abl_fv_get_phase_times")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.PHASES)}
+
+    def check_state(self, U, stream=None) -> int:
+        self._check_tensor(U, "U")
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().abl_fv_check_state(self._h, ctypes.c_void_p(U.data_ptr()), ctypes.byref(n),
+                                                 self._stream_ptr(stream)), self._h, "abl_fv_check_state")
+        return int(n.value)

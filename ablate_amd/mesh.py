@@ -1,0 +1,117 @@
+"""Generated finite-volume meshes (DMPlex-style FV geometry) exposed as numpy views.
+
+``BoxMesh`` mirrors ablate::domain::BoxMesh + GhostBoundaryCells (src/domain/boxMesh.cpp:17-43,
+src/domain/modifiers/ghostBoundaryCells.cpp:4-8); ``TetBoxMesh`` is the jittered tetrahedral unit cube.
+Arrays are views of memory owned by the native mesh object and live as long as the Python object.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class MeshArrays:
+    dim: int
+    n_cells: int
+    n_interior: int
+    n_faces: int
+    face_cells: np.ndarray  # [F, 2] int32
+    face_normal: np.ndarray  # [F, 3]
+    face_centroid: np.ndarray  # [F, 3]
+    face_shift: np.ndarray | None  # [F, 3]
+    cell_centroid: np.ndarray  # [N, 3]
+    cell_volume: np.ndarray  # [N]
+    face_bc: np.ndarray | None  # [F] int32
+
+
+def _view(ptr, shape, dtype):
+    if not ptr:
+        return None
+    count = int(np.prod(shape))
+    ctype = ctypes.c_double if dtype == np.float64 else ctypes.c_int32
+    buf = ctypes.cast(ptr, ctypes.POINTER(ctype * count)).contents
+    return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+
+class _NativeMesh:
+    def __init__(self, handle: ctypes.c_void_p):
+        self._handle = handle
+        desc = _lib.MeshDesc()
+        _lib.check(_lib.lib().abl_fv_mesh_describe(handle, ctypes.byref(desc)), None, "abl_fv_mesh_describe")
+        N, F = desc.n_cells, desc.n_faces
+        self.arrays = MeshArrays(
+            dim=desc.dim,
+            n_cells=N,
+            n_interior=desc.n_interior,
+            n_faces=F,
+            face_cells=_view(desc.face_cells, (F, 2), np.int32),
+            face_normal=_view(desc.face_normal, (F, 3), np.float64),
+            face_centroid=_view(desc.face_centroid, (F, 3), np.float64),
+            face_shift=_view(desc.face_shift, (F, 3), np.float64),
+            cell_centroid=_view(desc.cell_centroid, (N, 3), np.float64),
+            cell_volume=_view(desc.cell_volume, (N,), np.float64),
+            face_bc=_view(desc.face_bc, (F,), np.int32),
+        )
+        self.desc = desc
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            _lib.lib().abl_fv_mesh_free(self._handle)
+            self._handle = None
+
+    # convenience accessors
+    @property
+    def dim(self):
+        return self.arrays.dim
+
+    @property
+    def n_cells(self):
+        return self.arrays.n_cells
+
+    @property
+    def n_interior(self):
+        return self.arrays.n_interior
+
+    @property
+    def n_faces(self):
+        return self.arrays.n_faces
+
+
+class BoxMesh(_NativeMesh):
+    """Uniform quad/hex box. ``periodic`` per direction; non-periodic sides get ghost cells whose boundary faces
+    carry boundary-condition slots 2*d (lower side) and 2*d+1 (upper side)."""
+
+    def __init__(self, faces, lower, upper, periodic=None, ordering: str = "brick"):
+        dim = len(faces)
+        periodic = periodic if periodic is not None else [True] * dim
+        cells = (ctypes.c_int32 * 3)(*[int(x) for x in faces], *([1] * (3 - dim)))
+        lo = (ctypes.c_double * 3)(*[float(x) for x in lower], *([0.0] * (3 - dim)))
+        hi = (ctypes.c_double * 3)(*[float(x) for x in upper], *([1.0] * (3 - dim)))
+        per = (ctypes.c_int32 * 3)(*[int(bool(x)) for x in periodic], *([0] * (3 - dim)))
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().abl_fv_mesh_box(dim, cells, lo, hi, per, _lib.ORDER[ordering], ctypes.byref(h)),
+            None,
+            "abl_fv_mesh_box",
+        )
+        self.faces, self.lower, self.upper, self.periodic = list(faces), list(lower), list(upper), list(periodic)
+        super().__init__(h)
+
+
+class TetBoxMesh(_NativeMesh):
+    """Unit cube of n^3 cubes x 6 Kuhn tetrahedra, interior nodes jittered by up to ``jitter`` * h."""
+
+    def __init__(self, n: int, jitter: float = 0.2, seed: int = 20261202, ordering: str = "rcm"):
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().abl_fv_mesh_tet_box(int(n), float(jitter), int(seed), _lib.ORDER[ordering], ctypes.byref(h)),
+            None,
+            "abl_fv_mesh_tet_box",
+        )
+        self.n, self.jitter, self.seed = n, jitter, seed
+        super().__init__(h)

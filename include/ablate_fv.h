@@ -1,0 +1,200 @@
+/*
+ * ablate_fv.h -- C ABI of the MI355X (gfx950) finite-volume compressible-flow right-hand side.
+ *
+ * The library replaces the per-face CPU loops behind
+ *   ablate::finiteVolume::FiniteVolumeSolver::ComputeRHSFunction(PetscReal time, Vec locX, Vec locF)
+ *   (src/finiteVolume/finiteVolumeSolver.cpp:233-291, declared finiteVolumeSolver.hpp:106)
+ * for the process set NavierStokesTransport (+ SpeciesTransport) with a PerfectGas EOS, a Constant transport
+ * model and the Ausm / AverageFlux / OffFlux flux calculators (plus the build-defined Rusanov flux).
+ *
+ * Conventions (mirroring PETSc / ABLATE):
+ *   - Every entry point returns an int status: 0 = success (PetscErrorCode semantics), nonzero = error;
+ *     abl_fv_last_error() returns the message of the last failure on that handle (or the global one when the
+ *     handle is NULL).  No C++ exception crosses this boundary.
+ *   - State and residual arrays are DEVICE pointers in structure-of-arrays layout U[comp * n_cells + cell].
+ *     Component order is the reference's field order: euler {rho, rhoE, rhoU, rhoV[, rhoW]} then densityYi
+ *     {rhoY_0 .. rhoY_{ns-1}} (src/finiteVolume/compressibleFlowFields.hpp:16, compressibleFlowFields.cpp:11-60).
+ *   - Mesh descriptors are HOST pointers (setup is host-side, like DMPlexComputeGeometryFVM).
+ *   - Boundary ghost cells (DM_POLYTOPE_FV_GHOST) occupy the trailing cell range [n_interior, n_cells).
+ *   - A handle is thread-compatible, not thread-safe: calls on one handle must be serialised by the caller.
+ *   - Results are deterministic: no floating-point atomics; every cell gathers its faces in a fixed order.
+ */
+#ifndef ABLATE_FV_H
+#define ABLATE_FV_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct abl_fv_solver* abl_fv_handle;
+typedef struct abl_fv_mesh* abl_fv_mesh_handle;
+/* hipStream_t without pulling in the HIP headers; pass NULL for the default stream. */
+typedef void* abl_fv_stream;
+
+/* ---- enumerations (values are part of the ABI) ------------------------------------------------------------ */
+
+/* Arithmetic form of the RHS. */
+enum { ABL_FV_FORM_FP64 = 0, ABL_FV_FORM_MIXED = 1 };
+
+/* Flux calculators: fluxCalculator::Ausm (ausm.cpp:2-43), AverageFlux (averageFlux.cpp:2-10),
+ * OffFlux (offFlux.cpp:2-8); RUSANOV is build-defined (absent from the reference). */
+enum { ABL_FV_FLUX_NONE = 0, ABL_FV_FLUX_AUSM = 1, ABL_FV_FLUX_AVERAGE = 2, ABL_FV_FLUX_OFF = 3, ABL_FV_FLUX_RUSANOV = 4 };
+
+/* Cell reconstruction (PETSc petscfv_type): upwind = first order, leastsquares = LSQ gradients. */
+enum { ABL_FV_RECON_NONE = 0, ABL_FV_RECON_LEASTSQUARES = 1 };
+
+/* Gradient limiters (PETSc petsclimiter_type, symmetric form of cellInterpolant.cpp:282-328) plus the
+ * build-defined Barth-Jespersen limiter. */
+enum {
+    ABL_FV_LIMITER_NONE = 0,
+    ABL_FV_LIMITER_ZERO = 1,
+    ABL_FV_LIMITER_MINMOD = 2,
+    ABL_FV_LIMITER_VANLEER = 3,
+    ABL_FV_LIMITER_VANALBADA = 4,
+    ABL_FV_LIMITER_SUPERBEE = 5,
+    ABL_FV_LIMITER_MC = 6,
+    ABL_FV_LIMITER_SIN = 7,
+    ABL_FV_LIMITER_BARTH_JESPERSEN = 8
+};
+
+/* Boundary (ghost-cell) conditions. */
+enum {
+    ABL_FV_BC_NONE = 0,
+    ABL_FV_BC_INFLOW = 1,  /* EssentialGhost (essentialGhost.cpp:5-23) with a constant (rho, u, T, Yi) state */
+    ABL_FV_BC_OUTFLOW = 2  /* zero-gradient extrapolation ghost: U_ghost = U_interior (build-defined) */
+};
+
+/* ---- descriptors ------------------------------------------------------------------------------------------ */
+
+/* DMPlex-style FV geometry (PetscFVFaceGeom / PetscFVCellGeom, DMPlexComputeGeometryFVM).
+ * All vectors are stored with 3 entries even when dim == 2 (the third is ignored). */
+typedef struct abl_fv_mesh_desc {
+    int32_t dim;                 /* 2 or 3 */
+    int64_t n_cells;             /* interior + ghost cells */
+    int64_t n_interior;          /* cells [0, n_interior) are interior; [n_interior, n_cells) are boundary ghosts */
+    int64_t n_faces;             /* faces with two supporting cells (interior and boundary faces) */
+    const int32_t* face_cells;   /* [n_faces][2] support: face_cells[2f] = left (support[0]), [2f+1] = right */
+    const double* face_normal;   /* [n_faces][3] area-weighted normal pointing left -> right */
+    const double* face_centroid; /* [n_faces][3] face centroid in the frame of the left cell */
+    const double* face_shift;    /* [n_faces][3] translation mapping the right cell into the left cell's frame
+                                    (periodic images); NULL = all zero */
+    const double* cell_centroid; /* [n_cells][3] */
+    const double* cell_volume;   /* [n_cells] */
+    const int32_t* face_bc;      /* [n_faces] boundary-condition slot of a boundary face (right cell is a ghost),
+                                    -1 for interior faces; NULL = no boundary conditions */
+} abl_fv_mesh_desc;
+
+/* Physics of one NavierStokesTransport (+ SpeciesTransport) process set.  Key names follow the YAML surface:
+ * PerfectGas{gamma, Rgas} (perfectGas.cpp:4-7), Constant{k, mu, diff} (constant.cpp:3-5),
+ * NavierStokesTransport{cfl} (navierStokesTransport.cpp:20), CompressibleFlowSolver{maxLimGrad}
+ * (compressibleFlowSolver.cpp:59-68), petscfv_type / petsclimiter_type. */
+typedef struct abl_fv_physics_desc {
+    double gamma;            /* default 1.4 */
+    double Rgas;             /* default 287 */
+    int32_t n_species;       /* transported species (densityYi components) */
+    int32_t flux_calculator; /* ABL_FV_FLUX_* ; NONE = no advection */
+    int32_t reconstruction;  /* ABL_FV_RECON_* (applies to euler and densityYi) */
+    int32_t limiter;         /* ABL_FV_LIMITER_* */
+    double max_lim_grad;     /* maxLimGrad (finiteVolumeSolver.hpp:84 default Constants::large = 1e10) */
+    double k;                /* thermal conductivity */
+    double mu;               /* dynamic viscosity */
+    double diff;             /* species diffusivity */
+    double cfl;              /* advection CFL (default 0.5) */
+} abl_fv_physics_desc;
+
+/* One boundary-condition slot. */
+typedef struct abl_fv_bc_desc {
+    int32_t kind;            /* ABL_FV_BC_* */
+    double rho;              /* inflow density */
+    double vel[3];           /* inflow velocity */
+    double T;                /* inflow temperature */
+    const double* Yi;        /* [n_species] inflow mass fractions (host pointer, copied), NULL = zeros */
+} abl_fv_bc_desc;
+
+/* Read-only facts about a created solver. */
+typedef struct abl_fv_info {
+    int32_t dim;
+    int32_t n_comp;              /* dim + 2 + n_species */
+    int32_t max_faces_per_cell;  /* width of the cell -> face table */
+    int32_t form;
+    int64_t n_cells, n_interior, n_faces;
+    int64_t device_bytes;        /* device memory held by the handle (mesh + workspace) */
+    double alg_bytes_per_cell;   /* algorithmic HBM bytes per RHS per interior cell (state + mesh) */
+} abl_fv_info;
+
+/* ---- mesh generation / ordering (domain helpers; replace BoxMesh + GhostBoundaryCells, boxMesh.cpp:17-43,
+ *      modifiers/ghostBoundaryCells.cpp:4-8, and DMPlex RCM reordering) --------------------------------------- */
+
+/* Ordering of generated cells. */
+enum { ABL_FV_ORDER_NATURAL = 0, ABL_FV_ORDER_BRICK = 1, ABL_FV_ORDER_RCM = 2 };
+
+/* Uniform quad (dim 2) / hex (dim 3) box: cells[d] cells per direction on [lower, upper], periodic[d] != 0 for
+ * periodic directions.  Non-periodic directions get one ghost cell per boundary face; boundary faces carry
+ * face_bc = 2*d (lower side) or 2*d+1 (upper side).  Faces are numbered by their left cell. */
+int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* lower, const double* upper, const int32_t* periodic,
+                    int32_t ordering, abl_fv_mesh_handle* out);
+/* Unit-cube tetrahedral mesh: n^3 cubes split into 6 tets each, interior nodes jittered by up to
+ * jitter * h (uniform, seeded), boundary faces with ghost cells (face_bc = 0..5 by cube side). */
+int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int32_t ordering, abl_fv_mesh_handle* out);
+/* Descriptor view of a generated mesh (pointers stay valid until abl_fv_mesh_free). */
+int abl_fv_mesh_describe(abl_fv_mesh_handle mesh, abl_fv_mesh_desc* desc);
+void abl_fv_mesh_free(abl_fv_mesh_handle mesh);
+
+/* ---- solver lifecycle -------------------------------------------------------------------------------------- */
+
+/* Builds the device mesh (cell->face table, LSQ weights, geometry in the form's precision) and workspace.
+ * Replaces FiniteVolumeSolver::Setup/Initialize + the lazy CellInterpolant/FaceInterpolant construction
+ * (finiteVolumeSolver.cpp:28-231, cellInterpolant.cpp:5-33,641-889, faceInterpolant.cpp:7-73). */
+int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_desc* physics, int32_t form, abl_fv_handle* out);
+int abl_fv_destroy(abl_fv_handle h);
+const char* abl_fv_last_error(abl_fv_handle h);
+int abl_fv_get_info(abl_fv_handle h, abl_fv_info* info);
+/* Registers the state of boundary-condition slot `slot` (Ghost/EssentialGhost, ghost.cpp:17-28). */
+int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_bc_desc* bc);
+
+/* ---- the hot path ------------------------------------------------------------------------------------------- */
+
+/* One RHS evaluation: ghost fill (when the mesh has boundary faces, writing the ghost rows of U), aux decode,
+ * gradients + limiter, face fluxes and per-cell residual gather.  With accumulate = 1 the residual is ADDED into
+ * R (reference semantics: locF is zeroed by the caller, timeStepper.cpp:363, and ComputeRHSFunction adds into
+ * it); with accumulate = 0, R = rhs.  Ghost rows of R are left untouched (accumulate = 1) or zeroed. */
+int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accumulate, abl_fv_stream stream);
+
+/* Vector-Jacobian product of the RHS (build-defined; the reference has no adjoint):
+ *   Ubar = Rbar^T dR/dU   (interior rows; ghost rows receive zero)
+ *   theta_bar[3 * slot + {0,1,2}] += Rbar^T dR/d(rho_in, u_in, T_in) for every INFLOW slot (host array sized
+ *   3 * number of slots, or NULL).  Fluxes are recomputed, not stored.  Supported for first-order
+ *   reconstruction without diffusion. */
+int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar, double* theta_bar,
+                   abl_fv_stream stream);
+
+/* DMPlex local-vector layout (AoS per cell, fields contiguous) <-> SoA. Device pointers. */
+int abl_fv_pack_aos(abl_fv_handle h, const double* aos, double* soa, abl_fv_stream stream);
+int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_stream stream);
+
+/* Number of interior cells whose state holds a non-finite value or a non-positive density
+ * (mirrors the verboseSourceCheck scan, timeStepper.cpp:385-389). */
+int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_bad, abl_fv_stream stream);
+
+/* ---- per-phase timing (replaces the Loggable / PetscLogEvent brackets around each RHS sub-phase,
+ *      finiteVolumeSolver.cpp:239-288, src/utilities/loggable.hpp:26-48) ------------------------------------- */
+enum {
+    ABL_FV_PHASE_GHOST = 0,
+    ABL_FV_PHASE_GRADIENTS = 1,
+    ABL_FV_PHASE_SPECIES_GRADIENTS = 2,
+    ABL_FV_PHASE_FLUX = 3,
+    ABL_FV_PHASE_VJP = 4,
+    ABL_FV_N_PHASES = 5
+};
+/* When enabled, every kernel of abl_fv_rhs / abl_fv_rhs_vjp is bracketed by events on its stream. */
+int abl_fv_set_profiling(abl_fv_handle h, int32_t enable);
+/* Accumulated device milliseconds and launch counts per phase since the last reset (waits for the events). */
+int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABLATE_FV_H */

@@ -1,0 +1,283 @@
+"""ctypes binding of the CPU oracle (oracle/fv_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the bench's CPU-baseline leg as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle_fv.so"
+
+FLUX = {"none": 0, "ausm": 1, "average": 2, "off": 3, "rusanov": 4}
+RECON = {"upwind": 0, "none": 0, "leastsquares": 1}
+LIMITER = {"none": 0, "zero": 1, "minmod": 2, "vanleer": 3, "vanalbada": 4, "superbee": 5, "mc": 6, "sin": 7,
+           "barthjespersen": 8}
+LEFT, RIGHT, NA = 1, 2, 0
+
+_dp = POINTER(c_double)
+
+
+class OrcMesh(ctypes.Structure):
+    _fields_ = [
+        ("dim", c_int),
+        ("n_cells", c_int64),
+        ("n_interior", c_int64),
+        ("n_faces", c_int64),
+        ("face_cells", POINTER(c_int32)),
+        ("face_normal", _dp),
+        ("face_centroid", _dp),
+        ("face_shift", _dp),
+        ("cell_centroid", _dp),
+        ("cell_volume", _dp),
+        ("face_bc", POINTER(c_int32)),
+    ]
+
+
+class OrcPhysics(ctypes.Structure):
+    _fields_ = [
+        ("gamma", c_double),
+        ("Rgas", c_double),
+        ("n_species", c_int),
+        ("flux", c_int),
+        ("recon", c_int),
+        ("limiter", c_int),
+        ("max_lim_grad", c_double),
+        ("k", c_double),
+        ("mu", c_double),
+        ("diff", c_double),
+    ]
+
+
+class OrcBc(ctypes.Structure):
+    _fields_ = [("kind", c_int), ("rho", c_double), ("vel", c_double * 3), ("T", c_double), ("Yi", _dp)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        d = c_double
+        for name in ("orc_ausm", "orc_average_flux", "orc_off_flux"):
+            getattr(L, name).argtypes = [d] * 8 + [_dp, _dp]
+            getattr(L, name).restype = c_int
+        for name, args in {
+            "orc_pg_pressure": [_dp, c_int, d, d],
+            "orc_pg_temperature": [_dp, c_int, d, d],
+            "orc_pg_internal_energy": [_dp, c_int],
+            "orc_pg_internal_energy_T": [d, d, d],
+            "orc_pg_speed_of_sound": [_dp, c_int, d, d],
+            "orc_pg_speed_of_sound_T": [_dp, d, d, d],
+            "orc_pg_pressure_T": [_dp, d, d],
+            "orc_pg_sensible_enthalpy_T": [d, d, d],
+            "orc_pg_cv": [d, d],
+            "orc_pg_cp": [d, d],
+            "orc_limiter": [c_int, d],
+        }.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = d
+        L.orc_ns_advection_flux.argtypes = [c_int, _dp, _dp, _dp, d, d, c_int, _dp]
+        L.orc_ns_advection_flux.restype = c_int
+        L.orc_stress_tensor.argtypes = [c_int, d, _dp, _dp]
+        L.orc_stress_tensor.restype = None
+        L.orc_lsq_weights.argtypes = [c_int, c_int, _dp, _dp]
+        L.orc_lsq_weights.restype = c_int
+        L.orc_rhs.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(OrcBc), c_int, d, _dp, _dp]
+        L.orc_rhs.restype = c_int
+        L.orc_create.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(ctypes.c_void_p)]
+        L.orc_create.restype = c_int
+        L.orc_rhs_ctx.argtypes = [ctypes.c_void_p, POINTER(OrcBc), c_int, d, _dp, _dp]
+        L.orc_rhs_ctx.restype = c_int
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_destroy.restype = None
+        L.orc_cell_gradients.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp]
+        L.orc_cell_gradients.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _mesh_struct(m):
+    keep = []
+
+    def arr(x, dtype):
+        if x is None:
+            return None
+        y = np.ascontiguousarray(x, dtype=dtype)
+        keep.append(y)
+        return y
+
+    fc = arr(m.face_cells, np.int32)
+    fn, fcen = arr(m.face_normal, np.float64), arr(m.face_centroid, np.float64)
+    fsh = arr(m.face_shift, np.float64)
+    cc, cv = arr(m.cell_centroid, np.float64), arr(m.cell_volume, np.float64)
+    fbc = arr(m.face_bc, np.int32)
+    s = OrcMesh(
+        dim=m.dim, n_cells=m.n_cells, n_interior=m.n_interior, n_faces=m.n_faces,
+        face_cells=fc.ctypes.data_as(POINTER(c_int32)), face_normal=_ptr(fn), face_centroid=_ptr(fcen),
+        face_shift=_ptr(fsh), cell_centroid=_ptr(cc), cell_volume=_ptr(cv),
+        face_bc=fbc.ctypes.data_as(POINTER(c_int32)) if fbc is not None else None,
+    )
+    return s, keep
+
+
+def physics(gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", recon="upwind", limiter="none", max_lim_grad=1e10,
+            k=0.0, mu=0.0, diff=0.0) -> OrcPhysics:
+    return OrcPhysics(gamma=gamma, Rgas=Rgas, n_species=n_species, flux=FLUX[flux], recon=RECON[recon],
+                      limiter=LIMITER[limiter], max_lim_grad=max_lim_grad, k=k, mu=mu, diff=diff)
+
+
+def rhs(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, bcs=None, time: float = 0.0):
+    """Oracle RHS from a zeroed locF. Returns (R_soa, U_soa with ghost rows filled)."""
+    nc, N = U_soa.shape
+    X = np.ascontiguousarray(U_soa.T)  # AoS [N][nc]
+    F = np.zeros_like(X)
+    ms, keep = _mesh_struct(mesh_arrays)
+    bc_arr = None
+    n_bcs = 0
+    ykeep = []
+    if bcs:
+        n_bcs = max(bcs) + 1
+        bc_arr = (OrcBc * n_bcs)()
+        for slot, b in bcs.items():
+            bc_arr[slot].kind = b["kind"]
+            bc_arr[slot].rho = b.get("rho", 0.0)
+            v = list(b.get("vel", [0.0, 0.0, 0.0])) + [0.0] * 3
+            bc_arr[slot].vel = (c_double * 3)(*v[:3])
+            bc_arr[slot].T = b.get("T", 0.0)
+            if b.get("Yi") is not None:
+                y = np.ascontiguousarray(b["Yi"], dtype=np.float64)
+                ykeep.append(y)
+                bc_arr[slot].Yi = _ptr(y)
+    rc = lib().orc_rhs(ctypes.byref(ms), ctypes.byref(phys), bc_arr, n_bcs, time, _ptr(X), _ptr(F))
+    if rc:
+        raise RuntimeError(f"orc_rhs failed with status {rc}")
+    return np.ascontiguousarray(F.T), np.ascontiguousarray(X.T)
+
+
+class Context:
+    """Oracle setup (LSQ weights, face stencils) kept for repeated RHS evaluations (CPU-baseline timing)."""
+
+    def __init__(self, mesh_arrays, phys: OrcPhysics):
+        self._ms, self._keep = _mesh_struct(mesh_arrays)
+        self._phys = phys
+        self._ctx = ctypes.c_void_p()
+        rc = lib().orc_create(ctypes.byref(self._ms), ctypes.byref(self._phys), ctypes.byref(self._ctx))
+        if rc:
+            raise RuntimeError(f"orc_create failed with status {rc}")
+
+    def rhs_aos(self, X: np.ndarray, F: np.ndarray) -> None:
+        """locF += RHS(locX) on AoS arrays [N][nc] (periodic meshes: no boundary conditions)."""
+        rc = lib().orc_rhs_ctx(self._ctx, None, 0, 0.0, _ptr(X), _ptr(F))
+        if rc:
+            raise RuntimeError(f"orc_rhs_ctx failed with status {rc}")
+
+    def close(self):
+        if self._ctx:
+            lib().orc_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def cell_gradients(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray) -> np.ndarray:
+    nc, N = U_soa.shape
+    X = np.ascontiguousarray(U_soa.T)
+    G = np.zeros((N, nc, mesh_arrays.dim))
+    ms, keep = _mesh_struct(mesh_arrays)
+    rc = lib().orc_cell_gradients(ctypes.byref(ms), ctypes.byref(phys), _ptr(X), _ptr(G))
+    if rc:
+        raise RuntimeError(f"orc_cell_gradients failed with status {rc}")
+    return G
+
+
+# ---- leaf functions
+def ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR):
+    m, p = c_double(), c_double()
+    d = lib().orc_ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, ctypes.byref(m), ctypes.byref(p))
+    return d, m.value, p.value
+
+
+def average_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR):
+    m, p = c_double(), c_double()
+    d = lib().orc_average_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR, ctypes.byref(m), ctypes.byref(p))
+    return d, m.value, p.value
+
+
+def off_flux(*args):
+    m, p = c_double(), c_double()
+    d = lib().orc_off_flux(*args, ctypes.byref(m), ctypes.byref(p))
+    return d, m.value, p.value
+
+
+def ns_advection_flux(normal, xL, xR, gamma=1.4, R=287.0, flux="ausm"):
+    dim = len(normal)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    a = np.ascontiguousarray(xL, dtype=np.float64)
+    b = np.ascontiguousarray(xR, dtype=np.float64)
+    out = np.zeros(dim + 2)
+    lib().orc_ns_advection_flux(dim, _ptr(n), _ptr(a), _ptr(b), gamma, R, FLUX[flux], _ptr(out))
+    return out
+
+
+def stress_tensor(dim, mu, grad_vel):
+    g = np.ascontiguousarray(grad_vel, dtype=np.float64)
+    out = np.zeros(dim * dim)
+    lib().orc_stress_tensor(dim, mu, _ptr(g), _ptr(out))
+    return out
+
+
+def lsq_weights(dx: np.ndarray) -> np.ndarray:
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    n, dim = dx.shape
+    out = np.zeros((n, dim))
+    lib().orc_lsq_weights(n, dim, _ptr(dx), _ptr(out))
+    return out
+
+
+def limiter(name: str, f: float) -> float:
+    return lib().orc_limiter(LIMITER[name], f)
+
+
+def pg(prop: str, euler, gamma=1.4, R=287.0, T=None):
+    u = np.ascontiguousarray(euler, dtype=np.float64)
+    dim = len(u) - 2
+    L = lib()
+    if prop == "pressure":
+        return L.orc_pg_pressure(_ptr(u), dim, gamma, R)
+    if prop == "temperature":
+        return L.orc_pg_temperature(_ptr(u), dim, gamma, R)
+    if prop == "internal_energy":
+        return L.orc_pg_internal_energy(_ptr(u), dim) if T is None else L.orc_pg_internal_energy_T(T, gamma, R)
+    if prop == "speed_of_sound":
+        return L.orc_pg_speed_of_sound(_ptr(u), dim, gamma, R) if T is None else L.orc_pg_speed_of_sound_T(_ptr(u), T, gamma, R)
+    if prop == "sensible_enthalpy":
+        if T is None:
+            T = L.orc_pg_temperature(_ptr(u), dim, gamma, R)
+        return L.orc_pg_sensible_enthalpy_T(T, gamma, R)
+    if prop == "cv":
+        return L.orc_pg_cv(gamma, R)
+    if prop == "cp":
+        return L.orc_pg_cp(gamma, R)
+    if prop == "density":
+        return float(u[0])
+    raise KeyError(prop)

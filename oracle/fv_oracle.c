@@ -1,0 +1,952 @@
+/*
+ * fv_oracle.c -- CPU FP64 restatement of ABLATE's FiniteVolumeSolver::ComputeRHSFunction for the
+ * NavierStokesTransport + SpeciesTransport process set with a PerfectGas EOS and Constant transport.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and the bench's CPU-baseline leg may
+ * call this file; the product library never links it.
+ *
+ * Restated reference files (paths relative to the ABLATE tree):
+ *   src/finiteVolume/fluxCalculator/ausm.cpp:2-43, averageFlux.cpp:2-10, offFlux.cpp:2-8
+ *   src/eos/perfectGas.cpp:49-197
+ *   src/finiteVolume/processes/navierStokesTransport.cpp:99-207 (AdvectionFlux), 454-532 (DiffusionFlux, stress)
+ *   src/finiteVolume/processes/speciesTransport.cpp:98-110 (aux Yi), 196-228 (species diffusion), 263-341
+ *   src/finiteVolume/cellInterpolant.cpp:282-328 (limiter), 330-497 (gradients), 499-639 (face loop),
+ *                                       641-715 (LSQ weights), 890-950 (ProjectToFace)
+ *   src/finiteVolume/faceInterpolant.cpp:109-249, 262-448; stencils/leastSquaresAverage.cpp:5-155
+ *   src/solver/cellSolver.cpp:62-139 (aux update); src/finiteVolume/boundaryConditions/essentialGhost.cpp:5-23
+ *   src/utilities/mathUtilities.hpp:39-57 (NormVector / MagVector)
+ * Third-party arithmetic restated from its published definition (PETSc, version unpinned by the reference):
+ *   PetscFVComputeGradient (least-squares pseudo-inverse), PetscLimiterLimit_* (symmetric limiter forms).
+ * Build-defined parts (no reference counterpart): Rusanov flux, Barth-Jespersen limiter, ghost-cell gradients
+ * (taken as zero), the OUTFLOW extrapolation ghost, periodic-image shifts of the right cell (face_shift).
+ */
+#include "fv_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define PETSC_MAX_REAL DBL_MAX
+/* PETSc's min/max macros: the comparison order decides NaN propagation, keep it. */
+static inline double pmin(double a, double b) { return a < b ? a : b; }
+static inline double pmax(double a, double b) { return a < b ? b : a; }
+static inline double psqr(double a) { return a * a; }
+
+enum { RHO = 0, RHOE = 1, RHOU = 2 };
+
+/* ============================================================== flux calculators */
+
+/* Liou & Steffen AUSM, ausm.cpp:2-43 */
+int orc_ausm(double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR, double pR, double* massFlux,
+             double* p12) {
+    double Mm = uR / aR;
+    double sMm, sPm;
+    if (fabs(Mm) <= 1.) {
+        sMm = -0.25 * psqr(Mm - 1);
+        sPm = -(sMm) * (2 + Mm);
+    } else {
+        sMm = 0.5 * (Mm - fabs(Mm));
+        sPm = (sMm) / Mm;
+    }
+    double Mp = uL / aL;
+    double sMp, sPp;
+    if (fabs(Mp) <= 1.) {
+        sMp = 0.25 * psqr(Mp + 1);
+        sPp = (sMp) * (2 - Mp);
+    } else {
+        sMp = 0.5 * (Mp + fabs(Mp));
+        sPp = (sMp) / Mp;
+    }
+    double m = sMm + sMp;
+    int dir;
+    if (m < 0) {
+        *massFlux = m * aR * rhoR;
+        dir = ORC_RIGHT;
+    } else {
+        *massFlux = m * aL * rhoL;
+        dir = ORC_LEFT;
+    }
+    if (p12) *p12 = pR * sPm + pL * sPp;
+    return dir;
+}
+
+/* averageFlux.cpp:2-10 */
+int orc_average_flux(double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR, double pR,
+                     double* massFlux, double* p12) {
+    (void)aL;
+    (void)aR;
+    *massFlux = 0.5 * (uL * rhoL + uR * rhoR);
+    if (p12) *p12 = 0.5 * (pL + pR);
+    return ORC_NA;
+}
+
+/* offFlux.cpp:2-8 */
+int orc_off_flux(double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR, double pR,
+                 double* massFlux, double* p12) {
+    (void)uL; (void)aL; (void)rhoL; (void)pL; (void)uR; (void)aR; (void)rhoR; (void)pR;
+    *massFlux = 0.0;
+    if (p12) *p12 = 0.0;
+    return ORC_NA;
+}
+
+static int call_flux_calculator(int type, double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR,
+                                double pR, double* massFlux, double* p12) {
+    switch (type) {
+        case ORC_FLUX_AUSM: return orc_ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
+        case ORC_FLUX_AVERAGE: return orc_average_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
+        default: return orc_off_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
+    }
+}
+
+/* ============================================================== PerfectGas, perfectGas.cpp */
+
+double orc_pg_internal_energy(const double* euler, int dim) { /* :91-106 */
+    double density = euler[RHO];
+    double ke = 0.0;
+    for (int d = 0; d < dim; d++) ke += psqr(euler[RHOU + d] / density);
+    ke *= 0.5;
+    return euler[RHOE] / density - ke;
+}
+double orc_pg_pressure(const double* euler, int dim, double gamma, double R) { /* :49-65 */
+    (void)R;
+    double density = euler[RHO];
+    double ke = 0.0;
+    for (int d = 0; d < dim; d++) ke += psqr(euler[RHOU + d] / density);
+    ke *= 0.5;
+    double internalEnergy = euler[RHOE] / density - ke;
+    return (gamma - 1.0) * density * internalEnergy;
+}
+double orc_pg_cv(double gamma, double R) { return R / (gamma - 1.0); }                 /* :128-133 */
+double orc_pg_cp(double gamma, double R) { return gamma * R / (gamma - 1.0); }         /* :138-143 */
+double orc_pg_temperature(const double* euler, int dim, double gamma, double R) {     /* :76-86 */
+    double internalEnergy = orc_pg_internal_energy(euler, dim);
+    double cv = R / (gamma - 1.0);
+    return internalEnergy / cv;
+}
+double orc_pg_internal_energy_T(double T, double gamma, double R) { /* :107-113 */
+    double cv = R / (gamma - 1.0);
+    return T * cv;
+}
+double orc_pg_pressure_T(const double* euler, double T, double R) { return R * euler[RHO] * T; } /* :67-74 */
+double orc_pg_speed_of_sound(const double* euler, int dim, double gamma, double R) {            /* :147-164 */
+    double density = euler[RHO];
+    double p = orc_pg_pressure(euler, dim, gamma, R);
+    return sqrt(gamma * p / density);
+}
+double orc_pg_speed_of_sound_T(const double* euler, double T, double gamma, double R) { /* :165-174 */
+    double density = euler[RHO];
+    double p = orc_pg_pressure_T(euler, T, R);
+    return sqrt(gamma * (p) / density);
+}
+double orc_pg_sensible_enthalpy_T(double T, double gamma, double R) { /* :121-127 */
+    double cp = gamma * R / (gamma - 1.0);
+    return T * cp;
+}
+
+/* ============================================================== physics plug-ins */
+
+/* MathUtilities::NormVector / MagVector, mathUtilities.hpp:39-57 */
+static void norm_vector(int dim, const double* in, double* out) {
+    double mag = 0.0;
+    for (int d = 0; d < dim; d++) mag += in[d] * in[d];
+    mag = sqrt(mag);
+    for (int d = 0; d < dim; d++) out[d] = in[d] / mag;
+}
+static double mag_vector(int dim, const double* in) {
+    double mag = 0.0;
+    for (int d = 0; d < dim; d++) mag += in[d] * in[d];
+    return sqrt(mag);
+}
+
+/* Per-side decode used by AdvectionFlux, navierStokesTransport.cpp:122-140: the temperature guess from the aux
+ * field is ignored by PerfectGas (perfectGas.cpp:88-90), so T comes from the conserved state. */
+typedef struct {
+    double rho, un, vel[3], e, a, p;
+} side_state;
+
+static void decode_side(int dim, const double* field, const double* norm, double gamma, double R, side_state* s) {
+    s->rho = field[RHO];
+    double T = orc_pg_temperature(field, dim, gamma, R);
+    s->un = 0.0;
+    for (int d = 0; d < dim; d++) {
+        s->vel[d] = field[RHOU + d] / s->rho;
+        s->un += s->vel[d] * norm[d];
+    }
+    s->e = orc_pg_internal_energy_T(T, gamma, R);
+    s->a = orc_pg_speed_of_sound_T(field, T, gamma, R);
+    s->p = orc_pg_pressure_T(field, T, R);
+}
+
+/* NavierStokesTransport::AdvectionFlux, navierStokesTransport.cpp:99-207 */
+int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, const double* fieldR, double gamma,
+                          double R, int flux_calculator, double* flux) {
+    double norm[3];
+    norm_vector(dim, normal, norm);
+    const double areaMag = mag_vector(dim, normal);
+    side_state L, Rs;
+    decode_side(dim, fieldL, norm, gamma, R, &L);
+    decode_side(dim, fieldR, norm, gamma, R, &Rs);
+    double massFlux, p12;
+    int direction = call_flux_calculator(flux_calculator, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, &p12);
+    if (direction == ORC_LEFT) {
+        flux[RHO] = massFlux * areaMag;
+        double velMagL = mag_vector(dim, L.vel);
+        double HL = L.e + velMagL * velMagL / 2.0 + L.p / L.rho;
+        flux[RHOE] = HL * massFlux * areaMag;
+        for (int n = 0; n < dim; n++) flux[RHOU + n] = L.vel[n] * massFlux * areaMag + p12 * normal[n];
+    } else if (direction == ORC_RIGHT) {
+        flux[RHO] = massFlux * areaMag;
+        double velMagR = mag_vector(dim, Rs.vel);
+        double HR = Rs.e + velMagR * velMagR / 2.0 + Rs.p / Rs.rho;
+        flux[RHOE] = HR * massFlux * areaMag;
+        for (int n = 0; n < dim; n++) flux[RHOU + n] = Rs.vel[n] * massFlux * areaMag + p12 * normal[n];
+    } else {
+        flux[RHO] = massFlux * areaMag;
+        double velMagL = mag_vector(dim, L.vel);
+        double HL = L.e + velMagL * velMagL / 2.0 + L.p / L.rho;
+        double velMagR = mag_vector(dim, Rs.vel);
+        double HR = Rs.e + velMagR * velMagR / 2.0 + Rs.p / Rs.rho;
+        flux[RHOE] = 0.5 * (HL + HR) * massFlux * areaMag;
+        for (int n = 0; n < dim; n++) flux[RHOU + n] = 0.5 * (L.vel[n] + Rs.vel[n]) * massFlux * areaMag + p12 * normal[n];
+    }
+    return 0;
+}
+
+/* SpeciesTransport::AdvectionFlux, speciesTransport.cpp:263-341. The calculator is called with p12 = NULL and any
+ * non-LEFT direction (including NA) takes the right state. fieldL/R point at the euler field; yiL/yiR at
+ * densityYi. */
+static void species_advection_flux(int dim, const double* normal, const double* fieldL, const double* yiL,
+                                   const double* fieldR, const double* yiR, double gamma, double R, int flux_calculator,
+                                   int ns, double* flux) {
+    double norm[3];
+    norm_vector(dim, normal, norm);
+    const double areaMag = mag_vector(dim, normal);
+    side_state L, Rs;
+    /* same decode as the euler flux (T, u_n, a, p); the internal energy is not needed here */
+    decode_side(dim, fieldL, norm, gamma, R, &L);
+    decode_side(dim, fieldR, norm, gamma, R, &Rs);
+    double massFlux;
+    if (call_flux_calculator(flux_calculator, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, NULL) == ORC_LEFT) {
+        for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiL[sp] / L.rho) * areaMag;
+    } else {
+        for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiR[sp] / Rs.rho) * areaMag;
+    }
+}
+
+/* Rusanov (local Lax-Friedrichs) full-vector flux, build-defined (Rusanov 1961; Toro 2009 sec. 10.5):
+ * F = A * [ 1/2 (f(U_L) + f(U_R)).n_hat - 1/2 lambda (U_R - U_L) ],  lambda = max(|u_n| + a) over both sides.
+ * Pressure and sound speed use the same PerfectGas T-route as AdvectionFlux. */
+static void rusanov_flux(int dim, const double* normal, const double* uL, const double* uR, double gamma, double R,
+                         int ns, double* flux) {
+    double norm[3];
+    norm_vector(dim, normal, norm);
+    const double areaMag = mag_vector(dim, normal);
+    side_state L, Rs;
+    decode_side(dim, uL, norm, gamma, R, &L);
+    decode_side(dim, uR, norm, gamma, R, &Rs);
+    double lambda = pmax(fabs(L.un) + L.a, fabs(Rs.un) + Rs.a);
+    int nc = dim + 2 + ns;
+    for (int c = 0; c < nc; c++) {
+        double fL, fR;
+        if (c == RHO) {
+            fL = uL[RHO] * L.un;
+            fR = uR[RHO] * Rs.un;
+        } else if (c == RHOE) {
+            fL = (uL[RHOE] + L.p) * L.un;
+            fR = (uR[RHOE] + Rs.p) * Rs.un;
+        } else if (c < dim + 2) {
+            int d = c - RHOU;
+            fL = uL[c] * L.un + L.p * norm[d];
+            fR = uR[c] * Rs.un + Rs.p * norm[d];
+        } else {
+            fL = uL[c] * L.un;
+            fR = uR[c] * Rs.un;
+        }
+        flux[c] = areaMag * (0.5 * (fL + fR) - 0.5 * lambda * (uR[c] - uL[c]));
+    }
+}
+
+/* CompressibleFlowComputeStressTensor, navierStokesTransport.cpp:510-532 */
+void orc_stress_tensor(int dim, double mu, const double* gradVel, double* tau) {
+    double divVel = 0.0;
+    for (int c = 0; c < dim; ++c) divVel += gradVel[c * dim + c];
+    for (int c = 0; c < dim; ++c) {
+        for (int d = 0; d < dim; ++d) {
+            if (d == c) {
+                tau[c * dim + d] = 2.0 * mu * ((gradVel[c * dim + d]) - divVel / 3.0);
+            } else {
+                tau[c * dim + d] = mu * ((gradVel[c * dim + d]) + (gradVel[d * dim + c]));
+            }
+        }
+    }
+}
+
+/* NavierStokesTransport::DiffusionFlux, navierStokesTransport.cpp:454-508. vel = face velocity, gradT[dim],
+ * gradVel[dim*dim] (component-major). Writes dim+2 values. */
+static void ns_diffusion_flux(int dim, const double* normal, double mu, double k, const double* vel, const double* gradT,
+                              const double* gradVel, double* flux) {
+    double tau[9];
+    orc_stress_tensor(dim, mu, gradVel, tau);
+    for (int c = 0; c < dim; ++c) {
+        double viscousFlux = 0.0;
+        for (int d = 0; d < dim; ++d) viscousFlux += -normal[d] * tau[c * dim + d];
+        flux[RHOU + c] = viscousFlux;
+    }
+    flux[RHOE] = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        double heatFlux = 0.0;
+        for (int c = 0; c < dim; ++c) heatFlux += vel[c] * tau[d * dim + c];
+        heatFlux += k * gradT[d];
+        heatFlux *= -normal[d];
+        flux[RHOE] += heatFlux;
+    }
+    flux[RHO] = 0.0;
+}
+
+/* ============================================================== least squares (PetscFVComputeGradient) */
+
+/* Symmetric Jacobi eigen-decomposition of a dim x dim matrix (dim <= 3): A = V diag(w) V^T. */
+static void jacobi_eigen(int n, double* A, double* V, double* w) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) V[i * n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; i++) {
+            diag += A[i * n + i] * A[i * n + i];
+            for (int j = i + 1; j < n; j++) off += A[i * n + j] * A[i * n + j];
+        }
+        if (off <= 1e-34 * diag || off == 0.0) break;
+        for (int p = 0; p < n; p++) {
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) { /* A = J^T A J */
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+}
+
+/* Minimum-norm least-squares pseudo-inverse of the n x dim matrix dx (rows = neighbour offsets), the quantity
+ * PetscFVComputeGradient_LeastSquares returns (LAPACK gelss-style, singular values below eps * s_max dropped):
+ * grad[f*dim + d] = pinv(dx)[d][f].  Computed as V diag(1/lambda) V^T dx^T from the eigen-decomposition of
+ * dx^T dx. */
+int orc_lsq_weights(int n, int dim, const double* dx, double* grad) {
+    double M[9] = {0}, V[9], w[3];
+    for (int i = 0; i < dim; i++)
+        for (int j = 0; j < dim; j++) {
+            double s = 0.0;
+            for (int f = 0; f < n; f++) s += dx[f * dim + i] * dx[f * dim + j];
+            M[i * dim + j] = s;
+        }
+    jacobi_eigen(dim, M, V, w);
+    double wmax = 0.0;
+    for (int i = 0; i < dim; i++) wmax = pmax(wmax, fabs(w[i]));
+    double Minv[9] = {0};
+    for (int k = 0; k < dim; k++) {
+        /* lambda = s^2 of dx; eigenvalues of dx^T dx carry an absolute error ~eps * lambda_max, so singular
+         * values below 1e-6 s_max are treated as zero (exact rank deficiency of geometric stencils). */
+        if (!(w[k] > 1e-12 * wmax) || wmax == 0.0) continue;
+        for (int i = 0; i < dim; i++)
+            for (int j = 0; j < dim; j++) Minv[i * dim + j] += V[i * dim + k] * V[j * dim + k] / w[k];
+    }
+    for (int f = 0; f < n; f++)
+        for (int d = 0; d < dim; d++) {
+            double s = 0.0;
+            for (int j = 0; j < dim; j++) s += Minv[d * dim + j] * dx[f * dim + j];
+            grad[f * dim + d] = s;
+        }
+    return 0;
+}
+
+/* ============================================================== limiters */
+
+/* PETSc scalar limiters in the symmetric form used by cellInterpolant.cpp:316-324 (phi(1/2) = 1 is the smooth
+ * case). Restated from PETSc's PetscLimiterLimit_* definitions; parity unpinned (PETSc absent). */
+double orc_limiter(int type, double f) {
+    switch (type) {
+        case ORC_LIM_NONE: return 1.0;
+        case ORC_LIM_ZERO: return 0.0;
+        case ORC_LIM_MINMOD: return 2 * pmax(0, pmin(f, 1 - f));
+        case ORC_LIM_VANLEER: return pmax(0, 4 * f * (1 - f));
+        case ORC_LIM_VANALBADA: return pmax(0, 2 * f * (1 - f) / (psqr(f) + psqr(1 - f)));
+        case ORC_LIM_SUPERBEE: return 2 * pmax(0, pmax(pmin(2 * f, 1 - f), pmin(f, 2 * (1 - f))));
+        case ORC_LIM_MC: return pmin(1, 4 * pmax(0, pmin(f, 1 - f)));
+        case ORC_LIM_SIN: return sin(M_PI * pmax(pmin(f, 1), 0));
+        default: return 1.0;
+    }
+}
+
+/* ============================================================== mesh helpers */
+
+typedef struct {
+    const orc_mesh* m;
+    int dim, ns, nc, naux;
+    int64_t N, NI, F;
+    int32_t* cone_off; /* cell -> faces (increasing face number), the DMPlex cone order used here */
+    int32_t* cone;
+} topo;
+
+static int is_boundary_face(const topo* t, int64_t f) { return t->m->face_cells[2 * f + 1] >= t->NI; }
+
+/* centroid of the neighbour of cell c across face f, expressed in c's frame */
+static void neighbor_centroid(const topo* t, int64_t f, int64_t c, double* x) {
+    const double* sh = t->m->face_shift ? &t->m->face_shift[3 * f] : NULL;
+    int32_t l = t->m->face_cells[2 * f], r = t->m->face_cells[2 * f + 1];
+    if (c == l) {
+        for (int d = 0; d < 3; d++) x[d] = t->m->cell_centroid[3 * r + d] + (sh ? sh[d] : 0.0);
+    } else {
+        for (int d = 0; d < 3; d++) x[d] = t->m->cell_centroid[3 * l + d] - (sh ? sh[d] : 0.0);
+    }
+}
+/* face centroid expressed in cell c's frame */
+static void face_centroid_in(const topo* t, int64_t f, int64_t c, double* x) {
+    const double* sh = t->m->face_shift ? &t->m->face_shift[3 * f] : NULL;
+    int32_t l = t->m->face_cells[2 * f];
+    for (int d = 0; d < 3; d++) x[d] = t->m->face_centroid[3 * f + d] - ((c != l && sh) ? sh[d] : 0.0);
+}
+
+static int build_topo(const orc_mesh* m, int ns, topo* t) {
+    t->m = m;
+    t->dim = m->dim;
+    t->ns = ns;
+    t->nc = m->dim + 2 + ns;
+    t->naux = 1 + m->dim + ns;
+    t->N = m->n_cells;
+    t->NI = m->n_interior;
+    t->F = m->n_faces;
+    t->cone_off = (int32_t*)calloc((size_t)t->N + 1, sizeof(int32_t));
+    t->cone = (int32_t*)malloc(sizeof(int32_t) * (size_t)(2 * t->F + 1));
+    if (!t->cone_off || !t->cone) return 1;
+    for (int64_t f = 0; f < t->F; f++) {
+        t->cone_off[m->face_cells[2 * f] + 1]++;
+        t->cone_off[m->face_cells[2 * f + 1] + 1]++;
+    }
+    for (int64_t c = 0; c < t->N; c++) t->cone_off[c + 1] += t->cone_off[c];
+    int32_t* fill = (int32_t*)calloc((size_t)t->N, sizeof(int32_t));
+    for (int64_t f = 0; f < t->F; f++) {
+        for (int s = 0; s < 2; s++) {
+            int32_t c = m->face_cells[2 * f + s];
+            t->cone[t->cone_off[c] + fill[c]++] = (int32_t)f;
+        }
+    }
+    free(fill);
+    return 0;
+}
+static void free_topo(topo* t) {
+    free(t->cone_off);
+    free(t->cone);
+}
+
+/* ============================================================== RHS stages */
+
+/* Ghost fill, DMPlexInsertBoundaryValuesRiemann -> EssentialGhostUpdate (essentialGhost.cpp:33-51) with
+ * enforceAtFace = false, or the build-defined extrapolation (outflow) ghost. */
+static int ghost_fill(const topo* t, const orc_physics* ph, const orc_bc* bcs, int n_bcs, double* X) {
+    if (!t->m->face_bc) return 0;
+    int nc = t->nc, dim = t->dim;
+    for (int64_t f = 0; f < t->F; f++) {
+        int slot = t->m->face_bc[f];
+        if (slot < 0) continue;
+        if (!bcs || slot >= n_bcs) return 2;
+        const orc_bc* bc = &bcs[slot];
+        const double* xI = &X[(size_t)t->m->face_cells[2 * f] * nc];
+        double* xG = &X[(size_t)t->m->face_cells[2 * f + 1] * nc];
+        if (bc->kind == ORC_BC_INFLOW) {
+            double ke = 0.0;
+            for (int d = 0; d < dim; d++) ke += bc->vel[d] * bc->vel[d];
+            xG[RHO] = bc->rho;
+            xG[RHOE] = bc->rho * (orc_pg_internal_energy_T(bc->T, ph->gamma, ph->Rgas) + 0.5 * ke);
+            for (int d = 0; d < dim; d++) xG[RHOU + d] = bc->rho * bc->vel[d];
+            for (int sp = 0; sp < t->ns; sp++) xG[dim + 2 + sp] = bc->rho * (bc->Yi ? bc->Yi[sp] : 0.0);
+        } else if (bc->kind == ORC_BC_OUTFLOW) {
+            for (int c = 0; c < nc; c++) xG[c] = xI[c];
+        } else {
+            return 3;
+        }
+    }
+    return 0;
+}
+
+/* CellSolver::UpdateAuxFields (cellSolver.cpp:62-139) for every cell incl. ghosts:
+ * temperature (UpdateAuxTemperatureField), velocity (UpdateAuxVelocityField), Yi (UpdateAuxMassFractionField). */
+static void update_aux(const topo* t, const orc_physics* ph, const double* X, double* aux) {
+    int nc = t->nc, na = t->naux, dim = t->dim;
+    for (int64_t c = 0; c < t->N; c++) {
+        const double* u = &X[(size_t)c * nc];
+        double* a = &aux[(size_t)c * na];
+        a[0] = orc_pg_temperature(u, dim, ph->gamma, ph->Rgas);
+        double density = u[RHO];
+        for (int d = 0; d < dim; d++) a[1 + d] = u[RHOU + d] / density;
+        for (int sp = 0; sp < t->ns; sp++) a[1 + dim + sp] = u[dim + 2 + sp] / density;
+    }
+}
+
+/* LSQ weights per face side (fg->grad[side]), BuildGradientReconstruction_Internal, cellInterpolant.cpp:641-715:
+ * for each interior cell, its non-boundary faces, dx = x_neighbour - x_cell. */
+static int build_cell_lsq(const topo* t, double* fgrad /* [F][2][3] */) {
+    int dim = t->dim;
+    memset(fgrad, 0, sizeof(double) * (size_t)t->F * 6);
+    double dx[64 * 3], grad[64 * 3];
+    int used[64];
+    for (int64_t c = 0; c < t->NI; c++) {
+        int nu = 0;
+        double xc[3];
+        for (int d = 0; d < 3; d++) xc[d] = t->m->cell_centroid[3 * c + d];
+        for (int32_t k = t->cone_off[c]; k < t->cone_off[c + 1]; k++) {
+            int64_t f = t->cone[k];
+            if (is_boundary_face(t, f)) continue;
+            if (nu >= 64) return 4;
+            double xn[3];
+            neighbor_centroid(t, f, c, xn);
+            for (int d = 0; d < dim; d++) dx[nu * dim + d] = xn[d] - xc[d];
+            used[nu++] = (int)f;
+        }
+        if (nu == 0) return 5;
+        orc_lsq_weights(nu, dim, dx, grad);
+        for (int i = 0; i < nu; i++) {
+            int64_t f = used[i];
+            int side = (c != t->m->face_cells[2 * f]);
+            for (int d = 0; d < dim; d++) fgrad[f * 6 + side * 3 + d] = grad[i * dim + d];
+        }
+    }
+    return 0;
+}
+
+/* ComputeFieldGradients for one field (components [off, off+dof) of the local state), cellInterpolant.cpp:330-497:
+ * face loop accumulation (skipping boundary faces), then the cell loop limiter with the maxLimGrad cancel rule. */
+static void field_gradients(const topo* t, const orc_physics* ph, const double* X, const double* fgrad, int off, int dof,
+                            double* grad /* [N][nc][dim] */) {
+    int nc = t->nc, dim = t->dim;
+    for (int64_t f = 0; f < t->F; f++) {
+        if (is_boundary_face(t, f)) continue;
+        int32_t cl = t->m->face_cells[2 * f], cr = t->m->face_cells[2 * f + 1];
+        const double* cx0 = &X[(size_t)cl * nc + off];
+        const double* cx1 = &X[(size_t)cr * nc + off];
+        double* g0 = &grad[((size_t)cl * nc + off) * dim];
+        double* g1 = &grad[((size_t)cr * nc + off) * dim];
+        for (int pd = 0; pd < dof; ++pd) {
+            double delta = cx1[pd] - cx0[pd];
+            for (int d = 0; d < dim; ++d) {
+                g0[pd * dim + d] += fgrad[f * 6 + 0 + d] * delta;
+                g1[pd * dim + d] -= fgrad[f * 6 + 3 + d] * delta;
+            }
+        }
+    }
+    if (ph->limiter == ORC_LIM_NONE) return; /* phi = 1: the loop below would leave the gradient bit-identical */
+    double cellPhi[64];
+    for (int64_t c = 0; c < t->NI; c++) {
+        const double* cx = &X[(size_t)c * nc + off];
+        double* cg = &grad[((size_t)c * nc + off) * dim];
+        double xc[3];
+        for (int d = 0; d < 3; d++) xc[d] = t->m->cell_centroid[3 * c + d];
+        for (int d = 0; d < dof; ++d) cellPhi[d] = PETSC_MAX_REAL;
+        if (ph->limiter == ORC_LIM_BJ) {
+            /* Barth-Jespersen (build-defined): phi = min_f min(1, (u_max - u_c)/D_f | (u_min - u_c)/D_f),
+             * D_f = grad.(x_f - x_c), u_min/u_max over the cell and its face neighbours. */
+            for (int pd = 0; pd < dof; ++pd) {
+                double umin = cx[pd], umax = cx[pd];
+                for (int32_t k = t->cone_off[c]; k < t->cone_off[c + 1]; k++) {
+                    int64_t f = t->cone[k];
+                    int32_t n = t->m->face_cells[2 * f] == c ? t->m->face_cells[2 * f + 1] : t->m->face_cells[2 * f];
+                    double un = X[(size_t)n * nc + off + pd];
+                    umin = fmin(umin, un);
+                    umax = fmax(umax, un);
+                }
+                for (int32_t k = t->cone_off[c]; k < t->cone_off[c + 1]; k++) {
+                    int64_t f = t->cone[k];
+                    double xf[3];
+                    face_centroid_in(t, f, c, xf);
+                    double D = 0.0;
+                    for (int d = 0; d < dim; d++) D += cg[pd * dim + d] * (xf[d] - xc[d]);
+                    double phi = 1.0;
+                    if (D > 0) phi = fmin(1.0, (umax - cx[pd]) / D);
+                    else if (D < 0) phi = fmin(1.0, (umin - cx[pd]) / D);
+                    cellPhi[pd] = pmin(cellPhi[pd], phi);
+                }
+            }
+        } else {
+            /* DMPlexApplyLimiter_Internal, cellInterpolant.cpp:282-328 */
+            for (int32_t k = t->cone_off[c]; k < t->cone_off[c + 1]; k++) {
+                int64_t f = t->cone[k];
+                int32_t n = t->m->face_cells[2 * f] == c ? t->m->face_cells[2 * f + 1] : t->m->face_cells[2 * f];
+                const double* ncx = &X[(size_t)n * nc + off];
+                double xn[3], v[3];
+                neighbor_centroid(t, f, c, xn);
+                for (int d = 0; d < dim; d++) v[d] = xn[d] - xc[d];
+                for (int d = 0; d < dof; ++d) {
+                    double denom = 0.0;
+                    for (int e = 0; e < dim; e++) denom += cg[d * dim + e] * v[e];
+                    double flim = 0.5 * (ncx[d] - cx[d]) / denom;
+                    double phi = orc_limiter(ph->limiter, flim);
+                    cellPhi[d] = pmin(cellPhi[d], phi);
+                }
+            }
+        }
+        int cancel = 0;
+        for (int pd = 0; pd < dof; ++pd) {
+            if (cellPhi[pd] == 0) {
+                for (int d = 0; d < dim; d++)
+                    if (fabs(cg[pd * dim + d]) > ph->max_lim_grad) cancel = 1;
+            }
+        }
+        for (int pd = 0; pd < dof; ++pd)
+            for (int d = 0; d < dim; ++d) {
+                if (cancel)
+                    cg[pd * dim + d] *= 0;
+                else
+                    cg[pd * dim + d] *= cellPhi[pd];
+            }
+    }
+}
+
+int orc_cell_gradients(const orc_mesh* mesh, const orc_physics* ph, const double* X, double* grad) {
+    topo t;
+    if (build_topo(mesh, ph->n_species, &t)) return 1;
+    memset(grad, 0, sizeof(double) * (size_t)t.N * t.nc * t.dim);
+    int rc = 0;
+    if (ph->recon == ORC_RECON_LSQ) {
+        double* fgrad = (double*)malloc(sizeof(double) * (size_t)t.F * 6);
+        rc = build_cell_lsq(&t, fgrad);
+        if (!rc) {
+            field_gradients(&t, ph, X, fgrad, 0, t.dim + 2, grad);
+            if (t.ns) field_gradients(&t, ph, X, fgrad, t.dim + 2, t.ns, grad);
+        }
+        free(fgrad);
+    }
+    free_topo(&t);
+    return rc;
+}
+
+/* ProjectToFace for one cell (cellInterpolant.cpp:890-950): u = x_c + grad.(x_f - x_c) when gradients exist. */
+static void project_to_face(const topo* t, int64_t f, int64_t c, const double* X, const double* grad, double* u) {
+    int nc = t->nc, dim = t->dim;
+    const double* xc = &X[(size_t)c * nc];
+    if (!grad) {
+        for (int k = 0; k < nc; k++) u[k] = xc[k];
+        return;
+    }
+    double xf[3], dx[3];
+    face_centroid_in(t, f, c, xf);
+    for (int d = 0; d < dim; d++) dx[d] = xf[d] - t->m->cell_centroid[3 * c + d];
+    const double* g = &grad[(size_t)c * nc * dim];
+    for (int k = 0; k < nc; k++) {
+        double dot = 0.0;
+        for (int d = 0; d < dim; d++) dot += g[k * dim + d] * dx[d];
+        u[k] = xc[k] + dot;
+    }
+}
+
+/* ---- face interpolant (LeastSquaresAverage), faceInterpolant.cpp:7-73,109-249 + leastSquaresAverage.cpp ---- */
+
+typedef struct {
+    int n;
+    int32_t cell[128];
+    double w[128];
+    double gw[128 * 3];
+} stencil_t;
+
+/* ComputeNeighborCellStencil, leastSquaresAverage.cpp:69-155: the cell plus every neighbour across a face with two
+ * supports (ghost neighbours included); LSQ gradient weights, then the self weight -sum(w). */
+static int neighbor_cell_stencil(const topo* t, int64_t c, stencil_t* s) {
+    int dim = t->dim;
+    s->n = 0;
+    s->cell[s->n++] = (int32_t)c;
+    double dxs[128 * 3], xc[3];
+    for (int d = 0; d < 3; d++) xc[d] = t->m->cell_centroid[3 * c + d];
+    for (int32_t k = t->cone_off[c]; k < t->cone_off[c + 1]; k++) {
+        int64_t f = t->cone[k];
+        int32_t n = t->m->face_cells[2 * f] == c ? t->m->face_cells[2 * f + 1] : t->m->face_cells[2 * f];
+        if (s->n >= 128) return 6;
+        double xn[3];
+        neighbor_centroid(t, f, c, xn);
+        for (int d = 0; d < dim; d++) dxs[(s->n - 1) * dim + d] = xn[d] - xc[d];
+        s->cell[s->n++] = n;
+    }
+    for (int i = 0; i < s->n; i++) {
+        s->w[i] = 0.0;
+        for (int d = 0; d < dim; d++) s->gw[i * dim + d] = 0.0;
+    }
+    s->w[0] = 1.0;
+    orc_lsq_weights(s->n - 1, dim, dxs, s->gw + dim);
+    double org[128 * 3];
+    memcpy(org, s->gw, sizeof(double) * (size_t)s->n * dim);
+    for (int gn = 0; gn < s->n; gn++)
+        for (int fn = 0; fn < s->n; fn++)
+            for (int d = 0; d < dim; ++d) s->gw[fn * dim + d] -= org[gn * dim + d] * s->w[fn];
+    return 0;
+}
+
+static int cmp_i32(const void* a, const void* b) {
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* LeastSquaresAverage::Generate, leastSquaresAverage.cpp:5-67: merged, sorted stencil of the two cells. The
+ * ghost side of a boundary face contributes its own one-neighbour stencil. */
+static int face_stencil(const topo* t, int64_t f, stencil_t* out) {
+    int dim = t->dim;
+    stencil_t L, R;
+    int rc = neighbor_cell_stencil(t, t->m->face_cells[2 * f], &L);
+    if (!rc) rc = neighbor_cell_stencil(t, t->m->face_cells[2 * f + 1], &R);
+    if (rc) return rc;
+    int32_t all[256];
+    int n = 0;
+    for (int i = 0; i < L.n; i++) all[n++] = L.cell[i];
+    for (int i = 0; i < R.n; i++) all[n++] = R.cell[i];
+    qsort(all, (size_t)n, sizeof(int32_t), cmp_i32);
+    int u = 0;
+    for (int i = 0; i < n; i++)
+        if (u == 0 || all[i] != all[u - 1]) all[u++] = all[i];
+    out->n = u;
+    for (int i = 0; i < u; i++) {
+        out->cell[i] = all[i];
+        out->w[i] = 0.0;
+        for (int d = 0; d < dim; d++) out->gw[i * dim + d] = 0.0;
+    }
+    int numberStencils = (L.n ? 1 : 0) + (R.n ? 1 : 0);
+    const stencil_t* parts[2] = {&L, &R};
+    for (int p = 0; p < 2; p++) {
+        const stencil_t* s = parts[p];
+        for (int i = 0; i < s->n; i++) {
+            int loc = 0;
+            while (out->cell[loc] != s->cell[i]) loc++;
+            out->w[loc] += s->w[i] / numberStencils;
+            for (int d = 0; d < dim; d++) out->gw[loc * dim + d] += s->gw[i * dim + d] / numberStencils;
+        }
+    }
+    return 0;
+}
+
+/* ============================================================== top level */
+
+/* Setup state (the reference builds these once: the CellInterpolant LSQ weights and the FaceInterpolant
+ * stencils are constructed lazily on the first RHS call, finiteVolumeSolver.cpp:241-243,269-271). */
+struct orc_ctx {
+    topo t;
+    const orc_mesh* mesh;
+    orc_physics ph;
+    double* fgrad;     /* [F][2][3] cell LSQ weights, or NULL */
+    int64_t* st_off;   /* face stencils (CSR), or NULL */
+    int32_t* st_cell;
+    double* st_w;
+    double* st_gw;
+    double* aux;       /* [N][naux] */
+    double* grad;      /* [N][nc][dim] */
+};
+
+void orc_destroy(orc_ctx* c) {
+    if (!c) return;
+    free_topo(&c->t);
+    free(c->fgrad);
+    free(c->st_off);
+    free(c->st_cell);
+    free(c->st_w);
+    free(c->st_gw);
+    free(c->aux);
+    free(c->grad);
+    free(c);
+}
+
+int orc_create(const orc_mesh* mesh, const orc_physics* ph, orc_ctx** out) {
+    orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+    if (!c) return 1;
+    *out = NULL;
+    c->mesh = mesh;
+    c->ph = *ph;
+    if (build_topo(mesh, ph->n_species, &c->t)) {
+        orc_destroy(c);
+        return 1;
+    }
+    topo* t = &c->t;
+    c->aux = (double*)malloc(sizeof(double) * (size_t)t->N * t->naux);
+    if (ph->flux != ORC_FLUX_NONE && ph->recon == ORC_RECON_LSQ) {
+        c->fgrad = (double*)malloc(sizeof(double) * (size_t)t->F * 6);
+        c->grad = (double*)malloc(sizeof(double) * (size_t)t->N * t->nc * t->dim);
+        int rc = build_cell_lsq(t, c->fgrad);
+        if (rc) {
+            orc_destroy(c);
+            return rc;
+        }
+    }
+    int transport_active = (ph->k != 0.0) || (ph->mu != 0.0) || (ph->diff != 0.0);
+    if (transport_active) {
+        stencil_t* st = (stencil_t*)malloc(sizeof(stencil_t));
+        int64_t cap = 16 * t->F + 16;
+        c->st_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(t->F + 1));
+        c->st_cell = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
+        c->st_w = (double*)malloc(sizeof(double) * (size_t)cap);
+        c->st_gw = (double*)malloc(sizeof(double) * (size_t)cap * t->dim);
+        c->st_off[0] = 0;
+        for (int64_t f = 0; f < t->F; f++) {
+            int rc = face_stencil(t, f, st);
+            if (rc) {
+                free(st);
+                orc_destroy(c);
+                return rc;
+            }
+            int64_t o = c->st_off[f];
+            if (o + st->n > cap) {
+                cap = 2 * (o + st->n);
+                c->st_cell = (int32_t*)realloc(c->st_cell, sizeof(int32_t) * (size_t)cap);
+                c->st_w = (double*)realloc(c->st_w, sizeof(double) * (size_t)cap);
+                c->st_gw = (double*)realloc(c->st_gw, sizeof(double) * (size_t)cap * t->dim);
+            }
+            for (int i = 0; i < st->n; i++) {
+                c->st_cell[o + i] = st->cell[i];
+                c->st_w[o + i] = st->w[i];
+                for (int d = 0; d < t->dim; d++) c->st_gw[(o + i) * t->dim + d] = st->gw[i * t->dim + d];
+            }
+            c->st_off[f + 1] = o + st->n;
+        }
+        free(st);
+    }
+    *out = c;
+    return 0;
+}
+
+int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double* X, double* locF) {
+    (void)time;
+    const topo* tp = &ctx->t;
+    const topo t = *tp;
+    const orc_mesh* mesh = ctx->mesh;
+    const orc_physics* ph = &ctx->ph;
+    const int dim = t.dim, nc = t.nc, na = t.naux, ns = t.ns;
+    int rc = ghost_fill(&t, ph, bcs, n_bcs, X);
+    if (rc) return rc;
+    double* aux = ctx->aux;
+    update_aux(&t, ph, X, aux);
+
+    /* ---- discontinuous flux functions (CellInterpolant::ComputeRHS, cellInterpolant.cpp:43-140) ---- */
+    if (ph->flux != ORC_FLUX_NONE) {
+        double* grad = NULL;
+        if (ph->recon == ORC_RECON_LSQ) {
+            grad = ctx->grad;
+            memset(grad, 0, sizeof(double) * (size_t)t.N * nc * dim);
+            field_gradients(&t, ph, X, ctx->fgrad, 0, dim + 2, grad); /* euler field */
+            if (ns) field_gradients(&t, ph, X, ctx->fgrad, dim + 2, ns, grad); /* densityYi field */
+        }
+        double uL[64], uR[64], flux[64];
+        /* ComputeFluxSourceTerms, cellInterpolant.cpp:561-631 */
+        for (int64_t f = 0; f < t.F; f++) {
+            int32_t cl = mesh->face_cells[2 * f], cr = mesh->face_cells[2 * f + 1];
+            const double* normal = &mesh->face_normal[3 * f];
+            project_to_face(&t, f, cl, X, grad, uL);
+            project_to_face(&t, f, cr, X, (cr < t.NI) ? grad : NULL, uR); /* ghost gradients are zero */
+            double vL = mesh->cell_volume[cl], vR = mesh->cell_volume[cr];
+            double* fLp = (cl < t.NI) ? &locF[(size_t)cl * nc] : NULL;
+            double* fRp = (cr < t.NI) ? &locF[(size_t)cr * nc] : NULL;
+            if (ph->flux == ORC_FLUX_RUSANOV) {
+                for (int k = 0; k < nc; k++) flux[k] = 0.0;
+                rusanov_flux(dim, normal, uL, uR, ph->gamma, ph->Rgas, ns, flux);
+                for (int d = 0; d < nc; ++d) {
+                    if (fLp) fLp[d] -= flux[d] / vL;
+                    if (fRp) fRp[d] += flux[d] / vR;
+                }
+                continue;
+            }
+            /* NavierStokesTransport::AdvectionFlux on the euler field */
+            for (int k = 0; k < nc; k++) flux[k] = 0.0;
+            orc_ns_advection_flux(dim, normal, uL, uR, ph->gamma, ph->Rgas, ph->flux, flux);
+            for (int d = 0; d < dim + 2; ++d) {
+                if (fLp) fLp[d] -= flux[d] / vL;
+                if (fRp) fRp[d] += flux[d] / vR;
+            }
+            /* SpeciesTransport::AdvectionFlux on densityYi */
+            if (ns) {
+                for (int k = 0; k < nc; k++) flux[k] = 0.0;
+                species_advection_flux(dim, normal, uL, uL + dim + 2, uR, uR + dim + 2, ph->gamma, ph->Rgas, ph->flux, ns, flux);
+                for (int d = 0; d < ns; ++d) {
+                    if (fLp) fLp[dim + 2 + d] -= flux[d] / vL;
+                    if (fRp) fRp[dim + 2 + d] += flux[d] / vR;
+                }
+            }
+        }
+    }
+
+    /* ---- continuous flux functions (FaceInterpolant::ComputeRHS, faceInterpolant.cpp:262-448) ---- */
+    if (ctx->st_off) {
+        double faceSol[64], faceAux[64], faceSolGrad[64 * 3], faceAuxGrad[64 * 3], flux[64];
+        for (int64_t f = 0; f < t.F; f++) {
+            /* GetInterpolatedFaceVectors, faceInterpolant.cpp:178-236 */
+            for (int k = 0; k < nc; k++) faceSol[k] = 0.0;
+            for (int k = 0; k < na; k++) faceAux[k] = 0.0;
+            for (int k = 0; k < nc * dim; k++) faceSolGrad[k] = 0.0;
+            for (int k = 0; k < na * dim; k++) faceAuxGrad[k] = 0.0;
+            for (int64_t c = ctx->st_off[f]; c < ctx->st_off[f + 1]; c++) {
+                int32_t cell = ctx->st_cell[c];
+                const double w = ctx->st_w[c];
+                const double* gw = &ctx->st_gw[c * dim];
+                const double* sv = &X[(size_t)cell * nc];
+                const double* av = &aux[(size_t)cell * na];
+                for (int k = 0; k < nc; k++) faceSol[k] += sv[k] * w; /* AddToArray */
+                for (int k = 0; k < na; k++) faceAux[k] += av[k] * w;
+                int offset = 0;
+                for (int cc = 0; cc < nc; cc++)
+                    for (int d = 0; d < dim; ++d) faceSolGrad[offset++] += gw[d] * sv[cc];
+                offset = 0;
+                for (int cc = 0; cc < na; cc++)
+                    for (int d = 0; d < dim; ++d) faceAuxGrad[offset++] += gw[d] * av[cc];
+            }
+            int32_t cl = mesh->face_cells[2 * f], cr = mesh->face_cells[2 * f + 1];
+            const double* normal = &mesh->face_normal[3 * f];
+            double vL = mesh->cell_volume[cl], vR = mesh->cell_volume[cr];
+            double* fLp = (cl < t.NI) ? &locF[(size_t)cl * nc] : NULL;
+            double* fRp = (cr < t.NI) ? &locF[(size_t)cr * nc] : NULL;
+            /* NavierStokesTransport::DiffusionFlux: aux = {T, vel}, gradAux = {dT, dvel} */
+            for (int k = 0; k < nc; k++) flux[k] = 0.0;
+            ns_diffusion_flux(dim, normal, ph->mu, ph->k, faceAux + 1, faceAuxGrad + 0, faceAuxGrad + dim, flux);
+            for (int d = 0; d < dim + 2; ++d) {
+                if (fLp) fLp[d] -= flux[d] / vL;
+                if (fRp) fRp[d] += flux[d] / vR;
+            }
+            if (ns) {
+                /* SpeciesTransport::DiffusionEnergyFlux adds -n.rho.D.h_i.gradY_i with h_i = 0 for PerfectGas
+                 * (perfectGas.cpp:175-185): an exact zero contribution, omitted. */
+                /* SpeciesTransport::DiffusionSpeciesFlux, speciesTransport.cpp:196-228 */
+                const double density = faceSol[RHO];
+                for (int sp = 0; sp < ns; ++sp) {
+                    flux[sp] = 0;
+                    for (int d = 0; d < dim; ++d) {
+                        const int offset = (1 + dim + sp) * dim + d;
+                        double speciesFlux = -normal[d] * density * ph->diff * faceAuxGrad[offset];
+                        flux[sp] += speciesFlux;
+                    }
+                }
+                for (int d = 0; d < ns; ++d) {
+                    if (fLp) fLp[dim + 2 + d] -= flux[d] / vL;
+                    if (fRp) fRp[dim + 2 + d] += flux[d] / vR;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+int orc_rhs(const orc_mesh* mesh, const orc_physics* ph, const orc_bc* bcs, int n_bcs, double time, double* X,
+            double* locF) {
+    orc_ctx* ctx = NULL;
+    int rc = orc_create(mesh, ph, &ctx);
+    if (rc) return rc;
+    rc = orc_rhs_ctx(ctx, bcs, n_bcs, time, X, locF);
+    orc_destroy(ctx);
+    return rc;
+}

@@ -1,0 +1,154 @@
+"""Extracts the reference's own unit-test tables for the FV RHS leaf functions into JSON fixtures.
+
+Run in a container that has the ABLATE source tree (default /root/reference); the fixtures it writes are data
+(inputs and expected outputs), committed next to this script:
+  flux_calculators.json   fluxCalculatorTests.cpp: Ausm, AverageFlux and OffFlux tables
+  ns_advection_flux.json  navierStokesTransportTests.cpp: AdvectionFlux cases
+  stress_tensor.json      navierStokesTransportTests.cpp: CompressibleFlowComputeStressTensor cases
+  perfect_gas.json        perfectGasTests.cpp: PerfectGas thermodynamic property cases
+Each record keeps the source file and line of its table so that a reader can find it.
+"""
+from __future__ import annotations
+
+import json
+import math
+import re
+import sys
+from pathlib import Path
+
+REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+OUT = Path(__file__).resolve().parent
+UT = REF / "tests" / "unitTests"
+
+_NUM = re.compile(r"^[\s0-9eE+\-.*/()]*$")
+
+
+def num(expr: str) -> float:
+    """Evaluates a numeric C++ literal expression of the tables (literals, * and /, NAN)."""
+    expr = expr.strip()
+    if expr in ("NAN", "(double)NAN"):
+        return math.nan
+    if not _NUM.match(expr):
+        raise ValueError(f"unexpected expression {expr!r}")
+    return float(eval(expr, {"__builtins__": {}}, {}))  # arithmetic literal only (checked above)
+
+
+def braces(text: str, start: int) -> tuple[str, int]:
+    """Returns the content of the {...} block opening at text[start] and the index after it."""
+    assert text[start] == "{"
+    depth = 0
+    for i in range(start, len(text)):
+        if text[i] == "{":
+            depth += 1
+        elif text[i] == "}":
+            depth -= 1
+            if depth == 0:
+                return text[start + 1 : i], i + 1
+    raise ValueError("unbalanced braces")
+
+
+def field_list(block: str, name: str):
+    m = re.search(r"\." + name + r"\s*=\s*(std::vector<PetscReal>)?\s*\{", block)
+    if not m:
+        return None
+    inner, _ = braces(block, m.end() - 1)
+    items = [x for x in inner.replace("\n", " ").split(",") if x.strip()]
+    return items
+
+
+def line_of(text: str, idx: int) -> int:
+    return text.count("\n", 0, idx) + 1
+
+
+def flux_calculators():
+    path = UT / "finiteVolume" / "fluxCalculator" / "fluxCalculatorTests.cpp"
+    text = path.read_text()
+    out = {}
+    for m in re.finditer(r"\(FluxCalculatorTestParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        name = re.search(r'\.testName\s*=\s*"([^"]+)"', block).group(1)
+        if name not in ("Ausm", "AverageFlux", "OffFlux"):
+            continue
+        rec = {"source": f"tests/unitTests/finiteVolume/fluxCalculator/fluxCalculatorTests.cpp:{line_of(text, m.start())}"}
+        for key in ("uL", "aL", "rhoL", "pL", "uR", "aR", "rhoR", "pR", "expectedMassFlux", "expectedInterfacePressure"):
+            rec[key] = [num(x) for x in field_list(block, key)]
+        rec["expectedDirection"] = [x.strip() for x in field_list(block, "expectedDirection")]
+        rec["tolerance"] = {"massFlux": 1e-2, "interfacePressure": 1.0}
+        out[name] = rec
+    return out
+
+
+def ns_cases():
+    path = UT / "finiteVolume" / "processes" / "navierStokesTransportTests.cpp"
+    text = path.read_text()
+    adv, stress = [], []
+    for m in re.finditer(r"\(NavierStokesTransportFluxTestParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        flux = re.search(r"fluxCalculator::(\w+)>", block).group(1)
+        adv.append({
+            "source": f"tests/unitTests/finiteVolume/processes/navierStokesTransportTests.cpp:{line_of(text, m.start())}",
+            "fluxCalculator": flux,
+            "area": [num(x) for x in field_list(block, "area")],
+            "xLeft": [num(x) for x in field_list(block, "xLeft")],
+            "xRight": [num(x) for x in field_list(block, "xRight")],
+            "expectedFlux": [num(x) for x in field_list(block, "expectedFlux")],
+            "tolerance": 1e-3,
+            "eos": {"gamma": 1.4, "Rgas": 287.0},
+        })
+    for m in re.finditer(r"\(StressTensorTestParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        stress.append({
+            "source": f"tests/unitTests/finiteVolume/processes/navierStokesTransportTests.cpp:{line_of(text, m.start())}",
+            "dim": int(re.search(r"\.dim\s*=\s*(\d+)", block).group(1)),
+            "mu": num(re.search(r"\.mu\s*=\s*([0-9.eE+\-]+)", block).group(1)),
+            "gradVel": [num(x) for x in field_list(block, "gradVel")],
+            "expectedStressTensor": [num(x) for x in field_list(block, "expectedStressTensor")],
+            "tolerance": 1e-8,
+        })
+    return adv, stress
+
+
+def perfect_gas():
+    path = UT / "eos" / "perfectGasTests.cpp"
+    text = path.read_text()
+    out = []
+    for m in re.finditer(r"\(PGTestParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        opts = dict(re.findall(r'\{"(\w+)",\s*"([^"]+)"\}', block))
+        prop = re.search(r"ThermodynamicProperty::(\w+)", block).group(1)
+        fm = re.search(r'MockField::Create\("euler",\s*(\d+)(?:,\s*(\d+))?\)', block)
+        species = re.search(r"\.species\s*=\s*(std::vector<std::string>)?\{([^}]*)\}", block)
+        sp = [s.strip().strip('"') for s in species.group(2).split(",") if s.strip()] if species else []
+        tm = re.search(r"\.expectedTemperature\s*=\s*([^,}\n]+)", block)
+        T = None
+        if tm and tm.group(1).strip() not in ("{}", "{"):
+            T = num(tm.group(1))
+        out.append({
+            "source": f"tests/unitTests/eos/perfectGasTests.cpp:{line_of(text, m.start())}",
+            "gamma": float(opts.get("gamma", 1.4)),
+            "Rgas": float(opts.get("Rgas", 287.0)),
+            "species": sp,
+            "property": prop,
+            "eulerComponents": int(fm.group(1)),
+            "eulerOffset": int(fm.group(2) or 0),
+            "conservedValues": [num(x) for x in field_list(block, "conservedValues")],
+            "temperature": T,
+            "expectedValue": [num(x) for x in field_list(block, "expectedValue")],
+            "tolerance": 1e-6,
+        })
+    return out
+
+
+def main():
+    fc = flux_calculators()
+    adv, stress = ns_cases()
+    pgt = perfect_gas()
+    (OUT / "flux_calculators.json").write_text(json.dumps(fc, indent=1))
+    (OUT / "ns_advection_flux.json").write_text(json.dumps(adv, indent=1))
+    (OUT / "stress_tensor.json").write_text(json.dumps(stress, indent=1))
+    (OUT / "perfect_gas.json").write_text(json.dumps(pgt, indent=1, allow_nan=True))
+    print({k: len(v["uL"]) for k, v in fc.items()}, len(adv), len(stress), len(pgt))
+
+
+if __name__ == "__main__":
+    main()
