@@ -115,6 +115,12 @@ def lib() -> ctypes.CDLL:
                 f"native library {LIB_PATH} is missing: build it with `python -m ablate_amd.build` "
                 "(there is no CPU fallback)"
             )
+        try:
+            # One HIP runtime per process: torch ships its own libamdhip64 (same soname). Loading it first makes
+            # the dynamic linker bind this library to that runtime instead of starting a second one.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -26,11 +26,10 @@ struct MeshView {
     int K;
     const int32_t* __restrict__ ell;
     const int32_t* __restrict__ fcell;
-    const G* __restrict__ fnhat;
-    const G* __restrict__ farea;
-    const G* __restrict__ fdx;
-    const G* __restrict__ fw;
-    const G* __restrict__ fwa;
+    const G* __restrict__ fna;  // [F][4]: unit normal, area
+    const G* __restrict__ fdx;  // [F][2][DIM]: x_face - x_cell per side
+    const G* __restrict__ fw;   // [F][2][DIM]: cell least-squares weights per side
+    const G* __restrict__ fwa;  // [F][2][DIM]: aux (face-stencil) weights, or null = fw
     const double* __restrict__ inv_vol;
 };
 
@@ -41,6 +40,32 @@ __device__ inline int64_t xcd_cell_index() {
     const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
     const uint32_t logical = x * q + (x < r ? x : r) + (b >> 3);
     return (int64_t)logical * blockDim.x + threadIdx.x;
+}
+
+// Face records are 16-byte aligned: read them with 16-byte vector loads.
+__device__ inline void load_face_na(const double* __restrict__ fna, int64_t f, double* nhat, double& area) {
+    const double2* p = reinterpret_cast<const double2*>(fna + 4 * f);
+    const double2 a = p[0], b = p[1];
+    nhat[0] = a.x;
+    nhat[1] = a.y;
+    nhat[2] = b.x;
+    area = b.y;
+}
+// Loads the per-face pair record x[2][DIM] (both sides) of fdx / fw / fwa.
+template <int DIM>
+__device__ inline void load_face_pair(const double* __restrict__ arr, int64_t f, double (*x)[DIM]) {
+    const double2* p = reinterpret_cast<const double2*>(arr + 2 * DIM * f);
+    double t[2 * DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        const double2 v = p[i];
+        t[2 * i] = v.x;
+        t[2 * i + 1] = v.y;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++) x[s][d] = t[s * DIM + d];
 }
 
 // PETSc's PetscMin/PetscMax: the comparison order decides NaN propagation.

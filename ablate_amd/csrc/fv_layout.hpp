@@ -22,11 +22,10 @@ struct Layout {
     bool has_boundary = false;
     std::vector<int32_t> ell;      // [K][NI]
     std::vector<int32_t> fcell;    // [2][F]
-    std::vector<double> fnhat;     // [dim][F]
-    std::vector<double> farea;     // [F]
-    std::vector<double> fdx;       // [2][dim][F]
-    std::vector<double> fw;        // [2][dim][F] (empty when unused)
-    std::vector<double> fwa;       // [2][dim][F] (empty: same as fw)
+    std::vector<double> fna;       // [F][4]: unit normal, area
+    std::vector<double> fdx;       // [F][2][dim]
+    std::vector<double> fw;        // [F][2][dim] (empty when unused)
+    std::vector<double> fwa;       // [F][2][dim] (empty: same as fw)
     std::vector<double> inv_vol;   // [N]
     std::vector<BoundaryFace> bfaces;
 };

@@ -10,9 +10,10 @@
 //   ell[k * NI + c]        k-th face of interior cell c, encoded (face << 1) | side (side 1 = c is the right cell),
 //                          faces sorted by number (the reference's accumulation order), -1 padding
 //   fcell[s * F + f]       support cells
-//   fnhat[d * F + f]       unit normal (left -> right), farea[f] = |n|
-//   fdx[(s*dim + d)*F + f] x_face - x_cell(side s) in that cell's frame
-//   fw, fwa                least-squares weights of cell `side` for the neighbour across f (cell / aux stencils)
+//   fna[4 * f + {0,1,2}]   unit normal (left -> right), fna[4 * f + 3] = |n|   (one 32-byte record per face)
+//   fdx[2*dim*f + s*dim + d]  x_face - x_cell(side s) in that cell's frame    (one record per face)
+//   fw, fwa  [2*dim*f + s*dim + d]  least-squares weight of cell `side` for the neighbour across f (cell / aux
+//                          stencils)
 //   inv_vol[c]
 #include <algorithm>
 #include <cmath>
@@ -181,8 +182,7 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
 
     // ---- face geometry
     L.fcell.resize(2 * F);
-    L.fnhat.resize((size_t)dim * F);
-    L.farea.resize(F);
+    L.fna.assign((size_t)4 * F, 0.0);
     L.fdx.resize((size_t)2 * dim * F);
     auto shift = [&](int64_t f, int d) { return m.face_shift ? m.face_shift[3 * f + d] : 0.0; };
     for (int64_t f = 0; f < F; f++) {
@@ -192,11 +192,11 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
         double mag = 0.0;
         for (int d = 0; d < dim; d++) mag += m.face_normal[3 * f + d] * m.face_normal[3 * f + d];
         mag = std::sqrt(mag);
-        L.farea[f] = mag;
+        L.fna[4 * f + 3] = mag;
         for (int d = 0; d < dim; d++) {
-            L.fnhat[(size_t)d * F + f] = m.face_normal[3 * f + d] / mag;
-            L.fdx[(size_t)(0 * dim + d) * F + f] = m.face_centroid[3 * f + d] - m.cell_centroid[3 * a + d];
-            L.fdx[(size_t)(1 * dim + d) * F + f] = m.face_centroid[3 * f + d] - (m.cell_centroid[3 * b + d] + shift(f, d));
+            L.fna[4 * f + d] = m.face_normal[3 * f + d] / mag;
+            L.fdx[(size_t)2 * dim * f + 0 * dim + d] = m.face_centroid[3 * f + d] - m.cell_centroid[3 * a + d];
+            L.fdx[(size_t)2 * dim * f + 1 * dim + d] = m.face_centroid[3 * f + d] - (m.cell_centroid[3 * b + d] + shift(f, d));
         }
         if (b >= NI) L.bfaces.push_back({(int32_t)f, m.face_bc[f]});
     }
@@ -246,7 +246,7 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
                 w.resize(dx.size());
                 lsq_pinv_weights((int)used.size(), dim, dx.data(), w.data());
                 for (size_t i = 0; i < used.size(); i++)
-                    for (int d = 0; d < dim; d++) dst[(size_t)(sides[i] * dim + d) * F + used[i]] = w[i * dim + d];
+                    for (int d = 0; d < dim; d++) dst[(size_t)2 * dim * used[i] + sides[i] * dim + d] = w[i * dim + d];
             }
             if (pass == 1) {
                 // ghost cells: one-neighbour stencil, minimum-norm solution dx / |dx|^2 (ghost is always side 1)
@@ -256,7 +256,7 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
                     nbr_dx(f, 1, v);
                     double wv[3];
                     lsq_pinv_weights(1, dim, v, wv);
-                    for (int d = 0; d < dim; d++) dst[(size_t)(1 * dim + d) * F + f] = wv[d];
+                    for (int d = 0; d < dim; d++) dst[(size_t)2 * dim * f + 1 * dim + d] = wv[d];
                 }
             }
         }

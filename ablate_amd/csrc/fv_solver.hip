@@ -59,6 +59,7 @@ template <int DIM>
 __device__ inline void aux_of(const double* u, const Phys& ph, double* a) {
     const double irho = 1.0 / u[kRho];
     double ke = 0.0;
+#pragma unroll
     for (int d = 0; d < DIM; d++) {
         const double v = u[kRhoU + d] * irho;
         a[1 + d] = v;
@@ -67,8 +68,11 @@ __device__ inline void aux_of(const double* u, const Phys& ph, double* a) {
     a[0] = (u[kRhoE] * irho - 0.5 * ke) * ph.icv;
 }
 
-// Gradients of the euler field (limited) and of the aux T/velocity (unlimited) for one interior cell per lane.
-template <int DIM>
+// Limiter families resolved at compile time.
+enum : int { kLimNone = 0, kLimPetsc = 1, kLimBJ = 2 };
+
+// Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
+template <int DIM, bool RECON, bool VISC, int LIM>
 __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                       double* __restrict__ gradC, double* __restrict__ gradA) {
     constexpr int NE = DIM + 2;
@@ -76,19 +80,22 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const bool recon = ph.recon != 0;
-    const bool aux = ph.visc != 0;
     const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
     double uc[NE], ac[NA];
+#pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
-    aux_of<DIM>(uc, ph, ac);
+    if (VISC) aux_of<DIM>(uc, ph, ac);
     double g[NE][DIM], ga[NA][DIM], umin[NE], umax[NE];
+#pragma unroll
     for (int p = 0; p < NE; p++) {
         umin[p] = uc[p];
         umax[p] = uc[p];
+#pragma unroll
         for (int d = 0; d < DIM; d++) g[p][d] = 0.0;
     }
+#pragma unroll
     for (int q = 0; q < NA; q++)
+#pragma unroll
         for (int d = 0; d < DIM; d++) ga[q][d] = 0.0;
 
     for (int k = 0; k < m.K; k++) {
@@ -99,45 +106,54 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
         const int64_t n = m.fcell[(1 - s) * F + f];
         const bool bnd = n >= NI;
         double un[NE];
+#pragma unroll
         for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
-        if (recon) {
-            for (int p = 0; p < NE; p++) {
-                umin[p] = fmin(umin[p], un[p]);
-                umax[p] = fmax(umax[p], un[p]);
+        if (RECON) {
+            if (LIM == kLimBJ) {
+#pragma unroll
+                for (int p = 0; p < NE; p++) {
+                    umin[p] = fmin(umin[p], un[p]);
+                    umax[p] = fmax(umax[p], un[p]);
+                }
             }
             if (!bnd) {
-                double w[DIM];
-                for (int d = 0; d < DIM; d++) w[d] = m.fw[(s * DIM + d) * F + f];
+                double w[2][DIM];
+                load_face_pair<DIM>(m.fw, f, w);
+#pragma unroll
                 for (int p = 0; p < NE; p++) {
                     // delta = u_R - u_L; the left cell adds w.delta, the right cell subtracts it
                     if (s == 0) {
                         const double delta = un[p] - uc[p];
-                        for (int d = 0; d < DIM; d++) g[p][d] += w[d] * delta;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) g[p][d] += w[0][d] * delta;
                     } else {
                         const double delta = uc[p] - un[p];
-                        for (int d = 0; d < DIM; d++) g[p][d] -= w[d] * delta;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) g[p][d] -= w[1][d] * delta;
                     }
                 }
             }
         }
-        if (aux) {
-            double an[NA], w[DIM];
+        if (VISC) {
+            double an[NA], w[2][DIM];
             aux_of<DIM>(un, ph, an);
-            for (int d = 0; d < DIM; d++) w[d] = wa_arr[(s * DIM + d) * F + f];
+            load_face_pair<DIM>(wa_arr, f, w);
+#pragma unroll
             for (int q = 0; q < NA; q++) {
                 const double delta = an[q] - ac[q];
-                for (int d = 0; d < DIM; d++) ga[q][d] += w[d] * delta;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) ga[q][d] += w[s][d] * delta;
             }
         }
     }
 
-    if (recon && ph.limiter != ABL_FV_LIMITER_NONE) {
+    if (RECON && LIM != kLimNone) {
         double phi[NE];
-        for (int p = 0; p < NE; p++) phi[p] = DBL_MAX;
-        if (ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN) {
-            // phi = min(1, (u_max - u_c)/max_f D_f+, (u_min - u_c)/min_f D_f-), D_f = grad.(x_f - x_c): the
-            // per-face minimum collapsed onto the extreme projections (division is monotone).
+        if (LIM == kLimBJ) {
+            // phi = min(1, (u_max - u_c)/max_f D_f+, (u_min - u_c)/min_f D_f-), D_f = grad.(x_f - x_c): the per-face
+            // minimum collapsed onto the extreme projections (division is monotone).
             double dpos[NE], dneg[NE];
+#pragma unroll
             for (int p = 0; p < NE; p++) {
                 dpos[p] = 0.0;
                 dneg[p] = 0.0;
@@ -147,33 +163,42 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
                 if (e < 0) break;
                 const int64_t f = e >> 1;
                 const int s = e & 1;
-                double xf[DIM];
-                for (int d = 0; d < DIM; d++) xf[d] = m.fdx[(s * DIM + d) * F + f];
+                double x[2][DIM];
+                load_face_pair<DIM>(m.fdx, f, x);
+#pragma unroll
                 for (int p = 0; p < NE; p++) {
                     double D = 0.0;
-                    for (int d = 0; d < DIM; d++) D += g[p][d] * xf[d];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) D += g[p][d] * x[s][d];
                     dpos[p] = fmax(dpos[p], D);
                     dneg[p] = fmin(dneg[p], D);
                 }
             }
+#pragma unroll
             for (int p = 0; p < NE; p++) {
-                double ph_ = 1.0;
-                if (dpos[p] > 0.0) ph_ = fmin(ph_, (umax[p] - uc[p]) / dpos[p]);
-                if (dneg[p] < 0.0) ph_ = fmin(ph_, (umin[p] - uc[p]) / dneg[p]);
-                phi[p] = ph_;
+                double v = 1.0;
+                if (dpos[p] > 0.0) v = fmin(v, (umax[p] - uc[p]) / dpos[p]);
+                if (dneg[p] < 0.0) v = fmin(v, (umin[p] - uc[p]) / dneg[p]);
+                phi[p] = v;
             }
         } else {
             // DMPlexApplyLimiter_Internal (cellInterpolant.cpp:282-328): f = 1/2 (u_n - u_c) / (grad.(x_n - x_c))
+#pragma unroll
+            for (int p = 0; p < NE; p++) phi[p] = DBL_MAX;
             for (int k = 0; k < m.K; k++) {
                 const int32_t e = m.ell[k * NI + c];
                 if (e < 0) break;
                 const int64_t f = e >> 1;
                 const int s = e & 1;
                 const int64_t n = m.fcell[(1 - s) * F + f];
-                double v[DIM];
-                for (int d = 0; d < DIM; d++) v[d] = m.fdx[(s * DIM + d) * F + f] - m.fdx[((1 - s) * DIM + d) * F + f];
+                double x[2][DIM], v[DIM];
+                load_face_pair<DIM>(m.fdx, f, x);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) v[d] = x[s][d] - x[1 - s][d];
+#pragma unroll
                 for (int p = 0; p < NE; p++) {
                     double denom = 0.0;
+#pragma unroll
                     for (int d = 0; d < DIM; d++) denom += g[p][d] * v[d];
                     const double flim = 0.5 * (U[p * N + n] - uc[p]) / denom;
                     phi[p] = pmin(phi[p], limiter_phi<double>(ph.limiter, flim));
@@ -181,18 +206,26 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
             }
         }
         bool cancel = false;
+#pragma unroll
         for (int p = 0; p < NE; p++)
             if (phi[p] == 0)
+#pragma unroll
                 for (int d = 0; d < DIM; d++)
                     if (fabs(g[p][d]) > ph.max_lim_grad) cancel = true;
+#pragma unroll
         for (int p = 0; p < NE; p++)
+#pragma unroll
             for (int d = 0; d < DIM; d++) g[p][d] *= cancel ? 0.0 : phi[p];
     }
-    if (recon)
+    if (RECON)
+#pragma unroll
         for (int p = 0; p < NE; p++)
+#pragma unroll
             for (int d = 0; d < DIM; d++) gradC[(p * DIM + d) * N + c] = g[p][d];
-    if (aux)
+    if (VISC)
+#pragma unroll
         for (int q = 0; q < NA; q++)
+#pragma unroll
             for (int d = 0; d < DIM; d++) gradA[(q * DIM + d) * N + c] = ga[q][d];
 }
 
@@ -214,6 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
         const double uc = U[comp * N + c];
         const double yc = uc * irho_c;
         double g[DIM], ga[DIM], umin = uc, umax = uc;
+#pragma unroll
         for (int d = 0; d < DIM; d++) {
             g[d] = 0.0;
             ga[d] = 0.0;
@@ -229,18 +263,25 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                 umin = fmin(umin, un);
                 umax = fmax(umax, un);
                 if (n < NI) {
+                    double w[2][DIM];
+                    load_face_pair<DIM>(m.fw, f, w);
                     if (s == 0) {
                         const double delta = un - uc;
-                        for (int d = 0; d < DIM; d++) g[d] += m.fw[(s * DIM + d) * F + f] * delta;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) g[d] += w[0][d] * delta;
                     } else {
                         const double delta = uc - un;
-                        for (int d = 0; d < DIM; d++) g[d] -= m.fw[(s * DIM + d) * F + f] * delta;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) g[d] -= w[1][d] * delta;
                     }
                 }
             }
             if (aux) {
+                double w[2][DIM];
+                load_face_pair<DIM>(wa_arr, f, w);
                 const double delta = un / U[kRho * N + n] - yc;
-                for (int d = 0; d < DIM; d++) ga[d] += wa_arr[(s * DIM + d) * F + f] * delta;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) ga[d] += w[s][d] * delta;
             }
         }
         if (recon) {
@@ -252,8 +293,11 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                     if (e < 0) break;
                     const int64_t f = e >> 1;
                     const int s = e & 1;
+                    double x[2][DIM];
+                    load_face_pair<DIM>(m.fdx, f, x);
                     double D = 0.0;
-                    for (int d = 0; d < DIM; d++) D += g[d] * m.fdx[(s * DIM + d) * F + f];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) D += g[d] * x[s][d];
                     dpos = fmax(dpos, D);
                     dneg = fmin(dneg, D);
                 }
@@ -267,9 +311,11 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                     const int64_t f = e >> 1;
                     const int s = e & 1;
                     const int64_t n = m.fcell[(1 - s) * F + f];
+                    double x[2][DIM];
+                    load_face_pair<DIM>(m.fdx, f, x);
                     double denom = 0.0;
-                    for (int d = 0; d < DIM; d++)
-                        denom += g[d] * (m.fdx[(s * DIM + d) * F + f] - m.fdx[((1 - s) * DIM + d) * F + f]);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) denom += g[d] * (x[s][d] - x[1 - s][d]);
                     const double flim = 0.5 * (U[comp * N + n] - uc) / denom;
                     phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
                 }
@@ -277,46 +323,52 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                 phi = 1.0;
             }
             if (phi == 0)
+#pragma unroll
                 for (int d = 0; d < DIM; d++)
                     if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
+#pragma unroll
             for (int d = 0; d < DIM; d++) gradC[(comp * DIM + d) * N + c] = g[d] * phi;
         }
         if (aux)
+#pragma unroll
             for (int d = 0; d < DIM; d++) gradA[((DIM + 1 + sp) * DIM + d) * N + c] = ga[d];
     }
     if (cancel)  // the maxLimGrad rule zeroes the whole densityYi field of the cell (cellInterpolant.cpp:459-481)
         for (int sp = 0; sp < ph.ns; sp++)
+#pragma unroll
             for (int d = 0; d < DIM; d++) gradC[((NE + sp) * DIM + d) * N + c] = 0.0;
 }
 
 // Face fluxes gathered per cell.  One lane per interior cell; both cells of a face evaluate its flux with the
 // reference orientation, so the gathered residual is conservative and order-deterministic.
-template <int DIM>
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                  const double* __restrict__ gradC, const double* __restrict__ gradA,
                                                  double* __restrict__ R, int accumulate) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
-    constexpr int MAXK = kMaxFacesPerCell;
+    constexpr int MAXK = SPECIES ? kMaxFacesPerCell : 1;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const bool recon = ph.recon != 0;
-    const bool visc = ph.visc != 0;
-    const int ns = ph.ns;
     const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
 
     double uc[NE], gc[NE][DIM], ac[NA], gac[NA][DIM];
+#pragma unroll
     for (int p = 0; p < NE; p++) {
         uc[p] = U[p * N + c];
-        for (int d = 0; d < DIM; d++) gc[p][d] = recon ? gradC[(p * DIM + d) * N + c] : 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) gc[p][d] = RECON ? gradC[(p * DIM + d) * N + c] : 0.0;
     }
-    if (visc) {
+    if (VISC) {
         aux_of<DIM>(uc, ph, ac);
+#pragma unroll
         for (int q = 0; q < NA; q++)
+#pragma unroll
             for (int d = 0; d < DIM; d++) gac[q][d] = gradA[(q * DIM + d) * N + c];
     }
     double acc[NE];
+#pragma unroll
     for (int p = 0; p < NE; p++) acc[p] = 0.0;
     // per-face coefficients for the species pass: contribution = cL * Y_L + cR * Y_R (+ diffusion)
     double sp_cl[MAXK], sp_cr[MAXK], sp_cd[MAXK];
@@ -330,29 +382,32 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         const int s = e & 1;
         const int64_t n = m.fcell[(1 - s) * F + f];
         const bool ghost = n >= NI;
+        double nh[3], area;
+        load_face_na(m.fna, f, nh, area);
         double nhat[DIM];
-        for (int d = 0; d < DIM; d++) nhat[d] = m.fnhat[d * F + f];
-        const double area = m.farea[f];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) nhat[d] = nh[d];
         double un[NE];
+#pragma unroll
         for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
         // ProjectToFace for both sides (ghost gradients are zero)
         double uself[NE], unbr[NE];
-        if (recon) {
-            double dxs[DIM], dxn[DIM];
-            for (int d = 0; d < DIM; d++) {
-                dxs[d] = m.fdx[(s * DIM + d) * F + f];
-                dxn[d] = m.fdx[((1 - s) * DIM + d) * F + f];
-            }
+        if (RECON) {
+            double x[2][DIM];
+            load_face_pair<DIM>(m.fdx, f, x);
+#pragma unroll
             for (int p = 0; p < NE; p++) {
                 double a = 0.0, b = 0.0;
+#pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    a += gc[p][d] * dxs[d];
-                    if (!ghost) b += gradC[(p * DIM + d) * N + n] * dxn[d];
+                    a += gc[p][d] * x[s][d];
+                    if (!ghost) b += gradC[(p * DIM + d) * N + n] * x[1 - s][d];
                 }
                 uself[p] = uc[p] + a;
                 unbr[p] = un[p] + b;
             }
         } else {
+#pragma unroll
             for (int p = 0; p < NE; p++) {
                 uself[p] = uc[p];
                 unbr[p] = un[p];
@@ -361,6 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         const double* uL = s == 0 ? uself : unbr;
         const double* uR = s == 0 ? unbr : uself;
         double flux[NE];
+#pragma unroll
         for (int p = 0; p < NE; p++) flux[p] = 0.0;
         double cl = 0.0, cr = 0.0;
         if (ph.flux == ABL_FV_FLUX_RUSANOV) {
@@ -378,24 +434,30 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
                 cl = mA / rl;
         }
         double cd = 0.0;
-        if (visc) {
+        if (VISC) {
             // face interpolant: value = 1/2 (a_L + a_R), gradient = 1/2 (G_L + G_R) of unlimited LSQ gradients
             double an[NA], gan[NA][DIM];
             aux_of<DIM>(un, ph, an);
             if (ghost) {
-                double w[DIM];
-                for (int d = 0; d < DIM; d++) w[d] = wa_arr[(1 * DIM + d) * F + f];
+                double w[2][DIM];
+                load_face_pair<DIM>(wa_arr, f, w);
+#pragma unroll
                 for (int q = 0; q < NA; q++)
-                    for (int d = 0; d < DIM; d++) gan[q][d] = w[d] * (ac[q] - an[q]);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) gan[q][d] = w[1][d] * (ac[q] - an[q]);
             } else {
+#pragma unroll
                 for (int q = 0; q < NA; q++)
+#pragma unroll
                     for (int d = 0; d < DIM; d++) gan[q][d] = gradA[(q * DIM + d) * N + n];
             }
             double vel[DIM], gT[DIM], gV[DIM][DIM], normal[DIM];
+#pragma unroll
             for (int d = 0; d < DIM; d++) {
                 vel[d] = 0.5 * ac[1 + d] + 0.5 * an[1 + d];
                 gT[d] = 0.5 * gac[0][d] + 0.5 * gan[0][d];
                 normal[d] = area * nhat[d];
+#pragma unroll
                 for (int q = 0; q < DIM; q++) gV[q][d] = 0.5 * gac[1 + q][d] + 0.5 * gan[1 + q][d];
             }
             diffusion_flux_add<double, DIM>(normal, ph.mu, ph.k, vel, gT, gV, flux);
@@ -403,8 +465,9 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
             cd = -ph.diff * (0.5 * uc[kRho] + 0.5 * un[kRho]) * 0.5;
         }
         const double sign = s == 0 ? -1.0 : 1.0;
+#pragma unroll
         for (int p = 0; p < NE; p++) acc[p] += sign * flux[p];
-        if (ns) {
+        if (SPECIES) {
             sp_cl[nk] = sign * cl;
             sp_cr[nk] = sign * cr;
             sp_cd[nk] = sign * cd;
@@ -413,20 +476,22 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         }
     }
     const double iv = m.inv_vol[c];
+#pragma unroll
     for (int p = 0; p < NE; p++) {
         const double r = acc[p] * iv;
         R[p * N + c] = accumulate ? R[p * N + c] + r : r;
     }
-    if (!ns) return;
+    if (!SPECIES) return;
     // species pass: reconstructed densityYi on each side, upwinded by the stored coefficients
     const double rho_c = uc[kRho];
-    for (int sp = 0; sp < ns; sp++) {
+    for (int sp = 0; sp < ph.ns; sp++) {
         const int64_t comp = NE + sp;
         const double yc = U[comp * N + c];
         double gyc[DIM], gac_s[DIM];
+#pragma unroll
         for (int d = 0; d < DIM; d++) {
-            gyc[d] = recon ? gradC[(comp * DIM + d) * N + c] : 0.0;
-            gac_s[d] = ph.sdiff ? gradA[((DIM + 1 + sp) * DIM + d) * N + c] : 0.0;
+            gyc[d] = RECON ? gradC[(comp * DIM + d) * N + c] : 0.0;
+            gac_s[d] = VISC ? gradA[((DIM + 1 + sp) * DIM + d) * N + c] : 0.0;
         }
         double a = 0.0;
         for (int k = 0; k < nk; k++) {
@@ -437,11 +502,14 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
             const bool ghost = n >= NI;
             const double yn = U[comp * N + n];
             double yself = yc, ynbr = yn;
-            if (recon) {
+            if (RECON) {
+                double x[2][DIM];
+                load_face_pair<DIM>(m.fdx, f, x);
                 double ps = 0.0, pn = 0.0;
+#pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    ps += gyc[d] * m.fdx[(s * DIM + d) * F + f];
-                    if (!ghost) pn += gradC[(comp * DIM + d) * N + n] * m.fdx[((1 - s) * DIM + d) * F + f];
+                    ps += gyc[d] * x[s][d];
+                    if (!ghost) pn += gradC[(comp * DIM + d) * N + n] * x[1 - s][d];
                 }
                 yself += ps;
                 ynbr += pn;
@@ -449,16 +517,23 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
             const double yL = s == 0 ? yself : ynbr;
             const double yR = s == 0 ? ynbr : yself;
             double contrib = sp_cl[k] * yL + sp_cr[k] * yR;
-            if (ph.sdiff) {
+            if (VISC) {
                 double gn[DIM];
                 if (ghost) {
+                    double w[2][DIM];
+                    load_face_pair<DIM>(wa_arr, f, w);
                     const double rho_n = U[kRho * N + n];
-                    for (int d = 0; d < DIM; d++) gn[d] = wa_arr[(1 * DIM + d) * F + f] * (yc / rho_c - yn / rho_n);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) gn[d] = w[1][d] * (yc / rho_c - yn / rho_n);
                 } else {
+#pragma unroll
                     for (int d = 0; d < DIM; d++) gn[d] = gradA[((DIM + 1 + sp) * DIM + d) * N + n];
                 }
+                double nh[3], area;
+                load_face_na(m.fna, f, nh, area);
                 double dot = 0.0;
-                for (int d = 0; d < DIM; d++) dot += m.farea[f] * m.fnhat[d * F + f] * (gac_s[d] + gn[d]);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) dot += area * nh[d] * (gac_s[d] + gn[d]);
                 contrib += sp_cd[k] * dot;
             }
             a += contrib;
@@ -466,6 +541,33 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         const double r = a * iv;
         R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
     }
+}
+
+// ---- compile-time variant selection
+using GradKernel = void (*)(MeshView<double>, Phys, const double*, double*, double*);
+using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, int);
+
+template <int DIM>
+GradKernel pick_gradients(bool recon, bool visc, int lim) {
+    if (!recon) return visc ? k_gradients<DIM, false, true, kLimNone> : nullptr;
+    if (visc) {
+        if (lim == kLimBJ) return k_gradients<DIM, true, true, kLimBJ>;
+        if (lim == kLimPetsc) return k_gradients<DIM, true, true, kLimPetsc>;
+        return k_gradients<DIM, true, true, kLimNone>;
+    }
+    if (lim == kLimBJ) return k_gradients<DIM, true, false, kLimBJ>;
+    if (lim == kLimPetsc) return k_gradients<DIM, true, false, kLimPetsc>;
+    return k_gradients<DIM, true, false, kLimNone>;
+}
+
+template <int DIM, bool RECON>
+FluxKernel pick_flux2(bool visc, bool species) {
+    if (visc) return species ? k_flux<DIM, RECON, true, true> : k_flux<DIM, RECON, true, false>;
+    return species ? k_flux<DIM, RECON, false, true> : k_flux<DIM, RECON, false, false>;
+}
+template <int DIM>
+FluxKernel pick_flux(bool recon, bool visc, bool species) {
+    return recon ? pick_flux2<DIM, true>(visc, species) : pick_flux2<DIM, false>(visc, species);
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_rows(double* __restrict__ R, int64_t N, int64_t start, int nc) {
@@ -543,7 +645,9 @@ struct abl_fv_solver {
     abl::Layout host;  // metadata only after upload (arrays released)
     abl::Phys ph{};
     abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind;
-    abl::DevBuf<double> fnhat, farea, fdx, fw, fwa, inv_vol, gradC, gradA, slot_state;
+    abl::DevBuf<double> fna, fdx, fw, fwa, inv_vol, gradC, gradA, slot_state;
+    abl::GradKernel grad_kernel = nullptr;
+    abl::FluxKernel flux_kernel = nullptr;
     abl::DevBuf<unsigned long long> counter;
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
@@ -603,8 +707,7 @@ struct abl_fv_solver {
         v.K = host.K;
         v.ell = ell.ptr;
         v.fcell = fcell.ptr;
-        v.fnhat = fnhat.ptr;
-        v.farea = farea.ptr;
+        v.fna = fna.ptr;
         v.fdx = fdx.ptr;
         v.fw = fw.ptr;
         v.fwa = fwa.ptr;
@@ -612,8 +715,7 @@ struct abl_fv_solver {
         return v;
     }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fnhat.bytes() +
-                         farea.bytes() + fdx.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
                          gradA.bytes() + slot_state.bytes() + counter.bytes());
     }
 };
@@ -671,11 +773,20 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         ph.ns = p.n_species;
         ph.visc = s->visc;
         ph.sdiff = s->visc && p.n_species > 0;
+        const int lim = p.limiter == ABL_FV_LIMITER_NONE ? abl::kLimNone
+                        : p.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN ? abl::kLimBJ : abl::kLimPetsc;
+        const bool species = p.n_species > 0;
+        if (mesh->dim == 3) {
+            s->grad_kernel = abl::pick_gradients<3>(s->recon, s->visc, lim);
+            s->flux_kernel = abl::pick_flux<3>(s->recon, s->visc, species);
+        } else {
+            s->grad_kernel = abl::pick_gradients<2>(s->recon, s->visc, lim);
+            s->flux_kernel = abl::pick_flux<2>(s->recon, s->visc, species);
+        }
 
         s->ell.upload(L.ell);
         s->fcell.upload(L.fcell);
-        s->fnhat.upload(L.fnhat);
-        s->farea.upload(L.farea);
+        s->fna.upload(L.fna);
         s->fdx.upload(L.fdx);
         s->fw.upload(L.fw);
         s->fwa.upload(L.fwa);
@@ -706,8 +817,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
         L.fcell.clear(); L.fcell.shrink_to_fit();
-        L.fnhat.clear(); L.fnhat.shrink_to_fit();
-        L.farea.clear(); L.farea.shrink_to_fit();
+        L.fna.clear(); L.fna.shrink_to_fit();
         L.fdx.clear(); L.fdx.shrink_to_fit();
         L.fw.clear(); L.fw.shrink_to_fit();
         L.fwa.clear(); L.fwa.shrink_to_fit();
@@ -790,10 +900,7 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
         const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
         if (h->recon || h->visc) {
             h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
-                if (L.dim == 3)
-                    hipLaunchKernelGGL(abl::k_gradients<3>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
-                else
-                    hipLaunchKernelGGL(abl::k_gradients<2>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
+                hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
             });
             if (L.ns)
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
@@ -806,10 +913,7 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                 });
         }
         h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-            if (L.dim == 3)
-                hipLaunchKernelGGL(abl::k_flux<3>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R, accumulate);
-            else
-                hipLaunchKernelGGL(abl::k_flux<2>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R, accumulate);
+            hipLaunchKernelGGL(h->flux_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R, accumulate);
         });
         if (!accumulate && L.N > L.NI)
             hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, R, L.N, L.NI, L.nc);
