@@ -99,9 +99,11 @@ SIGNATURES = {
     "abl_fv_unpack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_check_state": (c_int32, [c_void_p, c_void_p, POINTER(c_int64), c_void_p]),
     "abl_fv_set_profiling": (c_int32, [c_void_p, c_int32]),
+    "abl_fv_set_strategy": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_phase_times": (c_int32, [c_void_p, _dp, POINTER(c_int64), c_int32]),
 }
-PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp"]
+PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]
+STRATEGY = {"recompute": 1, "face_flux": 2}
 
 _lib = None
 

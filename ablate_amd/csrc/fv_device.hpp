@@ -42,7 +42,7 @@ __device__ inline int64_t xcd_cell_index() {
     return (int64_t)logical * blockDim.x + threadIdx.x;
 }
 
-// Face records are 16-byte aligned: read them with 16-byte vector loads.
+// Face records are read with vector loads (16 bytes for FP64 records, 8/16 bytes for FP32 ones).
 __device__ inline void load_face_na(const double* __restrict__ fna, int64_t f, double* nhat, double& area) {
     const double2* p = reinterpret_cast<const double2*>(fna + 4 * f);
     const double2 a = p[0], b = p[1];
@@ -51,14 +51,32 @@ __device__ inline void load_face_na(const double* __restrict__ fna, int64_t f, d
     nhat[2] = b.x;
     area = b.y;
 }
+__device__ inline void load_face_na(const float* __restrict__ fna, int64_t f, float* nhat, float& area) {
+    const float4 a = *reinterpret_cast<const float4*>(fna + 4 * f);
+    nhat[0] = a.x;
+    nhat[1] = a.y;
+    nhat[2] = a.z;
+    area = a.w;
+}
+template <typename G>
+struct Vec2T;
+template <>
+struct Vec2T<double> {
+    using type = double2;
+};
+template <>
+struct Vec2T<float> {
+    using type = float2;
+};
 // Loads the per-face pair record x[2][DIM] (both sides) of fdx / fw / fwa.
-template <int DIM>
-__device__ inline void load_face_pair(const double* __restrict__ arr, int64_t f, double (*x)[DIM]) {
-    const double2* p = reinterpret_cast<const double2*>(arr + 2 * DIM * f);
-    double t[2 * DIM];
+template <int DIM, typename G>
+__device__ inline void load_face_pair(const G* __restrict__ arr, int64_t f, G (*x)[DIM]) {
+    using V2 = typename Vec2T<G>::type;
+    const V2* p = reinterpret_cast<const V2*>(arr + 2 * DIM * f);
+    G t[2 * DIM];
 #pragma unroll
     for (int i = 0; i < DIM; i++) {
-        const double2 v = p[i];
+        const V2 v = p[i];
         t[2 * i] = v.x;
         t[2 * i + 1] = v.y;
     }
@@ -66,6 +84,25 @@ __device__ inline void load_face_pair(const double* __restrict__ arr, int64_t f,
     for (int s = 0; s < 2; s++)
 #pragma unroll
         for (int d = 0; d < DIM; d++) x[s][d] = t[s * DIM + d];
+}
+
+// Face slot k of interior cell c in the cell -> face table.  Loops over slots are unrolled to a compile-time
+// width so that every slot's loads issue up front; padding slots map to face 0 / the cell itself (valid addresses)
+// and are masked out of the arithmetic.
+struct Slot {
+    bool valid;
+    int s;
+    int64_t f, n;
+};
+template <typename MV>
+__device__ inline Slot face_slot(const MV& m, int k, int64_t c) {
+    const int32_t e = k < m.K ? m.ell[k * m.NI + c] : -1;
+    Slot sl;
+    sl.valid = e >= 0;
+    sl.f = sl.valid ? (int64_t)(e >> 1) : 0;
+    sl.s = sl.valid ? (e & 1) : 0;
+    sl.n = sl.valid ? (int64_t)m.fcell[(1 - sl.s) * m.F + sl.f] : c;
+    return sl;
 }
 
 // PETSc's PetscMin/PetscMax: the comparison order decides NaN propagation.
@@ -216,26 +253,54 @@ __device__ inline void rusanov_flux(const T* uL, const T* uR, const T* nhat, T a
     }
 }
 
-// NavierStokesTransport::DiffusionFlux + CompressibleFlowComputeStressTensor (navierStokesTransport.cpp:454-532)
-// with face values vel[] and face gradients gT[d], gV[c][d].  normal = area-weighted normal.  Adds into flux.
+// Per-cell viscous terms: the symmetric stress tensor (DIM(DIM+1)/2 entries) and the heat flux k grad T (DIM).
+// The face-interpolant viscous flux uses the face average 1/2 (G_L + G_R) of unlimited cell gradients
+// (stencils/leastSquaresAverage.cpp:5-67); tau and k grad T are linear in G, so the face values are the averages
+// of the per-cell tau and q.
+template <int DIM>
+struct ViscTerms {
+    static constexpr int NTAU = DIM * (DIM + 1) / 2;
+    static constexpr int NT = NTAU + DIM;
+};
+__host__ __device__ constexpr int sym_index(int dim, int c, int d) {
+    return c <= d ? c * dim - c * (c - 1) / 2 + (d - c) : d * dim - d * (d - 1) / 2 + (c - d);
+}
+
+// CompressibleFlowComputeStressTensor (navierStokesTransport.cpp:510-532) and the conduction term of
+// DiffusionFlux (:496-497) for one cell gradient: gA[0][d] = dT/dx_d, gA[1 + c][d] = du_c/dx_d.
 template <typename T, int DIM>
-__device__ inline void diffusion_flux_add(const T* normal, T mu, T k, const T* vel, const T* gT, const T (*gV)[DIM], T* flux) {
+__device__ inline void stress_heat(T mu, T k, const T (*gA)[DIM], T* tq) {
     T divVel = T(0);
-    for (int c = 0; c < DIM; c++) divVel += gV[c][c];
-    T tau[DIM][DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) divVel += gA[1 + c][c];
+#pragma unroll
     for (int c = 0; c < DIM; c++)
-        for (int d = 0; d < DIM; d++)
-            tau[c][d] = (c == d) ? T(2) * mu * (gV[c][d] - divVel / T(3)) : mu * (gV[c][d] + gV[d][c]);
+#pragma unroll
+        for (int d = c; d < DIM; d++)
+            tq[sym_index(DIM, c, d)] =
+                (c == d) ? T(2) * mu * (gA[1 + c][d] - divVel / T(3)) : mu * (gA[1 + c][d] + gA[1 + d][c]);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) tq[ViscTerms<DIM>::NTAU + d] = k * gA[0][d];
+}
+
+// NavierStokesTransport::DiffusionFlux (navierStokesTransport.cpp:454-508) from face tau/q and face velocity;
+// normal = area-weighted normal.  Adds into flux.
+template <typename T, int DIM>
+__device__ inline void viscous_flux_add(const T* normal, const T* vel, const T* tq, T* flux) {
+#pragma unroll
     for (int c = 0; c < DIM; c++) {
         T vf = T(0);
-        for (int d = 0; d < DIM; d++) vf += -normal[d] * tau[c][d];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) vf += -normal[d] * tq[sym_index(DIM, c, d)];
         flux[kRhoU + c] += vf;
     }
     T e = T(0);
+#pragma unroll
     for (int d = 0; d < DIM; d++) {
         T hf = T(0);
-        for (int c = 0; c < DIM; c++) hf += vel[c] * tau[d][c];
-        hf += k * gT[d];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) hf += vel[c] * tq[sym_index(DIM, d, c)];
+        hf += tq[ViscTerms<DIM>::NTAU + d];
         e += -normal[d] * hf;
     }
     flux[kRhoE] += e;

@@ -72,42 +72,48 @@ __device__ inline void aux_of(const double* u, const Phys& ph, double* a) {
 enum : int { kLimNone = 0, kLimPetsc = 1, kLimBJ = 2 };
 
 // Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
-template <int DIM, bool RECON, bool VISC, int LIM>
-__global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                      double* __restrict__ gradC, double* __restrict__ gradA) {
+// G is the storage/arithmetic type of weights and gradients (double: FP64 form, float: mixed form); state
+// differences are always taken in FP64 before rounding to G.
+template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                                      G* __restrict__ gradC, G* __restrict__ gradA) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    const G* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
     double uc[NE], ac[NA];
 #pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
     if (VISC) aux_of<DIM>(uc, ph, ac);
-    double g[NE][DIM], ga[NA][DIM], umin[NE], umax[NE];
+    G g[NE][DIM], ga[NA][DIM];
+    double umin[NE], umax[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         umin[p] = uc[p];
         umax[p] = uc[p];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) g[p][d] = 0.0;
+        for (int d = 0; d < DIM; d++) g[p][d] = G(0);
     }
 #pragma unroll
     for (int q = 0; q < NA; q++)
 #pragma unroll
-        for (int d = 0; d < DIM; d++) ga[q][d] = 0.0;
+        for (int d = 0; d < DIM; d++) ga[q][d] = G(0);
 
-    for (int k = 0; k < m.K; k++) {
-        const int32_t e = m.ell[k * NI + c];
-        if (e < 0) break;
-        const int64_t f = e >> 1;
-        const int s = e & 1;
-        const int64_t n = m.fcell[(1 - s) * F + f];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const Slot sl = face_slot(m, k, c);
+        const int64_t f = sl.f, n = sl.n;
+        const int s = sl.s;
         const bool bnd = n >= NI;
         double un[NE];
 #pragma unroll
         for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
+        G w[2][DIM], wa[2][DIM];
+        if (RECON) load_face_pair<DIM, G>(m.fw, f, w);
+        if (VISC) load_face_pair<DIM, G>(wa_arr, f, wa);
+        if (!sl.valid) continue;
         if (RECON) {
             if (LIM == kLimBJ) {
 #pragma unroll
@@ -117,17 +123,15 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
                 }
             }
             if (!bnd) {
-                double w[2][DIM];
-                load_face_pair<DIM>(m.fw, f, w);
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
                     // delta = u_R - u_L; the left cell adds w.delta, the right cell subtracts it
                     if (s == 0) {
-                        const double delta = un[p] - uc[p];
+                        const G delta = G(un[p] - uc[p]);
 #pragma unroll
                         for (int d = 0; d < DIM; d++) g[p][d] += w[0][d] * delta;
                     } else {
-                        const double delta = uc[p] - un[p];
+                        const G delta = G(uc[p] - un[p]);
 #pragma unroll
                         for (int d = 0; d < DIM; d++) g[p][d] -= w[1][d] * delta;
                     }
@@ -135,39 +139,38 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
             }
         }
         if (VISC) {
-            double an[NA], w[2][DIM];
+            double an[NA];
             aux_of<DIM>(un, ph, an);
-            load_face_pair<DIM>(wa_arr, f, w);
 #pragma unroll
             for (int q = 0; q < NA; q++) {
-                const double delta = an[q] - ac[q];
+                const G delta = G(an[q] - ac[q]);
 #pragma unroll
-                for (int d = 0; d < DIM; d++) ga[q][d] += w[s][d] * delta;
+                for (int d = 0; d < DIM; d++) ga[q][d] += wa[s][d] * delta;
             }
         }
     }
 
     if (RECON && LIM != kLimNone) {
-        double phi[NE];
+        G phi[NE];
         if (LIM == kLimBJ) {
             // phi = min(1, (u_max - u_c)/max_f D_f+, (u_min - u_c)/min_f D_f-), D_f = grad.(x_f - x_c): the per-face
             // minimum collapsed onto the extreme projections (division is monotone).
-            double dpos[NE], dneg[NE];
+            G dpos[NE], dneg[NE];
 #pragma unroll
             for (int p = 0; p < NE; p++) {
-                dpos[p] = 0.0;
-                dneg[p] = 0.0;
+                dpos[p] = G(0);
+                dneg[p] = G(0);
             }
-            for (int k = 0; k < m.K; k++) {
-                const int32_t e = m.ell[k * NI + c];
-                if (e < 0) break;
-                const int64_t f = e >> 1;
-                const int s = e & 1;
-                double x[2][DIM];
-                load_face_pair<DIM>(m.fdx, f, x);
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                const Slot sl = face_slot(m, k, c);
+                G x[2][DIM];
+                load_face_pair<DIM, G>(m.fdx, sl.f, x);
+                if (!sl.valid) continue;
+                const int s = sl.s;
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
-                    double D = 0.0;
+                    G D = G(0);
 #pragma unroll
                     for (int d = 0; d < DIM; d++) D += g[p][d] * x[s][d];
                     dpos[p] = fmax(dpos[p], D);
@@ -176,57 +179,58 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<double> m, Phys p
             }
 #pragma unroll
             for (int p = 0; p < NE; p++) {
-                double v = 1.0;
-                if (dpos[p] > 0.0) v = fmin(v, (umax[p] - uc[p]) / dpos[p]);
-                if (dneg[p] < 0.0) v = fmin(v, (umin[p] - uc[p]) / dneg[p]);
+                G v = G(1);
+                if (dpos[p] > G(0)) v = fmin(v, G(umax[p] - uc[p]) / dpos[p]);
+                if (dneg[p] < G(0)) v = fmin(v, G(umin[p] - uc[p]) / dneg[p]);
                 phi[p] = v;
             }
         } else {
             // DMPlexApplyLimiter_Internal (cellInterpolant.cpp:282-328): f = 1/2 (u_n - u_c) / (grad.(x_n - x_c))
 #pragma unroll
-            for (int p = 0; p < NE; p++) phi[p] = DBL_MAX;
-            for (int k = 0; k < m.K; k++) {
-                const int32_t e = m.ell[k * NI + c];
-                if (e < 0) break;
-                const int64_t f = e >> 1;
-                const int s = e & 1;
-                const int64_t n = m.fcell[(1 - s) * F + f];
-                double x[2][DIM], v[DIM];
-                load_face_pair<DIM>(m.fdx, f, x);
+            for (int p = 0; p < NE; p++) phi[p] = sizeof(G) == 8 ? G(DBL_MAX) : G(FLT_MAX);
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                const Slot sl = face_slot(m, k, c);
+                const int64_t n = sl.n;
+                const int s = sl.s;
+                G x[2][DIM], v[DIM];
+                load_face_pair<DIM, G>(m.fdx, sl.f, x);
+                if (!sl.valid) continue;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) v[d] = x[s][d] - x[1 - s][d];
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
-                    double denom = 0.0;
+                    G denom = G(0);
 #pragma unroll
                     for (int d = 0; d < DIM; d++) denom += g[p][d] * v[d];
-                    const double flim = 0.5 * (U[p * N + n] - uc[p]) / denom;
-                    phi[p] = pmin(phi[p], limiter_phi<double>(ph.limiter, flim));
+                    const G flim = G(0.5) * G(U[p * N + n] - uc[p]) / denom;
+                    phi[p] = pmin(phi[p], limiter_phi<G>(ph.limiter, flim));
                 }
             }
         }
         bool cancel = false;
 #pragma unroll
         for (int p = 0; p < NE; p++)
-            if (phi[p] == 0)
+            if (phi[p] == G(0))
 #pragma unroll
                 for (int d = 0; d < DIM; d++)
-                    if (fabs(g[p][d]) > ph.max_lim_grad) cancel = true;
+                    if (double(fabs(g[p][d])) > ph.max_lim_grad) cancel = true;
 #pragma unroll
         for (int p = 0; p < NE; p++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) g[p][d] *= cancel ? 0.0 : phi[p];
+            for (int d = 0; d < DIM; d++) g[p][d] *= cancel ? G(0) : phi[p];
     }
     if (RECON)
 #pragma unroll
         for (int p = 0; p < NE; p++)
 #pragma unroll
             for (int d = 0; d < DIM; d++) gradC[(p * DIM + d) * N + c] = g[p][d];
-    if (VISC)
+    if (VISC) {
+        G tq[ViscTerms<DIM>::NT];
+        stress_heat<G, DIM>(G(ph.mu), G(ph.k), ga, tq);
 #pragma unroll
-        for (int q = 0; q < NA; q++)
-#pragma unroll
-            for (int d = 0; d < DIM; d++) gradA[(q * DIM + d) * N + c] = ga[q][d];
+        for (int q = 0; q < ViscTerms<DIM>::NT; q++) gradA[q * N + c] = tq[q];
+    }
 }
 
 // densityYi gradients (limited, one field) and Yi aux gradients (unlimited).
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
         }
         if (aux)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradA[((DIM + 1 + sp) * DIM + d) * N + c] = ga[d];
+            for (int d = 0; d < DIM; d++) gradA[(ViscTerms<DIM>::NT + sp * DIM + d) * N + c] = ga[d];
     }
     if (cancel)  // the maxLimGrad rule zeroes the whole densityYi field of the cell (cellInterpolant.cpp:459-481)
         for (int sp = 0; sp < ph.ns; sp++)
@@ -339,141 +343,188 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
             for (int d = 0; d < DIM; d++) gradC[((NE + sp) * DIM + d) * N + c] = 0.0;
 }
 
-// Face fluxes gathered per cell.  One lane per interior cell; both cells of a face evaluate its flux with the
-// reference orientation, so the gathered residual is conservative and order-deterministic.
+// Everything a face flux needs from one side: the state projected to the face (ProjectToFace,
+// cellInterpolant.cpp:890-950), the cell density, temperature and velocity (face-interpolant values) and the cell's
+// viscous terms.
+template <int DIM, bool VISC>
+struct SideData {
+    static constexpr int NE = DIM + 2;
+    double uf[NE];
+    double rho;
+    double T;
+    double vel[DIM];
+    double tq[VISC ? ViscTerms<DIM>::NT : 1];
+};
+
+// Loads side data of cell c for a face whose centroid is at x_c + dx (ghost cells have zero gradients; their
+// viscous terms are filled by ghost_visc).
+template <int DIM, bool RECON, bool VISC>
+__device__ inline void load_side(const double* __restrict__ U, const double* __restrict__ gradC,
+                                 const double* __restrict__ gradA, int64_t N, int64_t c, bool ghost, const double* dx,
+                                 SideData<DIM, VISC>& sd) {
+    constexpr int NE = DIM + 2;
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        const double u = U[p * N + c];
+        double a = 0.0;
+        if (RECON && !ghost)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) a += gradC[(p * DIM + d) * N + c] * dx[d];
+        sd.uf[p] = u + a;
+        if (p == kRho) sd.rho = u;
+        if (VISC) {
+            if (p == kRhoE) sd.T = u;
+            if (p >= kRhoU) sd.vel[p - kRhoU] = u;
+        }
+    }
+    if (VISC) {
+        const double irho = 1.0 / sd.rho;
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            sd.vel[d] *= irho;
+            ke += sd.vel[d] * sd.vel[d];
+        }
+        sd.T = sd.T * irho - 0.5 * ke;  // internal energy; scaled to T by the caller's cv only where needed
+        if (!ghost)
+#pragma unroll
+            for (int q = 0; q < ViscTerms<DIM>::NT; q++) sd.tq[q] = gradA[q * N + c];
+    }
+}
+
+// Viscous terms of a ghost cell: its one-neighbour least-squares gradient w (a_interior - a_ghost)
+// (the LeastSquaresAverage stencil of a ghost cell, stencils/leastSquaresAverage.cpp:69-155).
+template <int DIM, bool VISC>
+__device__ inline void ghost_visc(const Phys& ph, const double* w, const SideData<DIM, VISC>& in, SideData<DIM, VISC>& gh) {
+    double gA[DIM + 1][DIM];
+    const double dT = (in.T - gh.T) * ph.icv;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        gA[0][d] = w[d] * dT;
+#pragma unroll
+        for (int q = 0; q < DIM; q++) gA[1 + q][d] = w[d] * (in.vel[q] - gh.vel[q]);
+    }
+    stress_heat<double, DIM>(ph.mu, ph.k, gA, gh.tq);
+}
+
+// The flux through one face oriented left -> right (ComputeFluxSourceTerms + the continuous face loop,
+// cellInterpolant.cpp:561-631 and faceInterpolant.cpp:356-435): the advection flux of the projected states and
+// the viscous flux from face-averaged viscous terms.  Also returns the coefficients of the species fluxes:
+// F_sp = cl * rhoY_L + cr * rhoY_R + cd * n.(gradY_L + gradY_R).
+template <int DIM, bool VISC>
+__device__ inline void face_flux(const Phys& ph, const double* nhat, double area, const SideData<DIM, VISC>& L,
+                                 const SideData<DIM, VISC>& R, double* flux, double& cl, double& cr, double& cd) {
+    constexpr int NE = DIM + 2;
+#pragma unroll
+    for (int p = 0; p < NE; p++) flux[p] = 0.0;
+    cl = 0.0;
+    cr = 0.0;
+    cd = 0.0;
+    if (ph.flux == ABL_FV_FLUX_RUSANOV) {
+        double unL, unR, lam;
+        rusanov_flux<double, DIM>(L.uf, R.uf, nhat, area, ph, flux, unL, unR, lam);
+        cl = area * (0.5 * unL + 0.5 * lam);
+        cr = area * (0.5 * unR - 0.5 * lam);
+    } else if (ph.flux != ABL_FV_FLUX_NONE) {
+        double mA, rl, rr;
+        int right;
+        advection_flux<double, DIM>(ph.flux, L.uf, R.uf, nhat, area, ph, flux, mA, right, rl, rr);
+        if (right)
+            cr = mA / rr;
+        else
+            cl = mA / rl;
+    }
+    if (VISC) {
+        double vel[DIM], tq[ViscTerms<DIM>::NT], normal[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            vel[d] = 0.5 * L.vel[d] + 0.5 * R.vel[d];
+            normal[d] = area * nhat[d];
+        }
+#pragma unroll
+        for (int q = 0; q < ViscTerms<DIM>::NT; q++) tq[q] = 0.5 * L.tq[q] + 0.5 * R.tq[q];
+        viscous_flux_add<double, DIM>(normal, vel, tq, flux);
+        // species diffusion: -D rho_face n.(1/2 (gradY_L + gradY_R)), rho_face = 1/2 (rho_L + rho_R)
+        cd = -ph.diff * (0.5 * L.rho + 0.5 * R.rho) * 0.5 * area;
+    }
+}
+
+// Species fluxes of one face (SpeciesTransport AdvectionFlux / DiffusionSpeciesFlux, speciesTransport.cpp:196-341)
+// from the coefficients of face_flux: sum over species of a per-species scalar expression.
+template <int DIM, bool RECON, bool VISC>
+__device__ inline double species_face_flux(const MeshView<double>& m, const double* __restrict__ U,
+                                           const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                           int64_t comp, int sp, int64_t f, int64_t L, int64_t R, bool ghost,
+                                           const double (*x)[DIM], const double* nhat, double cl, double cr, double cd,
+                                           const double* wghost) {
+    const int64_t N = m.N;
+    double yL = U[comp * N + L], yR = U[comp * N + R];
+    const double rL = U[kRho * N + L], rR = U[kRho * N + R];
+    double flux = 0.0;
+    if (RECON) {
+        double pl = 0.0, pr = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            pl += gradC[(comp * DIM + d) * N + L] * x[0][d];
+            if (!ghost) pr += gradC[(comp * DIM + d) * N + R] * x[1][d];
+        }
+        flux = cl * (yL + pl) + cr * (yR + pr);
+    } else {
+        flux = cl * yL + cr * yR;
+    }
+    if (VISC) {
+        double dot = 0.0;
+        const int64_t base = ViscTerms<DIM>::NT + (int64_t)sp * DIM;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double gl = gradA[(base + d) * N + L];
+            const double gr = ghost ? wghost[d] * (yL / rL - yR / rR) : gradA[(base + d) * N + R];
+            dot += nhat[d] * (gl + gr);
+        }
+        flux += cd * dot;
+    }
+    return flux;
+}
+
+// Strategy 1 (recompute): face fluxes gathered per cell.  One lane per interior cell; both cells of a face
+// evaluate its flux with the reference orientation, so the gathered residual is conservative and
+// order-deterministic, with no face-flux storage.
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                  const double* __restrict__ gradC, const double* __restrict__ gradA,
                                                  double* __restrict__ R, int accumulate) {
     constexpr int NE = DIM + 2;
-    constexpr int NA = DIM + 1;
-    constexpr int MAXK = SPECIES ? kMaxFacesPerCell : 1;
+    using SD = SideData<DIM, VISC>;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
     const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
-
-    double uc[NE], gc[NE][DIM], ac[NA], gac[NA][DIM];
-#pragma unroll
-    for (int p = 0; p < NE; p++) {
-        uc[p] = U[p * N + c];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) gc[p][d] = RECON ? gradC[(p * DIM + d) * N + c] : 0.0;
-    }
-    if (VISC) {
-        aux_of<DIM>(uc, ph, ac);
-#pragma unroll
-        for (int q = 0; q < NA; q++)
-#pragma unroll
-            for (int d = 0; d < DIM; d++) gac[q][d] = gradA[(q * DIM + d) * N + c];
-    }
     double acc[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) acc[p] = 0.0;
-    // per-face coefficients for the species pass: contribution = cL * Y_L + cR * Y_R (+ diffusion)
-    double sp_cl[MAXK], sp_cr[MAXK], sp_cd[MAXK];
-    int sp_face[MAXK];
-    int nk = 0;
-
     for (int k = 0; k < m.K; k++) {
         const int32_t e = m.ell[k * NI + c];
         if (e < 0) break;
         const int64_t f = e >> 1;
         const int s = e & 1;
-        const int64_t n = m.fcell[(1 - s) * F + f];
-        const bool ghost = n >= NI;
-        double nh[3], area;
+        const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+        const bool ghost = Rc >= NI;  // a ghost is always the right cell
+        double nh[3], area, x[2][DIM];
         load_face_na(m.fna, f, nh, area);
-        double nhat[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) nhat[d] = nh[d];
-        double un[NE];
-#pragma unroll
-        for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
-        // ProjectToFace for both sides (ghost gradients are zero)
-        double uself[NE], unbr[NE];
-        if (RECON) {
-            double x[2][DIM];
-            load_face_pair<DIM>(m.fdx, f, x);
-#pragma unroll
-            for (int p = 0; p < NE; p++) {
-                double a = 0.0, b = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    a += gc[p][d] * x[s][d];
-                    if (!ghost) b += gradC[(p * DIM + d) * N + n] * x[1 - s][d];
-                }
-                uself[p] = uc[p] + a;
-                unbr[p] = un[p] + b;
-            }
-        } else {
-#pragma unroll
-            for (int p = 0; p < NE; p++) {
-                uself[p] = uc[p];
-                unbr[p] = un[p];
-            }
+        load_face_pair<DIM>(m.fdx, f, x);
+        SD sl, sr;
+        load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
+        load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+        if (VISC && ghost) {
+            double w[2][DIM];
+            load_face_pair<DIM>(wa_arr, f, w);
+            ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
         }
-        const double* uL = s == 0 ? uself : unbr;
-        const double* uR = s == 0 ? unbr : uself;
-        double flux[NE];
-#pragma unroll
-        for (int p = 0; p < NE; p++) flux[p] = 0.0;
-        double cl = 0.0, cr = 0.0;
-        if (ph.flux == ABL_FV_FLUX_RUSANOV) {
-            double unL, unR, lam;
-            rusanov_flux<double, DIM>(uL, uR, nhat, area, ph, flux, unL, unR, lam);
-            cl = area * (0.5 * unL + 0.5 * lam);
-            cr = area * (0.5 * unR - 0.5 * lam);
-        } else if (ph.flux != ABL_FV_FLUX_NONE) {
-            double mA, rl, rr;
-            int right;
-            advection_flux<double, DIM>(ph.flux, uL, uR, nhat, area, ph, flux, mA, right, rl, rr);
-            if (right)
-                cr = mA / rr;
-            else
-                cl = mA / rl;
-        }
-        double cd = 0.0;
-        if (VISC) {
-            // face interpolant: value = 1/2 (a_L + a_R), gradient = 1/2 (G_L + G_R) of unlimited LSQ gradients
-            double an[NA], gan[NA][DIM];
-            aux_of<DIM>(un, ph, an);
-            if (ghost) {
-                double w[2][DIM];
-                load_face_pair<DIM>(wa_arr, f, w);
-#pragma unroll
-                for (int q = 0; q < NA; q++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) gan[q][d] = w[1][d] * (ac[q] - an[q]);
-            } else {
-#pragma unroll
-                for (int q = 0; q < NA; q++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) gan[q][d] = gradA[(q * DIM + d) * N + n];
-            }
-            double vel[DIM], gT[DIM], gV[DIM][DIM], normal[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                vel[d] = 0.5 * ac[1 + d] + 0.5 * an[1 + d];
-                gT[d] = 0.5 * gac[0][d] + 0.5 * gan[0][d];
-                normal[d] = area * nhat[d];
-#pragma unroll
-                for (int q = 0; q < DIM; q++) gV[q][d] = 0.5 * gac[1 + q][d] + 0.5 * gan[1 + q][d];
-            }
-            diffusion_flux_add<double, DIM>(normal, ph.mu, ph.k, vel, gT, gV, flux);
-            // species diffusion: -D rho_face sum_d n_d gradY_d, rho_face = 1/2 (rho_L + rho_R)
-            cd = -ph.diff * (0.5 * uc[kRho] + 0.5 * un[kRho]) * 0.5;
-        }
+        double flux[NE], cl, cr, cd;
+        face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
         const double sign = s == 0 ? -1.0 : 1.0;
 #pragma unroll
         for (int p = 0; p < NE; p++) acc[p] += sign * flux[p];
-        if (SPECIES) {
-            sp_cl[nk] = sign * cl;
-            sp_cr[nk] = sign * cr;
-            sp_cd[nk] = sign * cd;
-            sp_face[nk] = e;
-            nk++;
-        }
     }
     const double iv = m.inv_vol[c];
 #pragma unroll
@@ -482,82 +533,399 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         R[p * N + c] = accumulate ? R[p * N + c] + r : r;
     }
     if (!SPECIES) return;
-    // species pass: reconstructed densityYi on each side, upwinded by the stored coefficients
-    const double rho_c = uc[kRho];
+    // species: each face's coefficients are re-evaluated per species pass (register-light)
     for (int sp = 0; sp < ph.ns; sp++) {
         const int64_t comp = NE + sp;
-        const double yc = U[comp * N + c];
-        double gyc[DIM], gac_s[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            gyc[d] = RECON ? gradC[(comp * DIM + d) * N + c] : 0.0;
-            gac_s[d] = VISC ? gradA[((DIM + 1 + sp) * DIM + d) * N + c] : 0.0;
-        }
         double a = 0.0;
-        for (int k = 0; k < nk; k++) {
-            const int32_t e = sp_face[k];
+        for (int k = 0; k < m.K; k++) {
+            const int32_t e = m.ell[k * NI + c];
+            if (e < 0) break;
             const int64_t f = e >> 1;
             const int s = e & 1;
-            const int64_t n = m.fcell[(1 - s) * F + f];
-            const bool ghost = n >= NI;
-            const double yn = U[comp * N + n];
-            double yself = yc, ynbr = yn;
-            if (RECON) {
-                double x[2][DIM];
-                load_face_pair<DIM>(m.fdx, f, x);
-                double ps = 0.0, pn = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    ps += gyc[d] * x[s][d];
-                    if (!ghost) pn += gradC[(comp * DIM + d) * N + n] * x[1 - s][d];
-                }
-                yself += ps;
-                ynbr += pn;
-            }
-            const double yL = s == 0 ? yself : ynbr;
-            const double yR = s == 0 ? ynbr : yself;
-            double contrib = sp_cl[k] * yL + sp_cr[k] * yR;
-            if (VISC) {
-                double gn[DIM];
-                if (ghost) {
-                    double w[2][DIM];
-                    load_face_pair<DIM>(wa_arr, f, w);
-                    const double rho_n = U[kRho * N + n];
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) gn[d] = w[1][d] * (yc / rho_c - yn / rho_n);
-                } else {
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) gn[d] = gradA[((DIM + 1 + sp) * DIM + d) * N + n];
-                }
-                double nh[3], area;
-                load_face_na(m.fna, f, nh, area);
-                double dot = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) dot += area * nh[d] * (gac_s[d] + gn[d]);
-                contrib += sp_cd[k] * dot;
-            }
-            a += contrib;
+            const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+            const bool ghost = Rc >= NI;
+            double nh[3], area, x[2][DIM], w[2][DIM];
+            load_face_na(m.fna, f, nh, area);
+            load_face_pair<DIM>(m.fdx, f, x);
+            if (VISC && ghost) load_face_pair<DIM>(wa_arr, f, w);
+            SD sl, sr;
+            load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
+            load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+            if (VISC && ghost) ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
+            double flux[NE], cl, cr, cd;
+            face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
+            const double fs = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, comp, sp, f, L, Rc, ghost, x, nh,
+                                                                   cl, cr, cd, w[1]);
+            a += (s == 0 ? -1.0 : 1.0) * fs;
         }
         const double r = a * iv;
         R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
     }
 }
 
-// ---- compile-time variant selection
-using GradKernel = void (*)(MeshView<double>, Phys, const double*, double*, double*);
-using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, int);
+// Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored
+// (euler components as a padded record [F][FS], species as [ns][F]).
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
+__global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                                      double* __restrict__ Fe, double* __restrict__ Fs) {
+    constexpr int NE = DIM + 2;
+    constexpr int FS = (NE + 1) & ~1;
+    using SD = SideData<DIM, VISC>;
+    const int64_t f = xcd_cell_index();
+    if (f >= m.F) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+    const bool ghost = Rc >= NI;
+    double nh[3], area, x[2][DIM], w[2][DIM];
+    load_face_na(m.fna, f, nh, area);
+    load_face_pair<DIM>(m.fdx, f, x);
+    SD sl, sr;
+    load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
+    load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+    if (VISC && ghost) {
+        load_face_pair<DIM>(m.fwa ? m.fwa : m.fw, f, w);
+        ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
+    }
+    double flux[FS], cl, cr, cd;
+    face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
+    if (FS > NE) flux[FS - 1] = 0.0;
+    double2* out = reinterpret_cast<double2*>(Fe + FS * f);
+#pragma unroll
+    for (int i = 0; i < FS / 2; i++) out[i] = make_double2(flux[2 * i], flux[2 * i + 1]);
+    if (SPECIES)
+        for (int sp = 0; sp < ph.ns; sp++)
+            Fs[(int64_t)sp * F + f] = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, NE + sp, sp, f, L, Rc,
+                                                                          ghost, x, nh, cl, cr, cd, w[1]);
+}
+
+// Strategy 2, pass 2: each interior cell gathers its faces' stored fluxes in face-number order.
+template <int DIM, bool SPECIES, int KM>
+__global__ __launch_bounds__(kBlock) void k_gather(MeshView<double> m, int ns, const double* __restrict__ Fe,
+                                                   const double* __restrict__ Fs, double* __restrict__ R, int accumulate) {
+    constexpr int NE = DIM + 2;
+    constexpr int FS = (NE + 1) & ~1;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    int64_t fk[KM];
+    double sg[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
+        fk[k] = e >= 0 ? (int64_t)(e >> 1) : 0;
+        sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
+    }
+    double v[KM][FS];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const double2* rec = reinterpret_cast<const double2*>(Fe + FS * fk[k]);
+#pragma unroll
+        for (int i = 0; i < FS / 2; i++) {
+            const double2 t = rec[i];
+            v[k][2 * i] = t.x;
+            v[k][2 * i + 1] = t.y;
+        }
+    }
+    double acc[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        acc[p] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++)
+            if (sg[k] != 0.0) acc[p] += sg[k] * v[k][p];
+    }
+    const double iv = m.inv_vol[c];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        const double r = acc[p] * iv;
+        R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+    }
+    if (!SPECIES) return;
+    for (int sp = 0; sp < ns; sp++) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const double fv = Fs[(int64_t)sp * F + fk[k]];
+            if (sg[k] != 0.0) a += sg[k] * fv;
+        }
+        const int64_t comp = NE + sp;
+        const double r = a * iv;
+        R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
+    }
+}
+
+// ============================================================================================ mixed form
+//
+// The residual of a cell is a sum of face fluxes that cancel to O(1 %) of their size (and far more at low Mach),
+// so FP32 fluxes would lose ~2 digits.  The mixed form therefore evaluates, per face, the PERTURBATION of the flux
+// against the flux of the owning cell's own state, dF = F(U_L, U_R) - F(U_c).n: the baseline terms cancel exactly
+// over a closed cell (sum_f n_f A_f = 0), and dF is small, so FP32 keeps ~7 digits of it.  The cell state is
+// decoded in FP64 (pressure / temperature recovery from total energy); face-side states enter as FP32 perturbations
+// dU (FP64 differences of cell states rounded to FP32, plus FP32 reconstruction increments), turned into
+// perturbations of the primitive variables by exact identities; the AUSM split functions enter through factored
+// differences, e.g. P+(M_L) - P+(M_c) = 1/4 (M_L - M_c)(3 - M_L^2 - M_L M_c - M_c^2).  Faces with a supersonic
+// side, and the non-AUSM calculators, take the FP64 route (full flux minus FP64 baseline).  Accumulation is FP64.
 
 template <int DIM>
-GradKernel pick_gradients(bool recon, bool visc, int lim) {
-    if (!recon) return visc ? k_gradients<DIM, false, true, kLimNone> : nullptr;
-    if (visc) {
-        if (lim == kLimBJ) return k_gradients<DIM, true, true, kLimBJ>;
-        if (lim == kLimPetsc) return k_gradients<DIM, true, true, kLimPetsc>;
-        return k_gradients<DIM, true, true, kLimNone>;
+struct Baseline {  // FP64 EOS recovery of a cell (perfectGas.cpp:67-113, 147-174)
+    double rho, vel[DIM], eps, T, p, a, H;
+};
+
+template <int DIM>
+__device__ inline void decode_baseline(const double* u, const Phys& ph, Baseline<DIM>& b) {
+    b.rho = u[kRho];
+    const double irho = 1.0 / b.rho;
+    double ke = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        b.vel[d] = u[kRhoU + d] * irho;
+        ke += b.vel[d] * b.vel[d];
     }
-    if (lim == kLimBJ) return k_gradients<DIM, true, false, kLimBJ>;
-    if (lim == kLimPetsc) return k_gradients<DIM, true, false, kLimPetsc>;
-    return k_gradients<DIM, true, false, kLimNone>;
+    b.eps = u[kRhoE] * irho;
+    b.T = (b.eps - 0.5 * ke) * ph.icv;
+    b.p = ph.R * b.rho * b.T;
+    b.a = sqrt(ph.gamma * ph.R * b.T);
+    b.H = b.eps + ph.R * b.T;
+}
+
+template <int DIM>
+struct BaselineF {  // FP32 copies of the baseline for products with perturbations
+    float rho, vel[DIM], eps, T, p, a, H;
+};
+
+template <int DIM>
+struct SidePert {
+    float drho, rho, du[DIM], dT, dp, a, da, dM, M, dH;
+};
+
+// Perturbations of the primitive variables of a face side from its conserved perturbation dU.
+template <int DIM>
+__device__ inline void side_perturbation(const float* dU, const BaselineF<DIM>& b, const float* nhat, float unc, float Mc,
+                                         float gR, float R, float icv, SidePert<DIM>& sp) {
+    sp.drho = dU[kRho];
+    sp.rho = b.rho + sp.drho;
+    const float irho = 1.0f / sp.rho;
+    float dun = 0.f, dke = 0.f, dd = 0.f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        sp.du[d] = (dU[kRhoU + d] - b.vel[d] * sp.drho) * irho;
+        dun += sp.du[d] * nhat[d];
+        dke += b.vel[d] * sp.du[d];
+        dd += sp.du[d] * sp.du[d];
+    }
+    const float deps = (dU[kRhoE] - b.eps * sp.drho) * irho;
+    const float de = deps - (dke + 0.5f * dd);
+    sp.dT = de * icv;
+    sp.dp = R * (sp.rho * sp.dT + b.T * sp.drho);
+    sp.a = sqrtf(gR * (b.T + sp.dT));
+    sp.da = gR * sp.dT / (sp.a + b.a);
+    sp.dM = (dun * b.a - unc * sp.da) / (sp.a * b.a);
+    sp.M = Mc + sp.dM;
+    sp.dH = deps + R * sp.dT;
+}
+
+template <int DIM, bool RECON, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
+                                                       const float* __restrict__ gradC, const float* __restrict__ gradA,
+                                                       double* __restrict__ R, int accumulate) {
+    constexpr int NE = DIM + 2;
+    constexpr int NT = ViscTerms<DIM>::NT;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    const float* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    double uc[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
+    Baseline<DIM> b;
+    decode_baseline<DIM>(uc, ph, b);
+    BaselineF<DIM> bf;
+    bf.rho = float(b.rho);
+    bf.eps = float(b.eps);
+    bf.T = float(b.T);
+    bf.p = float(b.p);
+    bf.a = float(b.a);
+    bf.H = float(b.H);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) bf.vel[d] = float(b.vel[d]);
+    const float gR = float(ph.gamma * ph.R), Rg = float(ph.R), icv = float(ph.icv);
+    float tqc[VISC ? NT : 1];
+    if (VISC)
+#pragma unroll
+        for (int q = 0; q < NT; q++) tqc[q] = gradA[q * N + c];
+    double acc[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) acc[p] = 0.0;
+
+    for (int k = 0; k < m.K; k++) {
+        const int32_t e = m.ell[k * NI + c];
+        if (e < 0) break;
+        const int64_t f = e >> 1;
+        const int s = e & 1;
+        const int64_t n = m.fcell[(1 - s) * F + f];
+        const bool ghost = n >= NI;
+        float nh[3], area, x[2][DIM];
+        load_face_na(m.fna, f, nh, area);
+        load_face_pair<DIM, float>(m.fdx, f, x);
+        float nhat[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) nhat[d] = nh[d];
+        // conserved perturbations of both face sides relative to the owning cell
+        float dself[NE], dnbr[NE], dcell[NE];
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            const double un = U[p * N + n];
+            dcell[p] = float(un - uc[p]);
+            float a = 0.f, bb = 0.f;
+            if (RECON)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    a += gradC[(p * DIM + d) * N + c] * x[s][d];
+                    if (!ghost) bb += gradC[(p * DIM + d) * N + n] * x[1 - s][d];
+                }
+            dself[p] = a;
+            dnbr[p] = dcell[p] + bb;
+        }
+        const float* dL = s == 0 ? dself : dnbr;
+        const float* dR = s == 0 ? dnbr : dself;
+        float unc = 0.f;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) unc += bf.vel[d] * nhat[d];
+        const float Mc = unc / bf.a;
+        SidePert<DIM> L, Rs;
+        side_perturbation<DIM>(dL, bf, nhat, unc, Mc, gR, Rg, icv, L);
+        side_perturbation<DIM>(dR, bf, nhat, unc, Mc, gR, Rg, icv, Rs);
+        float dF[NE];
+#pragma unroll
+        for (int p = 0; p < NE; p++) dF[p] = 0.f;
+        const bool subsonic = fabsf(L.M) <= 1.f && fabsf(Rs.M) <= 1.f && fabsf(Mc) <= 1.f;
+        if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
+            const float dm = 0.25f * L.dM * (L.M + Mc + 2.f) - 0.25f * Rs.dM * (Rs.M + Mc - 2.f);
+            const float mm = Mc + dm;
+            const SidePert<DIM>& up = mm < 0.f ? Rs : L;
+            const float mdot_c = unc * bf.rho;
+            const float dmdot = dm * up.a * up.rho + Mc * (up.da * up.rho + bf.a * up.drho);
+            const float mdot = mdot_c + dmdot;
+            const float PpL = 0.25f * (L.M + 1.f) * (L.M + 1.f) * (2.f - L.M);
+            const float PmR = 0.25f * (Rs.M - 1.f) * (Rs.M - 1.f) * (2.f + Rs.M);
+            const float dPpL = 0.25f * L.dM * (3.f - (L.M * L.M + L.M * Mc + Mc * Mc));
+            const float dPpR = 0.25f * Rs.dM * (3.f - (Rs.M * Rs.M + Rs.M * Mc + Mc * Mc));
+            const float dp12 = PpL * L.dp + PmR * Rs.dp + bf.p * (dPpL - dPpR);
+            dF[kRho] = dmdot;
+            dF[kRhoE] = bf.H * dmdot + up.dH * mdot;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) dF[kRhoU + d] = bf.vel[d] * dmdot + up.du[d] * mdot + dp12 * nhat[d];
+#pragma unroll
+            for (int p = 0; p < NE; p++) dF[p] *= area;
+        } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF) {
+            // FP64 route: full face flux of the reconstructed states minus the FP64 baseline flux
+            double uLd[NE], uRd[NE], nhd[DIM], fl[NE];
+#pragma unroll
+            for (int p = 0; p < NE; p++) {
+                uLd[p] = uc[p] + double(dL[p]);
+                uRd[p] = uc[p] + double(dR[p]);
+                fl[p] = 0.0;
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++) nhd[d] = double(nhat[d]);
+            const double ad = double(area);
+            if (ph.flux == ABL_FV_FLUX_RUSANOV) {
+                double unL, unR, lam;
+                rusanov_flux<double, DIM>(uLd, uRd, nhd, ad, ph, fl, unL, unR, lam);
+            } else {
+                double mA, rl, rr;
+                int right;
+                advection_flux<double, DIM>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rl, rr);
+            }
+            double und = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) und += b.vel[d] * nhd[d];
+            const double mc = b.rho * und * ad;
+            fl[kRho] -= mc;
+            fl[kRhoE] -= b.H * mc;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) fl[kRhoU + d] -= b.vel[d] * mc + b.p * ad * nhd[d];
+#pragma unroll
+            for (int p = 0; p < NE; p++) dF[p] = float(fl[p]);
+        }
+        if (VISC) {
+            // face-averaged viscous terms and face velocity (cell values) in FP32
+            float tq[NT], vel[DIM], normal[DIM];
+            const float irn = 1.0f / (bf.rho + dcell[kRho]);
+            if (ghost) {
+                float w[2][DIM], gA[DIM + 1][DIM];
+                load_face_pair<DIM, float>(wa_arr, f, w);
+                float dvel[DIM], dke = 0.f, dd = 0.f;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    dvel[d] = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
+                    dke += bf.vel[d] * dvel[d];
+                    dd += dvel[d] * dvel[d];
+                }
+                const float dT = ((dcell[kRhoE] - bf.eps * dcell[kRho]) * irn - (dke + 0.5f * dd)) * icv;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    gA[0][d] = -w[1][d] * dT;
+#pragma unroll
+                    for (int q = 0; q < DIM; q++) gA[1 + q][d] = -w[1][d] * dvel[q];
+                }
+                stress_heat<float, DIM>(float(ph.mu), float(ph.k), gA, tq);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NT; q++) tq[q] = gradA[q * N + n];
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) tq[q] = 0.5f * tqc[q] + 0.5f * tq[q];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const float dv = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
+                vel[d] = bf.vel[d] + 0.5f * dv;
+                normal[d] = area * nhat[d];
+            }
+            viscous_flux_add<float, DIM>(normal, vel, tq, dF);
+        }
+        const double sign = s == 0 ? -1.0 : 1.0;
+#pragma unroll
+        for (int p = 0; p < NE; p++) acc[p] += sign * double(dF[p]);
+    }
+    const double iv = m.inv_vol[c];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        const double r = acc[p] * iv;
+        R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+    }
+}
+
+// ---- compile-time variant selection
+template <typename G>
+using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*);
+using GradKernel = GradKernelT<double>;
+using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, int);
+using FaceFluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, double*);
+using GatherKernel = void (*)(MeshView<double>, int, const double*, const double*, double*, int);
+
+template <typename G, int DIM, int KM>
+GradKernelT<G> pick_gradients_k(bool recon, bool visc, int lim) {
+    if (!recon) return visc ? k_gradients<G, DIM, false, true, kLimNone, KM> : nullptr;
+    if (visc) {
+        if (lim == kLimBJ) return k_gradients<G, DIM, true, true, kLimBJ, KM>;
+        if (lim == kLimPetsc) return k_gradients<G, DIM, true, true, kLimPetsc, KM>;
+        return k_gradients<G, DIM, true, true, kLimNone, KM>;
+    }
+    if (lim == kLimBJ) return k_gradients<G, DIM, true, false, kLimBJ, KM>;
+    if (lim == kLimPetsc) return k_gradients<G, DIM, true, false, kLimPetsc, KM>;
+    return k_gradients<G, DIM, true, false, kLimNone, KM>;
+}
+// Compile-time slot width of the cell -> face table: 4 (quads, tets), 6 (hexes) or 8.
+inline int slot_width(int K) { return K <= 4 ? 4 : (K <= 6 ? 6 : 8); }
+template <typename G, int DIM>
+GradKernelT<G> pick_gradients(bool recon, bool visc, int lim, int K) {
+    switch (slot_width(K)) {
+        case 4: return pick_gradients_k<G, DIM, 4>(recon, visc, lim);
+        case 6: return pick_gradients_k<G, DIM, 6>(recon, visc, lim);
+        default: return pick_gradients_k<G, DIM, 8>(recon, visc, lim);
+    }
 }
 
 template <int DIM, bool RECON>
@@ -568,6 +936,29 @@ FluxKernel pick_flux2(bool visc, bool species) {
 template <int DIM>
 FluxKernel pick_flux(bool recon, bool visc, bool species) {
     return recon ? pick_flux2<DIM, true>(visc, species) : pick_flux2<DIM, false>(visc, species);
+}
+template <int DIM, bool RECON>
+FaceFluxKernel pick_face_flux2(bool visc, bool species) {
+    if (visc) return species ? k_face_flux<DIM, RECON, true, true> : k_face_flux<DIM, RECON, true, false>;
+    return species ? k_face_flux<DIM, RECON, false, true> : k_face_flux<DIM, RECON, false, false>;
+}
+template <int DIM>
+FaceFluxKernel pick_face_flux(bool recon, bool visc, bool species) {
+    return recon ? pick_face_flux2<DIM, true>(visc, species) : pick_face_flux2<DIM, false>(visc, species);
+}
+using MixedFluxKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, double*, int);
+template <int DIM>
+MixedFluxKernel pick_flux_mixed(bool recon, bool visc) {
+    if (recon) return visc ? k_flux_mixed<DIM, true, true> : k_flux_mixed<DIM, true, false>;
+    return visc ? k_flux_mixed<DIM, false, true> : k_flux_mixed<DIM, false, false>;
+}
+template <int DIM>
+GatherKernel pick_gather(bool species, int K) {
+    switch (slot_width(K)) {
+        case 4: return species ? k_gather<DIM, true, 4> : k_gather<DIM, false, 4>;
+        case 6: return species ? k_gather<DIM, true, 6> : k_gather<DIM, false, 6>;
+        default: return species ? k_gather<DIM, true, 8> : k_gather<DIM, false, 8>;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_rows(double* __restrict__ R, int64_t N, int64_t start, int nc) {
@@ -646,8 +1037,16 @@ struct abl_fv_solver {
     abl::Phys ph{};
     abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind;
     abl::DevBuf<double> fna, fdx, fw, fwa, inv_vol, gradC, gradA, slot_state;
+    abl::DevBuf<double> face_flux_e, face_flux_s;  // strategy 2 face-flux storage (allocated on first use)
+    // mixed form: FP32 geometry and gradients
+    abl::DevBuf<float> fna_f, fdx_f, fw_f, fwa_f, gradC_f, gradA_f;
+    abl::GradKernelT<float> grad_kernel_f = nullptr;
+    abl::MixedFluxKernel flux_kernel_mixed = nullptr;
     abl::GradKernel grad_kernel = nullptr;
     abl::FluxKernel flux_kernel = nullptr;
+    abl::FaceFluxKernel face_flux_kernel = nullptr;
+    abl::GatherKernel gather_kernel = nullptr;
+    int strategy = ABL_FV_STRATEGY_RECOMPUTE;
     abl::DevBuf<unsigned long long> counter;
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
@@ -714,9 +1113,25 @@ struct abl_fv_solver {
         v.inv_vol = inv_vol.ptr;
         return v;
     }
+    abl::MeshView<float> view_f() const {
+        abl::MeshView<float> v;
+        v.N = host.N;
+        v.NI = host.NI;
+        v.F = host.F;
+        v.K = host.K;
+        v.ell = ell.ptr;
+        v.fcell = fcell.ptr;
+        v.fna = fna_f.ptr;
+        v.fdx = fdx_f.ptr;
+        v.fw = fw_f.ptr;
+        v.fwa = fwa_f.ptr;
+        v.inv_vol = inv_vol.ptr;
+        return v;
+    }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + slot_state.bytes() + counter.bytes());
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+                         gradA.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         fw_f.bytes() + fwa_f.bytes() + gradC_f.bytes() + gradA_f.bytes());
     }
 };
 
@@ -741,7 +1156,9 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
                              abl_fv_handle* out) {
     if (!mesh || !physics || !out) return fail(nullptr, "abl_fv_create: null argument");
     *out = nullptr;
-    if (form != ABL_FV_FORM_FP64) return fail(nullptr, "abl_fv_create: only the FP64 form is available in this build");
+    if (form != ABL_FV_FORM_FP64 && form != ABL_FV_FORM_MIXED) return fail(nullptr, "abl_fv_create: unknown form");
+    if (form == ABL_FV_FORM_MIXED && physics && physics->n_species > 0)
+        return fail(nullptr, "abl_fv_create: the mixed form does not transport species in this build");
     const abl_fv_physics_desc& p = *physics;
     if (!(p.gamma > 1.0) || !(p.Rgas > 0.0)) return fail(nullptr, "abl_fv_create: PerfectGas needs gamma > 1 and Rgas > 0");
     if (p.flux_calculator < ABL_FV_FLUX_NONE || p.flux_calculator > ABL_FV_FLUX_RUSANOV)
@@ -777,28 +1194,56 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
                         : p.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN ? abl::kLimBJ : abl::kLimPetsc;
         const bool species = p.n_species > 0;
         if (mesh->dim == 3) {
-            s->grad_kernel = abl::pick_gradients<3>(s->recon, s->visc, lim);
+            s->grad_kernel = abl::pick_gradients<double, 3>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<3>(s->recon, s->visc, species);
+            s->face_flux_kernel = abl::pick_face_flux<3>(s->recon, s->visc, species);
+            s->gather_kernel = abl::pick_gather<3>(species, L.K);
         } else {
-            s->grad_kernel = abl::pick_gradients<2>(s->recon, s->visc, lim);
+            s->grad_kernel = abl::pick_gradients<double, 2>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<2>(s->recon, s->visc, species);
+            s->face_flux_kernel = abl::pick_face_flux<2>(s->recon, s->visc, species);
+            s->gather_kernel = abl::pick_gather<2>(species, L.K);
         }
 
         s->ell.upload(L.ell);
         s->fcell.upload(L.fcell);
-        s->fna.upload(L.fna);
-        s->fdx.upload(L.fdx);
-        s->fw.upload(L.fw);
-        s->fwa.upload(L.fwa);
         s->inv_vol.upload(L.inv_vol);
         const int64_t N = L.N;
-        if (s->recon) {
-            s->gradC.alloc((size_t)L.nc * L.dim * N);
-            HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost rows stay zero
-        }
-        if (s->visc) {
-            s->gradA.alloc((size_t)(1 + L.dim + L.ns) * L.dim * N);
-            HIP_CHECK(hipMemset(s->gradA.ptr, 0, s->gradA.bytes()));
+        const int nt = L.dim * (L.dim + 1) / 2 + L.dim;
+        if (form == ABL_FV_FORM_FP64) {
+            s->fna.upload(L.fna);
+            s->fdx.upload(L.fdx);
+            s->fw.upload(L.fw);
+            s->fwa.upload(L.fwa);
+            if (s->recon) {
+                s->gradC.alloc((size_t)L.nc * L.dim * N);
+                HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost rows stay zero
+            }
+            if (s->visc) {
+                s->gradA.alloc((size_t)(nt + L.ns * L.dim) * N);
+                HIP_CHECK(hipMemset(s->gradA.ptr, 0, s->gradA.bytes()));
+            }
+        } else {
+            auto to_f = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+            s->fna_f.upload(to_f(L.fna));
+            s->fdx_f.upload(to_f(L.fdx));
+            s->fw_f.upload(to_f(L.fw));
+            s->fwa_f.upload(to_f(L.fwa));
+            if (s->recon) {
+                s->gradC_f.alloc((size_t)L.nc * L.dim * N);
+                HIP_CHECK(hipMemset(s->gradC_f.ptr, 0, s->gradC_f.bytes()));
+            }
+            if (s->visc) {
+                s->gradA_f.alloc((size_t)nt * N);
+                HIP_CHECK(hipMemset(s->gradA_f.ptr, 0, s->gradA_f.bytes()));
+            }
+            if (mesh->dim == 3) {
+                s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
+                s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc);
+            } else {
+                s->grad_kernel_f = abl::pick_gradients<float, 2>(s->recon, s->visc, lim, L.K);
+                s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc);
+            }
         }
         std::vector<int32_t> bf, bs;
         for (auto& b : L.bfaces) {
@@ -896,8 +1341,19 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                                    h->slot_state.ptr, U);
             });
         }
-        const auto v = h->view();
         const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
+        if (h->form == ABL_FV_FORM_MIXED) {
+            const auto vf = h->view_f();
+            if (h->recon || h->visc)
+                h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
+                    hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr, h->gradA_f.ptr);
+                });
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->flux_kernel_mixed, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr,
+                                   h->gradA_f.ptr, R, accumulate);
+            });
+        } else {
+        const auto v = h->view();
         if (h->recon || h->visc) {
             h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
                 hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
@@ -912,15 +1368,41 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                                            h->gradA.ptr);
                 });
         }
-        h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-            hipLaunchKernelGGL(h->flux_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R, accumulate);
-        });
+        if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
+            const int fs = ((L.dim + 2) + 1) & ~1;
+            if (!h->face_flux_e.ptr) {
+                h->face_flux_e.alloc((size_t)fs * L.F);
+                if (L.ns) h->face_flux_s.alloc((size_t)L.ns * L.F);
+            }
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->face_flux_kernel, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, h->gradC.ptr,
+                                   h->gradA.ptr, h->face_flux_e.ptr, h->face_flux_s.ptr);
+            });
+            h->timed(ABL_FV_PHASE_GATHER, st, [&] {
+                hipLaunchKernelGGL(h->gather_kernel, grid, block, 0, st, v, L.ns, h->face_flux_e.ptr, h->face_flux_s.ptr,
+                                   R, accumulate);
+            });
+        } else {
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->flux_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R,
+                                   accumulate);
+            });
+        }
+        }
         if (!accumulate && L.N > L.NI)
             hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, R, L.N, L.NI, L.nc);
         HIP_CHECK(hipGetLastError());
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_rhs: ") + e.what());
     }
+    return 0;
+}
+
+extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
+    if (!h) return fail(h, "abl_fv_set_strategy: null handle");
+    if (strategy != ABL_FV_STRATEGY_RECOMPUTE && strategy != ABL_FV_STRATEGY_FACE_FLUX)
+        return fail(h, "abl_fv_set_strategy: unknown strategy");
+    h->strategy = strategy;
     return 0;
 }
 

@@ -247,6 +247,10 @@ class FiniteVolumeSolver:
         )
         return theta.reshape(16, 3)
 
+    def set_strategy(self, strategy: str) -> None:
+        """"recompute" (each face flux evaluated by both cells) or "face_flux" (stored once, gathered per cell)."""
+        _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]), self._h, "abl_fv_set_strategy")
+
     def set_profiling(self, enable: bool) -> None:
         _lib.check(_lib.lib().abl_fv_set_profiling(self._h, int(bool(enable))), self._h, "abl_fv_set_profiling")
 

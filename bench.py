@@ -35,7 +35,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (default: the workload's 256)")
+    ap.add_argument("--strategy", default="recompute", choices=["recompute", "face_flux"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--cpu-n", type=int, default=48, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget")
     return ap.parse_args()
@@ -81,6 +83,54 @@ def cpu_baseline(n, seconds, mu, k, mach):
     }
 
 
+def per_field_rel(a, b):
+    return [float(x) for x in ((a - b).norm(dim=1) / b.norm(dim=1)).tolist()]
+
+
+def mixed_leg(fv, synthetic, mesh, mu, k, R64, U, mach, args, fp64_ms):
+    """Mixed form on the same mesh: timed like the FP64 leg; per-field error against the FP64 GPU residual at
+    Mach 0.3 and at the low-Mach variant (Mach 0.01)."""
+    import torch
+
+    solver = fv.FiniteVolumeSolver(mesh, ns_processes(fv, mu, k),
+                                   {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"},
+                                   form="mixed")
+    Rm = torch.zeros_like(U)
+    for _ in range(args.warmup):
+        solver.compute_rhs(0.0, U, Rm)
+    torch.cuda.synchronize()
+    solver.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.compute_rhs(0.0, U, Rm)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    solver.set_profiling(False)
+    phases = solver.phase_times(reset=True)
+    err = per_field_rel(Rm, R64)
+    # low-Mach variant: FP64 reference and mixed on the Mach 0.01 state
+    U2 = torch.tensor(synthetic.taylor_green(mesh, 0.01), dtype=torch.float64, device="cuda")
+    ref = fv.FiniteVolumeSolver(mesh, ns_processes(fv, mu, k),
+                                {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"})
+    R2d, R2m = torch.zeros_like(U2), torch.zeros_like(U2)
+    ref.compute_rhs(0.0, U2, R2d)
+    solver.compute_rhs(0.0, U2, R2m)
+    torch.cuda.synchronize()
+    err_low = per_field_rel(R2m, R2d)
+    del ref
+    cells = mesh.n_interior
+    return {
+        "value": round(cells / (ms * 1e-3) / 1e6, 3),
+        "unit": "Mcell/s",
+        "ms_per_step": round(ms, 4),
+        "speedup_vs_fp64": round(fp64_ms / ms, 3),
+        "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in phases.items() if c},
+        "rel_err_vs_fp64_mach0.3": [float(f"{e:.3e}") for e in err],
+        "rel_err_vs_fp64_mach0.01": [float(f"{e:.3e}") for e in err_low],
+        "error_norm": "per conserved field ||R_mixed - R_fp64|| / ||R_fp64|| over all cells (rho, rhoE, rhoU, rhoV, rhoW)",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -102,6 +152,7 @@ def main():
     solver = fv.FiniteVolumeSolver(mesh, ns_processes(fv, mu, k),
                                    {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"},
                                    form=cfg["form"])
+    solver.set_strategy(args.strategy)
     U = torch.tensor(synthetic.taylor_green(mesh, cfg["mach"]), dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     info = solver.info()
@@ -152,6 +203,7 @@ def main():
             "faces": info.n_faces,
             "mesh": f"{n}^3 periodic hex, brick-ordered",
             "form": cfg["form"],
+            "strategy": args.strategy,
             "physics": "NavierStokesTransport(AUSM) + Constant(mu=1.8e-5, Pr=0.72), LSQ + Barth-Jespersen, Mach "
                        f"{cfg['mach']}",
         },
@@ -171,6 +223,8 @@ def main():
         "device_bytes": info.device_bytes,
         "finite": finite,
     }
+    if cfg["form"] == "fp64" and not args.no_mixed:
+        out["mixed"] = mixed_leg(fv, synthetic, mesh, mu, k, R, U, cfg["mach"], args, wall_ms)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds, mu, k, cfg["mach"])
     print(json.dumps(out), flush=True)

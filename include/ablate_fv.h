@@ -179,6 +179,13 @@ int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_st
  * (mirrors the verboseSourceCheck scan, timeStepper.cpp:385-389). */
 int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_bad, abl_fv_stream stream);
 
+/* ---- evaluation strategy (both deterministic, no float atomics) ----------------------------------------------- */
+enum {
+    ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice */
+    ABL_FV_STRATEGY_FACE_FLUX = 2  /* one lane per face stores its flux once; a second pass gathers per cell */
+};
+int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);
+
 /* ---- per-phase timing (replaces the Loggable / PetscLogEvent brackets around each RHS sub-phase,
  *      finiteVolumeSolver.cpp:239-288, src/utilities/loggable.hpp:26-48) ------------------------------------- */
 enum {
@@ -187,7 +194,8 @@ enum {
     ABL_FV_PHASE_SPECIES_GRADIENTS = 2,
     ABL_FV_PHASE_FLUX = 3,
     ABL_FV_PHASE_VJP = 4,
-    ABL_FV_N_PHASES = 5
+    ABL_FV_PHASE_GATHER = 5,
+    ABL_FV_N_PHASES = 6
 };
 /* When enabled, every kernel of abl_fv_rhs / abl_fv_rhs_vjp is bracketed by events on its stream. */
 int abl_fv_set_profiling(abl_fv_handle h, int32_t enable);

@@ -11,13 +11,15 @@ TOL_FP64 = 1e-12
 CASES = standard_cases()
 
 
+@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
 @pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
-def test_rhs_matches_oracle(case, torch_cuda):
+def test_rhs_matches_oracle(case, strategy, torch_cuda):
     torch = torch_cuda
-    print(f"[parity] {case.name} cells={case.mesh.n_interior} faces={case.mesh.n_faces}", flush=True)
+    print(f"[parity] {case.name} {strategy} cells={case.mesh.n_interior} faces={case.mesh.n_faces}", flush=True)
     U0 = case.initial_state()
     Rref, Ufilled = case.oracle_rhs(U0.copy())
     solver = case.solver()
+    solver.set_strategy(strategy)
     U = torch.tensor(U0, dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -34,9 +36,11 @@ def test_rhs_matches_oracle(case, torch_cuda):
 
 @pytest.mark.parametrize("case", [c for c in CASES if c.name in ("c2_ns_lsq_bj", "c5_tets_rusanov", "c4_species_inflow")],
                          ids=lambda c: c.name)
-def test_rhs_deterministic_and_accumulates(case, torch_cuda):
+@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
+def test_rhs_deterministic_and_accumulates(case, strategy, torch_cuda):
     torch = torch_cuda
     solver = case.solver()
+    solver.set_strategy(strategy)
     U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
     R1 = torch.zeros_like(U)
     R2 = torch.zeros_like(U)
@@ -52,11 +56,13 @@ def test_rhs_deterministic_and_accumulates(case, torch_cuda):
     assert torch.equal(R3[:, case.mesh.n_interior :], base[:, case.mesh.n_interior :])
 
 
-def test_conservation_periodic(torch_cuda):
+@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
+def test_conservation_periodic(strategy, torch_cuda):
     """sum_c V_c R_c = 0 on a periodic mesh (every face flux enters two cells with opposite signs)."""
     torch = torch_cuda
     case = next(c for c in CASES if c.name == "c2_ns_lsq_bj")
     solver = case.solver()
+    solver.set_strategy(strategy)
     U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -64,6 +70,23 @@ def test_conservation_periodic(torch_cuda):
     tot = (R * vol).sum(dim=1)
     scale = (R.abs() * vol).sum(dim=1)
     assert torch.all(tot.abs() <= 1e-13 * scale), (tot, scale)
+
+
+def test_strategies_agree_bitwise_per_face(torch_cuda):
+    """Both strategies evaluate every face flux with the same arithmetic; the per-cell sums may differ only in
+    summation association, so they agree to rounding."""
+    torch = torch_cuda
+    case = next(c for c in CASES if c.name == "c5_tets_rusanov")
+    solver = case.solver()
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Ra, Rb = torch.zeros_like(U), torch.zeros_like(U)
+    solver.set_strategy("recompute")
+    solver.compute_rhs(0.0, U, Ra)
+    solver.set_strategy("face_flux")
+    solver.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    scale = Ra.abs().max(dim=1, keepdim=True).values
+    assert torch.all((Ra - Rb).abs() <= 1e-14 * scale)
 
 
 def test_pack_unpack_roundtrip(torch_cuda):
