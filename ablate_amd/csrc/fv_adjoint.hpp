@@ -207,12 +207,13 @@ __device__ inline void rusanov_flux_vjp(const double* uL, const double* uR, doub
     lam_b -= 0.5 * area * QdL;  // QdL = sum_sp fb_sp (rhoY_R - rhoY_L)
     coefL = area * (0.5 * L.un + 0.5 * lam);
     coefR = area * (0.5 * R.un - 0.5 * lam);
+    // d|u_n|/du_n = sign(u_n) with sign(0) = 0 (the central-difference value at the kink)
     double aL_b = 0.0, aR_b = 0.0;
     if (left_max) {
-        unL_b += lam_b * (L.un < 0 ? -1.0 : 1.0);
+        unL_b += lam_b * (double)((L.un > 0) - (L.un < 0));
         aL_b += lam_b;
     } else {
-        unR_b += lam_b * (R.un < 0 ? -1.0 : 1.0);
+        unR_b += lam_b * (double)((R.un > 0) - (R.un < 0));
         aR_b += lam_b;
     }
     double velL_b[DIM], velR_b[DIM];

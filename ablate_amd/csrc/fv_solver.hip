@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "fv_adjoint.hpp"
 #include "fv_device.hpp"
 #include "fv_internal.hpp"
 #include "fv_layout.hpp"
@@ -897,6 +898,164 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
     }
 }
 
+// ============================================================================================ adjoint
+//
+// Vector-Jacobian product of the first-order (no reconstruction) inviscid RHS: Ubar = Rbar^T dR/dU and
+// theta_bar = Rbar^T dR/dtheta for inflow ghosts, theta = (rho_in, |u_in|, T_in).  R_c = (1/V_c) sum_f sign F_f, so
+// every face contributes dF/dU_side^T g_f with g_f = Rbar_R / V_R - Rbar_L / V_L.  One lane per interior cell
+// recomputes the forward flux of each of its faces and keeps its own side's adjoint (deterministic gather).
+
+// Adjoint seeds of face f: g[p] for euler components and the species sums QL = sum g_sp rhoY_L,
+// QR = sum g_sp rhoY_R, QdL = sum g_sp (rhoY_R - rhoY_L).
+template <int DIM>
+__device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U,
+                                 const double* __restrict__ Rb, int64_t L, int64_t Rc, bool ghost, double* fb,
+                                 double& QL, double& QR, double& QdL) {
+    constexpr int NE = DIM + 2;
+    const int64_t N = m.N;
+    const double ivL = m.inv_vol[L], ivR = ghost ? 0.0 : m.inv_vol[Rc];
+#pragma unroll
+    for (int p = 0; p < NE; p++) fb[p] = (ghost ? 0.0 : Rb[p * N + Rc] * ivR) - Rb[p * N + L] * ivL;
+    QL = 0.0;
+    QR = 0.0;
+    QdL = 0.0;
+    for (int sp = 0; sp < ph.ns; sp++) {
+        const int64_t comp = NE + sp;
+        const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+        const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+        QL += g * yl;
+        QR += g * yr;
+        QdL += g * (yr - yl);
+    }
+}
+
+template <int DIM, bool RUS>
+__device__ inline void face_vjp(const double* uL, const double* uR, double QL, double QR, double QdL, const double* nh,
+                                double area, const Phys& ph, const double* fb, double* ubL, double* ubR, double& cL,
+                                double& cR) {
+    if (RUS)
+        rusanov_flux_vjp<DIM>(uL, uR, QL, QR, QdL, nh, area, ph, fb, ubL, ubR, cL, cR);
+    else
+        ausm_flux_vjp<DIM>(uL, uR, QL, QR, nh, area, ph, fb, ubL, ubR, cL, cR);
+}
+
+template <int DIM, bool RUS, int KM>
+__global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                const double* __restrict__ Rb, const int32_t* __restrict__ ghost_slot,
+                                                const int32_t* __restrict__ slot_kind, double* __restrict__ Ub) {
+    constexpr int NE = DIM + 2;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    double ub[NE], coef[KM];
+#pragma unroll
+    for (int p = 0; p < NE; p++) ub[p] = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const Slot sl = face_slot(m, k, c);
+        const int64_t f = sl.f;
+        const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+        const bool ghost = Rc >= NI;
+        double nh[3], area, uL[NE], uR[NE], fb[NE], QL, QR, QdL;
+        load_face_na(m.fna, f, nh, area);
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            uL[p] = U[p * N + L];
+            uR[p] = U[p * N + Rc];
+        }
+        face_seed<DIM>(m, ph, U, Rb, L, Rc, ghost, fb, QL, QR, QdL);
+        double ubL[NE], ubR[NE], cL, cR;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            ubL[p] = 0.0;
+            ubR[p] = 0.0;
+        }
+        face_vjp<DIM, RUS>(uL, uR, QL, QR, QdL, nh, area, ph, fb, ubL, ubR, cL, cR);
+        coef[k] = 0.0;
+        if (!sl.valid) continue;
+        double cf = sl.s == 0 ? cL : cR;
+#pragma unroll
+        for (int p = 0; p < NE; p++) ub[p] += sl.s == 0 ? ubL[p] : ubR[p];
+        // an outflow ghost copies this cell's state: its adjoint flows back here
+        if (ghost && slot_kind[ghost_slot[Rc - NI]] == ABL_FV_BC_OUTFLOW) {
+#pragma unroll
+            for (int p = 0; p < NE; p++) ub[p] += ubR[p];
+            cf += cR;
+        }
+        coef[k] = cf;
+    }
+#pragma unroll
+    for (int p = 0; p < NE; p++) Ub[p * N + c] = ub[p];
+    for (int sp = 0; sp < ph.ns; sp++) {
+        const int64_t comp = NE + sp;
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const Slot sl = face_slot(m, k, c);
+            const int64_t L = m.fcell[sl.f], Rc = m.fcell[F + sl.f];
+            const bool ghost = Rc >= NI;
+            const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * m.inv_vol[Rc]) - Rb[comp * N + L] * m.inv_vol[L];
+            if (sl.valid) a += coef[k] * g;
+        }
+        Ub[comp * N + c] = a;
+    }
+}
+
+// theta_bar contributions of each inflow boundary face: dU_ghost/dtheta^T (dF/dU_ghost)^T g_f.
+// slot_param[slot] = {rho_in, |u_in|, T_in, dir_x, dir_y, dir_z}; slot_Y[slot][ns] = inflow mass fractions.
+template <int DIM, bool RUS>
+__global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ Rb, int64_t nb,
+                                                      const int32_t* __restrict__ bface, const int32_t* __restrict__ bslot,
+                                                      const int32_t* __restrict__ slot_kind,
+                                                      const double* __restrict__ slot_param,
+                                                      const double* __restrict__ slot_Y, double* __restrict__ contrib) {
+    constexpr int NE = DIM + 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int64_t f = bface[i];
+    const int slot = bslot[i];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (slot_kind[slot] == ABL_FV_BC_INFLOW) {
+        const int64_t N = m.N, F = m.F;
+        const int64_t L = m.fcell[f], G = m.fcell[F + f];
+        double nh[3], area, uL[NE], uG[NE], fb[NE], QL, QR, QdL;
+        load_face_na(m.fna, f, nh, area);
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            uL[p] = U[p * N + L];
+            uG[p] = U[p * N + G];
+        }
+        face_seed<DIM>(m, ph, U, Rb, L, G, true, fb, QL, QR, QdL);
+        double ubL[NE], ubG[NE], cL, cR;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            ubL[p] = 0.0;
+            ubG[p] = 0.0;
+        }
+        face_vjp<DIM, RUS>(uL, uG, QL, QR, QdL, nh, area, ph, fb, ubL, ubG, cL, cR);
+        const double* sp_ = slot_param + 6 * slot;
+        const double rho = sp_[0], um = sp_[1], T = sp_[2];
+        // U_ghost = [rho, rho (cv T + u^2/2), rho u dir, rho Y]
+        t0 = ubG[kRho] + ubG[kRhoE] * (ph.cv * T + 0.5 * um * um);
+        t1 = ubG[kRhoE] * rho * um;
+        t2 = ubG[kRhoE] * rho * ph.cv;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            t0 += ubG[kRhoU + d] * um * sp_[3 + d];
+            t1 += ubG[kRhoU + d] * rho * sp_[3 + d];
+        }
+        for (int sp = 0; sp < ph.ns; sp++) {
+            const int64_t comp = NE + sp;
+            const double g = -Rb[comp * N + L] * m.inv_vol[L];
+            t0 += cR * g * slot_Y[slot * ph.ns + sp];
+        }
+    }
+    contrib[3 * i + 0] = t0;
+    contrib[3 * i + 1] = t1;
+    contrib[3 * i + 2] = t2;
+}
+
 // ---- compile-time variant selection
 template <typename G>
 using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*);
@@ -951,6 +1110,22 @@ template <int DIM>
 MixedFluxKernel pick_flux_mixed(bool recon, bool visc) {
     if (recon) return visc ? k_flux_mixed<DIM, true, true> : k_flux_mixed<DIM, true, false>;
     return visc ? k_flux_mixed<DIM, false, true> : k_flux_mixed<DIM, false, false>;
+}
+using VjpKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const int32_t*, const int32_t*,
+                          double*);
+using VjpThetaKernel = void (*)(MeshView<double>, Phys, const double*, const double*, int64_t, const int32_t*,
+                               const int32_t*, const int32_t*, const double*, const double*, double*);
+template <int DIM>
+VjpKernel pick_vjp(bool rusanov, int K) {
+    switch (slot_width(K)) {
+        case 4: return rusanov ? k_vjp<DIM, true, 4> : k_vjp<DIM, false, 4>;
+        case 6: return rusanov ? k_vjp<DIM, true, 6> : k_vjp<DIM, false, 6>;
+        default: return rusanov ? k_vjp<DIM, true, 8> : k_vjp<DIM, false, 8>;
+    }
+}
+template <int DIM>
+VjpThetaKernel pick_vjp_theta(bool rusanov) {
+    return rusanov ? k_vjp_theta<DIM, true> : k_vjp_theta<DIM, false>;
 }
 template <int DIM>
 GatherKernel pick_gather(bool species, int K) {
@@ -1042,6 +1217,12 @@ struct abl_fv_solver {
     abl::DevBuf<float> fna_f, fdx_f, fw_f, fwa_f, gradC_f, gradA_f;
     abl::GradKernelT<float> grad_kernel_f = nullptr;
     abl::MixedFluxKernel flux_kernel_mixed = nullptr;
+    // adjoint
+    abl::DevBuf<int32_t> ghost_slot;
+    abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
+    std::vector<double> h_slot_param, h_slot_Y;
+    abl::VjpKernel vjp_kernel = nullptr;
+    abl::VjpThetaKernel vjp_theta_kernel = nullptr;
     abl::GradKernel grad_kernel = nullptr;
     abl::FluxKernel flux_kernel = nullptr;
     abl::FaceFluxKernel face_flux_kernel = nullptr;
@@ -1251,6 +1432,25 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             bs.push_back(b.slot);
         }
         s->n_bfaces = (int64_t)bf.size();
+        {
+            std::vector<int32_t> gs(std::max<int64_t>(L.N - L.NI, 1), 0);
+            for (auto& b : L.bfaces) gs[L.fcell[L.F + b.face] - L.NI] = b.slot;
+            s->ghost_slot.upload(gs);
+        }
+        s->h_slot_param.assign((size_t)abl::kMaxBcSlots * 6, 0.0);
+        s->h_slot_Y.assign((size_t)abl::kMaxBcSlots * std::max(L.ns, 1), 0.0);
+        s->slot_param.upload(s->h_slot_param);
+        s->slot_Y.upload(s->h_slot_Y);
+        if (form == ABL_FV_FORM_FP64) {
+            const bool rus = p.flux_calculator == ABL_FV_FLUX_RUSANOV;
+            if (mesh->dim == 3) {
+                s->vjp_kernel = abl::pick_vjp<3>(rus, L.K);
+                s->vjp_theta_kernel = abl::pick_vjp_theta<3>(rus);
+            } else {
+                s->vjp_kernel = abl::pick_vjp<2>(rus, L.K);
+                s->vjp_theta_kernel = abl::pick_vjp_theta<2>(rus);
+            }
+        }
         s->bface.upload(bf);
         s->bslot.upload(bs);
         s->h_slot_kind.assign(abl::kMaxBcSlots, ABL_FV_BC_NONE);
@@ -1319,7 +1519,23 @@ extern "C" int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_b
         for (int sp = 0; sp < L.ns; sp++) st[dim + 2 + sp] = bc->rho * (bc->Yi ? bc->Yi[sp] : 0.0);
     }
     h->h_slot_kind[slot] = bc->kind;
+    {
+        double* sp = &h->h_slot_param[(size_t)slot * 6];
+        for (int i = 0; i < 6; i++) sp[i] = 0.0;
+        if (bc->kind == ABL_FV_BC_INFLOW) {
+            double um = 0.0;
+            for (int d = 0; d < dim; d++) um += bc->vel[d] * bc->vel[d];
+            um = std::sqrt(um);
+            sp[0] = bc->rho;
+            sp[1] = um;
+            sp[2] = bc->T;
+            for (int d = 0; d < dim; d++) sp[3 + d] = um > 0.0 ? bc->vel[d] / um : 0.0;
+            for (int q = 0; q < L.ns; q++) h->h_slot_Y[(size_t)slot * L.ns + q] = bc->Yi ? bc->Yi[q] : 0.0;
+        }
+    }
     try {
+        HIP_CHECK(hipMemcpy(h->slot_param.ptr, h->h_slot_param.data(), h->slot_param.bytes(), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->slot_Y.ptr, h->h_slot_Y.data(), h->slot_Y.bytes(), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(h->slot_kind.ptr, h->h_slot_kind.data(), h->slot_kind.bytes(), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(h->slot_state.ptr, h->h_slot_state.data(), h->slot_state.bytes(), hipMemcpyHostToDevice));
     } catch (const std::exception& e) {
@@ -1430,8 +1646,62 @@ extern "C" int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* laun
     return 0;
 }
 
-extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double, double*, const double*, double*, double*, abl_fv_stream) {
-    return fail(h, "abl_fv_rhs_vjp: not available in this build");
+extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
+                              double* theta_bar, abl_fv_stream stream) {
+    (void)time;
+    if (!h || !U || !Rbar || !Ubar) return fail(h, "abl_fv_rhs_vjp: null argument");
+    const abl::Layout& L = h->host;
+    const int fl = h->phys.flux_calculator;
+    if (h->form != ABL_FV_FORM_FP64 || h->recon || h->visc ||
+        (fl != ABL_FV_FLUX_AUSM && fl != ABL_FV_FLUX_RUSANOV && fl != ABL_FV_FLUX_OFF && fl != ABL_FV_FLUX_NONE))
+        return fail(h, "abl_fv_rhs_vjp: supported for the FP64 form, first-order reconstruction, no diffusion and the "
+                       "Ausm / Rusanov / Off flux calculators");
+    try {
+        hipStream_t st = (hipStream_t)stream;
+        if (h->n_bfaces) {
+            h->timed(ABL_FV_PHASE_GHOST, st, [&] {
+                hipLaunchKernelGGL(abl::k_ghost_fill, dim3(grid_for(h->n_bfaces)), dim3(abl::kBlock), 0, st, h->n_bfaces,
+                                   h->bface.ptr, h->bslot.ptr, h->fcell.ptr, L.F, L.N, L.nc, h->slot_kind.ptr,
+                                   h->slot_state.ptr, U);
+            });
+        }
+        const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
+        const bool noflux = fl == ABL_FV_FLUX_OFF || fl == ABL_FV_FLUX_NONE;
+        if (noflux) {
+            HIP_CHECK(hipMemsetAsync(Ubar, 0, sizeof(double) * (size_t)L.nc * L.N, st));
+        } else {
+            const auto v = h->view();
+            h->timed(ABL_FV_PHASE_VJP, st, [&] {
+                hipLaunchKernelGGL(h->vjp_kernel, grid, block, 0, st, v, h->ph, U, Rbar, h->ghost_slot.ptr,
+                                   h->slot_kind.ptr, Ubar);
+            });
+            if (L.N > L.NI)
+                hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
+                                   L.nc);
+        }
+        HIP_CHECK(hipGetLastError());
+        if (theta_bar && h->n_bfaces && !noflux) {
+            if (!h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
+            const auto v = h->view();
+            hipLaunchKernelGGL(h->vjp_theta_kernel, dim3(grid_for(h->n_bfaces)), block, 0, st, v, h->ph, U, Rbar,
+                               h->n_bfaces, h->bface.ptr, h->bslot.ptr, h->slot_kind.ptr, h->slot_param.ptr,
+                               h->slot_Y.ptr, h->theta_contrib.ptr);
+            HIP_CHECK(hipGetLastError());
+            std::vector<double> contrib((size_t)3 * h->n_bfaces);
+            std::vector<int32_t> slots(h->n_bfaces);
+            HIP_CHECK(hipMemcpyAsync(contrib.data(), h->theta_contrib.ptr, contrib.size() * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                     st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            // fixed-order host reduction per slot (deterministic)
+            for (int64_t i = 0; i < h->n_bfaces; i++)
+                for (int j = 0; j < 3; j++) theta_bar[3 * slots[i] + j] += contrib[3 * i + j];
+        }
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
+    }
+    return 0;
 }
 
 extern "C" int abl_fv_pack_aos(abl_fv_handle h, const double* aos, double* soa, abl_fv_stream stream) {

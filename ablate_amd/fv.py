@@ -169,7 +169,10 @@ class FiniteVolumeSolver:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            _lib.lib().abl_fv_destroy(self._h)
+            try:
+                _lib.lib().abl_fv_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @property

@@ -61,7 +61,10 @@ class _NativeMesh:
 
     def __del__(self):
         if getattr(self, "_handle", None):
-            _lib.lib().abl_fv_mesh_free(self._handle)
+            try:
+                _lib.lib().abl_fv_mesh_free(self._handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
             self._handle = None
 
     # convenience accessors

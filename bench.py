@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--strategy", default="recompute", choices=["recompute", "face_flux"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
+    ap.add_argument("--adjoint-n", type=int, default=128, help="cells per direction of the adjoint workload")
     ap.add_argument("--cpu-n", type=int, default=48, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget")
     return ap.parse_args()
@@ -128,6 +130,74 @@ def mixed_leg(fv, synthetic, mesh, mu, k, R64, U, mach, args, fp64_ms):
         "rel_err_vs_fp64_mach0.3": [float(f"{e:.3e}") for e in err],
         "rel_err_vs_fp64_mach0.01": [float(f"{e:.3e}") for e in err_low],
         "error_norm": "per conserved field ||R_mixed - R_fp64|| / ||R_fp64|| over all cells (rho, rhoE, rhoU, rhoV, rhoW)",
+    }
+
+
+def adjoint_leg(fv, synthetic, args):
+    """C4: VJP of the first-order FP64 RHS on a 128^3 hex mesh, Euler + 10 species, inflow (rho 1.2, u 100 m/s,
+    T 300 K) on x-, outflow on x+, periodic y/z; base flow the Taylor-Green field at Mach 0.3 (phase-shifted off
+    its symmetry planes) with Dirichlet(1) mass fractions; seed Rbar ~ N(0, 1).  Timed like the forward legs, plus
+    a full-size directional-derivative check: Rbar.(R(U+hV) - R(U-hV))/2h of the GPU FP64 forward, h = 1e-6."""
+    import torch
+
+    from ablate_amd.mesh import BoxMesh
+
+    n, ns = args.adjoint_n, 10
+    mesh = BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [False, True, True], ordering="brick")
+    species = [f"Y{i}" for i in range(ns)]
+    eos = fv.PerfectGas({"gamma": 1.4, "Rgas": 287.0}, species)
+    procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, fv.Ausm()), fv.SpeciesTransport(eos, fv.Ausm())]
+    bcs = [fv.InflowGhost([0], 1.2, [100.0, 0.0, 0.0], 300.0, [1.0 / ns] * ns), fv.OutflowGhost([1])]
+    solver = fv.FiniteVolumeSolver(mesh, procs, {"petscfv_type": "upwind"}, bcs)
+    U0 = synthetic.taylor_green(mesh, 0.3, n_species=ns, phase=(0.37, 0.21, 0.53))
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior), device="cuda")
+    Ub = torch.empty_like(U)
+    for _ in range(args.warmup):
+        solver.rhs_vjp(0.0, U, Rb, Ub)
+    torch.cuda.synchronize()
+    solver.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.rhs_vjp(0.0, U, Rb, Ub)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    solver.set_profiling(False)
+    phases = solver.phase_times(reset=True)
+    theta_bar = solver.rhs_vjp(0.0, U, Rb, Ub)[0]
+    # forward timing on the same workload (for the adjoint / forward cost ratio)
+    R = torch.zeros_like(U)
+    solver.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # directional derivative check at full size
+    g = torch.Generator(device="cuda").manual_seed(5)
+    NI = mesh.n_interior
+    V = torch.zeros_like(U)
+    scale = U[:, :NI].abs().mean(dim=1, keepdim=True)
+    V[:, :NI] = torch.randn(U.shape[0], NI, dtype=torch.float64, device="cuda", generator=g) * scale
+    h = 1e-6
+    Rp, Rm = torch.zeros_like(U), torch.zeros_like(U)
+    solver.compute_rhs(0.0, U + h * V, Rp)
+    solver.compute_rhs(0.0, U - h * V, Rm)
+    fd = float((Rb * (Rp - Rm)).sum().item() / (2 * h))
+    ad = float((Ub * V).sum().item())
+    cells = NI
+    return {
+        "value": round(cells / (ms * 1e-3) / 1e6, 3),
+        "unit": "Mcell/s",
+        "ms_per_step": round(ms, 4),
+        "forward_ms_per_step": round(fwd_ms, 4),
+        "adjoint_over_forward": round(ms / fwd_ms, 3),
+        "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in phases.items() if c},
+        "workload": f"vjp_fp64_{n}cube_euler_10species_inflow_ausm_first_order",
+        "cells": cells,
+        "directional_check": {"fd_central_h1e-6": fd, "vjp": ad, "rel_diff": abs(fd - ad) / abs(fd)},
+        "theta_bar_inflow": [float(x) for x in theta_bar],
     }
 
 
@@ -225,6 +295,9 @@ def main():
     }
     if cfg["form"] == "fp64" and not args.no_mixed:
         out["mixed"] = mixed_leg(fv, synthetic, mesh, mu, k, R, U, cfg["mach"], args, wall_ms)
+    if cfg["form"] == "fp64" and not args.no_adjoint:
+        del solver
+        out["adjoint"] = adjoint_leg(fv, synthetic, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds, mu, k, cfg["mach"])
     print(json.dumps(out), flush=True)

@@ -27,6 +27,7 @@ class Case:
     gamma: float = 1.4
     Rgas: float = 287.0
     max_lim_grad: float = 1e10
+    phase: tuple = (0.0, 0.0, 0.0)
 
     def processes(self):
         species = [f"S{i}" for i in range(self.n_species)]
@@ -63,7 +64,8 @@ class Case:
                 U = np.concatenate([U, np.zeros((self.n_species, U.shape[1]))])
                 U[4:, : self.mesh.n_interior] = U[0, : self.mesh.n_interior] * Y
             return U
-        return synthetic.taylor_green(self.mesh, self.mach, self.gamma, self.Rgas, seed=seed, n_species=self.n_species)
+        return synthetic.taylor_green(self.mesh, self.mach, self.gamma, self.Rgas, seed=seed, n_species=self.n_species,
+                                      phase=self.phase)
 
     def oracle_physics(self):
         return orc.physics(self.gamma, self.Rgas, self.n_species, self.flux, self.recon, self.limiter,

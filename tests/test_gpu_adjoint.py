@@ -1,0 +1,136 @@
+"""Hand-written adjoint (abl_fv_rhs_vjp) of the first-order RHS against (1) torch.autograd over an independent
+PyTorch FP64 restatement (tests/torch_rhs.py, itself pinned to the oracle) and (2) central finite differences
+(h = 1e-6) of the CPU oracle, including the inflow parameters theta = (rho_in, |u_in|, T_in)."""
+import numpy as np
+import pytest
+import torch as _torch
+
+from ablate_amd import autograd as fv_autograd
+from ablate_amd import synthetic
+from ablate_amd.mesh import BoxMesh, TetBoxMesh
+from tests.fvcases import Case, all_outflow, inflow_x
+from tests.test_torch_restatement import PHASE, thetas_of, torch_model
+
+pytestmark = pytest.mark.gpu
+
+
+def adjoint_cases():
+    chan = lambda n: BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [False, True, True])
+    return [
+        Case("c4_ausm_species_inflow", chan(8), n_species=10, bcs=inflow_x(10), phase=PHASE),
+        Case("rusanov_species_inflow", chan(6), flux="rusanov", n_species=3, bcs=inflow_x(3), phase=PHASE),
+        Case("periodic_ausm", BoxMesh([6, 6, 6], [0, 0, 0], [1, 1, 1], [True] * 3), phase=PHASE),
+        Case("quad2d_outflow", BoxMesh([9, 7], [0, 0], [1, 1], [False, True]), state="random2d",
+             bcs=all_outflow(2)),
+        Case("tets_rusanov_outflow", TetBoxMesh(3, 0.2), flux="rusanov", n_species=2, bcs=all_outflow(6),
+             phase=PHASE),
+    ]
+
+
+CASES = adjoint_cases()
+
+
+def _seed(case, U0):
+    return synthetic.adjoint_seed(U0.shape[0], case.mesh.n_cells, case.mesh.n_interior)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c.name)
+def test_vjp_matches_autograd_of_restatement(case, torch_cuda):
+    torch = torch_cuda
+    print(f"[adjoint] {case.name}", flush=True)
+    U0 = case.initial_state()
+    Rb = _seed(case, U0)
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ub = torch.empty_like(U)
+    theta_bar = solver.rhs_vjp(0.0, U, torch.tensor(Rb, device="cuda"), Ub)
+    torch.cuda.synchronize()
+    Ug = Ub.cpu().numpy()
+    # reference gradient: autograd through the PyTorch restatement
+    Ut = _torch.tensor(U0, requires_grad=True)
+    thetas = {s: t.clone().requires_grad_(True) for s, t in thetas_of(case).items()}
+    out = torch_model(case)(Ut, thetas)
+    loss = (out * _torch.tensor(Rb)).sum()
+    grads = _torch.autograd.grad(loss, [Ut] + list(thetas.values()))
+    ref = grads[0].numpy()
+    NI = case.mesh.n_interior
+    err = np.linalg.norm(Ug[:, :NI] - ref[:, :NI]) / np.linalg.norm(ref[:, :NI])
+    assert err <= 1e-10, err
+    assert not np.any(Ug[:, NI:])
+    for (slot, t), g in zip(thetas.items(), grads[1:]):
+        np.testing.assert_allclose(theta_bar[slot], g.numpy(), rtol=1e-10, atol=1e-12 * np.abs(g.numpy()).max())
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=lambda c: c.name)
+def test_vjp_matches_oracle_central_differences(case, torch_cuda):
+    """Rbar . (R(U + hV) - R(U - hV)) / 2h of the oracle vs Ubar . V, and the same for each inflow parameter."""
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    U0 = case.initial_state()
+    NI = case.mesh.n_interior
+    Rb = _seed(case, U0)
+    V = np.zeros_like(U0)
+    V[:, :NI] = rng.normal(size=(U0.shape[0], NI)) * np.abs(U0[:, :NI]).mean(axis=1, keepdims=True)
+    h = 1e-6
+    Rp, _ = case.oracle_rhs(U0 + h * V)
+    Rm, _ = case.oracle_rhs(U0 - h * V)
+    fd = np.sum(Rb * (Rp - Rm)) / (2 * h)
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ub = torch.empty_like(U)
+    theta_bar = solver.rhs_vjp(0.0, U, torch.tensor(Rb, device="cuda"), Ub)
+    ad = float((Ub.cpu().numpy() * V).sum())
+    assert ad == pytest.approx(fd, rel=1e-6)
+    # inflow parameters (slot 0): rho_in, |u_in|, T_in
+    b = case.bcs[0]
+    base = np.array([b["rho"], np.linalg.norm(b["vel"]), b["T"]])
+    d = np.asarray(b["vel"]) / np.linalg.norm(b["vel"])
+    for j in range(3):
+        th = base.copy()
+        hj = h * abs(base[j])
+        vals = []
+        for sgn in (+1, -1):
+            th[j] = base[j] + sgn * hj
+            bcs = dict(case.oracle_bcs())
+            bcs[0] = dict(bcs[0], rho=th[0], vel=list(th[1] * d), T=th[2])
+            from oracle import binding as orc
+            R, _ = orc.rhs(case.mesh.arrays, case.oracle_physics(), U0.copy(), bcs)
+            vals.append(np.sum(Rb * R))
+        fdj = (vals[0] - vals[1]) / (2 * hj)
+        assert theta_bar[0][j] == pytest.approx(fdj, rel=1e-6, abs=1e-9 * abs(fdj) + 1e-12)
+
+
+def test_autograd_function_wrapper(torch_cuda):
+    """torch.autograd through ablate_amd.autograd.rhs returns the hand-written VJP (state and inflow theta)."""
+    torch = torch_cuda
+    case = CASES[0]
+    U0 = case.initial_state()
+    Rb = torch.tensor(_seed(case, U0), device="cuda")
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda", requires_grad=True)
+    theta = torch.tensor([1.2, 100.0, 300.0], dtype=torch.float64, requires_grad=True)
+    R = fv_autograd.rhs(U, solver, theta, slot=0, direction=(1.0, 0.0, 0.0), Yi=case.bcs[0]["Yi"])
+    gU, gT = torch.autograd.grad((R * Rb).sum(), [U, theta])
+    Ub = torch.empty_like(U)
+    tb = solver.rhs_vjp(0.0, U.detach().clone(), Rb, Ub)
+    assert torch.equal(gU, Ub)
+    np.testing.assert_array_equal(gT.numpy(), tb[0])
+
+
+def test_vjp_deterministic_and_rejects_unsupported(torch_cuda):
+    torch = torch_cuda
+    case = CASES[0]
+    U0 = case.initial_state()
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Rb = torch.tensor(_seed(case, U0), device="cuda")
+    a, b = torch.empty_like(U), torch.empty_like(U)
+    solver.rhs_vjp(0.0, U, Rb, a)
+    solver.rhs_vjp(0.0, U, Rb, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    recon = Case("lsq", case.mesh, recon="leastsquares", n_species=10, bcs=case.bcs).solver()
+    from ablate_amd._lib import NativeError
+
+    with pytest.raises(NativeError, match="first-order"):
+        recon.rhs_vjp(0.0, U, Rb, a)
