@@ -1212,7 +1212,7 @@ struct abl_fv_solver {
     abl::Phys ph{};
     abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind;
     abl::DevBuf<double> fna, fdx, fw, fwa, inv_vol, gradC, gradA, slot_state;
-    abl::DevBuf<double> face_flux_e, face_flux_s;  // strategy 2 face-flux storage (allocated on first use)
+    abl::DevBuf<double> face_flux_e, face_flux_s;  // face-flux records (allocated when that strategy is selected)
     // mixed form: FP32 geometry and gradients
     abl::DevBuf<float> fna_f, fdx_f, fw_f, fwa_f, gradC_f, gradA_f;
     abl::GradKernelT<float> grad_kernel_f = nullptr;
@@ -1227,7 +1227,7 @@ struct abl_fv_solver {
     abl::FluxKernel flux_kernel = nullptr;
     abl::FaceFluxKernel face_flux_kernel = nullptr;
     abl::GatherKernel gather_kernel = nullptr;
-    int strategy = ABL_FV_STRATEGY_RECOMPUTE;
+    int strategy = ABL_FV_STRATEGY_FACE_FLUX;
     abl::DevBuf<unsigned long long> counter;
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
@@ -1308,6 +1308,13 @@ struct abl_fv_solver {
         v.fwa = fwa_f.ptr;
         v.inv_vol = inv_vol.ptr;
         return v;
+    }
+    // Face-flux records [F][FS] (FS = n_euler rounded up to even) and species fluxes [ns][F]; FP64 form only.
+    void ensure_face_flux() {
+        if (form != ABL_FV_FORM_FP64 || face_flux_e.ptr) return;
+        const int fs = ((host.dim + 2) + 1) & ~1;
+        face_flux_e.alloc((size_t)fs * host.F);
+        if (host.ns) face_flux_s.alloc((size_t)host.ns * host.F);
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
@@ -1459,6 +1466,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         s->slot_kind.upload(s->h_slot_kind);
         s->slot_state.upload(s->h_slot_state);
         s->counter.alloc(1);
+        if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
         L.fcell.clear(); L.fcell.shrink_to_fit();
@@ -1585,11 +1593,6 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                 });
         }
         if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
-            const int fs = ((L.dim + 2) + 1) & ~1;
-            if (!h->face_flux_e.ptr) {
-                h->face_flux_e.alloc((size_t)fs * L.F);
-                if (L.ns) h->face_flux_s.alloc((size_t)L.ns * L.F);
-            }
             h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                 hipLaunchKernelGGL(h->face_flux_kernel, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, h->gradC.ptr,
                                    h->gradA.ptr, h->face_flux_e.ptr, h->face_flux_s.ptr);
@@ -1618,6 +1621,11 @@ extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
     if (!h) return fail(h, "abl_fv_set_strategy: null handle");
     if (strategy != ABL_FV_STRATEGY_RECOMPUTE && strategy != ABL_FV_STRATEGY_FACE_FLUX)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
+    try {
+        if (strategy == ABL_FV_STRATEGY_FACE_FLUX) h->ensure_face_flux();
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_set_strategy: ") + e.what());
+    }
     h->strategy = strategy;
     return 0;
 }

@@ -35,14 +35,55 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (default: the workload's 256)")
-    ap.add_argument("--strategy", default="recompute", choices=["recompute", "face_flux"])
+    ap.add_argument("--strategy", default="face_flux", choices=["recompute", "face_flux"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
     ap.add_argument("--adjoint-n", type=int, default=128, help="cells per direction of the adjoint workload")
     ap.add_argument("--cpu-n", type=int, default=48, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget")
+    ap.add_argument("--copy-gib", type=float, default=2.0,
+                    help="size of the plain device copy timed before the RHS (HBM reference and counter calibration; "
+                         "0 = skip)")
     return ap.parse_args()
+
+
+def copy_probe(gib, reps=5):
+    """Times a plain device-to-device copy of `gib` GiB (torch copy_, one read + one write per byte) on the current
+    stream.  It is the achievable-bandwidth reference for the RHS and, under a hardware-counter profiler, the
+    calibration kernel for FETCH_SIZE / WRITE_SIZE (its algorithmic traffic is exactly `gib` read + `gib` written)."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * n * 8
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": n * 8, "bytes_written": n * 8, "ms": round(ms, 4),
+            "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+KERNELS = {"face_flux": ["k_gradients", "k_face_flux", "k_gather"], "recompute": ["k_gradients", "k_flux"]}
+
+# HBM bytes per RHS from hardware counters (FETCH_SIZE + WRITE_SIZE summed over the RHS kernels of one step, each
+# scaled by the copy-kernel calibration factor), measured with the commands recorded in
+# profiles/rhs_ns_fp64_256cube_traffic.md.  Keyed by (workload, cells per direction, strategy).
+TRAFFIC_BYTES = {}
+
+
+def measured_traffic(cfg, n, strategy):
+    return TRAFFIC_BYTES.get((cfg["workload"], n, strategy))
 
 
 def ns_processes(fv, mu, k):
@@ -214,6 +255,7 @@ def main():
     from ablate_amd import fv, synthetic
     from ablate_amd.mesh import BoxMesh
 
+    copy = copy_probe(args.copy_gib) if args.copy_gib > 0 else None
     mu = 1.8e-5
     cp = 1.4 * 287.0 / 0.4
     k = mu * cp / 0.72
@@ -283,13 +325,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "RHS kernels (k_gradients + k_flux), algorithmic bytes per cell "
-                      f"{info.alg_bytes_per_cell:.1f} B",
+            "traffic": measured_traffic(cfg, n, args.strategy),
+            "kernel": "one RHS launch sequence (" + " + ".join(KERNELS[args.strategy]) + "), event-timed per "
+                      f"launch on its stream; algorithmic bytes {info.alg_bytes_per_cell:.1f} B/cell x {cells} cells",
         },
         "phases_ms": {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in phases.items() if cnt},
         "event_ms_per_step": round(event_ms, 4),
         "setup_s": round(setup_s, 2),
+        "hbm_copy_probe": copy,
         "device_bytes": info.device_bytes,
         "finite": finite,
     }
