@@ -103,7 +103,7 @@ SIGNATURES = {
     "abl_fv_get_phase_times": (c_int32, [c_void_p, _dp, POINTER(c_int64), c_int32]),
 }
 PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]
-STRATEGY = {"recompute": 1, "face_flux": 2}
+STRATEGY = {"recompute": 1, "face_flux": 2, "tile": 3}
 
 _lib = None
 

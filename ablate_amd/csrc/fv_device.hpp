@@ -28,19 +28,25 @@ struct MeshView {
     const int32_t* __restrict__ fcell;
     const G* __restrict__ fna;  // [F][4]: unit normal, area
     const G* __restrict__ fdx;  // [F][2][DIM]: x_face - x_cell per side
-    const G* __restrict__ fw;   // [F][2][DIM]: cell least-squares weights per side
-    const G* __restrict__ fwa;  // [F][2][DIM]: aux (face-stencil) weights, or null = fw
+    const G* __restrict__ fw;   // [F][2][DIM]: cell least-squares weights per side (unused by the kernels)
+    const G* __restrict__ fwa;  // [F][2][DIM]: aux (face-stencil) weights, or null when there are no boundaries
     const double* __restrict__ inv_vol;
+    // Slot-major copies of the per-side geometry, entry (k, d, c) at (k * DIM + d) * NI + c, for KM = slot width
+    // slots (padding slots: neighbour -1, zero weights).  A lane per cell streams its own records.
+    const int32_t* __restrict__ snbr;  // [KM][NI] cell across slot k, -1 = no face
+    const G* __restrict__ sw;          // [KM][DIM][NI] cell least-squares weight of the slot's side
+    const G* __restrict__ swa;         // [KM][DIM][NI] aux weights, or null = sw
+    const G* __restrict__ slim;        // [KM][DIM][NI] limiter geometry: x_f - x_c (BJ) or x_n - x_c (PETSc family)
 };
 
 // Block index remapped so that each XCD (workgroups are dispatched round-robin over 8 XCDs) walks a contiguous
 // range of cells, keeping neighbour reuse inside one XCD's L2.
-__device__ inline int64_t xcd_cell_index() {
+__device__ inline int64_t xcd_block_index() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
-    const uint32_t logical = x * q + (x < r ? x : r) + (b >> 3);
-    return (int64_t)logical * blockDim.x + threadIdx.x;
+    return (int64_t)(x * q + (x < r ? x : r) + (b >> 3));
 }
+__device__ inline int64_t xcd_cell_index() { return xcd_block_index() * blockDim.x + threadIdx.x; }
 
 // Face records are read with vector loads (16 bytes for FP64 records, 8/16 bytes for FP32 ones).
 __device__ inline void load_face_na(const double* __restrict__ fna, int64_t f, double* nhat, double& area) {

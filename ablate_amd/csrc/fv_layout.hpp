@@ -33,4 +33,16 @@ struct Layout {
 // Validates the descriptor and fills the layout; throws std::invalid_argument on bad input.
 void build_layout(const abl_fv_mesh_desc& mesh, const abl_fv_physics_desc& phys, Layout& out);
 
+// Cell tiles for the fused face-flux + gather kernel: tile t owns interior cells [t*T, min((t+1)*T, NI)).
+//   start[t] .. start[t+1]   its faces (every face with a cell in the tile), ascending face number
+//   slot[k * NI + c]         (tile-local index of the k-th face of c) << 1 | side, -1 padding; KM slots
+//   nf_max                   largest face count of a tile (sizes the workgroup's LDS)
+struct Tiles {
+    int T = 0;
+    int64_t n = 0;
+    int nf_max = 0;
+    std::vector<int32_t> start, faces, slot;
+};
+void build_tiles(const Layout& L, int KM, int T, Tiles& out);
+
 }  // namespace abl

@@ -263,4 +263,39 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
     }
 }
 
+void build_tiles(const Layout& L, int KM, int T, Tiles& out) {
+    const int64_t NI = L.NI;
+    out.T = T;
+    out.n = (NI + T - 1) / T;
+    out.nf_max = 0;
+    out.start.assign((size_t)out.n + 1, 0);
+    out.faces.clear();
+    out.faces.reserve((size_t)NI * 4);
+    out.slot.assign((size_t)KM * NI, -1);
+    std::vector<int32_t> list;
+    for (int64_t t = 0; t < out.n; t++) {
+        const int64_t c0 = t * T, c1 = std::min<int64_t>(c0 + T, NI);
+        list.clear();
+        for (int k = 0; k < L.K; k++)
+            for (int64_t c = c0; c < c1; c++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e >= 0) list.push_back(e >> 1);
+            }
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        for (int k = 0; k < L.K; k++)
+            for (int64_t c = c0; c < c1; c++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e < 0) continue;
+                const int32_t local = (int32_t)(std::lower_bound(list.begin(), list.end(), e >> 1) - list.begin());
+                out.slot[(size_t)k * NI + c] = (local << 1) | (e & 1);
+            }
+        if (out.faces.size() + list.size() > (size_t)INT32_MAX)
+            throw std::invalid_argument("mesh too large for 32-bit tile face lists");
+        out.faces.insert(out.faces.end(), list.begin(), list.end());
+        out.start[(size_t)t + 1] = (int32_t)out.faces.size();
+        out.nf_max = std::max(out.nf_max, (int)list.size());
+    }
+}
+
 }  // namespace abl

@@ -37,6 +37,8 @@ thread_local std::string g_last_error;
 void set_global_error(const std::string& msg) { g_last_error = msg; }
 
 constexpr int kBlock = 256;
+constexpr int kTileMax = 256;  // largest tile (workgroup) of the tile strategy
+constexpr int kTileLdsMax = 64 * 1024;  // LDS budget of one tile workgroup
 
 // ============================================================================================ kernels
 
@@ -72,18 +74,34 @@ __device__ inline void aux_of(const double* u, const Phys& ph, double* a) {
 // Limiter families resolved at compile time.
 enum : int { kLimNone = 0, kLimPetsc = 1, kLimBJ = 2 };
 
+// Cell record of the FP64 form, written by the gradient pass for every interior cell and read by the face
+// fluxes: [0, NE) the state, [G0, G0 + NE*DIM) the limited euler gradients (p * DIM + d), [T0, T0 + NT) the
+// viscous terms (zero where unused), padded to RS doubles (256 bytes in 3-D) so that a neighbour's whole record
+// arrives in whole cache lines instead of one line per structure-of-arrays row.
+template <int DIM>
+struct CellRec {
+    static constexpr int NE = DIM + 2;
+    static constexpr int G0 = NE;
+    static constexpr int T0 = NE + NE * DIM;
+    static constexpr int END = T0 + ViscTerms<DIM>::NT;
+    static constexpr int RS = DIM == 3 ? 32 : 18;
+    static_assert(END <= RS && RS % 2 == 0, "record layout");
+};
+
 // Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
 // G is the storage/arithmetic type of weights and gradients (double: FP64 form, float: mixed form); state
-// differences are always taken in FP64 before rounding to G.
+// differences are always taken in FP64 before rounding to G.  The FP64 form writes the cell record `rec`; the
+// mixed form writes structure-of-arrays gradC [NE*DIM][N] and gradA [NT][N].
 template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
 __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
-                                                      G* __restrict__ gradC, G* __restrict__ gradA) {
+                                                      G* __restrict__ gradC, G* __restrict__ gradA,
+                                                      double* __restrict__ rec) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
-    const int64_t N = m.N, F = m.F, NI = m.NI;
-    const G* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    const int64_t N = m.N, NI = m.NI;
+    const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     double uc[NE], ac[NA];
 #pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
@@ -104,17 +122,20 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
 
 #pragma unroll
     for (int k = 0; k < KM; k++) {
-        const Slot sl = face_slot(m, k, c);
-        const int64_t f = sl.f, n = sl.n;
-        const int s = sl.s;
+        const int32_t e = m.snbr[k * NI + c];
+        const bool valid = e >= 0;
+        const int64_t n = valid ? (int64_t)e : c;
         const bool bnd = n >= NI;
         double un[NE];
 #pragma unroll
         for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
-        G w[2][DIM], wa[2][DIM];
-        if (RECON) load_face_pair<DIM, G>(m.fw, f, w);
-        if (VISC) load_face_pair<DIM, G>(wa_arr, f, wa);
-        if (!sl.valid) continue;
+        G w[DIM], wa[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            if (RECON) w[d] = m.sw[(k * DIM + d) * NI + c];
+            if (VISC) wa[d] = wa_arr[(k * DIM + d) * NI + c];
+        }
+        if (!valid) continue;
         if (RECON) {
             if (LIM == kLimBJ) {
 #pragma unroll
@@ -124,18 +145,13 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
                 }
             }
             if (!bnd) {
+                // cellInterpolant.cpp:405-411 with delta = u_R - u_L: the left cell adds w0.delta and the right cell
+                // subtracts w1.delta, i.e. both add w.(u_n - u_c) with their own side's weight (bitwise the same)
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
-                    // delta = u_R - u_L; the left cell adds w.delta, the right cell subtracts it
-                    if (s == 0) {
-                        const G delta = G(un[p] - uc[p]);
+                    const G delta = G(un[p] - uc[p]);
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) g[p][d] += w[0][d] * delta;
-                    } else {
-                        const G delta = G(uc[p] - un[p]);
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) g[p][d] -= w[1][d] * delta;
-                    }
+                    for (int d = 0; d < DIM; d++) g[p][d] += w[d] * delta;
                 }
             }
         }
@@ -146,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
             for (int q = 0; q < NA; q++) {
                 const G delta = G(an[q] - ac[q]);
 #pragma unroll
-                for (int d = 0; d < DIM; d++) ga[q][d] += wa[s][d] * delta;
+                for (int d = 0; d < DIM; d++) ga[q][d] += wa[d] * delta;
             }
         }
     }
@@ -164,16 +180,16 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
             }
 #pragma unroll
             for (int k = 0; k < KM; k++) {
-                const Slot sl = face_slot(m, k, c);
-                G x[2][DIM];
-                load_face_pair<DIM, G>(m.fdx, sl.f, x);
-                if (!sl.valid) continue;
-                const int s = sl.s;
+                const bool valid = m.snbr[k * NI + c] >= 0;
+                G x[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) x[d] = m.slim[(k * DIM + d) * NI + c];
+                if (!valid) continue;
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
                     G D = G(0);
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) D += g[p][d] * x[s][d];
+                    for (int d = 0; d < DIM; d++) D += g[p][d] * x[d];
                     dpos[p] = fmax(dpos[p], D);
                     dneg[p] = fmin(dneg[p], D);
                 }
@@ -191,14 +207,12 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
             for (int p = 0; p < NE; p++) phi[p] = sizeof(G) == 8 ? G(DBL_MAX) : G(FLT_MAX);
 #pragma unroll
             for (int k = 0; k < KM; k++) {
-                const Slot sl = face_slot(m, k, c);
-                const int64_t n = sl.n;
-                const int s = sl.s;
-                G x[2][DIM], v[DIM];
-                load_face_pair<DIM, G>(m.fdx, sl.f, x);
-                if (!sl.valid) continue;
+                const int32_t e = m.snbr[k * NI + c];
+                const int64_t n = e >= 0 ? (int64_t)e : c;
+                G v[DIM];  // x_n - x_c
 #pragma unroll
-                for (int d = 0; d < DIM; d++) v[d] = x[s][d] - x[1 - s][d];
+                for (int d = 0; d < DIM; d++) v[d] = m.slim[(k * DIM + d) * NI + c];
+                if (e < 0) continue;
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
                     G denom = G(0);
@@ -221,17 +235,36 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
 #pragma unroll
             for (int d = 0; d < DIM; d++) g[p][d] *= cancel ? G(0) : phi[p];
     }
+    G tq[ViscTerms<DIM>::NT];
+    if (VISC) stress_heat<G, DIM>(G(ph.mu), G(ph.k), ga, tq);
+    if (sizeof(G) == 8) {
+        using CR = CellRec<DIM>;
+        double r[CR::RS];
+#pragma unroll
+        for (int i = 0; i < CR::RS; i++) r[i] = 0.0;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            r[p] = uc[p];
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                if (RECON) r[CR::G0 + p * DIM + d] = double(g[p][d]);
+        }
+#pragma unroll
+        for (int q = 0; q < ViscTerms<DIM>::NT; q++)
+            if (VISC) r[CR::T0 + q] = double(tq[q]);
+        double2* out = reinterpret_cast<double2*>(rec + c * CR::RS);
+#pragma unroll
+        for (int i = 0; i < CR::RS / 2; i++) out[i] = make_double2(r[2 * i], r[2 * i + 1]);
+        return;
+    }
     if (RECON)
 #pragma unroll
         for (int p = 0; p < NE; p++)
 #pragma unroll
             for (int d = 0; d < DIM; d++) gradC[(p * DIM + d) * N + c] = g[p][d];
-    if (VISC) {
-        G tq[ViscTerms<DIM>::NT];
-        stress_heat<G, DIM>(G(ph.mu), G(ph.k), ga, tq);
+    if (VISC)
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++) gradA[q * N + c] = tq[q];
-    }
 }
 
 // densityYi gradients (limited, one field) and Yi aux gradients (unlimited).
@@ -241,10 +274,10 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
     constexpr int NE = DIM + 2;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
-    const int64_t N = m.N, F = m.F, NI = m.NI;
+    const int64_t N = m.N, NI = m.NI;
     const bool recon = ph.recon != 0;
     const bool aux = ph.sdiff != 0;
-    const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     const double irho_c = 1.0 / U[kRho * N + c];
     bool cancel = false;
     for (int sp = 0; sp < ph.ns; sp++) {
@@ -258,35 +291,23 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
             ga[d] = 0.0;
         }
         for (int k = 0; k < m.K; k++) {
-            const int32_t e = m.ell[k * NI + c];
+            const int32_t e = m.snbr[k * NI + c];
             if (e < 0) break;
-            const int64_t f = e >> 1;
-            const int s = e & 1;
-            const int64_t n = m.fcell[(1 - s) * F + f];
+            const int64_t n = e;
             const double un = U[comp * N + n];
             if (recon) {
                 umin = fmin(umin, un);
                 umax = fmax(umax, un);
                 if (n < NI) {
-                    double w[2][DIM];
-                    load_face_pair<DIM>(m.fw, f, w);
-                    if (s == 0) {
-                        const double delta = un - uc;
+                    const double delta = un - uc;  // both sides: + w.(u_n - u_c), see k_gradients
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) g[d] += w[0][d] * delta;
-                    } else {
-                        const double delta = uc - un;
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) g[d] -= w[1][d] * delta;
-                    }
+                    for (int d = 0; d < DIM; d++) g[d] += m.sw[(k * DIM + d) * NI + c] * delta;
                 }
             }
             if (aux) {
-                double w[2][DIM];
-                load_face_pair<DIM>(wa_arr, f, w);
                 const double delta = un / U[kRho * N + n] - yc;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) ga[d] += w[s][d] * delta;
+                for (int d = 0; d < DIM; d++) ga[d] += wa_arr[(k * DIM + d) * NI + c] * delta;
             }
         }
         if (recon) {
@@ -294,15 +315,10 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
             if (ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN) {
                 double dpos = 0.0, dneg = 0.0;
                 for (int k = 0; k < m.K; k++) {
-                    const int32_t e = m.ell[k * NI + c];
-                    if (e < 0) break;
-                    const int64_t f = e >> 1;
-                    const int s = e & 1;
-                    double x[2][DIM];
-                    load_face_pair<DIM>(m.fdx, f, x);
+                    if (m.snbr[k * NI + c] < 0) break;
                     double D = 0.0;
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) D += g[d] * x[s][d];
+                    for (int d = 0; d < DIM; d++) D += g[d] * m.slim[(k * DIM + d) * NI + c];
                     dpos = fmax(dpos, D);
                     dneg = fmin(dneg, D);
                 }
@@ -311,16 +327,12 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                 if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
             } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
                 for (int k = 0; k < m.K; k++) {
-                    const int32_t e = m.ell[k * NI + c];
+                    const int32_t e = m.snbr[k * NI + c];
                     if (e < 0) break;
-                    const int64_t f = e >> 1;
-                    const int s = e & 1;
-                    const int64_t n = m.fcell[(1 - s) * F + f];
-                    double x[2][DIM];
-                    load_face_pair<DIM>(m.fdx, f, x);
+                    const int64_t n = e;
                     double denom = 0.0;
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) denom += g[d] * (x[s][d] - x[1 - s][d]);
+                    for (int d = 0; d < DIM; d++) denom += g[d] * m.slim[(k * DIM + d) * NI + c];
                     const double flim = 0.5 * (U[comp * N + n] - uc) / denom;
                     phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
                 }
@@ -357,39 +369,54 @@ struct SideData {
     double tq[VISC ? ViscTerms<DIM>::NT : 1];
 };
 
-// Loads side data of cell c for a face whose centroid is at x_c + dx (ghost cells have zero gradients; their
-// viscous terms are filled by ghost_visc).
+// Loads side data of cell c for a face whose centroid is at x_c + dx: from the cell record when the gradient pass
+// ran (reconstruction or diffusion), else from the state.  Ghost cells have zero gradients and read the state;
+// their viscous terms are filled by ghost_visc.
 template <int DIM, bool RECON, bool VISC>
-__device__ inline void load_side(const double* __restrict__ U, const double* __restrict__ gradC,
-                                 const double* __restrict__ gradA, int64_t N, int64_t c, bool ghost, const double* dx,
-                                 SideData<DIM, VISC>& sd) {
+__device__ inline void load_side(const double* __restrict__ U, const double* __restrict__ rec, int64_t N, int64_t c,
+                                 bool ghost, const double* dx, SideData<DIM, VISC>& sd) {
+    using CR = CellRec<DIM>;
     constexpr int NE = DIM + 2;
+    double u[NE];
+    if ((RECON || VISC) && !ghost) {
+        constexpr int NL = ((VISC ? CR::END : CR::T0) + 1) & ~1;
+        double r[NL];
+        const double2* p = reinterpret_cast<const double2*>(rec + c * CR::RS);
 #pragma unroll
-    for (int p = 0; p < NE; p++) {
-        const double u = U[p * N + c];
-        double a = 0.0;
-        if (RECON && !ghost)
+        for (int i = 0; i < NL / 2; i++) {
+            const double2 t = p[i];
+            r[2 * i] = t.x;
+            r[2 * i + 1] = t.y;
+        }
 #pragma unroll
-            for (int d = 0; d < DIM; d++) a += gradC[(p * DIM + d) * N + c] * dx[d];
-        sd.uf[p] = u + a;
-        if (p == kRho) sd.rho = u;
-        if (VISC) {
-            if (p == kRhoE) sd.T = u;
-            if (p >= kRhoU) sd.vel[p - kRhoU] = u;
+        for (int p = 0; p < NE; p++) {
+            double a = 0.0;
+            if (RECON)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) a += r[CR::G0 + p * DIM + d] * dx[d];
+            u[p] = r[p];
+            sd.uf[p] = r[p] + a;
+        }
+        if (VISC)
+#pragma unroll
+            for (int q = 0; q < ViscTerms<DIM>::NT; q++) sd.tq[q] = r[CR::T0 + q];
+    } else {
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            u[p] = U[p * N + c];
+            sd.uf[p] = u[p];
         }
     }
+    sd.rho = u[kRho];
     if (VISC) {
         const double irho = 1.0 / sd.rho;
         double ke = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            sd.vel[d] *= irho;
+            sd.vel[d] = u[kRhoU + d] * irho;
             ke += sd.vel[d] * sd.vel[d];
         }
-        sd.T = sd.T * irho - 0.5 * ke;  // internal energy; scaled to T by the caller's cv only where needed
-        if (!ghost)
-#pragma unroll
-            for (int q = 0; q < ViscTerms<DIM>::NT; q++) sd.tq[q] = gradA[q * N + c];
+        sd.T = u[kRhoE] * irho - 0.5 * ke;  // internal energy; scaled to T by the caller's cv only where needed
     }
 }
 
@@ -493,13 +520,14 @@ __device__ inline double species_face_flux(const MeshView<double>& m, const doub
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                  const double* __restrict__ gradC, const double* __restrict__ gradA,
-                                                 double* __restrict__ R, int accumulate) {
+                                                 const double* __restrict__ rec, double* __restrict__ R,
+                                                 int accumulate) {
     constexpr int NE = DIM + 2;
     using SD = SideData<DIM, VISC>;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const double* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    const double* __restrict__ wa_arr = m.fwa;  // present whenever ghosts meet diffusion
     double acc[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) acc[p] = 0.0;
@@ -514,8 +542,8 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
         load_face_na(m.fna, f, nh, area);
         load_face_pair<DIM>(m.fdx, f, x);
         SD sl, sr;
-        load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
-        load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+        load_side<DIM, RECON, VISC>(U, rec, N, L, false, x[0], sl);
+        load_side<DIM, RECON, VISC>(U, rec, N, Rc, ghost, x[1], sr);
         if (VISC && ghost) {
             double w[2][DIM];
             load_face_pair<DIM>(wa_arr, f, w);
@@ -550,8 +578,8 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
             load_face_pair<DIM>(m.fdx, f, x);
             if (VISC && ghost) load_face_pair<DIM>(wa_arr, f, w);
             SD sl, sr;
-            load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
-            load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+            load_side<DIM, RECON, VISC>(U, rec, N, L, false, x[0], sl);
+            load_side<DIM, RECON, VISC>(U, rec, N, Rc, ghost, x[1], sr);
             if (VISC && ghost) ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
             double flux[NE], cl, cr, cd;
             face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
@@ -564,17 +592,14 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
     }
 }
 
-// Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored
-// (euler components as a padded record [F][FS], species as [ns][F]).
+// The flux through face f with the reference orientation: euler components into fe[0, NE), species into
+// fs[sp * fs_stride] (shared by the face-flux and tile strategies, which therefore agree bitwise).
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
-__global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                      const double* __restrict__ gradC, const double* __restrict__ gradA,
-                                                      double* __restrict__ Fe, double* __restrict__ Fs) {
-    constexpr int NE = DIM + 2;
-    constexpr int FS = (NE + 1) & ~1;
+__device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U,
+                                 const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                 const double* __restrict__ rec, int64_t f, double* fe, double* fs,
+                                 int64_t fs_stride) {
     using SD = SideData<DIM, VISC>;
-    const int64_t f = xcd_cell_index();
-    if (f >= m.F) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
     const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
     const bool ghost = Rc >= NI;
@@ -582,22 +607,97 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys p
     load_face_na(m.fna, f, nh, area);
     load_face_pair<DIM>(m.fdx, f, x);
     SD sl, sr;
-    load_side<DIM, RECON, VISC>(U, gradC, gradA, N, L, false, x[0], sl);
-    load_side<DIM, RECON, VISC>(U, gradC, gradA, N, Rc, ghost, x[1], sr);
+    load_side<DIM, RECON, VISC>(U, rec, N, L, false, x[0], sl);
+    load_side<DIM, RECON, VISC>(U, rec, N, Rc, ghost, x[1], sr);
     if (VISC && ghost) {
-        load_face_pair<DIM>(m.fwa ? m.fwa : m.fw, f, w);
+        load_face_pair<DIM>(m.fwa, f, w);
         ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
     }
-    double flux[FS], cl, cr, cd;
-    face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
+    double cl, cr, cd;
+    face_flux<DIM, VISC>(ph, nh, area, sl, sr, fe, cl, cr, cd);
+    if (SPECIES)
+        for (int sp = 0; sp < ph.ns; sp++)
+            fs[sp * fs_stride] = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, DIM + 2 + sp, sp, f, L, Rc,
+                                                                     ghost, x, nh, cl, cr, cd, w[1]);
+}
+
+// Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored
+// (euler components as a padded record [F][FS], species as [ns][F]).
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
+__global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                                      const double* __restrict__ rec, double* __restrict__ Fe,
+                                                      double* __restrict__ Fs) {
+    constexpr int NE = DIM + 2;
+    constexpr int FS = (NE + 1) & ~1;
+    const int64_t f = xcd_cell_index();
+    if (f >= m.F) return;
+    double flux[FS];
+    eval_face<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, f, flux, Fs + f, m.F);
     if (FS > NE) flux[FS - 1] = 0.0;
     double2* out = reinterpret_cast<double2*>(Fe + FS * f);
 #pragma unroll
     for (int i = 0; i < FS / 2; i++) out[i] = make_double2(flux[2 * i], flux[2 * i + 1]);
-    if (SPECIES)
-        for (int sp = 0; sp < ph.ns; sp++)
-            Fs[(int64_t)sp * F + f] = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, NE + sp, sp, f, L, Rc,
-                                                                          ghost, x, nh, cl, cr, cd, w[1]);
+}
+
+// Strategy 3 (tile): one workgroup per tile of T consecutive interior cells.  Its lanes evaluate every face of the
+// tile once into LDS (faces shared with a neighbouring tile are evaluated by both), then each lane gathers its
+// cell's faces from LDS in face-number order.  No face-flux round trip through HBM; the face fluxes and the
+// gather order are those of strategy 2, so the residual is bitwise the same.
+template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM>
+__global__ __launch_bounds__(kTileMax) void k_tile_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                        const double* __restrict__ gradC,
+                                                        const double* __restrict__ gradA,
+                                                        const double* __restrict__ rec,
+                                                        const int32_t* __restrict__ tstart,
+                                                        const int32_t* __restrict__ tfaces,
+                                                        const int32_t* __restrict__ tslot, int nfm,
+                                                        double* __restrict__ R, int accumulate) {
+    constexpr int NE = DIM + 2;
+    extern __shared__ double lds[];
+    double* le = lds;              // [NE][nfm]
+    double* ls = lds + NE * nfm;   // [ns][nfm]
+    const int64_t t = xcd_block_index();
+    const int32_t f0 = tstart[t], nf = tstart[t + 1] - f0;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        double flux[NE];
+        eval_face<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, (int64_t)tfaces[f0 + i], flux, ls + i,
+                                             nfm);
+#pragma unroll
+        for (int p = 0; p < NE; p++) le[p * nfm + i] = flux[p];
+    }
+    __syncthreads();
+    const int64_t c = t * blockDim.x + threadIdx.x;
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI;
+    int li[KM];
+    double sg[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = tslot[k * NI + c];
+        li[k] = e >= 0 ? (e >> 1) : 0;
+        sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
+    }
+    const double iv = m.inv_vol[c];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++)
+            if (sg[k] != 0.0) acc += sg[k] * le[p * nfm + li[k]];
+        const double r = acc * iv;
+        R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+    }
+    if (!SPECIES) return;
+    for (int sp = 0; sp < ph.ns; sp++) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++)
+            if (sg[k] != 0.0) a += sg[k] * ls[sp * nfm + li[k]];
+        const int64_t comp = NE + sp;
+        const double r = a * iv;
+        R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
+    }
 }
 
 // Strategy 2, pass 2: each interior cell gathers its faces' stored fluxes in face-number order.
@@ -735,7 +835,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const float* __restrict__ wa_arr = m.fwa ? m.fwa : m.fw;
+    const float* __restrict__ wa_arr = m.fwa;  // present whenever ghosts meet diffusion
     double uc[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
@@ -1058,10 +1158,12 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
 
 // ---- compile-time variant selection
 template <typename G>
-using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*);
+using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*, double*);
 using GradKernel = GradKernelT<double>;
-using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, int);
-using FaceFluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, double*, double*);
+using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
+                            double*, int);
+using FaceFluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
+                                double*, double*);
 using GatherKernel = void (*)(MeshView<double>, int, const double*, const double*, double*, int);
 
 template <typename G, int DIM, int KM>
@@ -1126,6 +1228,25 @@ VjpKernel pick_vjp(bool rusanov, int K) {
 template <int DIM>
 VjpThetaKernel pick_vjp_theta(bool rusanov) {
     return rusanov ? k_vjp_theta<DIM, true> : k_vjp_theta<DIM, false>;
+}
+using TileKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
+                           const int32_t*, const int32_t*, const int32_t*, int, double*, int);
+template <int DIM, bool RECON, bool VISC, int KM>
+TileKernel pick_tile3(bool species) {
+    return species ? k_tile_flux<DIM, RECON, VISC, true, KM> : k_tile_flux<DIM, RECON, VISC, false, KM>;
+}
+template <int DIM, bool RECON, bool VISC>
+TileKernel pick_tile2(bool species, int K) {
+    switch (slot_width(K)) {
+        case 4: return pick_tile3<DIM, RECON, VISC, 4>(species);
+        case 6: return pick_tile3<DIM, RECON, VISC, 6>(species);
+        default: return pick_tile3<DIM, RECON, VISC, 8>(species);
+    }
+}
+template <int DIM>
+TileKernel pick_tile(bool recon, bool visc, bool species, int K) {
+    if (recon) return visc ? pick_tile2<DIM, true, true>(species, K) : pick_tile2<DIM, true, false>(species, K);
+    return visc ? pick_tile2<DIM, false, true>(species, K) : pick_tile2<DIM, false, false>(species, K);
 }
 template <int DIM>
 GatherKernel pick_gather(bool species, int K) {
@@ -1202,6 +1323,42 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Slot-major copies of the per-side face geometry for the gradient kernels: for slot k (< KM, the compile-time
+// slot width) of interior cell c, the neighbour across the face, the cell's least-squares weight vectors (cell
+// and aux sets) and the limiter geometry (x_f - x_c for Barth-Jespersen, x_n - x_c for the PETSc family, both
+// in G arithmetic from the G-rounded face records, as the face-major kernels form them).
+template <typename G>
+struct SlotGeometry {
+    std::vector<int32_t> nbr;
+    std::vector<G> w, wa, lim;
+};
+template <typename G>
+void build_slot_geometry(const Layout& L, int KM, int lim, SlotGeometry<G>& out) {
+    const int dim = L.dim;
+    const int64_t NI = L.NI, F = L.F;
+    out.nbr.assign((size_t)KM * NI, -1);
+    if (!L.fw.empty()) out.w.assign((size_t)KM * dim * NI, G(0));
+    if (!L.fwa.empty()) out.wa.assign((size_t)KM * dim * NI, G(0));
+    if (lim != kLimNone) out.lim.assign((size_t)KM * dim * NI, G(0));
+    for (int k = 0; k < L.K; k++)
+        for (int64_t c = 0; c < NI; c++) {
+            const int32_t e = L.ell[(size_t)k * NI + c];
+            if (e < 0) continue;
+            const int64_t f = e >> 1;
+            const int s = e & 1;
+            out.nbr[(size_t)k * NI + c] = L.fcell[(size_t)(1 - s) * F + f];
+            for (int d = 0; d < dim; d++) {
+                const size_t o = ((size_t)k * dim + d) * NI + c;
+                const size_t a = (size_t)2 * dim * f + (size_t)s * dim + d;
+                const size_t b = (size_t)2 * dim * f + (size_t)(1 - s) * dim + d;
+                if (!out.w.empty()) out.w[o] = G(L.fw[a]);
+                if (!out.wa.empty()) out.wa[o] = G(L.fwa[a]);
+                if (lim == kLimBJ) out.lim[o] = G(L.fdx[a]);
+                else if (lim == kLimPetsc) out.lim[o] = G(L.fdx[a]) - G(L.fdx[b]);
+            }
+        }
+}
+
 }  // namespace abl
 
 struct abl_fv_solver {
@@ -1210,11 +1367,19 @@ struct abl_fv_solver {
     int form = 0;
     abl::Layout host;  // metadata only after upload (arrays released)
     abl::Phys ph{};
-    abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind;
-    abl::DevBuf<double> fna, fdx, fw, fwa, inv_vol, gradC, gradA, slot_state;
+    abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind, snbr;
+    abl::DevBuf<double> fna, fdx, fwa, inv_vol, gradC, gradA, slot_state;
+    abl::DevBuf<double> sw, swa, slim;  // slot-major gradient geometry (FP64 form)
+    abl::DevBuf<double> rec;            // cell records [NI][RS] of the FP64 form
     abl::DevBuf<double> face_flux_e, face_flux_s;  // face-flux records (allocated when that strategy is selected)
+    abl::DevBuf<int32_t> tile_start, tile_faces, tile_slot;  // tile strategy (built when selected)
+    int tile_T = 0, tile_nfm = 0;
+    int64_t tile_n = 0;
+    size_t tile_lds = 0;
+    abl::TileKernel tile_kernel = nullptr;
     // mixed form: FP32 geometry and gradients
-    abl::DevBuf<float> fna_f, fdx_f, fw_f, fwa_f, gradC_f, gradA_f;
+    abl::DevBuf<float> fna_f, fdx_f, fwa_f, gradC_f, gradA_f;
+    abl::DevBuf<float> sw_f, swa_f, slim_f;  // slot-major gradient geometry (mixed form)
     abl::GradKernelT<float> grad_kernel_f = nullptr;
     abl::MixedFluxKernel flux_kernel_mixed = nullptr;
     // adjoint
@@ -1227,7 +1392,7 @@ struct abl_fv_solver {
     abl::FluxKernel flux_kernel = nullptr;
     abl::FaceFluxKernel face_flux_kernel = nullptr;
     abl::GatherKernel gather_kernel = nullptr;
-    int strategy = ABL_FV_STRATEGY_FACE_FLUX;
+    int strategy = ABL_FV_STRATEGY_TILE;
     abl::DevBuf<unsigned long long> counter;
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
@@ -1289,9 +1454,13 @@ struct abl_fv_solver {
         v.fcell = fcell.ptr;
         v.fna = fna.ptr;
         v.fdx = fdx.ptr;
-        v.fw = fw.ptr;
+        v.fw = nullptr;
         v.fwa = fwa.ptr;
         v.inv_vol = inv_vol.ptr;
+        v.snbr = snbr.ptr;
+        v.sw = sw.ptr;
+        v.swa = swa.ptr;
+        v.slim = slim.ptr;
         return v;
     }
     abl::MeshView<float> view_f() const {
@@ -1304,9 +1473,13 @@ struct abl_fv_solver {
         v.fcell = fcell.ptr;
         v.fna = fna_f.ptr;
         v.fdx = fdx_f.ptr;
-        v.fw = fw_f.ptr;
+        v.fw = nullptr;
         v.fwa = fwa_f.ptr;
         v.inv_vol = inv_vol.ptr;
+        v.snbr = snbr.ptr;
+        v.sw = sw_f.ptr;
+        v.swa = swa_f.ptr;
+        v.slim = slim_f.ptr;
         return v;
     }
     // Face-flux records [F][FS] (FS = n_euler rounded up to even) and species fluxes [ns][F]; FP64 form only.
@@ -1316,10 +1489,30 @@ struct abl_fv_solver {
         face_flux_e.alloc((size_t)fs * host.F);
         if (host.ns) face_flux_s.alloc((size_t)host.ns * host.F);
     }
+    // Tiles for strategy 3 (FP64 form): the largest tile (256, 128 or 64 cells) whose face fluxes fit the LDS
+    // budget.  Needs the host cell -> face table, so it is built at creation.
+    void build_tiles() {
+        const int KM = abl::slot_width(host.K);
+        const int per_face = (host.dim + 2 + host.ns) * (int)sizeof(double);
+        abl::Tiles tl;
+        for (int T = abl::kTileMax; T >= 64; T /= 2) {
+            abl::build_tiles(host, KM, T, tl);
+            if ((size_t)tl.nf_max * per_face <= (size_t)abl::kTileLdsMax) break;
+            tl.n = 0;
+        }
+        if (tl.n == 0) return;  // no tile size fits: the strategy stays unavailable
+        tile_start.upload(tl.start);
+        tile_faces.upload(tl.faces);
+        tile_slot.upload(tl.slot);
+        tile_T = tl.T;
+        tile_n = tl.n;
+        tile_nfm = tl.nf_max;
+        tile_lds = (size_t)tl.nf_max * per_face;
+    }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + fw.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
-                         fw_f.bytes() + fwa_f.bytes() + gradC_f.bytes() + gradA_f.bytes());
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+                         gradA.bytes() + rec.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         fwa_f.bytes() + gradC_f.bytes() + gradA_f.bytes());
     }
 };
 
@@ -1385,11 +1578,13 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->grad_kernel = abl::pick_gradients<double, 3>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<3>(s->recon, s->visc, species);
             s->face_flux_kernel = abl::pick_face_flux<3>(s->recon, s->visc, species);
+            s->tile_kernel = abl::pick_tile<3>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<3>(species, L.K);
         } else {
             s->grad_kernel = abl::pick_gradients<double, 2>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<2>(s->recon, s->visc, species);
             s->face_flux_kernel = abl::pick_face_flux<2>(s->recon, s->visc, species);
+            s->tile_kernel = abl::pick_tile<2>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<2>(species, L.K);
         }
 
@@ -1401,13 +1596,21 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         if (form == ABL_FV_FORM_FP64) {
             s->fna.upload(L.fna);
             s->fdx.upload(L.fdx);
-            s->fw.upload(L.fw);
+            abl::SlotGeometry<double> sg;
+            abl::build_slot_geometry<double>(L, abl::slot_width(L.K), lim, sg);
+            s->snbr.upload(sg.nbr);
+            s->sw.upload(sg.w);
+            s->swa.upload(sg.wa);
+            s->slim.upload(sg.lim);
             s->fwa.upload(L.fwa);
-            if (s->recon) {
+            if (s->recon || s->visc)
+                s->rec.alloc((size_t)(L.dim == 3 ? abl::CellRec<3>::RS : abl::CellRec<2>::RS) * L.NI);
+            // species gradients stay structure-of-arrays (rows NE*DIM.. of gradC, NT.. of gradA)
+            if (s->recon && L.ns) {
                 s->gradC.alloc((size_t)L.nc * L.dim * N);
                 HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost rows stay zero
             }
-            if (s->visc) {
+            if (s->visc && L.ns) {
                 s->gradA.alloc((size_t)(nt + L.ns * L.dim) * N);
                 HIP_CHECK(hipMemset(s->gradA.ptr, 0, s->gradA.bytes()));
             }
@@ -1415,7 +1618,12 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             auto to_f = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
             s->fna_f.upload(to_f(L.fna));
             s->fdx_f.upload(to_f(L.fdx));
-            s->fw_f.upload(to_f(L.fw));
+            abl::SlotGeometry<float> sg;
+            abl::build_slot_geometry<float>(L, abl::slot_width(L.K), lim, sg);
+            s->snbr.upload(sg.nbr);
+            s->sw_f.upload(sg.w);
+            s->swa_f.upload(sg.wa);
+            s->slim_f.upload(sg.lim);
             s->fwa_f.upload(to_f(L.fwa));
             if (s->recon) {
                 s->gradC_f.alloc((size_t)L.nc * L.dim * N);
@@ -1466,6 +1674,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         s->slot_kind.upload(s->h_slot_kind);
         s->slot_state.upload(s->h_slot_state);
         s->counter.alloc(1);
+        if (form == ABL_FV_FORM_FP64) s->build_tiles();
+        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0) s->strategy = ABL_FV_STRATEGY_FACE_FLUX;
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
@@ -1570,7 +1780,8 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             const auto vf = h->view_f();
             if (h->recon || h->visc)
                 h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
-                    hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr, h->gradA_f.ptr);
+                    hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr, h->gradA_f.ptr,
+                                       nullptr);
                 });
             h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                 hipLaunchKernelGGL(h->flux_kernel_mixed, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr,
@@ -1580,7 +1791,8 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
         const auto v = h->view();
         if (h->recon || h->visc) {
             h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
-                hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
+                hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr,
+                                   h->rec.ptr);
             });
             if (L.ns)
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
@@ -1592,10 +1804,17 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                                            h->gradA.ptr);
                 });
         }
-        if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
+        if (h->strategy == ABL_FV_STRATEGY_TILE) {
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->tile_kernel, dim3((unsigned)h->tile_n), dim3(h->tile_T), h->tile_lds, st, v, h->ph,
+                                   U, h->gradC.ptr, h->gradA.ptr, h->rec.ptr, h->tile_start.ptr,
+                                   h->tile_faces.ptr,
+                                   h->tile_slot.ptr, h->tile_nfm, R, accumulate);
+            });
+        } else if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
             h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                 hipLaunchKernelGGL(h->face_flux_kernel, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, h->gradC.ptr,
-                                   h->gradA.ptr, h->face_flux_e.ptr, h->face_flux_s.ptr);
+                                   h->gradA.ptr, h->rec.ptr, h->face_flux_e.ptr, h->face_flux_s.ptr);
             });
             h->timed(ABL_FV_PHASE_GATHER, st, [&] {
                 hipLaunchKernelGGL(h->gather_kernel, grid, block, 0, st, v, L.ns, h->face_flux_e.ptr, h->face_flux_s.ptr,
@@ -1603,8 +1822,8 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             });
         } else {
             h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-                hipLaunchKernelGGL(h->flux_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr, R,
-                                   accumulate);
+                hipLaunchKernelGGL(h->flux_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr,
+                                   h->rec.ptr, R, accumulate);
             });
         }
         }
@@ -1619,8 +1838,12 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
 
 extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
     if (!h) return fail(h, "abl_fv_set_strategy: null handle");
-    if (strategy != ABL_FV_STRATEGY_RECOMPUTE && strategy != ABL_FV_STRATEGY_FACE_FLUX)
+    if (strategy != ABL_FV_STRATEGY_RECOMPUTE && strategy != ABL_FV_STRATEGY_FACE_FLUX &&
+        strategy != ABL_FV_STRATEGY_TILE)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
+    if (strategy == ABL_FV_STRATEGY_TILE && h->tile_n == 0)
+        return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (mixed form, or no tile "
+                       "size fits the LDS budget)");
     try {
         if (strategy == ABL_FV_STRATEGY_FACE_FLUX) h->ensure_face_flux();
     } catch (const std::exception& e) {

@@ -251,7 +251,8 @@ class FiniteVolumeSolver:
         return theta.reshape(16, 3)
 
     def set_strategy(self, strategy: str) -> None:
-        """"recompute" (each face flux evaluated by both cells) or "face_flux" (stored once, gathered per cell)."""
+        """"tile" (default: a workgroup per tile of cells evaluates the tile's faces into LDS and gathers its cells),
+        "face_flux" (stored once in HBM, gathered per cell) or "recompute" (each face flux evaluated by both cells)."""
         _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]), self._h, "abl_fv_set_strategy")
 
     def set_profiling(self, enable: bool) -> None:

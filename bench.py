@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (default: the workload's 256)")
-    ap.add_argument("--strategy", default="face_flux", choices=["recompute", "face_flux"])
+    ap.add_argument("--strategy", default="tile", choices=["recompute", "face_flux", "tile"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
@@ -74,7 +74,8 @@ def copy_probe(gib, reps=5):
             "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-KERNELS = {"face_flux": ["k_gradients", "k_face_flux", "k_gather"], "recompute": ["k_gradients", "k_flux"]}
+KERNELS = {"face_flux": ["k_gradients", "k_face_flux", "k_gather"], "recompute": ["k_gradients", "k_flux"],
+           "tile": ["k_gradients", "k_tile_flux"]}
 
 # HBM bytes per RHS from hardware counters (FETCH_SIZE + WRITE_SIZE summed over the RHS kernels of one step, each
 # scaled by the copy-kernel calibration factor), measured with the commands recorded in

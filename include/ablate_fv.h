@@ -182,11 +182,13 @@ int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_bad, abl_fv_
 /* ---- evaluation strategy (both deterministic, no float atomics) ----------------------------------------------- */
 enum {
     ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice */
-    ABL_FV_STRATEGY_FACE_FLUX = 2  /* one lane per face stores its flux once; a second pass gathers per cell
-                                      (default; holds 8 * (n_euler rounded up to even + n_species) bytes per face) */
+    ABL_FV_STRATEGY_FACE_FLUX = 2, /* one lane per face stores its flux once; a second pass gathers per cell
+                                      (holds 8 * (n_euler rounded up to even + n_species) bytes per face) */
+    ABL_FV_STRATEGY_TILE = 3       /* default: one workgroup per tile of consecutive cells evaluates the tile's faces
+                                      once into LDS and gathers its cells; bitwise equal to FACE_FLUX */
 };
 /* Applies to the FP64 form (the mixed form has one cell-centric strategy).  Selecting FACE_FLUX allocates its
- * face-flux storage here, never inside abl_fv_rhs. */
+ * face-flux storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
 int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);
 
 /* ---- per-phase timing (replaces the Loggable / PetscLogEvent brackets around each RHS sub-phase,

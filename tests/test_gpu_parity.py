@@ -9,9 +9,10 @@ pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-12
 CASES = standard_cases()
+STRATEGIES = ["recompute", "face_flux", "tile"]
 
 
-@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
+@pytest.mark.parametrize("strategy", STRATEGIES)
 @pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
 def test_rhs_matches_oracle(case, strategy, torch_cuda):
     torch = torch_cuda
@@ -36,7 +37,7 @@ def test_rhs_matches_oracle(case, strategy, torch_cuda):
 
 @pytest.mark.parametrize("case", [c for c in CASES if c.name in ("c2_ns_lsq_bj", "c5_tets_rusanov", "c4_species_inflow")],
                          ids=lambda c: c.name)
-@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
+@pytest.mark.parametrize("strategy", STRATEGIES)
 def test_rhs_deterministic_and_accumulates(case, strategy, torch_cuda):
     torch = torch_cuda
     solver = case.solver()
@@ -56,7 +57,7 @@ def test_rhs_deterministic_and_accumulates(case, strategy, torch_cuda):
     assert torch.equal(R3[:, case.mesh.n_interior :], base[:, case.mesh.n_interior :])
 
 
-@pytest.mark.parametrize("strategy", ["recompute", "face_flux"])
+@pytest.mark.parametrize("strategy", STRATEGIES)
 def test_conservation_periodic(strategy, torch_cuda):
     """sum_c V_c R_c = 0 on a periodic mesh (every face flux enters two cells with opposite signs)."""
     torch = torch_cuda
@@ -87,6 +88,22 @@ def test_strategies_agree_bitwise_per_face(torch_cuda):
     torch.cuda.synchronize()
     scale = Ra.abs().max(dim=1, keepdim=True).values
     assert torch.all((Ra - Rb).abs() <= 1e-14 * scale)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
+def test_tile_equals_face_flux_bitwise(case, torch_cuda):
+    """The tile strategy evaluates the same face fluxes (faces on tile borders twice) and gathers them in the same
+    order as the face-flux strategy, so the residuals are identical."""
+    torch = torch_cuda
+    solver = case.solver()
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Ra, Rb = torch.zeros_like(U), torch.zeros_like(U)
+    solver.set_strategy("face_flux")
+    solver.compute_rhs(0.0, U, Ra)
+    solver.set_strategy("tile")
+    solver.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    assert torch.equal(Ra, Rb), (Ra - Rb).abs().max().item()
 
 
 def test_pack_unpack_roundtrip(torch_cuda):
