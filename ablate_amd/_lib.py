@@ -98,6 +98,10 @@ SIGNATURES = {
     "abl_fv_pack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_unpack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_check_state": (c_int32, [c_void_p, c_void_p, POINTER(c_int64), c_void_p]),
+    "abl_fv_cfl_dt": (c_int32, [c_void_p, c_void_p, _dp, c_void_p]),
+    "abl_fv_axpy": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "abl_fv_ts_euler": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double, c_int64, _dp,
+                                  POINTER(c_int64), c_void_p]),
     "abl_fv_set_profiling": (c_int32, [c_void_p, c_int32]),
     "abl_fv_set_strategy": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_phase_times": (c_int32, [c_void_p, _dp, POINTER(c_int64), c_int32]),

@@ -1292,6 +1292,57 @@ __global__ __launch_bounds__(kBlock) void k_check_state(const double* __restrict
     if ((threadIdx.x & 63) == 0 && mask) atomicAdd(bad, (unsigned long long)__popcll(mask));
 }
 
+// NavierStokesTransport::ComputeCflTimeStep per interior cell (navierStokesTransport.cpp:246-281, pgsAlpha = 1):
+// dt = cfl * 2 r_min / (a + sum_d |rhoU_d| / rho), a through the PerfectGas T route (perfectGas.cpp:76-86,165-174),
+// reduced to one minimum per block (min is order-free, so the result does not depend on the launch shape; NaN
+// cells are skipped).
+__device__ inline double block_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    __shared__ double wmin[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = wmin[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmin(m, wmin[i]);
+    return m;
+}
+__global__ __launch_bounds__(kBlock) void k_cfl_partial(const double* __restrict__ U, const double* __restrict__ rmin,
+                                                        int64_t N, int64_t NI, int dim, Phys ph, double cfl,
+                                                        double* __restrict__ partial) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dt = 1e10;  // utilities::Constants::large, the reference's starting minimum
+    if (c < NI) {
+        const double rho = U[c];
+        double ke = 0.0, velSum = 0.0;
+        for (int d = 0; d < dim; d++) {
+            const double v = U[(kRhoU + d) * N + c] / rho;
+            ke += v * v;
+            velSum += fabs(U[(kRhoU + d) * N + c]) / rho;
+        }
+        ke *= 0.5;
+        const double T = (U[kRhoE * N + c] / rho - ke) / ph.cv;
+        const double p = ph.R * rho * T;
+        const double a = sqrt(ph.gamma * p / rho);
+        dt = cfl * (2.0 * rmin[c]) / (a / 1.0 + velSum);
+    }
+    const double m = block_min(dt);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+__global__ __launch_bounds__(kBlock) void k_min_final(const double* __restrict__ partial, int64_t n,
+                                                      double* __restrict__ out) {
+    double v = 1e10;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, partial[i]);
+    const double m = block_min(v);
+    if (threadIdx.x == 0) out[0] = m;
+}
+// Forward-Euler stage update of the interior rows in FP64: U[p][c] += dt * R[p][c] (blockIdx.y = component).
+__global__ __launch_bounds__(kBlock) void k_axpy(double dt, const double* __restrict__ R, double* __restrict__ U,
+                                                 int64_t N, int64_t NI) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= NI) return;
+    const int64_t i = (int64_t)blockIdx.y * N + c;
+    U[i] += dt * R[i];
+}
+
 // ============================================================================================ host side
 
 #define HIP_CHECK(x)                                                                                                 \
@@ -1394,6 +1445,7 @@ struct abl_fv_solver {
     abl::GatherKernel gather_kernel = nullptr;
     int strategy = ABL_FV_STRATEGY_TILE;
     abl::DevBuf<unsigned long long> counter;
+    abl::DevBuf<double> rmin, cfl_partial, cfl_out;  // MIN_CELL_RADIUS per interior cell; CFL reduction scratch
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
     std::vector<std::array<double, 3>> h_slot_theta;
@@ -1511,7 +1563,7 @@ struct abl_fv_solver {
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         gradA.bytes() + rec.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes() + gradC_f.bytes() + gradA_f.bytes());
     }
 };
@@ -1674,6 +1726,30 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         s->slot_kind.upload(s->h_slot_kind);
         s->slot_state.upload(s->h_slot_state);
         s->counter.alloc(1);
+        {
+            // MIN_CELL_RADIUS (finiteVolumeSolver.cpp:185-215): min over the cell's faces of |x_cell - x_face|
+            std::vector<double> r((size_t)L.NI, 0.0);
+            for (int64_t c = 0; c < L.NI; c++) {
+                bool first = true;
+                for (int k = 0; k < L.K; k++) {
+                    const int32_t e = L.ell[(size_t)k * L.NI + c];
+                    if (e < 0) break;
+                    const int64_t f = e >> 1;
+                    const int sd = e & 1;
+                    double mag = 0.0;
+                    for (int d = 0; d < L.dim; d++) {
+                        const double v = L.fdx[(size_t)2 * L.dim * f + (size_t)sd * L.dim + d];
+                        mag += v * v;
+                    }
+                    const double rad = std::sqrt(mag);
+                    r[(size_t)c] = first ? rad : std::min(r[(size_t)c], rad);
+                    first = false;
+                }
+            }
+            s->rmin.upload(r);
+            s->cfl_partial.alloc((size_t)std::max<int64_t>(grid_for(L.NI), 1));
+            s->cfl_out.alloc(1);
+        }
         if (form == ABL_FV_FORM_FP64) s->build_tiles();
         if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0) s->strategy = ABL_FV_STRATEGY_FACE_FLUX;
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
@@ -1948,6 +2024,84 @@ extern "C" int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos
     const abl::Layout& L = h->host;
     hipLaunchKernelGGL(abl::k_pack, dim3(grid_for(L.N * L.nc)), dim3(abl::kBlock), 0, (hipStream_t)stream, soa, aos, L.N, L.nc, 0);
     if (hipGetLastError() != hipSuccess) return fail(h, "abl_fv_unpack_aos: launch failed");
+    return 0;
+}
+
+namespace {
+// CFL step size of state U (device), returned on the host (synchronises the stream).
+double cfl_dt_impl(abl_fv_handle h, const double* U, hipStream_t st) {
+    const abl::Layout& L = h->host;
+    const unsigned nb = grid_for(L.NI);
+    hipLaunchKernelGGL(abl::k_cfl_partial, dim3(nb), dim3(abl::kBlock), 0, st, U, h->rmin.ptr, L.N, L.NI, L.dim, h->ph,
+                       h->phys.cfl, h->cfl_partial.ptr);
+    hipLaunchKernelGGL(abl::k_min_final, dim3(1), dim3(abl::kBlock), 0, st, h->cfl_partial.ptr, (int64_t)nb,
+                       h->cfl_out.ptr);
+    HIP_CHECK(hipGetLastError());
+    double dt = 0.0;
+    HIP_CHECK(hipMemcpyAsync(&dt, h->cfl_out.ptr, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return dt;
+}
+void axpy_impl(abl_fv_handle h, double dt, const double* R, double* U, hipStream_t st) {
+    const abl::Layout& L = h->host;
+    hipLaunchKernelGGL(abl::k_axpy, dim3(grid_for(L.NI), (unsigned)L.nc), dim3(abl::kBlock), 0, st, dt, R, U, L.N, L.NI);
+    HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+extern "C" int abl_fv_cfl_dt(abl_fv_handle h, const double* U, double* dt, abl_fv_stream stream) {
+    if (!h || !U || !dt) return fail(h, "abl_fv_cfl_dt: null argument");
+    if (!(h->phys.cfl > 0.0)) return fail(h, "abl_fv_cfl_dt: the physics descriptor has no positive cfl");
+    try {
+        *dt = cfl_dt_impl(h, U, (hipStream_t)stream);
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_cfl_dt: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* U, abl_fv_stream stream) {
+    if (!h || !R || !U) return fail(h, "abl_fv_axpy: null argument");
+    try {
+        axpy_impl(h, dt, R, U, (hipStream_t)stream);
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_axpy: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max_time, double dt0,
+                               double safety, int64_t max_steps, double* t_out, int64_t* steps_out,
+                               abl_fv_stream stream) {
+    if (!h || !U || !R) return fail(h, "abl_fv_ts_euler: null argument");
+    if (!(h->phys.cfl > 0.0) || !(safety > 0.0) || !(dt0 > 0.0))
+        return fail(h, "abl_fv_ts_euler: cfl, safety and dt0 must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    double t = t0;
+    int64_t steps = 0;
+    try {
+        double dt = std::min(dt0, safety * cfl_dt_impl(h, U, st));  // AdaptPhysics::AdaptInitializer
+        while (t < max_time && steps < max_steps) {
+            if (!(dt > 0.0)) throw std::runtime_error("non-positive or NaN step size at t = " + std::to_string(t));
+            if (int rc = abl_fv_rhs(h, t, U, R, 0, stream)) return rc;
+            // TSStep_Euler asks the adaptor for the next step while the solution still holds U_n
+            double next = safety * cfl_dt_impl(h, U, st);
+            axpy_impl(h, dt, R, U, st);
+            const double t_end = t + dt, remaining = max_time - t_end;
+            // TSAdaptChoose with ts_exact_final_time matchstep (matchstep factors 0.01, 2)
+            if (t_end < max_time && t_end + next > max_time) next = remaining;
+            if (t_end < max_time && t_end + next < max_time && next * 2.0 > remaining) next = remaining / 2.0;
+            if (t_end < max_time && t_end + next < max_time && next * 1.01 > remaining) next = remaining;
+            t = t_end;
+            dt = next;
+            steps++;
+        }
+        HIP_CHECK(hipStreamSynchronize(st));
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_ts_euler: ") + e.what());
+    }
+    if (t_out) *t_out = t;
+    if (steps_out) *steps_out = steps;
     return 0;
 }
 

@@ -267,6 +267,39 @@ class FiniteVolumeSolver:
                    "abl_fv_get_phase_times")
         return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.PHASES)}
 
+    def compute_cfl_dt(self, U, stream=None) -> float:
+        """NavierStokesTransport::ComputeCflTimeStep on the device state U (host float; synchronises)."""
+        self._check_tensor(U, "U")
+        dt = ctypes.c_double()
+        _lib.check(_lib.lib().abl_fv_cfl_dt(self._h, ctypes.c_void_p(U.data_ptr()), ctypes.byref(dt),
+                                            self._stream_ptr(stream)), self._h, "abl_fv_cfl_dt")
+        return float(dt.value)
+
+    def axpy(self, dt: float, R, U, stream=None) -> None:
+        """Forward-Euler stage on the interior rows: U += dt * R (FP64, on the device)."""
+        self._check_tensor(U, "U")
+        self._check_tensor(R, "R")
+        _lib.check(_lib.lib().abl_fv_axpy(self._h, float(dt), ctypes.c_void_p(R.data_ptr()),
+                                          ctypes.c_void_p(U.data_ptr()), self._stream_ptr(stream)), self._h,
+                   "abl_fv_axpy")
+
+    def ts_euler(self, U, max_time: float, t0: float = 0.0, dt0: float = 0.1, safety: float = 1.0,
+                 max_steps: int = 10**9, R=None, stream=None):
+        """Forward-Euler integration of U in place to max_time under the CFL step-size adaptor with matchstep
+        (the reference TimeStepper with ts_adapt_type physics).  Returns (final time, steps)."""
+        import torch
+
+        self._check_tensor(U, "U")
+        R = torch.empty_like(U) if R is None else R
+        self._check_tensor(R, "R")
+        t = ctypes.c_double()
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().abl_fv_ts_euler(self._h, ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(R.data_ptr()),
+                                              float(t0), float(max_time), float(dt0), float(safety), int(max_steps),
+                                              ctypes.byref(t), ctypes.byref(n), self._stream_ptr(stream)),
+                   self._h, "abl_fv_ts_euler")
+        return float(t.value), int(n.value)
+
     def check_state(self, U, stream=None) -> int:
         self._check_tensor(U, "U")
         n = ctypes.c_int64()

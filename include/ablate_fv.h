@@ -179,6 +179,24 @@ int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_st
  * (mirrors the verboseSourceCheck scan, timeStepper.cpp:385-389). */
 int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_bad, abl_fv_stream stream);
 
+/* ---- time step (the reference's CFL step size and forward-Euler stage, on the device) ------------------------- */
+
+/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285, pgsAlpha = 1): the minimum over
+ * interior cells of cfl * 2 r_min / (a + sum_d |rhoU_d / rho|), r_min the MIN_CELL_RADIUS mesh characteristic
+ * (finiteVolumeSolver.cpp:185-215), cfl from the physics descriptor.  *dt is a host value (synchronises the
+ * stream).  Cells with a NaN step are skipped. */
+int abl_fv_cfl_dt(abl_fv_handle h, const double* U, double* dt, abl_fv_stream stream);
+/* Forward-Euler stage in FP64 on the interior rows: U += dt * R (ghost rows untouched). */
+int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* U, abl_fv_stream stream);
+/* Integrates U from t0 to max_time with forward Euler as the reference's TimeStepper runs it under
+ * ts_adapt_type physics (safety factor `safety`), ts_exact_final_time matchstep and PETSc's TSEULER step
+ * (timeStepper.cpp:80-130, adaptPhysics.cpp:3-59): first step min(dt0, safety * cfl_dt(U0)); each step evaluates
+ * the RHS into the caller's workspace R [n_comp][n_cells], takes the next step size from U_n, updates U; steps
+ * that would pass max_time are cut to it.  Stops at max_time or after max_steps; reports the final time and
+ * the number of steps taken. */
+int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max_time, double dt0, double safety,
+                    int64_t max_steps, double* t_out, int64_t* steps_out, abl_fv_stream stream);
+
 /* ---- evaluation strategy (both deterministic, no float atomics) ----------------------------------------------- */
 enum {
     ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice */

@@ -107,6 +107,10 @@ def lib() -> ctypes.CDLL:
         L.orc_destroy.restype = None
         L.orc_cell_gradients.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp]
         L.orc_cell_gradients.restype = c_int
+        L.orc_min_cell_radius.argtypes = [POINTER(OrcMesh), _dp]
+        L.orc_min_cell_radius.restype = None
+        L.orc_cfl_dt.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), d, _dp, _dp]
+        L.orc_cfl_dt.restype = d
         _lib = L
     return _lib
 
@@ -281,3 +285,54 @@ def pg(prop: str, euler, gamma=1.4, R=287.0, T=None):
     if prop == "density":
         return float(u[0])
     raise KeyError(prop)
+
+
+def min_cell_radius(mesh_arrays) -> np.ndarray:
+    """MIN_CELL_RADIUS mesh characteristic per cell (finiteVolumeSolver.cpp:185-215)."""
+    ms, keep = _mesh_struct(mesh_arrays)
+    r = np.zeros(mesh_arrays.n_cells)
+    lib().orc_min_cell_radius(ctypes.byref(ms), _ptr(r))
+    return r
+
+
+def cfl_dt(mesh_arrays, phys: OrcPhysics, cfl: float, r: np.ndarray, U_soa: np.ndarray) -> float:
+    """NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285) on a SoA state."""
+    ms, keep = _mesh_struct(mesh_arrays)
+    X = np.ascontiguousarray(U_soa.T)
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    return float(lib().orc_cfl_dt(ctypes.byref(ms), ctypes.byref(phys), cfl, _ptr(rr), _ptr(X)))
+
+
+def petsc_euler_solve(rhs_fn, dt_fn, U0: np.ndarray, max_time: float, interior: int, dt0: float = 0.1,
+                      safety: float = 1.0, matchstepfac=(0.01, 2.0), max_steps: int = 1000000,
+                      adapt_on_updated_state: bool = False):
+    """Forward-Euler time loop of the reference's TimeStepper with ts_adapt_type physics and
+    ts_exact_final_time matchstep (timeStepper.cpp:80-130, adaptPhysics.cpp:3-59), as PETSc runs it:
+      - the first step is min(dt0, safety * dt(U0)) (AdaptPhysics::AdaptInitializer);
+      - each step U += dt * rhs(t, U); the adaptor then picks the next step from the state BEFORE the update
+        (TSStep_Euler evaluates the step size while the solution vector still holds U_n);
+      - matchstep: a next step that would pass max_time is cut to the remaining time; one that would leave less
+        than matchstepfac[1] steps is halved, and within (1 + matchstepfac[0]) of the remainder it is stretched.
+    adapt_on_updated_state=True is a deliberately different protocol (step size from U_{n+1}) for tests that show
+    the reference tables tell the two apart.  Returns (U, t, steps)."""
+    U = U0.copy()
+    t = 0.0
+    dt = min(dt0, safety * dt_fn(U))
+    steps = 0
+    while t < max_time and steps < max_steps:
+        R = rhs_fn(t, U)
+        h = safety * dt_fn(U)
+        U[:, :interior] = U[:, :interior] + dt * R[:, :interior]
+        if adapt_on_updated_state:
+            h = safety * dt_fn(U)
+        t_end = t + dt
+        remaining = max_time - t_end
+        if t_end < max_time and t_end + h > max_time:
+            h = remaining
+        if t_end < max_time and t_end + h < max_time and h * matchstepfac[1] > remaining:
+            h = remaining / 2
+        if t_end < max_time and t_end + h < max_time and h * (1.0 + matchstepfac[0]) > remaining:
+            h = remaining
+        t, dt = t_end, h
+        steps += 1
+    return U, t, steps

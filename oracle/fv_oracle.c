@@ -950,3 +950,43 @@ int orc_rhs(const orc_mesh* mesh, const orc_physics* ph, const orc_bc* bcs, int 
     orc_destroy(ctx);
     return rc;
 }
+
+void orc_min_cell_radius(const orc_mesh* m, double* r) {
+    const int dim = m->dim;
+    for (int64_t c = 0; c < m->n_cells; ++c) r[c] = NAN;
+    for (int64_t f = 0; f < m->n_faces; ++f) {
+        for (int s = 0; s < 2; ++s) {
+            const int64_t c = m->face_cells[2 * f + s];
+            /* face centroid in this cell's frame (periodic images carry the face shift on the right side) */
+            double v[3];
+            for (int d = 0; d < dim; ++d) {
+                const double xc = m->cell_centroid[3 * c + d] + ((s == 1 && m->face_shift) ? m->face_shift[3 * f + d] : 0.0);
+                v[d] = xc - m->face_centroid[3 * f + d];
+            }
+            double mag = 0.0;
+            for (int d = 0; d < dim; ++d) mag += v[d] * v[d];
+            const double radius = sqrt(mag);
+            if (isnan(r[c]))
+                r[c] = radius;
+            else
+                r[c] = radius < r[c] ? radius : r[c];
+        }
+    }
+}
+
+double orc_cfl_dt(const orc_mesh* m, const orc_physics* ph, double cfl, const double* r, const double* X) {
+    const int dim = m->dim, nc = dim + 2 + ph->n_species;
+    double dtMin = 1e10; /* utilities::Constants::large (constants.hpp:31) */
+    for (int64_t c = 0; c < m->n_interior; ++c) {
+        const double* euler = X + c * nc;
+        const double rho = euler[0];
+        const double T = orc_pg_temperature(euler, dim, ph->gamma, ph->Rgas);
+        const double a = orc_pg_speed_of_sound_T(euler, T, ph->gamma, ph->Rgas);
+        const double dx = 2.0 * r[c];
+        double velSum = 0.0;
+        for (int d = 0; d < dim; d++) velSum += fabs(euler[2 + d]) / rho;
+        const double dt = cfl * dx / (a / 1.0 + velSum);
+        dtMin = dt < dtMin ? dt : dtMin;
+    }
+    return dtMin;
+}

@@ -103,6 +103,15 @@ void orc_destroy(orc_ctx* ctx);
  * [n_cells][totDim][dim] (zero for ghost cells). For tests of the reconstruction stage. */
 int orc_cell_gradients(const orc_mesh* mesh, const orc_physics* phys, const double* locX, double* grad);
 
+/* Minimum cell radius per cell: min over the cell's faces of |x_cell - x_face| (the MIN_CELL_RADIUS mesh
+ * characteristic, finiteVolumeSolver.cpp:185-215; every face with two supporting cells counts).  r[n_cells]. */
+void orc_min_cell_radius(const orc_mesh* mesh, double* r);
+
+/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285) with pgsAlpha = 1:
+ * min over interior cells of cfl * 2 r_min / (a + sum_d |rhoU_d / rho|), a from the PerfectGas T route.
+ * locX is AoS [n_cells][dim+2+ns]. */
+double orc_cfl_dt(const orc_mesh* mesh, const orc_physics* phys, double cfl, const double* r, const double* locX);
+
 #ifdef __cplusplus
 }
 #endif

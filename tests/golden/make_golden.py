@@ -6,6 +6,8 @@ Run in a container that has the ABLATE source tree (default /root/reference); th
   ns_advection_flux.json  navierStokesTransportTests.cpp: AdvectionFlux cases
   stress_tensor.json      navierStokesTransportTests.cpp: CompressibleFlowComputeStressTensor cases
   perfect_gas.json        perfectGasTests.cpp: PerfectGas thermodynamic property cases
+  shock_tube.json         compressibleShockTubeTests.cpp: first-order AUSM shock-tube cases (forward Euler, CFL
+                          time step) with the reference's x, rho, u, e profiles
 Each record keeps the source file and line of its table so that a reader can find it.
 """
 from __future__ import annotations
@@ -139,7 +141,39 @@ def perfect_gas():
     return out
 
 
+def shock_tube():
+    path = UT / "finiteVolume" / "compressibleShockTubeTests.cpp"
+    text = path.read_text()
+    out = []
+    for m in re.finditer(r"\(CompressibleShockTubeParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        ic_m = re.search(r"\.initialConditions\s*=\s*\{", block)
+        ic_block, _ = braces(block, ic_m.end() - 1)
+        ic = {k: num(v) for k, v in re.findall(r"\.(\w+)\s*=\s*([^,}]+)", ic_block)}
+        ev_m = re.search(r"\.expectedValues\s*=\s*\{", block)
+        ev_block, _ = braces(block, ev_m.end() - 1)
+        expected = {}
+        for vm in re.finditer(r'\{"(\w+)",\s*\{', ev_block):
+            inner, _ = braces(ev_block, vm.end() - 1)
+            expected[vm.group(1)] = [num(x) for x in inner.replace("\n", " ").split(",") if x.strip()]
+        out.append({
+            "source": f"tests/unitTests/finiteVolume/compressibleShockTubeTests.cpp:{line_of(text, m.start())}",
+            "name": re.search(r'MpiTestParameter\("([^"]+)"\)', block).group(1),
+            "fluxCalculator": re.search(r"fluxCalculator::(\w+)>", block).group(1),
+            "initialConditions": ic,
+            "nx": int(re.search(r"\.nx\s*=\s*(\d+)", block).group(1)),
+            "maxTime": num(re.search(r"\.maxTime\s*=\s*([0-9.eE+\-]+)", block).group(1)),
+            "cfl": num(re.search(r"\.cfl\s*=\s*([0-9.eE+\-]+)", block).group(1)),
+            "expectedValues": expected,
+            "tolerance": 1e-6,
+            "timeStepper": {"ts_adapt_type": "physics", "ts_adapt_safety": 1.0, "ts_exact_final_time": "matchstep"},
+        })
+    return out
+
+
 def main():
+    st = shock_tube()
+    (OUT / "shock_tube.json").write_text(json.dumps(st, indent=1))
     fc = flux_calculators()
     adv, stress = ns_cases()
     pgt = perfect_gas()
@@ -147,7 +181,7 @@ def main():
     (OUT / "ns_advection_flux.json").write_text(json.dumps(adv, indent=1))
     (OUT / "stress_tensor.json").write_text(json.dumps(stress, indent=1))
     (OUT / "perfect_gas.json").write_text(json.dumps(pgt, indent=1, allow_nan=True))
-    print({k: len(v["uL"]) for k, v in fc.items()}, len(adv), len(stress), len(pgt))
+    print({k: len(v["uL"]) for k, v in fc.items()}, len(adv), len(stress), len(pgt), len(st))
 
 
 if __name__ == "__main__":

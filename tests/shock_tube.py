@@ -1,0 +1,61 @@
+"""The reference's 1-D shock-tube cases (compressibleShockTubeTests.cpp:209-314) on this build's meshes.
+
+The reference runs a 1-D DMPlex box of nx cells with ghost cells at both ends, Ghost boundary conditions that
+hold the initial left / right states, PerfectGas (gamma, default Rgas), AUSM, first order, forward Euler with the
+physics (CFL) step-size adaptor.  Here the line is an nx x 1 quad strip of square cells: x- / x+ boundary slots
+(0, 1) hold the left / right states as constant ghost states, the y boundaries (slots 2, 3) are zero-gradient
+ghosts, whose pressure fluxes cancel exactly, so every cell sees the 1-D problem.
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "shock_tube.json"
+RGAS = 287.0  # PerfectGas default (perfectGas.cpp:4-7); the cases set gamma only
+
+
+def cases():
+    return json.loads(GOLDEN.read_text())
+
+
+def setup(case, ordering="natural"):
+    """Returns (mesh, U0 [4, N], boundary-condition dict slot -> state) for one golden case."""
+    from ablate_amd.mesh import BoxMesh
+
+    ic = case["initialConditions"]
+    g, L, nx = ic["gamma"], ic["length"], case["nx"]
+    h = L / nx
+    mesh = BoxMesh([nx, 1], [0.0, 0.0], [L, h], [False, False], ordering=ordering)
+    a = mesh.arrays
+    NI = a.n_interior
+    x = a.cell_centroid[:NI, 0]
+    left = x < L / 2.0
+    rho = np.where(left, ic["rhoL"], ic["rhoR"])
+    u = np.where(left, ic["uL"], ic["uR"])
+    p = np.where(left, ic["pL"], ic["pR"])
+    U = np.zeros((4, a.n_cells))
+    U[0, :NI] = rho
+    U[2, :NI] = rho * u
+    U[1, :NI] = rho * (p / ((g - 1.0) * rho) + 0.5 * u * u)  # SetInitialCondition, et = e + u^2/2
+    bcs = {
+        0: dict(kind="inflow", rho=ic["rhoL"], vel=[ic["uL"], 0.0, 0.0], T=ic["pL"] / (ic["rhoL"] * RGAS)),
+        1: dict(kind="inflow", rho=ic["rhoR"], vel=[ic["uR"], 0.0, 0.0], T=ic["pR"] / (ic["rhoR"] * RGAS)),
+        2: dict(kind="outflow"),
+        3: dict(kind="outflow"),
+    }
+    return mesh, U, bcs
+
+
+def primitives(mesh, U):
+    """Extract1DPrimitives (compressibleShockTubeTests.cpp:73-107): x, rho, u, e = rhoE/rho - u^2/2."""
+    NI = mesh.n_interior
+    rho = U[0, :NI]
+    u = U[2, :NI] / rho
+    return {"x": mesh.arrays.cell_centroid[:NI, 0], "rho": rho, "u": u, "e": U[1, :NI] / rho - 0.5 * u * u}
+
+
+def max_abs_error(case, prim):
+    return {k: float(np.max(np.abs(np.asarray(v) - prim[k]))) for k, v in case["expectedValues"].items()}
