@@ -88,6 +88,54 @@ struct CellRec {
     static_assert(END <= RS && RS % 2 == 0, "record layout");
 };
 
+// Cell record of the mixed form: [0, NE*DIM) FP32 limited gradients, [TF, TF + NT) FP32 viscous terms, then the
+// FP64 state at float offset U0 (16-byte aligned); RS floats per cell (144 bytes in 3-D).
+template <int DIM>
+struct CellRecF {
+    static constexpr int NE = DIM + 2;
+    static constexpr int TF = NE * DIM;
+    static constexpr int U0 = ((TF + ViscTerms<DIM>::NT) + 3) & ~3;
+    static constexpr int RS = (U0 + 2 * NE + 3) & ~3;
+};
+
+template <int DIM, bool RECON, bool VISC>
+struct RecF {
+    double u[DIM + 2];
+    float g[RECON ? (DIM + 2) * DIM : 1];
+    float tq[VISC ? ViscTerms<DIM>::NT : 1];
+};
+// Loads a mixed-form cell record (FP64 state, FP32 gradients and viscous terms).
+template <int DIM, bool RECON, bool VISC>
+__device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF<DIM, RECON, VISC>& r) {
+    using CR = CellRecF<DIM>;
+    constexpr int NE = DIM + 2;
+    const float* base = rec + c * CR::RS;
+    constexpr int NF = VISC ? CR::TF + ViscTerms<DIM>::NT : (RECON ? CR::TF : 0);
+    float t[NF > 0 ? ((NF + 3) & ~3) : 4];
+#pragma unroll
+    for (int i = 0; i < (NF + 3) / 4; i++) {
+        const float4 v = reinterpret_cast<const float4*>(base)[i];
+        t[4 * i] = v.x;
+        t[4 * i + 1] = v.y;
+        t[4 * i + 2] = v.z;
+        t[4 * i + 3] = v.w;
+    }
+    if (RECON)
+#pragma unroll
+        for (int i = 0; i < NE * DIM; i++) r.g[i] = t[i];
+    if (VISC)
+#pragma unroll
+        for (int q = 0; q < ViscTerms<DIM>::NT; q++) r.tq[q] = t[CR::TF + q];
+    const double* ud = reinterpret_cast<const double*>(base + CR::U0);
+#pragma unroll
+    for (int i = 0; i < NE / 2; i++) {
+        const double2 v = reinterpret_cast<const double2*>(ud)[i];
+        r.u[2 * i] = v.x;
+        r.u[2 * i + 1] = v.y;
+    }
+    if (NE & 1) r.u[NE - 1] = ud[NE - 1];
+}
+
 // Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
 // G is the storage/arithmetic type of weights and gradients (double: FP64 form, float: mixed form); state
 // differences are always taken in FP64 before rounding to G.  The FP64 form writes the cell record `rec`; the
@@ -95,7 +143,7 @@ struct CellRec {
 template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
 __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
                                                       G* __restrict__ gradC, G* __restrict__ gradA,
-                                                      double* __restrict__ rec) {
+                                                      void* __restrict__ rec) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
     const int64_t c = xcd_cell_index();
@@ -252,19 +300,33 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++)
             if (VISC) r[CR::T0 + q] = double(tq[q]);
-        double2* out = reinterpret_cast<double2*>(rec + c * CR::RS);
+        double2* out = reinterpret_cast<double2*>(reinterpret_cast<double*>(rec) + c * CR::RS);
 #pragma unroll
         for (int i = 0; i < CR::RS / 2; i++) out[i] = make_double2(r[2 * i], r[2 * i + 1]);
         return;
     }
-    if (RECON)
+    {
+        using CR = CellRecF<DIM>;
+        float t[CR::U0];
+#pragma unroll
+        for (int i = 0; i < CR::U0; i++) t[i] = 0.f;
 #pragma unroll
         for (int p = 0; p < NE; p++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradC[(p * DIM + d) * N + c] = g[p][d];
-    if (VISC)
+            for (int d = 0; d < DIM; d++)
+                if (RECON) t[p * DIM + d] = float(g[p][d]);
 #pragma unroll
-        for (int q = 0; q < ViscTerms<DIM>::NT; q++) gradA[q * N + c] = tq[q];
+        for (int q = 0; q < ViscTerms<DIM>::NT; q++)
+            if (VISC) t[CR::TF + q] = float(tq[q]);
+        float* base = reinterpret_cast<float*>(rec) + c * CR::RS;
+#pragma unroll
+        for (int i = 0; i < CR::U0 / 4; i++)
+            reinterpret_cast<float4*>(base)[i] = make_float4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
+        double* ud = reinterpret_cast<double*>(base + CR::U0);
+#pragma unroll
+        for (int i = 0; i < NE / 2; i++) reinterpret_cast<double2*>(ud)[i] = make_double2(uc[2 * i], uc[2 * i + 1]);
+        if (NE & 1) ud[NE - 1] = uc[NE - 1];
+    }
 }
 
 // densityYi gradients (limited, one field) and Yi aux gradients (unlimited).
@@ -797,7 +859,7 @@ struct BaselineF {  // FP32 copies of the baseline for products with perturbatio
 
 template <int DIM>
 struct SidePert {
-    float drho, rho, du[DIM], dT, dp, a, da, dM, M, dH;
+    float drho, rho, du[DIM], dun, dT, dp, a, da, dM, M, dH;
 };
 
 // Perturbations of the primitive variables of a face side from its conserved perturbation dU.
@@ -821,26 +883,46 @@ __device__ inline void side_perturbation(const float* dU, const BaselineF<DIM>& 
     sp.dp = R * (sp.rho * sp.dT + b.T * sp.drho);
     sp.a = sqrtf(gR * (b.T + sp.dT));
     sp.da = gR * sp.dT / (sp.a + b.a);
+    sp.dun = dun;
     sp.dM = (dun * b.a - unc * sp.da) / (sp.a * b.a);
     sp.M = Mc + sp.dM;
     sp.dH = deps + R * sp.dT;
 }
 
+// Mixed-form flux of face f with both baselines: dFL = F - F(U_L).n A and dFR = F - F(U_R).n A (undefined for a
+// ghost right cell).  The face flux is formed as a perturbation of the left cell's state; the right baseline
+// differs by the physical-flux perturbation of the cell difference U_R - U_L, also formed in FP32 from exact
+// identities (or in FP64 on the FP64 route).
 template <int DIM, bool RECON, bool VISC>
-__global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
-                                                       const float* __restrict__ gradC, const float* __restrict__ gradA,
-                                                       double* __restrict__ R, int accumulate) {
+__device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, const double* __restrict__ U,
+                                  const float* __restrict__ rec, int64_t f, float* dFL, float* dFR) {
     constexpr int NE = DIM + 2;
     constexpr int NT = ViscTerms<DIM>::NT;
-    const int64_t c = xcd_cell_index();
-    if (c >= m.NI) return;
+    constexpr bool USE_REC = RECON || VISC;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    const float* __restrict__ wa_arr = m.fwa;  // present whenever ghosts meet diffusion
-    double uc[NE];
+    const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+    const bool ghost = Rc >= NI;
+    float nh[3], area, x[2][DIM];
+    load_face_na(m.fna, f, nh, area);
+    load_face_pair<DIM, float>(m.fdx, f, x);
+    float nhat[DIM];
 #pragma unroll
-    for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
+    for (int d = 0; d < DIM; d++) nhat[d] = nh[d];
+    RecF<DIM, RECON, VISC> rl, rr;
+    if (USE_REC) {
+        load_rec_f<DIM, RECON, VISC>(rec, L, rl);
+    } else {
+#pragma unroll
+        for (int p = 0; p < NE; p++) rl.u[p] = U[p * N + L];
+    }
+    if (USE_REC && !ghost) {
+        load_rec_f<DIM, RECON, VISC>(rec, Rc, rr);
+    } else {
+#pragma unroll
+        for (int p = 0; p < NE; p++) rr.u[p] = U[p * N + Rc];
+    }
     Baseline<DIM> b;
-    decode_baseline<DIM>(uc, ph, b);
+    decode_baseline<DIM>(rl.u, ph, b);
     BaselineF<DIM> bf;
     bf.rho = float(b.rho);
     bf.eps = float(b.eps);
@@ -851,149 +933,231 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
 #pragma unroll
     for (int d = 0; d < DIM; d++) bf.vel[d] = float(b.vel[d]);
     const float gR = float(ph.gamma * ph.R), Rg = float(ph.R), icv = float(ph.icv);
-    float tqc[VISC ? NT : 1];
-    if (VISC)
+    // conserved perturbations relative to the left cell: the cell difference and both face states
+    float dcell[NE], dL[NE], dR[NE];
 #pragma unroll
-        for (int q = 0; q < NT; q++) tqc[q] = gradA[q * N + c];
+    for (int p = 0; p < NE; p++) {
+        dcell[p] = float(rr.u[p] - rl.u[p]);
+        float a = 0.f, bb = 0.f;
+        if (RECON)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                a += rl.g[p * DIM + d] * x[0][d];
+                if (!ghost) bb += rr.g[p * DIM + d] * x[1][d];
+            }
+        dL[p] = a;
+        dR[p] = dcell[p] + bb;
+    }
+    float unc = 0.f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) unc += bf.vel[d] * nhat[d];
+    const float Mc = unc / bf.a;
+    SidePert<DIM> sL, sR;
+    side_perturbation<DIM>(dL, bf, nhat, unc, Mc, gR, Rg, icv, sL);
+    side_perturbation<DIM>(dR, bf, nhat, unc, Mc, gR, Rg, icv, sR);
+    float dF[NE], dPhi[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        dF[p] = 0.f;
+        dPhi[p] = 0.f;
+    }
+    const bool subsonic = fabsf(sL.M) <= 1.f && fabsf(sR.M) <= 1.f && fabsf(Mc) <= 1.f;
+    if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
+        const float dm = 0.25f * sL.dM * (sL.M + Mc + 2.f) - 0.25f * sR.dM * (sR.M + Mc - 2.f);
+        const float mm = Mc + dm;
+        const SidePert<DIM>& up = mm < 0.f ? sR : sL;
+        const float mdot_c = unc * bf.rho;
+        const float dmdot = dm * up.a * up.rho + Mc * (up.da * up.rho + bf.a * up.drho);
+        const float mdot = mdot_c + dmdot;
+        const float PpL = 0.25f * (sL.M + 1.f) * (sL.M + 1.f) * (2.f - sL.M);
+        const float PmR = 0.25f * (sR.M - 1.f) * (sR.M - 1.f) * (2.f + sR.M);
+        const float dPpL = 0.25f * sL.dM * (3.f - (sL.M * sL.M + sL.M * Mc + Mc * Mc));
+        const float dPpR = 0.25f * sR.dM * (3.f - (sR.M * sR.M + sR.M * Mc + Mc * Mc));
+        const float dp12 = PpL * sL.dp + PmR * sR.dp + bf.p * (dPpL - dPpR);
+        dF[kRho] = dmdot;
+        dF[kRhoE] = bf.H * dmdot + up.dH * mdot;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) dF[kRhoU + d] = bf.vel[d] * dmdot + up.du[d] * mdot + dp12 * nhat[d];
+#pragma unroll
+        for (int p = 0; p < NE; p++) dF[p] *= area;
+        if (!ghost) {
+            // F(U_R).n - F(U_L).n from the cell difference: m = rho u_n, F = (m, m H, m u + p n)
+            SidePert<DIM> sC;
+            side_perturbation<DIM>(dcell, bf, nhat, unc, Mc, gR, Rg, icv, sC);
+            const float dmc = sC.drho * unc + sC.rho * sC.dun;
+            const float mcp = mdot_c + dmc;
+            dPhi[kRho] = dmc;
+            dPhi[kRhoE] = dmc * bf.H + mcp * sC.dH;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) dPhi[kRhoU + d] = dmc * bf.vel[d] + mcp * sC.du[d] + sC.dp * nhat[d];
+#pragma unroll
+            for (int p = 0; p < NE; p++) dPhi[p] *= area;
+        }
+    } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF) {
+        // FP64 route: full face flux of the reconstructed states minus the FP64 baselines
+        double uLd[NE], uRd[NE], nhd[DIM], fl[NE];
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            uLd[p] = rl.u[p] + double(dL[p]);
+            uRd[p] = rl.u[p] + double(dR[p]);
+            fl[p] = 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; d++) nhd[d] = double(nhat[d]);
+        const double ad = double(area);
+        if (ph.flux == ABL_FV_FLUX_RUSANOV) {
+            double unL, unR, lam;
+            rusanov_flux<double, DIM>(uLd, uRd, nhd, ad, ph, fl, unL, unR, lam);
+        } else {
+            double mA, rlo, rro;
+            int right;
+            advection_flux<double, DIM>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rlo, rro);
+        }
+        double und = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) und += b.vel[d] * nhd[d];
+        const double mc = b.rho * und * ad;
+        double base[NE];
+        base[kRho] = mc;
+        base[kRhoE] = b.H * mc;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) base[kRhoU + d] = b.vel[d] * mc + b.p * ad * nhd[d];
+#pragma unroll
+        for (int p = 0; p < NE; p++) dF[p] = float(fl[p] - base[p]);
+        if (!ghost) {
+            Baseline<DIM> br;
+            decode_baseline<DIM>(rr.u, ph, br);
+            double unr = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) unr += br.vel[d] * nhd[d];
+            const double mr = br.rho * unr * ad;
+            dPhi[kRho] = float(mr - base[kRho]);
+            dPhi[kRhoE] = float(br.H * mr - base[kRhoE]);
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                dPhi[kRhoU + d] = float(br.vel[d] * mr + br.p * ad * nhd[d] - base[kRhoU + d]);
+        }
+    }
+    if (VISC) {
+        // face-averaged viscous terms and face velocity (cell values) in FP32; a full flux, shared by both sides
+        float tq[NT], vel[DIM], normal[DIM];
+        const float irn = 1.0f / (bf.rho + dcell[kRho]);
+        if (ghost) {
+            float w[2][DIM], gA[DIM + 1][DIM];
+            load_face_pair<DIM, float>(m.fwa, f, w);
+            float dvel[DIM], dke = 0.f, dd = 0.f;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                dvel[d] = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
+                dke += bf.vel[d] * dvel[d];
+                dd += dvel[d] * dvel[d];
+            }
+            const float dT = ((dcell[kRhoE] - bf.eps * dcell[kRho]) * irn - (dke + 0.5f * dd)) * icv;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                gA[0][d] = -w[1][d] * dT;
+#pragma unroll
+                for (int q = 0; q < DIM; q++) gA[1 + q][d] = -w[1][d] * dvel[q];
+            }
+            stress_heat<float, DIM>(float(ph.mu), float(ph.k), gA, tq);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NT; q++) tq[q] = rr.tq[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) tq[q] = 0.5f * rl.tq[q] + 0.5f * tq[q];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float dv = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
+            vel[d] = bf.vel[d] + 0.5f * dv;
+            normal[d] = area * nhat[d];
+        }
+        viscous_flux_add<float, DIM>(normal, vel, tq, dF);
+    }
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        dFL[p] = dF[p];
+        dFR[p] = dF[p] - dPhi[p];
+    }
+}
+
+// Mixed form, cell-centric (strategy "recompute"): each cell evaluates its faces and keeps its own side's
+// perturbation; accumulation in FP64.
+template <int DIM, bool RECON, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
+                                                       const float* __restrict__ rec, double* __restrict__ R,
+                                                       int accumulate) {
+    constexpr int NE = DIM + 2;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI;
     double acc[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) acc[p] = 0.0;
-
     for (int k = 0; k < m.K; k++) {
         const int32_t e = m.ell[k * NI + c];
         if (e < 0) break;
-        const int64_t f = e >> 1;
-        const int s = e & 1;
-        const int64_t n = m.fcell[(1 - s) * F + f];
-        const bool ghost = n >= NI;
-        float nh[3], area, x[2][DIM];
-        load_face_na(m.fna, f, nh, area);
-        load_face_pair<DIM, float>(m.fdx, f, x);
-        float nhat[DIM];
+        float dFL[NE], dFR[NE];
+        mixed_face<DIM, RECON, VISC>(m, ph, U, rec, (int64_t)(e >> 1), dFL, dFR);
+        if (e & 1) {
 #pragma unroll
-        for (int d = 0; d < DIM; d++) nhat[d] = nh[d];
-        // conserved perturbations of both face sides relative to the owning cell
-        float dself[NE], dnbr[NE], dcell[NE];
+            for (int p = 0; p < NE; p++) acc[p] += double(dFR[p]);
+        } else {
 #pragma unroll
-        for (int p = 0; p < NE; p++) {
-            const double un = U[p * N + n];
-            dcell[p] = float(un - uc[p]);
-            float a = 0.f, bb = 0.f;
-            if (RECON)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    a += gradC[(p * DIM + d) * N + c] * x[s][d];
-                    if (!ghost) bb += gradC[(p * DIM + d) * N + n] * x[1 - s][d];
-                }
-            dself[p] = a;
-            dnbr[p] = dcell[p] + bb;
+            for (int p = 0; p < NE; p++) acc[p] -= double(dFL[p]);
         }
-        const float* dL = s == 0 ? dself : dnbr;
-        const float* dR = s == 0 ? dnbr : dself;
-        float unc = 0.f;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) unc += bf.vel[d] * nhat[d];
-        const float Mc = unc / bf.a;
-        SidePert<DIM> L, Rs;
-        side_perturbation<DIM>(dL, bf, nhat, unc, Mc, gR, Rg, icv, L);
-        side_perturbation<DIM>(dR, bf, nhat, unc, Mc, gR, Rg, icv, Rs);
-        float dF[NE];
-#pragma unroll
-        for (int p = 0; p < NE; p++) dF[p] = 0.f;
-        const bool subsonic = fabsf(L.M) <= 1.f && fabsf(Rs.M) <= 1.f && fabsf(Mc) <= 1.f;
-        if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
-            const float dm = 0.25f * L.dM * (L.M + Mc + 2.f) - 0.25f * Rs.dM * (Rs.M + Mc - 2.f);
-            const float mm = Mc + dm;
-            const SidePert<DIM>& up = mm < 0.f ? Rs : L;
-            const float mdot_c = unc * bf.rho;
-            const float dmdot = dm * up.a * up.rho + Mc * (up.da * up.rho + bf.a * up.drho);
-            const float mdot = mdot_c + dmdot;
-            const float PpL = 0.25f * (L.M + 1.f) * (L.M + 1.f) * (2.f - L.M);
-            const float PmR = 0.25f * (Rs.M - 1.f) * (Rs.M - 1.f) * (2.f + Rs.M);
-            const float dPpL = 0.25f * L.dM * (3.f - (L.M * L.M + L.M * Mc + Mc * Mc));
-            const float dPpR = 0.25f * Rs.dM * (3.f - (Rs.M * Rs.M + Rs.M * Mc + Mc * Mc));
-            const float dp12 = PpL * L.dp + PmR * Rs.dp + bf.p * (dPpL - dPpR);
-            dF[kRho] = dmdot;
-            dF[kRhoE] = bf.H * dmdot + up.dH * mdot;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) dF[kRhoU + d] = bf.vel[d] * dmdot + up.du[d] * mdot + dp12 * nhat[d];
-#pragma unroll
-            for (int p = 0; p < NE; p++) dF[p] *= area;
-        } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF) {
-            // FP64 route: full face flux of the reconstructed states minus the FP64 baseline flux
-            double uLd[NE], uRd[NE], nhd[DIM], fl[NE];
-#pragma unroll
-            for (int p = 0; p < NE; p++) {
-                uLd[p] = uc[p] + double(dL[p]);
-                uRd[p] = uc[p] + double(dR[p]);
-                fl[p] = 0.0;
-            }
-#pragma unroll
-            for (int d = 0; d < DIM; d++) nhd[d] = double(nhat[d]);
-            const double ad = double(area);
-            if (ph.flux == ABL_FV_FLUX_RUSANOV) {
-                double unL, unR, lam;
-                rusanov_flux<double, DIM>(uLd, uRd, nhd, ad, ph, fl, unL, unR, lam);
-            } else {
-                double mA, rl, rr;
-                int right;
-                advection_flux<double, DIM>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rl, rr);
-            }
-            double und = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) und += b.vel[d] * nhd[d];
-            const double mc = b.rho * und * ad;
-            fl[kRho] -= mc;
-            fl[kRhoE] -= b.H * mc;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) fl[kRhoU + d] -= b.vel[d] * mc + b.p * ad * nhd[d];
-#pragma unroll
-            for (int p = 0; p < NE; p++) dF[p] = float(fl[p]);
-        }
-        if (VISC) {
-            // face-averaged viscous terms and face velocity (cell values) in FP32
-            float tq[NT], vel[DIM], normal[DIM];
-            const float irn = 1.0f / (bf.rho + dcell[kRho]);
-            if (ghost) {
-                float w[2][DIM], gA[DIM + 1][DIM];
-                load_face_pair<DIM, float>(wa_arr, f, w);
-                float dvel[DIM], dke = 0.f, dd = 0.f;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    dvel[d] = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
-                    dke += bf.vel[d] * dvel[d];
-                    dd += dvel[d] * dvel[d];
-                }
-                const float dT = ((dcell[kRhoE] - bf.eps * dcell[kRho]) * irn - (dke + 0.5f * dd)) * icv;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    gA[0][d] = -w[1][d] * dT;
-#pragma unroll
-                    for (int q = 0; q < DIM; q++) gA[1 + q][d] = -w[1][d] * dvel[q];
-                }
-                stress_heat<float, DIM>(float(ph.mu), float(ph.k), gA, tq);
-            } else {
-#pragma unroll
-                for (int q = 0; q < NT; q++) tq[q] = gradA[q * N + n];
-            }
-#pragma unroll
-            for (int q = 0; q < NT; q++) tq[q] = 0.5f * tqc[q] + 0.5f * tq[q];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                const float dv = (dcell[kRhoU + d] - bf.vel[d] * dcell[kRho]) * irn;
-                vel[d] = bf.vel[d] + 0.5f * dv;
-                normal[d] = area * nhat[d];
-            }
-            viscous_flux_add<float, DIM>(normal, vel, tq, dF);
-        }
-        const double sign = s == 0 ? -1.0 : 1.0;
-#pragma unroll
-        for (int p = 0; p < NE; p++) acc[p] += sign * double(dF[p]);
     }
     const double iv = m.inv_vol[c];
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         const double r = acc[p] * iv;
+        R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+    }
+}
+
+// Mixed form, tile strategy: each face of a tile evaluated once into LDS (both baselines, FP32), gathered per cell
+// in FP64 in face-number order.
+template <int DIM, bool RECON, bool VISC, int KM>
+__global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
+                                                              const float* __restrict__ rec,
+                                                              const int32_t* __restrict__ tstart,
+                                                              const int32_t* __restrict__ tfaces,
+                                                              const int32_t* __restrict__ tslot, int nfm,
+                                                              double* __restrict__ R, int accumulate) {
+    constexpr int NE = DIM + 2;
+    extern __shared__ double lds_d[];
+    float* lf = reinterpret_cast<float*>(lds_d);  // [2][NE][nfm]: left-baseline then right-baseline perturbations
+    const int64_t t = xcd_block_index();
+    const int32_t f0 = tstart[t], nf = tstart[t + 1] - f0;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        float dFL[NE], dFR[NE];
+        mixed_face<DIM, RECON, VISC>(m, ph, U, rec, (int64_t)tfaces[f0 + i], dFL, dFR);
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            lf[p * nfm + i] = dFL[p];
+            lf[(NE + p) * nfm + i] = dFR[p];
+        }
+    }
+    __syncthreads();
+    const int64_t c = t * blockDim.x + threadIdx.x;
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI;
+    int off[KM];
+    double sg[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = tslot[k * NI + c];
+        off[k] = e >= 0 ? ((e & 1) ? NE * nfm : 0) + (e >> 1) : 0;
+        sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
+    }
+    const double iv = m.inv_vol[c];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++)
+            if (sg[k] != 0.0) acc += sg[k] * double(lf[p * nfm + off[k]]);
+        const double r = acc * iv;
         R[p * N + c] = accumulate ? R[p * N + c] + r : r;
     }
 }
@@ -1158,7 +1322,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
 
 // ---- compile-time variant selection
 template <typename G>
-using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*, double*);
+using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*, void*);
 using GradKernel = GradKernelT<double>;
 using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
                             double*, int);
@@ -1207,11 +1371,26 @@ template <int DIM>
 FaceFluxKernel pick_face_flux(bool recon, bool visc, bool species) {
     return recon ? pick_face_flux2<DIM, true>(visc, species) : pick_face_flux2<DIM, false>(visc, species);
 }
-using MixedFluxKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, double*, int);
+using MixedFluxKernel = void (*)(MeshView<float>, Phys, const double*, const float*, double*, int);
 template <int DIM>
 MixedFluxKernel pick_flux_mixed(bool recon, bool visc) {
     if (recon) return visc ? k_flux_mixed<DIM, true, true> : k_flux_mixed<DIM, true, false>;
     return visc ? k_flux_mixed<DIM, false, true> : k_flux_mixed<DIM, false, false>;
+}
+using TileMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const int32_t*, const int32_t*,
+                                 const int32_t*, int, double*, int);
+template <int DIM, bool RECON, bool VISC>
+TileMixedKernel pick_tile_mixed2(int K) {
+    switch (slot_width(K)) {
+        case 4: return k_tile_flux_mixed<DIM, RECON, VISC, 4>;
+        case 6: return k_tile_flux_mixed<DIM, RECON, VISC, 6>;
+        default: return k_tile_flux_mixed<DIM, RECON, VISC, 8>;
+    }
+}
+template <int DIM>
+TileMixedKernel pick_tile_mixed(bool recon, bool visc, int K) {
+    if (recon) return visc ? pick_tile_mixed2<DIM, true, true>(K) : pick_tile_mixed2<DIM, true, false>(K);
+    return visc ? pick_tile_mixed2<DIM, false, true>(K) : pick_tile_mixed2<DIM, false, false>(K);
 }
 using VjpKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const int32_t*, const int32_t*,
                           double*);
@@ -1429,10 +1608,12 @@ struct abl_fv_solver {
     size_t tile_lds = 0;
     abl::TileKernel tile_kernel = nullptr;
     // mixed form: FP32 geometry and gradients
-    abl::DevBuf<float> fna_f, fdx_f, fwa_f, gradC_f, gradA_f;
+    abl::DevBuf<float> fna_f, fdx_f, fwa_f;
     abl::DevBuf<float> sw_f, swa_f, slim_f;  // slot-major gradient geometry (mixed form)
     abl::GradKernelT<float> grad_kernel_f = nullptr;
     abl::MixedFluxKernel flux_kernel_mixed = nullptr;
+    abl::TileMixedKernel tile_kernel_mixed = nullptr;
+    abl::DevBuf<float> recf;  // cell records [NI][RS] of the mixed form
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
@@ -1545,7 +1726,9 @@ struct abl_fv_solver {
     // budget.  Needs the host cell -> face table, so it is built at creation.
     void build_tiles() {
         const int KM = abl::slot_width(host.K);
-        const int per_face = (host.dim + 2 + host.ns) * (int)sizeof(double);
+        // LDS per face: FP64 euler + species fluxes, or the mixed form's two FP32 perturbation records
+        const int per_face = form == ABL_FV_FORM_FP64 ? (host.dim + 2 + host.ns) * (int)sizeof(double)
+                                                      : 2 * (host.dim + 2) * (int)sizeof(float);
         abl::Tiles tl;
         for (int T = abl::kTileMax; T >= 64; T /= 2) {
             abl::build_tiles(host, KM, T, tl);
@@ -1563,8 +1746,8 @@ struct abl_fv_solver {
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
-                         fwa_f.bytes() + gradC_f.bytes() + gradA_f.bytes());
+                         gradA.bytes() + rec.bytes() + recf.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         fwa_f.bytes());
     }
 };
 
@@ -1677,20 +1860,16 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->swa_f.upload(sg.wa);
             s->slim_f.upload(sg.lim);
             s->fwa_f.upload(to_f(L.fwa));
-            if (s->recon) {
-                s->gradC_f.alloc((size_t)L.nc * L.dim * N);
-                HIP_CHECK(hipMemset(s->gradC_f.ptr, 0, s->gradC_f.bytes()));
-            }
-            if (s->visc) {
-                s->gradA_f.alloc((size_t)nt * N);
-                HIP_CHECK(hipMemset(s->gradA_f.ptr, 0, s->gradA_f.bytes()));
-            }
+            if (s->recon || s->visc)
+                s->recf.alloc((size_t)(L.dim == 3 ? abl::CellRecF<3>::RS : abl::CellRecF<2>::RS) * L.NI);
             if (mesh->dim == 3) {
                 s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, L.K);
             } else {
                 s->grad_kernel_f = abl::pick_gradients<float, 2>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, L.K);
             }
         }
         std::vector<int32_t> bf, bs;
@@ -1750,8 +1929,9 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->cfl_partial.alloc((size_t)std::max<int64_t>(grid_for(L.NI), 1));
             s->cfl_out.alloc(1);
         }
-        if (form == ABL_FV_FORM_FP64) s->build_tiles();
-        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0) s->strategy = ABL_FV_STRATEGY_FACE_FLUX;
+        s->build_tiles();
+        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0)
+            s->strategy = form == ABL_FV_FORM_FP64 ? ABL_FV_STRATEGY_FACE_FLUX : ABL_FV_STRATEGY_RECOMPUTE;
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
@@ -1856,19 +2036,27 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             const auto vf = h->view_f();
             if (h->recon || h->visc)
                 h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
-                    hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr, h->gradA_f.ptr,
-                                       nullptr);
+                    hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, nullptr, nullptr,
+                                       (void*)h->recf.ptr);
                 });
-            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-                hipLaunchKernelGGL(h->flux_kernel_mixed, grid, block, 0, st, vf, h->ph, U, h->gradC_f.ptr,
-                                   h->gradA_f.ptr, R, accumulate);
-            });
+            if (h->strategy == ABL_FV_STRATEGY_TILE) {
+                h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                    hipLaunchKernelGGL(h->tile_kernel_mixed, dim3((unsigned)h->tile_n), dim3(h->tile_T), h->tile_lds, st,
+                                       vf, h->ph, U, h->recf.ptr, h->tile_start.ptr, h->tile_faces.ptr,
+                                       h->tile_slot.ptr, h->tile_nfm, R, accumulate);
+                });
+            } else {
+                h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                    hipLaunchKernelGGL(h->flux_kernel_mixed, grid, block, 0, st, vf, h->ph, U, h->recf.ptr, R,
+                                       accumulate);
+                });
+            }
         } else {
         const auto v = h->view();
         if (h->recon || h->visc) {
             h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
                 hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr,
-                                   h->rec.ptr);
+                                   (void*)h->rec.ptr);
             });
             if (L.ns)
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
@@ -1918,8 +2106,10 @@ extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
         strategy != ABL_FV_STRATEGY_TILE)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
     if (strategy == ABL_FV_STRATEGY_TILE && h->tile_n == 0)
-        return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (mixed form, or no tile "
-                       "size fits the LDS budget)");
+        return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (no tile size fits the "
+                       "LDS budget)");
+    if (strategy == ABL_FV_STRATEGY_FACE_FLUX && h->form != ABL_FV_FORM_FP64)
+        return fail(h, "abl_fv_set_strategy: the mixed form has the recompute and tile strategies only");
     try {
         if (strategy == ABL_FV_STRATEGY_FACE_FLUX) h->ensure_face_flux();
     } catch (const std::exception& e) {

@@ -38,15 +38,18 @@ def mixed_cases():
 
 
 CASES = mixed_cases()
+STRATEGIES = ["tile", "recompute"]
 
 
+@pytest.mark.parametrize("strategy", STRATEGIES)
 @pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
-def test_mixed_matches_fp64_oracle(case, torch_cuda):
+def test_mixed_matches_fp64_oracle(case, strategy, torch_cuda):
     torch = torch_cuda
-    print(f"[mixed] {case.name} cells={case.mesh.n_interior}", flush=True)
+    print(f"[mixed] {case.name} {strategy} cells={case.mesh.n_interior}", flush=True)
     U0 = case.initial_state()
     Rref, _ = case.oracle_rhs(U0.copy())
     solver = case.solver(form="mixed")
+    solver.set_strategy(strategy)
     U = torch.tensor(U0, dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -58,19 +61,22 @@ def test_mixed_matches_fp64_oracle(case, torch_cuda):
     assert err.max() <= TOL_MIXED_COARSE, f"{case.name}: {err}"
 
 
+@pytest.mark.parametrize("strategy", STRATEGIES)
 @pytest.mark.parametrize("mach", [0.3, 0.01])
-def test_mixed_fine_mesh_within_1e6_of_fp64(mach, torch_cuda):
+def test_mixed_fine_mesh_within_1e6_of_fp64(mach, strategy, torch_cuda):
     """C3 at 128^3 (Taylor-Green, NS, LSQ + Barth-Jespersen, AUSM): mixed vs FP64 per field <= 1e-6."""
     torch = torch_cuda
     mesh = BoxMesh([128, 128, 128], [0, 0, 0], [1, 1, 1], [True] * 3)
     case = Case("c3_fine", mesh, recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125, mach=mach)
     U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
     Rm, Rd = torch.zeros_like(U), torch.zeros_like(U)
-    case.solver(form="mixed").compute_rhs(0.0, U, Rm)
+    sm = case.solver(form="mixed")
+    sm.set_strategy(strategy)
+    sm.compute_rhs(0.0, U, Rm)
     case.solver().compute_rhs(0.0, U, Rd)
     torch.cuda.synchronize()
     err = ((Rm - Rd).norm(dim=1) / Rd.norm(dim=1)).cpu().numpy()
-    print(f"[mixed] 128^3 mach {mach} per-field relative error vs FP64 {err}", flush=True)
+    print(f"[mixed] 128^3 mach {mach} {strategy} per-field relative error vs FP64 {err}", flush=True)
     assert err.max() <= TOL_MIXED, err
 
 
@@ -89,3 +95,27 @@ def test_mixed_is_not_fp64(torch_cuda):
     assert torch.equal(Rm1, Rm2)
     rel = ((Rm1 - Rd).norm(dim=1) / Rd.norm(dim=1)).max().item()
     assert 1e-12 < rel <= TOL_MIXED_COARSE, rel
+
+
+def test_mixed_strategies_agree(torch_cuda):
+    """Tile and recompute evaluate the same per-face perturbations (same arithmetic); only the FP64 summation
+    association of the gather can differ."""
+    torch = torch_cuda
+    case = CASES[0]
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Ra, Rb = torch.zeros_like(U), torch.zeros_like(U)
+    sm = case.solver(form="mixed")
+    sm.set_strategy("recompute")
+    sm.compute_rhs(0.0, U, Ra)
+    sm.set_strategy("tile")
+    sm.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    scale = Ra.abs().max(dim=1, keepdim=True).values
+    assert torch.all((Ra - Rb).abs() <= 1e-13 * scale)
+
+
+def test_mixed_rejects_face_flux_strategy(torch_cuda):
+    case = CASES[2]
+    sm = case.solver(form="mixed")
+    with pytest.raises(RuntimeError):
+        sm.set_strategy("face_flux")
