@@ -140,14 +140,50 @@ __device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF
 // G is the storage/arithmetic type of weights and gradients (double: FP64 form, float: mixed form); state
 // differences are always taken in FP64 before rounding to G.  The FP64 form writes the cell record `rec`; the
 // mixed form writes structure-of-arrays gradC [NE*DIM][N] and gradA [NT][N].
+// Stores each lane's cell record (RSW 64-bit words) at out + c * RSW.  The 64 records of a wave are contiguous, so
+// they are staged through LDS (in two halves) and written as consecutive 8-byte words by consecutive lanes: every
+// store instruction covers whole cache lines instead of 64 scattered 16-byte pieces.  Every thread of the block
+// must call it (it synchronises the block); lanes with active == false store nothing.
+template <int RSW>
+__device__ inline void store_records(const uint64_t* wd, uint64_t* __restrict__ out, int64_t c, bool active,
+                                     int64_t NI) {
+    constexpr int HALF = (RSW + 1) / 2;
+    constexpr int PITCH = HALF + 1;  // odd pitch: the lanes' staging writes spread over the banks
+    __shared__ uint64_t stage[kBlock / 64][64 * PITCH];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t cb = c - lane;  // first cell of the wave
+    (void)active;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        constexpr int H0 = 0;
+        const int lo = h * HALF;
+        const int hw = h == 0 ? HALF : RSW - HALF;
+        if (h == 1) __syncthreads();
+#pragma unroll
+        for (int i = 0; i < HALF; i++)
+            if (i < hw) stage[w][lane * PITCH + i] = wd[lo + i];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < HALF; j++) {
+            if (j < hw) {
+                const int idx = j * 64 + lane;
+                const int r = idx / hw, word = idx - r * hw;
+                if (cb + r < NI) out[(cb + r) * RSW + lo + word] = stage[w][r * PITCH + word];
+            }
+        }
+        (void)H0;
+    }
+}
+
 template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
 __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
                                                       G* __restrict__ gradC, G* __restrict__ gradA,
                                                       void* __restrict__ rec) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
-    const int64_t c = xcd_cell_index();
-    if (c >= m.NI) return;
+    const int64_t c_lane = xcd_cell_index();
+    const bool active = c_lane < m.NI;
+    const int64_t c = active ? c_lane : m.NI - 1;  // lanes past the end shadow the last cell and store nothing
     const int64_t N = m.N, NI = m.NI;
     const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     double uc[NE], ac[NA];
@@ -287,25 +323,22 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
     if (VISC) stress_heat<G, DIM>(G(ph.mu), G(ph.k), ga, tq);
     if (sizeof(G) == 8) {
         using CR = CellRec<DIM>;
-        double r[CR::RS];
+        uint64_t wd[CR::RS];
 #pragma unroll
-        for (int i = 0; i < CR::RS; i++) r[i] = 0.0;
+        for (int i = 0; i < CR::RS; i++) wd[i] = 0;
 #pragma unroll
         for (int p = 0; p < NE; p++) {
-            r[p] = uc[p];
+            wd[p] = (uint64_t)__double_as_longlong(uc[p]);
 #pragma unroll
             for (int d = 0; d < DIM; d++)
-                if (RECON) r[CR::G0 + p * DIM + d] = double(g[p][d]);
+                if (RECON) wd[CR::G0 + p * DIM + d] = (uint64_t)__double_as_longlong(double(g[p][d]));
         }
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++)
-            if (VISC) r[CR::T0 + q] = double(tq[q]);
-        double2* out = reinterpret_cast<double2*>(reinterpret_cast<double*>(rec) + c * CR::RS);
-#pragma unroll
-        for (int i = 0; i < CR::RS / 2; i++) out[i] = make_double2(r[2 * i], r[2 * i + 1]);
-        return;
-    }
-    {
+            if (VISC) wd[CR::T0 + q] = (uint64_t)__double_as_longlong(double(tq[q]));
+        store_records<CR::RS>(wd, reinterpret_cast<uint64_t*>(rec), c_lane, active, NI);
+    } else {
+        // FP32 record: float pairs packed little-endian into 64-bit words, then the FP64 state
         using CR = CellRecF<DIM>;
         float t[CR::U0];
 #pragma unroll
@@ -318,28 +351,31 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++)
             if (VISC) t[CR::TF + q] = float(tq[q]);
-        float* base = reinterpret_cast<float*>(rec) + c * CR::RS;
+        constexpr int RSW = CR::RS / 2;
+        uint64_t wd[RSW];
 #pragma unroll
-        for (int i = 0; i < CR::U0 / 4; i++)
-            reinterpret_cast<float4*>(base)[i] = make_float4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
-        double* ud = reinterpret_cast<double*>(base + CR::U0);
+        for (int i = 0; i < RSW; i++) wd[i] = 0;
 #pragma unroll
-        for (int i = 0; i < NE / 2; i++) reinterpret_cast<double2*>(ud)[i] = make_double2(uc[2 * i], uc[2 * i + 1]);
-        if (NE & 1) ud[NE - 1] = uc[NE - 1];
+        for (int i = 0; i < CR::U0 / 2; i++)
+            wd[i] = (uint64_t)__float_as_uint(t[2 * i]) | ((uint64_t)__float_as_uint(t[2 * i + 1]) << 32);
+#pragma unroll
+        for (int p = 0; p < NE; p++) wd[CR::U0 / 2 + p] = (uint64_t)__double_as_longlong(uc[p]);
+        store_records<RSW>(wd, reinterpret_cast<uint64_t*>(rec), c_lane, active, NI);
     }
 }
 
-// densityYi gradients (limited, one field) and Yi aux gradients (unlimited).
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                              double* __restrict__ gradC, double* __restrict__ gradA) {
+// densityYi gradients (limited, one field) into gS [ns*DIM][N] and Yi aux gradients (unlimited) into gY [ns*DIM][N];
+// FP64 arithmetic, stored as G (double: FP64 form, float: mixed form).
+template <typename G, int DIM>
+__global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                                              G* __restrict__ gradC, G* __restrict__ gradA) {
     constexpr int NE = DIM + 2;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, NI = m.NI;
     const bool recon = ph.recon != 0;
     const bool aux = ph.sdiff != 0;
-    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+    const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     const double irho_c = 1.0 / U[kRho * N + c];
     bool cancel = false;
     for (int sp = 0; sp < ph.ns; sp++) {
@@ -406,16 +442,16 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<double> m
                 for (int d = 0; d < DIM; d++)
                     if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradC[(comp * DIM + d) * N + c] = g[d] * phi;
+            for (int d = 0; d < DIM; d++) gradC[(sp * DIM + d) * N + c] = G(g[d] * phi);
         }
         if (aux)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradA[(ViscTerms<DIM>::NT + sp * DIM + d) * N + c] = ga[d];
+            for (int d = 0; d < DIM; d++) gradA[(sp * DIM + d) * N + c] = G(ga[d]);
     }
     if (cancel)  // the maxLimGrad rule zeroes the whole densityYi field of the cell (cellInterpolant.cpp:459-481)
         for (int sp = 0; sp < ph.ns; sp++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradC[((NE + sp) * DIM + d) * N + c] = 0.0;
+            for (int d = 0; d < DIM; d++) gradC[(sp * DIM + d) * N + c] = G(0);
 }
 
 // Everything a face flux needs from one side: the state projected to the face (ProjectToFace,
@@ -431,15 +467,32 @@ struct SideData {
     double tq[VISC ? ViscTerms<DIM>::NT : 1];
 };
 
+// Primitive face-interpolant values of a side from its cell state: density, velocity and internal energy
+// (scaled to T by the caller's cv only where needed).
+template <int DIM, bool VISC>
+__device__ inline void side_cell_values(const double* u, SideData<DIM, VISC>& sd) {
+    sd.rho = u[kRho];
+    if (VISC) {
+        const double irho = 1.0 / u[kRho];
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            sd.vel[d] = u[kRhoU + d] * irho;
+            ke += sd.vel[d] * sd.vel[d];
+        }
+        sd.T = u[kRhoE] * irho - 0.5 * ke;
+    }
+}
+
 // Loads side data of cell c for a face whose centroid is at x_c + dx: from the cell record when the gradient pass
 // ran (reconstruction or diffusion), else from the state.  Ghost cells have zero gradients and read the state;
-// their viscous terms are filled by ghost_visc.
+// their viscous terms are filled by ghost_visc.  (Each branch keeps its own arrays so that they stay in
+// registers.)
 template <int DIM, bool RECON, bool VISC>
 __device__ inline void load_side(const double* __restrict__ U, const double* __restrict__ rec, int64_t N, int64_t c,
                                  bool ghost, const double* dx, SideData<DIM, VISC>& sd) {
     using CR = CellRec<DIM>;
     constexpr int NE = DIM + 2;
-    double u[NE];
     if ((RECON || VISC) && !ghost) {
         constexpr int NL = ((VISC ? CR::END : CR::T0) + 1) & ~1;
         double r[NL];
@@ -451,34 +504,28 @@ __device__ inline void load_side(const double* __restrict__ U, const double* __r
             r[2 * i + 1] = t.y;
         }
 #pragma unroll
-        for (int p = 0; p < NE; p++) {
+        for (int p2 = 0; p2 < NE; p2++) {
             double a = 0.0;
             if (RECON)
 #pragma unroll
-                for (int d = 0; d < DIM; d++) a += r[CR::G0 + p * DIM + d] * dx[d];
-            u[p] = r[p];
-            sd.uf[p] = r[p] + a;
+                for (int d = 0; d < DIM; d++) a += r[CR::G0 + p2 * DIM + d] * dx[d];
+            sd.uf[p2] = r[p2] + a;
         }
         if (VISC)
 #pragma unroll
             for (int q = 0; q < ViscTerms<DIM>::NT; q++) sd.tq[q] = r[CR::T0 + q];
+        side_cell_values<DIM, VISC>(r, sd);
     } else {
+        double u[NE];
 #pragma unroll
-        for (int p = 0; p < NE; p++) {
-            u[p] = U[p * N + c];
-            sd.uf[p] = u[p];
+        for (int p2 = 0; p2 < NE; p2++) {
+            u[p2] = U[p2 * N + c];
+            sd.uf[p2] = u[p2];
         }
-    }
-    sd.rho = u[kRho];
-    if (VISC) {
-        const double irho = 1.0 / sd.rho;
-        double ke = 0.0;
+        if (VISC)
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            sd.vel[d] = u[kRhoU + d] * irho;
-            ke += sd.vel[d] * sd.vel[d];
-        }
-        sd.T = u[kRhoE] * irho - 0.5 * ke;  // internal energy; scaled to T by the caller's cv only where needed
+            for (int q = 0; q < ViscTerms<DIM>::NT; q++) sd.tq[q] = 0.0;
+        side_cell_values<DIM, VISC>(u, sd);
     }
 }
 
@@ -555,8 +602,8 @@ __device__ inline double species_face_flux(const MeshView<double>& m, const doub
         double pl = 0.0, pr = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            pl += gradC[(comp * DIM + d) * N + L] * x[0][d];
-            if (!ghost) pr += gradC[(comp * DIM + d) * N + R] * x[1][d];
+            pl += gradC[(sp * DIM + d) * N + L] * x[0][d];
+            if (!ghost) pr += gradC[(sp * DIM + d) * N + R] * x[1][d];
         }
         flux = cl * (yL + pl) + cr * (yR + pr);
     } else {
@@ -564,7 +611,7 @@ __device__ inline double species_face_flux(const MeshView<double>& m, const doub
     }
     if (VISC) {
         double dot = 0.0;
-        const int64_t base = ViscTerms<DIM>::NT + (int64_t)sp * DIM;
+        const int64_t base = (int64_t)sp * DIM;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             const double gl = gradA[(base + d) * N + L];
@@ -707,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys p
 // cell's faces from LDS in face-number order.  No face-flux round trip through HBM; the face fluxes and the
 // gather order are those of strategy 2, so the residual is bitwise the same.
 template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM>
-__global__ __launch_bounds__(kTileMax) void k_tile_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+__global__ __launch_bounds__(kTileMax, 2) void k_tile_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                         const double* __restrict__ gradC,
                                                         const double* __restrict__ gradA,
                                                         const double* __restrict__ rec,
@@ -1840,13 +1887,13 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->fwa.upload(L.fwa);
             if (s->recon || s->visc)
                 s->rec.alloc((size_t)(L.dim == 3 ? abl::CellRec<3>::RS : abl::CellRec<2>::RS) * L.NI);
-            // species gradients stay structure-of-arrays (rows NE*DIM.. of gradC, NT.. of gradA)
+            // species gradients: structure of arrays [ns*DIM][N], limited densityYi (gradC) and Yi aux (gradA)
             if (s->recon && L.ns) {
-                s->gradC.alloc((size_t)L.nc * L.dim * N);
+                s->gradC.alloc((size_t)L.ns * L.dim * N);
                 HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost rows stay zero
             }
             if (s->visc && L.ns) {
-                s->gradA.alloc((size_t)(nt + L.ns * L.dim) * N);
+                s->gradA.alloc((size_t)L.ns * L.dim * N);
                 HIP_CHECK(hipMemset(s->gradA.ptr, 0, s->gradA.bytes()));
             }
         } else {
@@ -2061,10 +2108,10 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             if (L.ns)
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
                     if (L.dim == 3)
-                        hipLaunchKernelGGL(abl::k_species_gradients<3>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
+                        hipLaunchKernelGGL((abl::k_species_gradients<double, 3>), grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
                                            h->gradA.ptr);
                     else
-                        hipLaunchKernelGGL(abl::k_species_gradients<2>, grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
+                        hipLaunchKernelGGL((abl::k_species_gradients<double, 2>), grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
                                            h->gradA.ptr);
                 });
         }
