@@ -940,15 +940,22 @@ __device__ inline void side_perturbation(const float* dU, const BaselineF<DIM>& 
 // ghost right cell).  The face flux is formed as a perturbation of the left cell's state; the right baseline
 // differs by the physical-flux perturbation of the cell difference U_R - U_L, also formed in FP32 from exact
 // identities (or in FP64 on the FP64 route).
-template <int DIM, bool RECON, bool VISC>
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, const double* __restrict__ U,
-                                  const float* __restrict__ rec, int64_t f, float* dFL, float* dFR) {
+                                  const float* __restrict__ rec, const float* __restrict__ gS,
+                                  const float* __restrict__ gY, int64_t f, float* dFL, float* dFR, float* sFL,
+                                  float* sFR, int64_t s_stride) {
     constexpr int NE = DIM + 2;
     constexpr int NT = ViscTerms<DIM>::NT;
     constexpr bool USE_REC = RECON || VISC;
     const int64_t N = m.N, F = m.F, NI = m.NI;
     const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
     const bool ghost = Rc >= NI;
+    if (SPECIES)
+        for (int sp = 0; sp < ph.ns; sp++) {
+            sFL[sp * s_stride] = 0.f;
+            sFR[sp * s_stride] = 0.f;
+        }
     float nh[3], area, x[2][DIM];
     load_face_na(m.fna, f, nh, area);
     load_face_pair<DIM, float>(m.fdx, f, x);
@@ -1040,6 +1047,31 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 #pragma unroll
             for (int p = 0; p < NE; p++) dPhi[p] *= area;
         }
+        if (SPECIES) {
+            // F_sp = mdot Y_up A (speciesTransport.cpp:325-337) as a perturbation of the left baseline rhoY_L u_n
+            SidePert<DIM> sC;
+            if (!ghost) side_perturbation<DIM>(dcell, bf, nhat, unc, Mc, gR, Rg, icv, sC);
+            for (int sp = 0; sp < ph.ns; sp++) {
+                const int64_t comp = NE + sp;
+                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+                const float YL = float(yl / rl.u[kRho]);
+                const float dcs = float(yr - yl);
+                float dsl = 0.f, dsr = dcs;
+                if (RECON)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        dsl += gS[(sp * DIM + d) * N + L] * x[0][d];
+                        if (!ghost) dsr += gS[(sp * DIM + d) * N + Rc] * x[1][d];
+                    }
+                const float dsu = mm < 0.f ? dsr : dsl;
+                const float dYu = (dsu - YL * up.drho) / up.rho;
+                const float adv = (dmdot * YL + mdot * dYu) * area;
+                float shift = 0.f;
+                if (!ghost) shift = (dcs * unc + (float(yl) + dcs) * sC.dun) * area;
+                sFL[sp * s_stride] = adv;
+                sFR[sp * s_stride] = adv - shift;
+            }
+        }
     } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF) {
         // FP64 route: full face flux of the reconstructed states minus the FP64 baselines
         double uLd[NE], uRd[NE], nhd[DIM], fl[NE];
@@ -1052,13 +1084,20 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 #pragma unroll
         for (int d = 0; d < DIM; d++) nhd[d] = double(nhat[d]);
         const double ad = double(area);
+        double clo = 0.0, cro = 0.0;  // species flux coefficients: F_sp = clo rhoY_L,f + cro rhoY_R,f
         if (ph.flux == ABL_FV_FLUX_RUSANOV) {
             double unL, unR, lam;
             rusanov_flux<double, DIM>(uLd, uRd, nhd, ad, ph, fl, unL, unR, lam);
+            clo = ad * (0.5 * unL + 0.5 * lam);
+            cro = ad * (0.5 * unR - 0.5 * lam);
         } else {
             double mA, rlo, rro;
             int right;
             advection_flux<double, DIM>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rlo, rro);
+            if (right)
+                cro = mA / rro;
+            else
+                clo = mA / rlo;
         }
         double und = 0.0;
 #pragma unroll
@@ -1083,6 +1122,29 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 #pragma unroll
             for (int d = 0; d < DIM; d++)
                 dPhi[kRhoU + d] = float(br.vel[d] * mr + br.p * ad * nhd[d] - base[kRhoU + d]);
+        }
+        if (SPECIES) {
+            double unr = 0.0;
+            if (!ghost) {
+                Baseline<DIM> br;
+                decode_baseline<DIM>(rr.u, ph, br);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) unr += br.vel[d] * nhd[d];
+            }
+            for (int sp = 0; sp < ph.ns; sp++) {
+                const int64_t comp = NE + sp;
+                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+                float dsl = 0.f, dsr = float(yr - yl);
+                if (RECON)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        dsl += gS[(sp * DIM + d) * N + L] * x[0][d];
+                        if (!ghost) dsr += gS[(sp * DIM + d) * N + Rc] * x[1][d];
+                    }
+                const double fsp = clo * (yl + double(dsl)) + cro * (yl + double(dsr)) - yl * und * ad;
+                sFL[sp * s_stride] = float(fsp);
+                sFR[sp * s_stride] = ghost ? 0.f : float(fsp - (yr * unr - yl * und) * ad);
+            }
         }
     }
     if (VISC) {
@@ -1120,6 +1182,30 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
             normal[d] = area * nhat[d];
         }
         viscous_flux_add<float, DIM>(normal, vel, tq, dF);
+        if (SPECIES && ph.sdiff) {
+            // DiffusionSpeciesFlux (speciesTransport.cpp:196-228): -D rho_face n.(1/2 (gradY_L + gradY_R)) A, a full
+            // flux (no baseline), shared by both sides
+            const float cd = -float(ph.diff) * (bf.rho + 0.5f * dcell[kRho]) * 0.5f * area;
+            float w[2][DIM];
+            if (ghost) load_face_pair<DIM, float>(m.fwa, f, w);
+            for (int sp = 0; sp < ph.ns; sp++) {
+                const int64_t comp = NE + sp;
+                float dot = 0.f;
+                float gyg = 0.f;
+                if (ghost) {
+                    const double yl = U[comp * N + L] / rl.u[kRho], yg = U[comp * N + Rc] / rr.u[kRho];
+                    gyg = float(yl - yg);
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    const float gl = gY[(sp * DIM + d) * N + L];
+                    const float gr = ghost ? w[1][d] * gyg : gY[(sp * DIM + d) * N + Rc];
+                    dot += nhat[d] * (gl + gr);
+                }
+                sFL[sp * s_stride] += cd * dot;
+                sFR[sp * s_stride] += cd * dot;
+            }
+        }
     }
 #pragma unroll
     for (int p = 0; p < NE; p++) {
@@ -1129,7 +1215,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 }
 
 // Mixed form, cell-centric (strategy "recompute"): each cell evaluates its faces and keeps its own side's
-// perturbation; accumulation in FP64.
+// perturbation; accumulation in FP64.  Euler components only (species need the tile strategy).
 template <int DIM, bool RECON, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                        const float* __restrict__ rec, double* __restrict__ R,
@@ -1145,7 +1231,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
         const int32_t e = m.ell[k * NI + c];
         if (e < 0) break;
         float dFL[NE], dFR[NE];
-        mixed_face<DIM, RECON, VISC>(m, ph, U, rec, (int64_t)(e >> 1), dFL, dFR);
+        mixed_face<DIM, RECON, VISC, false>(m, ph, U, rec, nullptr, nullptr, (int64_t)(e >> 1), dFL, dFR, nullptr,
+                                            nullptr, 0);
         if (e & 1) {
 #pragma unroll
             for (int p = 0; p < NE; p++) acc[p] += double(dFR[p]);
@@ -1162,27 +1249,32 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
     }
 }
 
-// Mixed form, tile strategy: each face of a tile evaluated once into LDS (both baselines, FP32), gathered per cell
-// in FP64 in face-number order.
-template <int DIM, bool RECON, bool VISC, int KM>
+// Mixed form, tile strategy: each face of a tile evaluated once into LDS (both baselines, FP32: rows [0, NE + ns)
+// left-baseline perturbations, rows [NE + ns, 2 (NE + ns)) right-baseline ones), gathered per cell in FP64 in
+// face-number order.
+template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM>
 __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                               const float* __restrict__ rec,
+                                                              const float* __restrict__ gS,
+                                                              const float* __restrict__ gY,
                                                               const int32_t* __restrict__ tstart,
                                                               const int32_t* __restrict__ tfaces,
                                                               const int32_t* __restrict__ tslot, int nfm,
                                                               double* __restrict__ R, int accumulate) {
     constexpr int NE = DIM + 2;
     extern __shared__ double lds_d[];
-    float* lf = reinterpret_cast<float*>(lds_d);  // [2][NE][nfm]: left-baseline then right-baseline perturbations
+    float* lf = reinterpret_cast<float*>(lds_d);
+    const int nrow = NE + (SPECIES ? ph.ns : 0);
     const int64_t t = xcd_block_index();
     const int32_t f0 = tstart[t], nf = tstart[t + 1] - f0;
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         float dFL[NE], dFR[NE];
-        mixed_face<DIM, RECON, VISC>(m, ph, U, rec, (int64_t)tfaces[f0 + i], dFL, dFR);
+        mixed_face<DIM, RECON, VISC, SPECIES>(m, ph, U, rec, gS, gY, (int64_t)tfaces[f0 + i], dFL, dFR,
+                                              lf + NE * nfm + i, lf + (nrow + NE) * nfm + i, nfm);
 #pragma unroll
         for (int p = 0; p < NE; p++) {
             lf[p * nfm + i] = dFL[p];
-            lf[(NE + p) * nfm + i] = dFR[p];
+            lf[(nrow + p) * nfm + i] = dFR[p];
         }
     }
     __syncthreads();
@@ -1194,12 +1286,11 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int32_t e = tslot[k * NI + c];
-        off[k] = e >= 0 ? ((e & 1) ? NE * nfm : 0) + (e >> 1) : 0;
+        off[k] = e >= 0 ? ((e & 1) ? nrow * nfm : 0) + (e >> 1) : 0;
         sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
     }
     const double iv = m.inv_vol[c];
-#pragma unroll
-    for (int p = 0; p < NE; p++) {
+    for (int p = 0; p < nrow; p++) {
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < KM; k++)
@@ -1424,20 +1515,25 @@ MixedFluxKernel pick_flux_mixed(bool recon, bool visc) {
     if (recon) return visc ? k_flux_mixed<DIM, true, true> : k_flux_mixed<DIM, true, false>;
     return visc ? k_flux_mixed<DIM, false, true> : k_flux_mixed<DIM, false, false>;
 }
-using TileMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const int32_t*, const int32_t*,
-                                 const int32_t*, int, double*, int);
-template <int DIM, bool RECON, bool VISC>
-TileMixedKernel pick_tile_mixed2(int K) {
+using TileMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, const float*,
+                                 const int32_t*, const int32_t*, const int32_t*, int, double*, int);
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
+TileMixedKernel pick_tile_mixed3(int K) {
     switch (slot_width(K)) {
-        case 4: return k_tile_flux_mixed<DIM, RECON, VISC, 4>;
-        case 6: return k_tile_flux_mixed<DIM, RECON, VISC, 6>;
-        default: return k_tile_flux_mixed<DIM, RECON, VISC, 8>;
+        case 4: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 4>;
+        case 6: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 6>;
+        default: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 8>;
     }
 }
+template <int DIM, bool RECON, bool VISC>
+TileMixedKernel pick_tile_mixed2(bool species, int K) {
+    return species ? pick_tile_mixed3<DIM, RECON, VISC, true>(K) : pick_tile_mixed3<DIM, RECON, VISC, false>(K);
+}
 template <int DIM>
-TileMixedKernel pick_tile_mixed(bool recon, bool visc, int K) {
-    if (recon) return visc ? pick_tile_mixed2<DIM, true, true>(K) : pick_tile_mixed2<DIM, true, false>(K);
-    return visc ? pick_tile_mixed2<DIM, false, true>(K) : pick_tile_mixed2<DIM, false, false>(K);
+TileMixedKernel pick_tile_mixed(bool recon, bool visc, bool species, int K) {
+    if (recon)
+        return visc ? pick_tile_mixed2<DIM, true, true>(species, K) : pick_tile_mixed2<DIM, true, false>(species, K);
+    return visc ? pick_tile_mixed2<DIM, false, true>(species, K) : pick_tile_mixed2<DIM, false, false>(species, K);
 }
 using VjpKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const int32_t*, const int32_t*,
                           double*);
@@ -1660,7 +1756,8 @@ struct abl_fv_solver {
     abl::GradKernelT<float> grad_kernel_f = nullptr;
     abl::MixedFluxKernel flux_kernel_mixed = nullptr;
     abl::TileMixedKernel tile_kernel_mixed = nullptr;
-    abl::DevBuf<float> recf;  // cell records [NI][RS] of the mixed form
+    abl::DevBuf<float> recf;            // cell records [NI][RS] of the mixed form
+    abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients [ns*DIM][N] (densityYi limited, Yi aux)
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
@@ -1775,7 +1872,7 @@ struct abl_fv_solver {
         const int KM = abl::slot_width(host.K);
         // LDS per face: FP64 euler + species fluxes, or the mixed form's two FP32 perturbation records
         const int per_face = form == ABL_FV_FORM_FP64 ? (host.dim + 2 + host.ns) * (int)sizeof(double)
-                                                      : 2 * (host.dim + 2) * (int)sizeof(float);
+                                                      : 2 * (host.dim + 2 + host.ns) * (int)sizeof(float);
         abl::Tiles tl;
         for (int T = abl::kTileMax; T >= 64; T /= 2) {
             abl::build_tiles(host, KM, T, tl);
@@ -1793,7 +1890,7 @@ struct abl_fv_solver {
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + recf.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         gradA.bytes() + rec.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
 };
@@ -1820,8 +1917,6 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
     if (!mesh || !physics || !out) return fail(nullptr, "abl_fv_create: null argument");
     *out = nullptr;
     if (form != ABL_FV_FORM_FP64 && form != ABL_FV_FORM_MIXED) return fail(nullptr, "abl_fv_create: unknown form");
-    if (form == ABL_FV_FORM_MIXED && physics && physics->n_species > 0)
-        return fail(nullptr, "abl_fv_create: the mixed form does not transport species in this build");
     const abl_fv_physics_desc& p = *physics;
     if (!(p.gamma > 1.0) || !(p.Rgas > 0.0)) return fail(nullptr, "abl_fv_create: PerfectGas needs gamma > 1 and Rgas > 0");
     if (p.flux_calculator < ABL_FV_FLUX_NONE || p.flux_calculator > ABL_FV_FLUX_RUSANOV)
@@ -1909,14 +2004,22 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->fwa_f.upload(to_f(L.fwa));
             if (s->recon || s->visc)
                 s->recf.alloc((size_t)(L.dim == 3 ? abl::CellRecF<3>::RS : abl::CellRecF<2>::RS) * L.NI);
+            if (s->recon && L.ns) {
+                s->gS_f.alloc((size_t)L.ns * L.dim * N);
+                HIP_CHECK(hipMemset(s->gS_f.ptr, 0, s->gS_f.bytes()));
+            }
+            if (s->visc && L.ns) {
+                s->gY_f.alloc((size_t)L.ns * L.dim * N);
+                HIP_CHECK(hipMemset(s->gY_f.ptr, 0, s->gY_f.bytes()));
+            }
             if (mesh->dim == 3) {
                 s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc);
-                s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, L.K);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, species, L.K);
             } else {
                 s->grad_kernel_f = abl::pick_gradients<float, 2>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc);
-                s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, L.K);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, species, L.K);
             }
         }
         std::vector<int32_t> bf, bs;
@@ -1977,8 +2080,12 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->cfl_out.alloc(1);
         }
         s->build_tiles();
-        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0)
+        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0) {
+            if (form != ABL_FV_FORM_FP64 && L.ns > 0)
+                throw std::invalid_argument("the mixed form with species needs the tile strategy, and no tile size "
+                                            "fits the LDS budget");
             s->strategy = form == ABL_FV_FORM_FP64 ? ABL_FV_STRATEGY_FACE_FLUX : ABL_FV_STRATEGY_RECOMPUTE;
+        }
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
@@ -2086,11 +2193,20 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                     hipLaunchKernelGGL(h->grad_kernel_f, grid, block, 0, st, vf, h->ph, U, nullptr, nullptr,
                                        (void*)h->recf.ptr);
                 });
+            if (L.ns && (h->recon || h->visc))
+                h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
+                    if (L.dim == 3)
+                        hipLaunchKernelGGL((abl::k_species_gradients<float, 3>), grid, block, 0, st, vf, h->ph, U,
+                                           h->gS_f.ptr, h->gY_f.ptr);
+                    else
+                        hipLaunchKernelGGL((abl::k_species_gradients<float, 2>), grid, block, 0, st, vf, h->ph, U,
+                                           h->gS_f.ptr, h->gY_f.ptr);
+                });
             if (h->strategy == ABL_FV_STRATEGY_TILE) {
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                     hipLaunchKernelGGL(h->tile_kernel_mixed, dim3((unsigned)h->tile_n), dim3(h->tile_T), h->tile_lds, st,
-                                       vf, h->ph, U, h->recf.ptr, h->tile_start.ptr, h->tile_faces.ptr,
-                                       h->tile_slot.ptr, h->tile_nfm, R, accumulate);
+                                       vf, h->ph, U, h->recf.ptr, h->gS_f.ptr, h->gY_f.ptr, h->tile_start.ptr,
+                                       h->tile_faces.ptr, h->tile_slot.ptr, h->tile_nfm, R, accumulate);
                 });
             } else {
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
@@ -2157,6 +2273,8 @@ extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
                        "LDS budget)");
     if (strategy == ABL_FV_STRATEGY_FACE_FLUX && h->form != ABL_FV_FORM_FP64)
         return fail(h, "abl_fv_set_strategy: the mixed form has the recompute and tile strategies only");
+    if (strategy == ABL_FV_STRATEGY_RECOMPUTE && h->form != ABL_FV_FORM_FP64 && h->host.ns > 0)
+        return fail(h, "abl_fv_set_strategy: the mixed form transports species with the tile strategy only");
     try {
         if (strategy == ABL_FV_STRATEGY_FACE_FLUX) h->ensure_face_flux();
     } catch (const std::exception& e) {

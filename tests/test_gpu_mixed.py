@@ -37,7 +37,21 @@ def mixed_cases():
     ]
 
 
+def mixed_species_cases():
+    chan = lambda n: BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [False, True, True])
+    return [
+        Case("mixed_c4_species_first_order", chan(8), n_species=10, bcs=inflow_x(10)),
+        Case("mixed_species_lsq_diffusion", chan(8), n_species=3, recon="leastsquares", limiter="barthjespersen",
+             mu=0.5, k=5.0, diff=0.3, bcs=inflow_x(3)),
+        Case("mixed_c5_tets_rusanov", TetBoxMesh(4, 0.2), flux="rusanov", n_species=3, recon="leastsquares",
+             limiter="barthjespersen", mu=0.05, k=0.5, diff=2e-2, bcs=all_outflow(6)),
+        Case("mixed_species_2d", BoxMesh([10, 7], [0, 0], [2, 1], [False, True]), n_species=2, recon="leastsquares",
+             limiter="barthjespersen", mu=0.2, k=1.0, diff=0.1, state="random2d", bcs=inflow_x(2)),
+    ]
+
+
 CASES = mixed_cases()
+SPECIES_CASES = mixed_species_cases()
 STRATEGIES = ["tile", "recompute"]
 
 
@@ -59,6 +73,32 @@ def test_mixed_matches_fp64_oracle(case, strategy, torch_cuda):
     print(f"[mixed] {case.name} per-field relative error {err}", flush=True)
     assert np.all(np.isfinite(Rg))
     assert err.max() <= TOL_MIXED_COARSE, f"{case.name}: {err}"
+
+
+@pytest.mark.parametrize("case", SPECIES_CASES, ids=[c.name for c in SPECIES_CASES])
+def test_mixed_species_matches_fp64_oracle(case, torch_cuda):
+    """Species transport in the mixed form (tile strategy): densityYi advection as a perturbation of the left
+    baseline rhoY u_n, species diffusion as a full FP32 flux."""
+    torch = torch_cuda
+    print(f"[mixed] {case.name} cells={case.mesh.n_interior} species={case.n_species}", flush=True)
+    U0 = case.initial_state()
+    Rref, _ = case.oracle_rhs(U0.copy())
+    solver = case.solver(form="mixed")
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    R = torch.zeros_like(U)
+    solver.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    Rg = R.cpu().numpy()
+    err = rel_err_per_field(Rg, Rref, case.mesh.n_interior)
+    print(f"[mixed] {case.name} per-field relative error {err}", flush=True)
+    assert np.all(np.isfinite(Rg))
+    assert err.max() <= TOL_MIXED_COARSE, f"{case.name}: {err}"
+
+
+def test_mixed_species_needs_tile_strategy(torch_cuda):
+    sm = SPECIES_CASES[0].solver(form="mixed")
+    with pytest.raises(RuntimeError):
+        sm.set_strategy("recompute")
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
