@@ -272,45 +272,25 @@ void build_tiles(const Layout& L, int KM, int T, Tiles& out) {
     out.faces.clear();
     out.faces.reserve((size_t)NI * 4);
     out.slot.assign((size_t)KM * NI, -1);
-    // Faces are listed round by round: round r holds the r-th owned face of every cell of the tile, where a cell
-    // owns the faces it is the left cell of, plus the faces whose left cell lies outside the tile.  A workgroup
-    // walking the list in chunks of T then touches every cell of the tile in each chunk, so the cell data a chunk
-    // loads is reused by the next chunk while it is still cached.
-    std::vector<std::pair<int64_t, int32_t>> keyed;  // (round * T + local owner, face)
-    std::vector<int32_t> list, sorted;
+    // Faces of a tile in ascending face number: with faces numbered by their left cell, consecutive lanes read
+    // consecutive face records (a measured alternative that listed each cell's owned faces round by round doubled
+    // the kernel's fetched bytes by breaking that coalescing).
+    std::vector<int32_t> list;
     for (int64_t t = 0; t < out.n; t++) {
         const int64_t c0 = t * T, c1 = std::min<int64_t>(c0 + T, NI);
-        keyed.clear();
-        for (int64_t c = c0; c < c1; c++) {
-            int rank = 0;
-            for (int k = 0; k < L.K; k++) {
-                const int32_t e = L.ell[(size_t)k * NI + c];
-                if (e < 0) break;
-                const int32_t f = e >> 1;
-                const int64_t left = L.fcell[f];
-                const bool owned = (e & 1) == 0 || left < c0 || left >= c1;
-                if (owned) keyed.emplace_back((int64_t)rank++ * T + (c - c0), f);
-            }
-        }
-        std::sort(keyed.begin(), keyed.end());
         list.clear();
-        for (auto& kf : keyed) list.push_back(kf.second);
-        // local index lookup by face number
-        sorted.assign(list.begin(), list.end());
-        std::sort(sorted.begin(), sorted.end());
-        std::vector<int32_t> pos(list.size());
-        {
-            std::vector<std::pair<int32_t, int32_t>> fi(list.size());
-            for (size_t i = 0; i < list.size(); i++) fi[i] = {list[i], (int32_t)i};
-            std::sort(fi.begin(), fi.end());
-            for (size_t i = 0; i < fi.size(); i++) pos[i] = fi[i].second;
-        }
+        for (int k = 0; k < L.K; k++)
+            for (int64_t c = c0; c < c1; c++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e >= 0) list.push_back(e >> 1);
+            }
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
         for (int k = 0; k < L.K; k++)
             for (int64_t c = c0; c < c1; c++) {
                 const int32_t e = L.ell[(size_t)k * NI + c];
                 if (e < 0) continue;
-                const size_t at = (size_t)(std::lower_bound(sorted.begin(), sorted.end(), e >> 1) - sorted.begin());
-                const int32_t local = pos[at];
+                const int32_t local = (int32_t)(std::lower_bound(list.begin(), list.end(), e >> 1) - list.begin());
                 out.slot[(size_t)k * NI + c] = (local << 1) | (e & 1);
             }
         if (out.faces.size() + list.size() > (size_t)INT32_MAX)
