@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fv_adjoint.hpp"
+#include "fv_adjoint2.hpp"
 #include "fv_device.hpp"
 #include "fv_internal.hpp"
 #include "fv_layout.hpp"
@@ -1458,7 +1459,660 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
     contrib[3 * i + 2] = t2;
 }
 
+// ============================================================================================ second-order adjoint
+//
+// VJP of the FP64 RHS with least-squares reconstruction (Barth-Jespersen or no limiter), the viscous flux and
+// species advection / diffusion, in three passes after the forward gradient pass (cell records, species gradient
+// arrays):
+//   1. k_adj_faces  (lane per interior cell): recomputes each face of the cell and reverses the face flux, keeping
+//      its own side's adjoints: state (direct), limited gradients (face state = U + G dx), viscous terms, species
+//      aux gradients.  The adjoint of a ghost state goes to a per-ghost array.
+//   2. k_adj_cells  (lane per interior cell): reverses the limiter, the least-squares gradients and the aux
+//      (temperature / velocity / mass fraction) gradients of the cell; writes its own adjoint and, per face slot,
+//      the contribution for the neighbour across the face.
+//   3. k_adj_gather (lane per interior cell): own adjoint + the contributions its neighbours wrote for it (mirror
+//      slot of each face) + the adjoint of its zero-gradient ghosts.
+// Every sum runs in a fixed order (deterministic, no atomics).  maxLimGrad's cancel rule (a gradient above maxLimGrad,
+// default 1e10, in a fully limited component) is taken as inactive.
+
+struct AdjBufs {
+    double* A;              // pass-1 record [NA][NI]: ub (nc), gb (nc * DIM), tqb (NT), gyb (ns * DIM)
+    double* Ughost;         // [nc][NG] adjoint of ghost states
+    double* Uself;          // [nc][NI]
+    double* contrib;        // [KM][nc][NI]
+    const int32_t* mirror;  // [KM][NI] slot of the same face in the neighbour's list, -1 when none / ghost
+    int km;                 // slot width of contrib / mirror
+};
+
+template <int DIM, bool RUS, int KM>
+__global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ Rb, const double* __restrict__ rec,
+                                                      const double* __restrict__ gS, const double* __restrict__ gY,
+                                                      AdjBufs b) {
+    constexpr int NE = DIM + 2;
+    constexpr int NT = ViscTerms<DIM>::NT;
+    using CR = CellRec<DIM>;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI, NG = N - NI;
+    const int ns = ph.ns, nc = NE + ns;
+    const bool recon = ph.recon != 0, visc = ph.visc != 0, sdiff = ph.sdiff != 0;
+    const bool useRec = recon || visc;
+    const int64_t oG = nc, oT = nc + (int64_t)nc * DIM, oY = oT + NT;
+    double* __restrict__ A = b.A;
+    double ub[NE], gb[NE][DIM], tqb[NT], velb[DIM], auxb[DIM + 1], rhob = 0.0;
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        ub[p] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) gb[p][d] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) tqb[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) velb[d] = 0.0;
+#pragma unroll
+    for (int q = 0; q <= DIM; q++) auxb[q] = 0.0;
+    for (int sp = 0; sp < ns; sp++) {
+        A[(int64_t)(NE + sp) * NI + c] = 0.0;
+        for (int d = 0; d < DIM; d++) {
+            A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = 0.0;
+            A[(oY + (int64_t)sp * DIM + d) * NI + c] = 0.0;
+        }
+    }
+    double uc[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
+
+    for (int k = 0; k < m.K; k++) {
+        const int32_t e = m.ell[k * NI + c];
+        if (e < 0) break;
+        const int64_t f = e >> 1;
+        const int s = e & 1;
+        const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+        const bool ghost = Rc >= NI;
+        double nh[3], area, x[2][DIM];
+        load_face_na(m.fna, f, nh, area);
+        load_face_pair<DIM>(m.fdx, f, x);
+        const double ivL = m.inv_vol[L], ivR = ghost ? 0.0 : m.inv_vol[Rc];
+        double g[NE];
+#pragma unroll
+        for (int p = 0; p < NE; p++) g[p] = (ghost ? 0.0 : Rb[p * N + Rc] * ivR) - Rb[p * N + L] * ivL;
+        // both sides: cell state, face state and viscous terms
+        double uL[NE], uR[NE], fL[NE], fR[NE], tqL[NT], tqR[NT];
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int64_t cc = sd == 0 ? L : Rc;
+            double* u = sd == 0 ? uL : uR;
+            double* fu = sd == 0 ? fL : fR;
+            double* tq = sd == 0 ? tqL : tqR;
+            if (useRec && !(sd == 1 && ghost)) {
+                const double* r = rec + cc * CR::RS;
+#pragma unroll
+                for (int p = 0; p < NE; p++) {
+                    u[p] = r[p];
+                    double a = 0.0;
+                    if (recon)
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) a += r[CR::G0 + p * DIM + d] * x[sd][d];
+                    fu[p] = u[p] + a;
+                }
+#pragma unroll
+                for (int q = 0; q < NT; q++) tq[q] = r[CR::T0 + q];
+            } else {
+#pragma unroll
+                for (int p = 0; p < NE; p++) {
+                    u[p] = U[p * N + cc];
+                    fu[p] = u[p];
+                }
+#pragma unroll
+                for (int q = 0; q < NT; q++) tq[q] = 0.0;
+            }
+        }
+        // species sums of the face values
+        double QL = 0.0, QR = 0.0, QdL = 0.0;
+        for (int sp = 0; sp < ns; sp++) {
+            const int64_t comp = NE + sp;
+            const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+            double yl = U[comp * N + L], yr = U[comp * N + Rc];
+            if (recon)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    yl += gS[((int64_t)sp * DIM + d) * N + L] * x[0][d];
+                    if (!ghost) yr += gS[((int64_t)sp * DIM + d) * N + Rc] * x[1][d];
+                }
+            QL += gsp * yl;
+            QR += gsp * yr;
+            QdL += gsp * (yr - yl);
+        }
+        double ubL[NE], ubR[NE], cL = 0.0, cR = 0.0;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            ubL[p] = 0.0;
+            ubR[p] = 0.0;
+        }
+        if (ph.flux == ABL_FV_FLUX_AUSM || ph.flux == ABL_FV_FLUX_RUSANOV)
+            face_vjp<DIM, RUS>(fL, fR, QL, QR, QdL, nh, area, ph, g, ubL, ubR, cL, cR);
+        // viscous flux
+        double tqbf[NT], velbf[DIM], wg[2][DIM], aL[DIM + 1], aG[DIM + 1];
+#pragma unroll
+        for (int q = 0; q < NT; q++) tqbf[q] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) velbf[d] = 0.0;
+        if (visc) {
+            double vL[DIM], vR[DIM], vf[DIM], tqf[NT], normal[DIM];
+            const double irL = 1.0 / uL[kRho], irR = 1.0 / uR[kRho];
+            double keL = 0.0, keR = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                vL[d] = uL[kRhoU + d] * irL;
+                vR[d] = uR[kRhoU + d] * irR;
+                keL += vL[d] * vL[d];
+                keR += vR[d] * vR[d];
+                vf[d] = 0.5 * vL[d] + 0.5 * vR[d];
+                normal[d] = area * nh[d];
+            }
+            if (ghost) {
+                load_face_pair<DIM>(m.fwa, f, wg);
+                aL[0] = (uL[kRhoE] * irL - 0.5 * keL) * ph.icv;
+                aG[0] = (uR[kRhoE] * irR - 0.5 * keR) * ph.icv;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    aL[1 + d] = vL[d];
+                    aG[1 + d] = vR[d];
+                }
+                double gA[DIM + 1][DIM];
+#pragma unroll
+                for (int q = 0; q <= DIM; q++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) gA[q][d] = wg[1][d] * (aL[q] - aG[q]);
+                stress_heat<double, DIM>(ph.mu, ph.k, gA, tqR);
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) tqf[q] = 0.5 * tqL[q] + 0.5 * tqR[q];
+            viscous_flux_vjp<DIM>(normal, vf, tqf, g, tqbf, velbf);
+        }
+        // own side
+        const double* ubS = s == 0 ? ubL : ubR;
+        const double cS = s == 0 ? cL : cR;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            ub[p] += ubS[p];
+            if (recon)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gb[p][d] += ubS[p] * x[s][d];
+        }
+        if (visc) {
+#pragma unroll
+            for (int q = 0; q < NT; q++) tqb[q] += 0.5 * tqbf[q];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) velb[d] += 0.5 * velbf[d];
+        }
+        // ghost side (a ghost is always the right cell of its boundary face)
+        double ugb[NE];
+#pragma unroll
+        for (int p = 0; p < NE; p++) ugb[p] = ubR[p];
+        if (ghost && visc) {
+            double gAb[DIM + 1][DIM], tqh[NT], aGb[DIM + 1];
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) {
+                aGb[q] = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gAb[q][d] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) tqh[q] = 0.5 * tqbf[q];
+            stress_heat_vjp<DIM>(ph.mu, ph.k, tqh, gAb);
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) {
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += wg[1][d] * gAb[q][d];
+                auxb[q] += t;  // a_L: this cell
+                aGb[q] -= t;
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++) aGb[1 + d] += 0.5 * velbf[d];
+            aux_vjp<DIM>(uR, ph, aGb, ugb);
+        }
+        // species: face-state adjoints and diffusion
+        for (int sp = 0; sp < ns; sp++) {
+            const int64_t comp = NE + sp;
+            const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+            const double yb = cS * gsp;
+            A[comp * NI + c] += yb;
+            if (recon)
+                for (int d = 0; d < DIM; d++) A[(oG + comp * DIM + d) * NI + c] += yb * x[s][d];
+            double ygb = ghost ? cR * gsp : 0.0, rgb = 0.0;
+            if (sdiff) {
+                const double rl = uL[kRho], rr = uR[kRho];
+                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+                const double YL = yl / rl, YR = yr / rr;
+                double GL[DIM], GR[DIM], dot = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    GL[d] = gY[((int64_t)sp * DIM + d) * N + L];
+                    GR[d] = ghost ? wg[1][d] * (YL - YR) : gY[((int64_t)sp * DIM + d) * N + Rc];
+                    dot += nh[d] * (GL[d] + GR[d]);
+                }
+                const double cdf = -ph.diff * (0.5 * rl + 0.5 * rr) * 0.5 * area;
+                const double drho = gsp * (-ph.diff * 0.25 * area) * dot;
+                rhob += drho;  // rho_L and rho_R enter alike; this cell's side
+                for (int d = 0; d < DIM; d++) A[(oY + (int64_t)sp * DIM + d) * NI + c] += gsp * cdf * nh[d];
+                if (ghost) {
+                    // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (this cell) and Y_G
+                    double t = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
+                    double yb_c = 0.0, rb_c = 0.0;
+                    massfrac_vjp(rl, yl, t, yb_c, rb_c);
+                    A[comp * NI + c] += yb_c;
+                    rhob += rb_c;
+                    massfrac_vjp(rr, yr, -t, ygb, rgb);
+                    rgb += drho;  // rho_G in the face density
+                }
+            }
+            if (ghost) {
+                b.Ughost[comp * NG + (Rc - NI)] = ygb;
+                ugb[kRho] += rgb;
+            }
+        }
+        if (ghost)
+#pragma unroll
+            for (int p = 0; p < NE; p++) b.Ughost[p * NG + (Rc - NI)] = ugb[p];
+    }
+    if (visc) {
+        double ab[DIM + 1];
+        ab[0] = auxb[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ab[1 + d] = velb[d] + auxb[1 + d];
+        aux_vjp<DIM>(uc, ph, ab, ub);
+    }
+    ub[kRho] += rhob;
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        A[(int64_t)p * NI + c] = ub[p];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) A[(oG + (int64_t)p * DIM + d) * NI + c] = gb[p][d];
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) A[(oT + q) * NI + c] = tqb[q];
+}
+
+// Barth-Jespersen limiter of one component with its active pieces: phi = min(1, (umax - uc) / dpos, (umin - uc) /
+// dneg); imax / imin = slot of the extreme neighbour (-1: the cell itself), kpos / kneg = slot of the extreme face
+// projection.
+struct BJFwd {
+    double umax, umin, dpos, dneg, phi;
+    int imax, imin, kpos, kneg, active;  // active: 0 none (phi = 1), 1 max side, 2 min side
+};
+__device__ inline void bj_finish(double uc, BJFwd& L) {
+    L.phi = 1.0;
+    L.active = 0;
+    if (L.dpos > 0.0) {
+        const double r = (L.umax - uc) / L.dpos;
+        if (r < L.phi) {
+            L.phi = r;
+            L.active = 1;
+        }
+    }
+    if (L.dneg < 0.0) {
+        const double r = (L.umin - uc) / L.dneg;
+        if (r < L.phi) {
+            L.phi = r;
+            L.active = 2;
+        }
+    }
+}
+
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const int32_t* __restrict__ ghost_slot,
+                                                      const int32_t* __restrict__ slot_kind, AdjBufs b) {
+    constexpr int NE = DIM + 2;
+    constexpr int NT = ViscTerms<DIM>::NT;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI, NG = N - NI;
+    const int ns = ph.ns, nc = NE + ns;
+    const bool recon = ph.recon != 0, visc = ph.visc != 0, sdiff = ph.sdiff != 0;
+    const bool bj = ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN;
+    const int64_t oG = nc, oT = nc + (int64_t)nc * DIM, oY = oT + NT;
+    const double* __restrict__ A = b.A;
+    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+    double uc[NE], ubc[NE], ce[KM][NE];
+    int64_t nb[KM];
+    bool valid[KM], bnd[KM];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        uc[p] = U[p * N + c];
+        ubc[p] = A[(int64_t)p * NI + c];
+    }
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = m.snbr[k * NI + c];
+        valid[k] = e >= 0;
+        nb[k] = valid[k] ? (int64_t)e : c;
+        bnd[k] = nb[k] >= NI;
+#pragma unroll
+        for (int p = 0; p < NE; p++) ce[k][p] = 0.0;
+    }
+    // ghost contributions are added to the ghost's adjoint (only this cell touches its ghosts)
+    auto to_ghost = [&](int64_t g, int comp, double v) { b.Ughost[(int64_t)comp * NG + (g - NI)] += v; };
+    for (int sp = 0; sp < ns; sp++)
+#pragma unroll
+        for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + NE + sp) * NI + c] = 0.0;
+
+    if (recon) {
+        for (int p = 0; p < NE; p++) {
+            double G[DIM], Gb[DIM], gbl[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                G[d] = 0.0;
+                gbl[d] = A[(oG + (int64_t)p * DIM + d) * NI + c];
+            }
+            BJFwd lim{uc[p], uc[p], 0.0, 0.0, 1.0, -1, -1, -1, -1, 0};
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k]) continue;
+                const double un = U[p * N + nb[k]];
+                if (!bnd[k])
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) G[d] += m.sw[(k * DIM + d) * NI + c] * (un - uc[p]);
+                if (un > lim.umax) {
+                    lim.umax = un;
+                    lim.imax = k;
+                }
+                if (un < lim.umin) {
+                    lim.umin = un;
+                    lim.imin = k;
+                }
+            }
+            if (bj) {
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
+                    double D = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) D += G[d] * m.slim[(k * DIM + d) * NI + c];
+                    if (D > lim.dpos) {
+                        lim.dpos = D;
+                        lim.kpos = k;
+                    }
+                    if (D < lim.dneg) {
+                        lim.dneg = D;
+                        lim.kneg = k;
+                    }
+                }
+                bj_finish(uc[p], lim);
+            }
+            const double phi = lim.phi;
+            double phib = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                Gb[d] = phi * gbl[d];
+                phib += gbl[d] * G[d];
+            }
+            if (bj && lim.active) {
+                const bool mx = lim.active == 1;
+                const double den = mx ? lim.dpos : lim.dneg;
+                const int ext = mx ? lim.imax : lim.imin;
+                const int kd = mx ? lim.kpos : lim.kneg;
+                const double ub_ext = phib / den;
+                ubc[p] -= ub_ext;
+                if (ext < 0)
+                    ubc[p] += ub_ext;
+                else if (bnd[ext])
+                    to_ghost(nb[ext], p, ub_ext);
+                else
+                    ce[ext][p] += ub_ext;
+                const double den_b = -phi * phib / den;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++) G[d] = Gb[d];
+            // least-squares gradient: G = sum_k w_k (u_n - u_c)
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k] || bnd[k]) continue;
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += m.sw[(k * DIM + d) * NI + c] * G[d];
+                ce[k][p] += t;
+                ubc[p] -= t;
+            }
+        }
+    }
+    if (visc) {
+        double tqbl[NT], gAb[DIM + 1][DIM], acb[DIM + 1];
+#pragma unroll
+        for (int q = 0; q < NT; q++) tqbl[q] = A[(oT + q) * NI + c];
+#pragma unroll
+        for (int q = 0; q <= DIM; q++) {
+            acb[q] = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) gAb[q][d] = 0.0;
+        }
+        stress_heat_vjp<DIM>(ph.mu, ph.k, tqbl, gAb);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            if (!valid[k]) continue;
+            double anb[DIM + 1], un[NE], unb[NE];
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) {
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += wa_arr[(k * DIM + d) * NI + c] * gAb[q][d];
+                anb[q] = t;
+                acb[q] -= t;
+            }
+#pragma unroll
+            for (int p = 0; p < NE; p++) {
+                un[p] = U[p * N + nb[k]];
+                unb[p] = 0.0;
+            }
+            aux_vjp<DIM>(un, ph, anb, unb);
+#pragma unroll
+            for (int p = 0; p < NE; p++) {
+                if (bnd[k])
+                    to_ghost(nb[k], p, unb[p]);
+                else
+                    ce[k][p] += unb[p];
+            }
+        }
+        aux_vjp<DIM>(uc, ph, acb, ubc);
+    }
+    // species: limited densityYi gradients and Yi aux gradients
+    for (int sp = 0; sp < ns; sp++) {
+        const int64_t comp = NE + sp;
+        const double ycs = U[comp * N + c];
+        double usb = A[comp * NI + c];
+        if (recon) {
+            double G[DIM], Gb[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) G[d] = 0.0;
+            BJFwd lim{ycs, ycs, 0.0, 0.0, 1.0, -1, -1, -1, -1, 0};
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k]) continue;
+                const double un = U[comp * N + nb[k]];
+                if (!bnd[k])
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) G[d] += m.sw[(k * DIM + d) * NI + c] * (un - ycs);
+                if (un > lim.umax) {
+                    lim.umax = un;
+                    lim.imax = k;
+                }
+                if (un < lim.umin) {
+                    lim.umin = un;
+                    lim.imin = k;
+                }
+            }
+            if (bj) {
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
+                    double D = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) D += G[d] * m.slim[(k * DIM + d) * NI + c];
+                    if (D > lim.dpos) {
+                        lim.dpos = D;
+                        lim.kpos = k;
+                    }
+                    if (D < lim.dneg) {
+                        lim.dneg = D;
+                        lim.kneg = k;
+                    }
+                }
+                bj_finish(ycs, lim);
+            }
+            double phib = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const double gbl = A[(oG + comp * DIM + d) * NI + c];
+                Gb[d] = lim.phi * gbl;
+                phib += gbl * G[d];
+            }
+            if (bj && lim.active) {
+                const bool mx = lim.active == 1;
+                const double den = mx ? lim.dpos : lim.dneg;
+                const int ext = mx ? lim.imax : lim.imin;
+                const int kd = mx ? lim.kpos : lim.kneg;
+                const double ub_ext = phib / den;
+                usb -= ub_ext;
+                if (ext < 0)
+                    usb += ub_ext;
+                else if (bnd[ext])
+                    to_ghost(nb[ext], (int)comp, ub_ext);
+                else
+                    b.contrib[((int64_t)ext * nc + comp) * NI + c] += ub_ext;
+                const double den_b = -lim.phi * phib / den;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
+            }
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k] || bnd[k]) continue;
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += m.sw[(k * DIM + d) * NI + c] * Gb[d];
+                b.contrib[((int64_t)k * nc + comp) * NI + c] += t;
+                usb -= t;
+            }
+        }
+        if (sdiff) {
+            double gyb[DIM], ycb = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) gyb[d] = A[(oY + (int64_t)sp * DIM + d) * NI + c];
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k]) continue;
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += wa_arr[(k * DIM + d) * NI + c] * gyb[d];
+                ycb -= t;
+                double yb_n = 0.0, rb_n = 0.0;
+                massfrac_vjp(U[nb[k]], U[comp * N + nb[k]], t, yb_n, rb_n);
+                if (bnd[k]) {
+                    to_ghost(nb[k], (int)comp, yb_n);
+                    to_ghost(nb[k], kRho, rb_n);
+                } else {
+                    b.contrib[((int64_t)k * nc + comp) * NI + c] += yb_n;
+                    ce[k][kRho] += rb_n;
+                }
+            }
+            double rb_c = 0.0;
+            massfrac_vjp(uc[kRho], ycs, ycb, usb, rb_c);
+            ubc[kRho] += rb_c;
+        }
+        b.Uself[comp * NI + c] = usb;
+    }
+#pragma unroll
+    for (int p = 0; p < NE; p++) b.Uself[(int64_t)p * NI + c] = ubc[p];
+#pragma unroll
+    for (int k = 0; k < KM; k++)
+#pragma unroll
+        for (int p = 0; p < NE; p++) b.contrib[((int64_t)k * nc + p) * NI + c] = ce[k][p];
+}
+
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<double> m, Phys ph,
+                                                       const int32_t* __restrict__ ghost_slot,
+                                                       const int32_t* __restrict__ slot_kind, AdjBufs b,
+                                                       double* __restrict__ Ub) {
+    constexpr int NE = DIM + 2;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI, NG = N - NI;
+    const int nc = NE + ph.ns;
+    int64_t nb[KM];
+    int mk[KM];
+    bool from_ghost[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = m.snbr[k * NI + c];
+        nb[k] = e;
+        mk[k] = b.mirror[k * NI + c];
+        from_ghost[k] = e >= NI && slot_kind[ghost_slot[e - NI]] == ABL_FV_BC_OUTFLOW;
+    }
+    for (int p = 0; p < nc; p++) {
+        double acc = b.Uself[(int64_t)p * NI + c];
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            if (nb[k] < 0) continue;
+            if (nb[k] < NI)
+                acc += b.contrib[((int64_t)mk[k] * nc + p) * NI + nb[k]];
+            else if (from_ghost[k])
+                acc += b.Ughost[(int64_t)p * NG + (nb[k] - NI)];
+        }
+        Ub[(int64_t)p * N + c] = acc;
+    }
+}
+
+// theta_bar contributions of each inflow boundary face from its ghost's adjoint: dU_ghost/dtheta^T Ubar_ghost.
+__global__ __launch_bounds__(kBlock) void k_adj_theta(int64_t nb, int dim, int ns, int64_t N, int64_t NI,
+                                                      const int32_t* __restrict__ bface, const int32_t* __restrict__ bslot,
+                                                      const int32_t* __restrict__ fcell, int64_t F,
+                                                      const int32_t* __restrict__ slot_kind,
+                                                      const double* __restrict__ slot_param,
+                                                      const double* __restrict__ slot_Y, Phys ph,
+                                                      const double* __restrict__ Ughost, double* __restrict__ contrib) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int slot = bslot[i];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    if (slot_kind[slot] == ABL_FV_BC_INFLOW) {
+        const int64_t g = fcell[F + bface[i]] - NI, NG = N - NI;
+        const double* sp_ = slot_param + 6 * slot;
+        const double rho = sp_[0], um = sp_[1], T = sp_[2];
+        const double ubr = Ughost[kRho * NG + g], ube = Ughost[kRhoE * NG + g];
+        t0 = ubr + ube * (ph.cv * T + 0.5 * um * um);
+        t1 = ube * rho * um;
+        t2 = ube * rho * ph.cv;
+        for (int d = 0; d < dim; d++) {
+            const double ubm = Ughost[(kRhoU + d) * NG + g];
+            t0 += ubm * um * sp_[3 + d];
+            t1 += ubm * rho * sp_[3 + d];
+        }
+        for (int sp = 0; sp < ns; sp++) t0 += Ughost[(dim + 2 + sp) * NG + g] * slot_Y[slot * ns + sp];
+    }
+    contrib[3 * i + 0] = t0;
+    contrib[3 * i + 1] = t1;
+    contrib[3 * i + 2] = t2;
+}
+
 // ---- compile-time variant selection
+using AdjFacesKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
+                                const double*, AdjBufs);
+using AdjCellsKernel = void (*)(MeshView<double>, Phys, const double*, const int32_t*, const int32_t*, AdjBufs);
+using AdjGatherKernel = void (*)(MeshView<double>, Phys, const int32_t*, const int32_t*, AdjBufs, double*);
+template <int DIM, int KM>
+void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjCellsKernel& ck, AdjGatherKernel& gk) {
+    fk = rusanov ? k_adj_faces<DIM, true, KM> : k_adj_faces<DIM, false, KM>;
+    ck = k_adj_cells<DIM, KM>;
+    gk = k_adj_gather<DIM, KM>;
+}
 template <typename G>
 using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*, void*);
 using GradKernel = GradKernelT<double>;
@@ -1482,6 +2136,14 @@ GradKernelT<G> pick_gradients_k(bool recon, bool visc, int lim) {
 }
 // Compile-time slot width of the cell -> face table: 4 (quads, tets), 6 (hexes) or 8.
 inline int slot_width(int K) { return K <= 4 ? 4 : (K <= 6 ? 6 : 8); }
+template <int DIM>
+void pick_adj(bool rusanov, int K, AdjFacesKernel& fk, AdjCellsKernel& ck, AdjGatherKernel& gk) {
+    switch (slot_width(K)) {
+        case 4: pick_adj_k<DIM, 4>(rusanov, fk, ck, gk); break;
+        case 6: pick_adj_k<DIM, 6>(rusanov, fk, ck, gk); break;
+        default: pick_adj_k<DIM, 8>(rusanov, fk, ck, gk); break;
+    }
+}
 template <typename G, int DIM>
 GradKernelT<G> pick_gradients(bool recon, bool visc, int lim, int K) {
     switch (slot_width(K)) {
@@ -1763,6 +2425,12 @@ struct abl_fv_solver {
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
     std::vector<double> h_slot_param, h_slot_Y;
     abl::VjpKernel vjp_kernel = nullptr;
+    // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
+    abl::AdjFacesKernel adj_faces = nullptr;
+    abl::AdjCellsKernel adj_cells = nullptr;
+    abl::AdjGatherKernel adj_gather = nullptr;
+    abl::DevBuf<int32_t> mirror;
+    abl::DevBuf<double> adjA, adjUghost, adjUself, adjContrib;
     abl::VjpThetaKernel vjp_theta_kernel = nullptr;
     abl::GradKernel grad_kernel = nullptr;
     abl::FluxKernel flux_kernel = nullptr;
@@ -1890,7 +2558,7 @@ struct abl_fv_solver {
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         gradA.bytes() + rec.bytes() + adjA.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
 };
@@ -2042,9 +2710,29 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             if (mesh->dim == 3) {
                 s->vjp_kernel = abl::pick_vjp<3>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<3>(rus);
+                abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_cells, s->adj_gather);
             } else {
                 s->vjp_kernel = abl::pick_vjp<2>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<2>(rus);
+                abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_cells, s->adj_gather);
+            }
+            if (s->recon || s->visc) {
+                // mirror slots: for slot k of cell c, the slot of the same face in the neighbour's list
+                const int KM = abl::slot_width(L.K);
+                std::vector<int32_t> mir((size_t)KM * L.NI, -1), at0((size_t)L.F, -1), at1((size_t)L.F, -1);
+                for (int k = 0; k < L.K; k++)
+                    for (int64_t c = 0; c < L.NI; c++) {
+                        const int32_t e = L.ell[(size_t)k * L.NI + c];
+                        if (e < 0) continue;
+                        ((e & 1) ? at1 : at0)[(size_t)(e >> 1)] = k;
+                    }
+                for (int k = 0; k < L.K; k++)
+                    for (int64_t c = 0; c < L.NI; c++) {
+                        const int32_t e = L.ell[(size_t)k * L.NI + c];
+                        if (e < 0) continue;
+                        mir[(size_t)k * L.NI + c] = ((e & 1) ? at0 : at1)[(size_t)(e >> 1)];
+                    }
+                s->mirror.upload(mir);
             }
         }
         s->bface.upload(bf);
@@ -2308,16 +2996,92 @@ extern "C" int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* laun
     return 0;
 }
 
+namespace {
+// theta_bar: fixed-order host reduction of per-boundary-face contributions (deterministic)
+void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
+    std::vector<double> contrib((size_t)3 * h->n_bfaces);
+    std::vector<int32_t> slots(h->n_bfaces);
+    HIP_CHECK(hipMemcpyAsync(contrib.data(), h->theta_contrib.ptr, contrib.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < h->n_bfaces; i++)
+        for (int j = 0; j < 3; j++) theta_bar[3 * slots[i] + j] += contrib[3 * i + j];
+}
+
+int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, hipStream_t st) {
+    try {
+        const abl::Layout& L = h->host;
+        const int KM = abl::slot_width(L.K);
+        const int nt = L.dim * (L.dim + 1) / 2 + L.dim;
+        const int64_t NG = L.N - L.NI;
+        const size_t na = (size_t)L.nc + (size_t)L.nc * L.dim + nt + (size_t)L.ns * L.dim;
+        if (!h->adjA.ptr) {
+            h->adjA.alloc(na * L.NI);
+            h->adjUghost.alloc((size_t)L.nc * std::max<int64_t>(NG, 1));
+            h->adjUself.alloc((size_t)L.nc * L.NI);
+            h->adjContrib.alloc((size_t)KM * L.nc * L.NI);
+        }
+        const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
+        if (h->n_bfaces)
+            h->timed(ABL_FV_PHASE_GHOST, st, [&] {
+                hipLaunchKernelGGL(abl::k_ghost_fill, dim3(grid_for(h->n_bfaces)), block, 0, st, h->n_bfaces,
+                                   h->bface.ptr, h->bslot.ptr, h->fcell.ptr, L.F, L.N, L.nc, h->slot_kind.ptr,
+                                   h->slot_state.ptr, U);
+            });
+        const auto v = h->view();
+        h->timed(ABL_FV_PHASE_GRADIENTS, st, [&] {
+            hipLaunchKernelGGL(h->grad_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr,
+                               (void*)h->rec.ptr);
+        });
+        if (L.ns)
+            h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
+                if (L.dim == 3)
+                    hipLaunchKernelGGL((abl::k_species_gradients<double, 3>), grid, block, 0, st, v, h->ph, U,
+                                       h->gradC.ptr, h->gradA.ptr);
+                else
+                    hipLaunchKernelGGL((abl::k_species_gradients<double, 2>), grid, block, 0, st, v, h->ph, U,
+                                       h->gradC.ptr, h->gradA.ptr);
+            });
+        abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM};
+        h->timed(ABL_FV_PHASE_VJP, st, [&] {
+            hipLaunchKernelGGL(h->adj_faces, grid, block, 0, st, v, h->ph, U, Rbar, h->rec.ptr, h->gradC.ptr,
+                               h->gradA.ptr, b);
+            hipLaunchKernelGGL(h->adj_cells, grid, block, 0, st, v, h->ph, U, h->ghost_slot.ptr, h->slot_kind.ptr, b);
+            hipLaunchKernelGGL(h->adj_gather, grid, block, 0, st, v, h->ph, h->ghost_slot.ptr, h->slot_kind.ptr, b,
+                               Ubar);
+        });
+        if (L.N > L.NI)
+            hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
+                               L.nc);
+        HIP_CHECK(hipGetLastError());
+        if (theta_bar && h->n_bfaces) {
+            if (!h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
+            hipLaunchKernelGGL(abl::k_adj_theta, dim3(grid_for(h->n_bfaces)), block, 0, st, h->n_bfaces, L.dim, L.ns,
+                               L.N, L.NI, h->bface.ptr, h->bslot.ptr, h->fcell.ptr, L.F, h->slot_kind.ptr,
+                               h->slot_param.ptr, h->slot_Y.ptr, h->ph, h->adjUghost.ptr, h->theta_contrib.ptr);
+            HIP_CHECK(hipGetLastError());
+            reduce_theta(h, st, theta_bar);
+        }
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
+    }
+    return 0;
+}
+}  // namespace
+
 extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
                               double* theta_bar, abl_fv_stream stream) {
     (void)time;
     if (!h || !U || !Rbar || !Ubar) return fail(h, "abl_fv_rhs_vjp: null argument");
     const abl::Layout& L = h->host;
     const int fl = h->phys.flux_calculator;
-    if (h->form != ABL_FV_FORM_FP64 || h->recon || h->visc ||
+    if (h->form != ABL_FV_FORM_FP64 ||
         (fl != ABL_FV_FLUX_AUSM && fl != ABL_FV_FLUX_RUSANOV && fl != ABL_FV_FLUX_OFF && fl != ABL_FV_FLUX_NONE))
-        return fail(h, "abl_fv_rhs_vjp: supported for the FP64 form, first-order reconstruction, no diffusion and the "
-                       "Ausm / Rusanov / Off flux calculators");
+        return fail(h, "abl_fv_rhs_vjp: supported for the FP64 form and the Ausm / Rusanov / Off flux calculators");
+    if (h->recon && h->phys.limiter != ABL_FV_LIMITER_NONE && h->phys.limiter != ABL_FV_LIMITER_BARTH_JESPERSEN)
+        return fail(h, "abl_fv_rhs_vjp: with reconstruction the adjoint supports the Barth-Jespersen limiter or none");
+    if (h->recon || h->visc) return vjp_second_order(h, U, Rbar, Ubar, theta_bar, (hipStream_t)stream);
     try {
         hipStream_t st = (hipStream_t)stream;
         if (h->n_bfaces) {

@@ -1,6 +1,7 @@
-"""Hand-written adjoint (abl_fv_rhs_vjp) of the first-order RHS against (1) torch.autograd over an independent
-PyTorch FP64 restatement (tests/torch_rhs.py, itself pinned to the oracle) and (2) central finite differences
-(h = 1e-6) of the CPU oracle, including the inflow parameters theta = (rho_in, |u_in|, T_in)."""
+"""Hand-written adjoint (abl_fv_rhs_vjp) of the RHS against (1) torch.autograd over independent PyTorch FP64
+restatements (tests/torch_rhs.py: first order, and second order with least squares + Barth-Jespersen, viscous and
+species diffusion terms; both pinned to the oracle) and (2) central finite differences (h = 1e-6) of the CPU
+oracle, including the inflow parameters theta = (rho_in, |u_in|, T_in)."""
 import numpy as np
 import pytest
 import torch as _torch
@@ -9,7 +10,7 @@ from ablate_amd import autograd as fv_autograd
 from ablate_amd import synthetic
 from ablate_amd.mesh import BoxMesh, TetBoxMesh
 from tests.fvcases import Case, all_outflow, inflow_x
-from tests.test_torch_restatement import PHASE, thetas_of, torch_model
+from tests.test_torch_restatement import PHASE, _cases2, thetas_of, torch_model, torch_model2
 
 pytestmark = pytest.mark.gpu
 
@@ -129,8 +130,87 @@ def test_vjp_deterministic_and_rejects_unsupported(torch_cuda):
     solver.rhs_vjp(0.0, U, Rb, b)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
-    recon = Case("lsq", case.mesh, recon="leastsquares", n_species=10, bcs=case.bcs).solver()
+    recon = Case("lsq", case.mesh, recon="leastsquares", limiter="vanleer", n_species=10, bcs=case.bcs).solver()
     from ablate_amd._lib import NativeError
 
-    with pytest.raises(NativeError, match="first-order"):
+    with pytest.raises(NativeError, match="Barth-Jespersen"):
         recon.rhs_vjp(0.0, U, Rb, a)
+
+
+# ---- second order (least squares + Barth-Jespersen or none, viscous flux, species diffusion)
+
+def adjoint_cases2():
+    chan = lambda n: BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [False, True, True])
+    return _cases2() + [
+        Case("c5_like_species10_lsq_bj_rusanov", TetBoxMesh(3, 0.2), flux="rusanov", n_species=10,
+             recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125, diff=2e-5, bcs=all_outflow(6),
+             phase=PHASE),
+        Case("ns_inflow_ausm_lsq_bj", chan(6), n_species=2, recon="leastsquares", limiter="barthjespersen", mu=0.5,
+             k=5.0, diff=0.3, bcs=inflow_x(2), phase=PHASE),
+        Case("viscous_first_order", chan(5), mu=0.5, k=5.0, bcs=inflow_x(), phase=PHASE),
+    ]
+
+
+CASES2 = adjoint_cases2()
+
+
+@pytest.mark.parametrize("case", CASES2, ids=lambda c: c.name)
+def test_second_order_vjp_matches_autograd(case, torch_cuda):
+    torch = torch_cuda
+    print(f"[adjoint2] {case.name}", flush=True)
+    U0 = case.initial_state()
+    Rb = _seed(case, U0)
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ub = torch.empty_like(U)
+    theta_bar = solver.rhs_vjp(0.0, U, torch.tensor(Rb, device="cuda"), Ub)
+    torch.cuda.synchronize()
+    Ug = Ub.cpu().numpy()
+    Ut = _torch.tensor(U0, requires_grad=True)
+    thetas = {s: t.clone().requires_grad_(True) for s, t in thetas_of(case).items()}
+    out = torch_model2(case)(Ut, thetas)
+    grads = _torch.autograd.grad((out * _torch.tensor(Rb)).sum(), [Ut] + list(thetas.values()))
+    ref = grads[0].numpy()
+    NI = case.mesh.n_interior
+    err = np.linalg.norm(Ug[:, :NI] - ref[:, :NI]) / np.linalg.norm(ref[:, :NI])
+    print(f"[adjoint2] {case.name} rel err vs autograd {err:.3e}", flush=True)
+    assert err <= 1e-10, err
+    assert not np.any(Ug[:, NI:])
+    for (slot, t), g in zip(thetas.items(), grads[1:]):
+        np.testing.assert_allclose(theta_bar[slot], g.numpy(), rtol=1e-9, atol=1e-12 * np.abs(g.numpy()).max())
+
+
+@pytest.mark.parametrize("case", [c for c in CASES2 if c.bcs and 0 in c.bcs and c.bcs[0]["kind"] == "inflow"][:2],
+                         ids=lambda c: c.name)
+def test_second_order_vjp_matches_oracle_central_differences(case, torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    U0 = case.initial_state()
+    NI = case.mesh.n_interior
+    Rb = _seed(case, U0)
+    V = np.zeros_like(U0)
+    V[:, :NI] = rng.normal(size=(U0.shape[0], NI)) * np.abs(U0[:, :NI]).mean(axis=1, keepdims=True)
+    h = 1e-6
+    Rp, _ = case.oracle_rhs(U0 + h * V)
+    Rm, _ = case.oracle_rhs(U0 - h * V)
+    fd = np.sum(Rb * (Rp - Rm)) / (2 * h)
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ub = torch.empty_like(U)
+    solver.rhs_vjp(0.0, U, torch.tensor(Rb, device="cuda"), Ub)
+    ad = float((Ub.cpu().numpy() * V).sum())
+    assert ad == pytest.approx(fd, rel=1e-5)
+
+
+def test_second_order_vjp_deterministic(torch_cuda):
+    torch = torch_cuda
+    case = CASES2[0]
+    U0 = case.initial_state()
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Rb = torch.tensor(_seed(case, U0), device="cuda")
+    a, b = torch.empty_like(U), torch.empty_like(U)
+    solver.rhs_vjp(0.0, U, Rb, a)
+    solver.rhs_vjp(0.0, U, Rb, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
