@@ -89,6 +89,7 @@ def _cases2():
              limiter="none", mu=0.3, k=2.0, bcs=all_outflow(6), phase=PHASE),
         Case("quad2d_species", BoxMesh([7, 6], [0, 0], [2, 1], [False, True]), n_species=2, recon="leastsquares",
              limiter="barthjespersen", mu=0.2, k=1.0, diff=0.1, bcs=inflow_x(2), phase=PHASE),
+        Case("viscous_first_order", chan, n_species=2, mu=0.5, k=5.0, diff=0.3, bcs=inflow_x(2), phase=PHASE),
     ]
 
 
@@ -103,7 +104,7 @@ def torch_model2(case):
         else:
             bcs[slot] = {"kind": "outflow"}
     return TorchRHS2(case.mesh, case.gamma, case.Rgas, case.n_species, case.flux, bcs, case.limiter, case.mu, case.k,
-                     case.diff)
+                     case.diff, recon=case.recon == "leastsquares")
 
 
 @pytest.mark.parametrize("case", _cases2(), ids=lambda c: c.name)

@@ -113,14 +113,14 @@ class TorchRHS2:
     Weights come from the oracle's least-squares pseudo-inverse."""
 
     def __init__(self, mesh, gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", bcs=None, limiter="barthjespersen",
-                 mu=0.0, k=0.0, diff=0.0, device="cpu"):
+                 mu=0.0, k=0.0, diff=0.0, device="cpu", recon=True):
         from oracle import binding as orc
 
         a = mesh.arrays
         self.base = TorchRHS(mesh, gamma, Rgas, n_species, flux, bcs, device)
         self.dim, self.N, self.NI, self.ns = a.dim, a.n_cells, a.n_interior, n_species
         self.nc = self.dim + 2 + n_species
-        self.limiter, self.mu, self.k, self.diff = limiter, mu, k, diff
+        self.limiter, self.mu, self.k, self.diff, self.recon = limiter, mu, k, diff, recon
         self.visc = bool(mu or k or diff)
         dim, NI, F = self.dim, self.NI, a.n_faces
         fc = a.face_cells
@@ -206,7 +206,7 @@ class TorchRHS2:
         Un = Ug[:, self.nbr]
         delta = (Un - Uc[:, :, None]) * self.mask[None]
         G = torch.einsum("pck,ckd->pcd", delta, self.w)
-        Glim = self._limited(Uc, Un, G)
+        Glim = self._limited(Uc, Un, G) if self.recon else torch.zeros_like(G)
         GN = torch.cat([Glim, torch.zeros((self.nc, N - NI, dim), dtype=U.dtype, device=U.device)], dim=1)
         uL = Ug[:, b.L] + (GN[:, b.L] * self.dxL[None]).sum(-1)
         uR = Ug[:, b.R] + (GN[:, b.R] * self.dxR[None]).sum(-1)
