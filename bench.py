@@ -180,8 +180,9 @@ def mixed_leg(fv, synthetic, mesh, mu, k, R64, U, mach, args, fp64_ms):
 def tets_leg(fv, synthetic, args):
     """C5: jittered tetrahedral unit cube (n^3 cubes x 6 Kuhn tets, interior nodes moved by up to 20 % of h, RCM
     order), Navier-Stokes + 10 species (D = 2e-5), LSQ + Barth-Jespersen, Rusanov, zero-gradient ghost cells on
-    all six sides; Taylor-Green state at Mach 0.3 with Dirichlet(1) mass fractions.  FP64 form, timed like the
-    headline leg."""
+    all six sides; Taylor-Green state at Mach 0.3 with Dirichlet(1) mass fractions.  FP64 forward timed like the
+    headline leg, the mixed forward with its error against FP64, and the FP64 adjoint (seed N(0, 1)) with a
+    directional check against central differences (h = 1e-6) of the GPU forward."""
     import torch
 
     from ablate_amd.mesh import TetBoxMesh
@@ -230,6 +231,32 @@ def tets_leg(fv, synthetic, args):
     ms_mixed = (time.perf_counter() - t2) * 1e3 / args.steps
     err = per_field_rel(Rm, R)
     del sm
+    # FP64 adjoint on the same workload (second order: LSQ + Barth-Jespersen, viscous, species diffusion)
+    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior), device="cuda")
+    Ub = torch.empty_like(U)
+    for _ in range(args.warmup):
+        solver.rhs_vjp(0.0, U, Rb, Ub)
+    torch.cuda.synchronize()
+    solver.phase_times(reset=True)
+    solver.set_profiling(True)
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.rhs_vjp(0.0, U, Rb, Ub)
+    torch.cuda.synchronize()
+    ms_adj = (time.perf_counter() - t3) * 1e3 / args.steps
+    solver.set_profiling(False)
+    adj_phases = solver.phase_times(reset=True)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    NI = mesh.n_interior
+    V = torch.zeros_like(U)
+    scale = U[:, :NI].abs().mean(dim=1, keepdim=True)
+    V[:, :NI] = torch.randn(U.shape[0], NI, dtype=torch.float64, device="cuda", generator=g) * scale
+    h = 1e-6
+    Rp, Rq = torch.zeros_like(U), torch.zeros_like(U)
+    solver.compute_rhs(0.0, U + h * V, Rp)
+    solver.compute_rhs(0.0, U - h * V, Rq)
+    fd = float((Rb * (Rp - Rq)).sum().item() / (2 * h))
+    ad = float((Ub * V).sum().item())
     return {
         "workload": f"ns_10species_fp64_tets{n}_lsq_bj_rusanov",
         "cells": cells,
@@ -247,6 +274,10 @@ def tets_leg(fv, synthetic, args):
                   "ms_per_step": round(ms_mixed, 4), "speedup_vs_fp64": round(ms / ms_mixed, 3),
                   "rel_err_vs_fp64_max": float(f"{max(err):.3e}"),
                   "error_norm": "max over the 15 conserved fields of ||R_mixed - R_fp64|| / ||R_fp64||"},
+        "adjoint": {"value": round(cells / (ms_adj * 1e-3) / 1e6, 3), "unit": "Mcell/s", "ms_per_step": round(ms_adj, 4),
+                    "adjoint_over_forward": round(ms_adj / ms, 3),
+                    "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in adj_phases.items() if c},
+                    "directional_check": {"fd_central_h1e-6": fd, "vjp": ad, "rel_diff": abs(fd - ad) / abs(fd)}},
     }
 
 
