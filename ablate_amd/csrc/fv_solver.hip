@@ -366,8 +366,9 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
 }
 
 // densityYi gradients (limited, one field) into gS [ns*DIM][N] and Yi aux gradients (unlimited) into gY [ns*DIM][N];
-// FP64 arithmetic, stored as G (double: FP64 form, float: mixed form).
-template <typename G, int DIM>
+// FP64 arithmetic, stored as G (double: FP64 form, float: mixed form).  The cell's slot geometry (neighbours,
+// weights, limiter vectors, neighbour densities) is loaded once for all species.
+template <typename G, int DIM, int KM>
 __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
                                                               G* __restrict__ gradC, G* __restrict__ gradA) {
     constexpr int NE = DIM + 2;
@@ -376,48 +377,68 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phy
     const int64_t N = m.N, NI = m.NI;
     const bool recon = ph.recon != 0;
     const bool aux = ph.sdiff != 0;
+    const bool bj = ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN;
     const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     const double irho_c = 1.0 / U[kRho * N + c];
+    int64_t nb[KM];
+    bool valid[KM], inner[KM];
+    double w[KM][DIM], wa[KM][DIM], xl[KM][DIM], irn[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = m.snbr[k * NI + c];
+        valid[k] = e >= 0;
+        nb[k] = valid[k] ? (int64_t)e : c;
+        inner[k] = valid[k] && nb[k] < NI;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            w[k][d] = recon ? double(m.sw[(k * DIM + d) * NI + c]) : 0.0;
+            wa[k][d] = aux ? double(wa_arr[(k * DIM + d) * NI + c]) : 0.0;
+            xl[k][d] = (recon && m.slim) ? double(m.slim[(k * DIM + d) * NI + c]) : 0.0;
+        }
+        irn[k] = aux ? 1.0 / U[kRho * N + nb[k]] : 0.0;
+    }
     bool cancel = false;
     for (int sp = 0; sp < ph.ns; sp++) {
         const int64_t comp = NE + sp;
         const double uc = U[comp * N + c];
         const double yc = uc * irho_c;
+        double un[KM];
+#pragma unroll
+        for (int k = 0; k < KM; k++) un[k] = U[comp * N + nb[k]];
         double g[DIM], ga[DIM], umin = uc, umax = uc;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             g[d] = 0.0;
             ga[d] = 0.0;
         }
-        for (int k = 0; k < m.K; k++) {
-            const int32_t e = m.snbr[k * NI + c];
-            if (e < 0) break;
-            const int64_t n = e;
-            const double un = U[comp * N + n];
-            if (recon) {
-                umin = fmin(umin, un);
-                umax = fmax(umax, un);
-                if (n < NI) {
-                    const double delta = un - uc;  // both sides: + w.(u_n - u_c), see k_gradients
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) g[d] += m.sw[(k * DIM + d) * NI + c] * delta;
+        for (int k = 0; k < KM; k++) {
+            if (!valid[k]) continue;
+            if (recon) {
+                umin = fmin(umin, un[k]);
+                umax = fmax(umax, un[k]);
+                if (inner[k]) {
+                    const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
                 }
             }
             if (aux) {
-                const double delta = un / U[kRho * N + n] - yc;
+                const double delta = un[k] * irn[k] - yc;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) ga[d] += wa_arr[(k * DIM + d) * NI + c] * delta;
+                for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
             }
         }
         if (recon) {
             double phi = DBL_MAX;
-            if (ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN) {
+            if (bj) {
                 double dpos = 0.0, dneg = 0.0;
-                for (int k = 0; k < m.K; k++) {
-                    if (m.snbr[k * NI + c] < 0) break;
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
                     double D = 0.0;
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) D += g[d] * m.slim[(k * DIM + d) * NI + c];
+                    for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
                     dpos = fmax(dpos, D);
                     dneg = fmin(dneg, D);
                 }
@@ -425,14 +446,13 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phy
                 if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
                 if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
             } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
-                for (int k = 0; k < m.K; k++) {
-                    const int32_t e = m.snbr[k * NI + c];
-                    if (e < 0) break;
-                    const int64_t n = e;
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
                     double denom = 0.0;
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) denom += g[d] * m.slim[(k * DIM + d) * NI + c];
-                    const double flim = 0.5 * (U[comp * N + n] - uc) / denom;
+                    for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
+                    const double flim = 0.5 * (un[k] - uc) / denom;
                     phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
                 }
             } else {
@@ -2116,6 +2136,16 @@ void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjCellsKernel& ck, AdjGatherK
 template <typename G>
 using GradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*, void*);
 using GradKernel = GradKernelT<double>;
+template <typename G>
+using SpeciesGradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*);
+template <typename G, int DIM>
+SpeciesGradKernelT<G> pick_species_gradients(int K) {
+    switch (K <= 4 ? 4 : (K <= 6 ? 6 : 8)) {
+        case 4: return k_species_gradients<G, DIM, 4>;
+        case 6: return k_species_gradients<G, DIM, 6>;
+        default: return k_species_gradients<G, DIM, 8>;
+    }
+}
 using FluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
                             double*, int);
 using FaceFluxKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
@@ -2433,6 +2463,8 @@ struct abl_fv_solver {
     abl::DevBuf<double> adjA, adjUghost, adjUself, adjContrib;
     abl::VjpThetaKernel vjp_theta_kernel = nullptr;
     abl::GradKernel grad_kernel = nullptr;
+    abl::SpeciesGradKernelT<double> species_kernel = nullptr;
+    abl::SpeciesGradKernelT<float> species_kernel_f = nullptr;
     abl::FluxKernel flux_kernel = nullptr;
     abl::FaceFluxKernel face_flux_kernel = nullptr;
     abl::GatherKernel gather_kernel = nullptr;
@@ -2620,12 +2652,16 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
                         : p.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN ? abl::kLimBJ : abl::kLimPetsc;
         const bool species = p.n_species > 0;
         if (mesh->dim == 3) {
+            s->species_kernel = abl::pick_species_gradients<double, 3>(L.K);
+            s->species_kernel_f = abl::pick_species_gradients<float, 3>(L.K);
             s->grad_kernel = abl::pick_gradients<double, 3>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<3>(s->recon, s->visc, species);
             s->face_flux_kernel = abl::pick_face_flux<3>(s->recon, s->visc, species);
             s->tile_kernel = abl::pick_tile<3>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<3>(species, L.K);
         } else {
+            s->species_kernel = abl::pick_species_gradients<double, 2>(L.K);
+            s->species_kernel_f = abl::pick_species_gradients<float, 2>(L.K);
             s->grad_kernel = abl::pick_gradients<double, 2>(s->recon, s->visc, lim, L.K);
             s->flux_kernel = abl::pick_flux<2>(s->recon, s->visc, species);
             s->face_flux_kernel = abl::pick_face_flux<2>(s->recon, s->visc, species);
@@ -2883,12 +2919,7 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                 });
             if (L.ns && (h->recon || h->visc))
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
-                    if (L.dim == 3)
-                        hipLaunchKernelGGL((abl::k_species_gradients<float, 3>), grid, block, 0, st, vf, h->ph, U,
-                                           h->gS_f.ptr, h->gY_f.ptr);
-                    else
-                        hipLaunchKernelGGL((abl::k_species_gradients<float, 2>), grid, block, 0, st, vf, h->ph, U,
-                                           h->gS_f.ptr, h->gY_f.ptr);
+                    hipLaunchKernelGGL(h->species_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gS_f.ptr, h->gY_f.ptr);
                 });
             if (h->strategy == ABL_FV_STRATEGY_TILE) {
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
@@ -2911,12 +2942,7 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             });
             if (L.ns)
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
-                    if (L.dim == 3)
-                        hipLaunchKernelGGL((abl::k_species_gradients<double, 3>), grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
-                                           h->gradA.ptr);
-                    else
-                        hipLaunchKernelGGL((abl::k_species_gradients<double, 2>), grid, block, 0, st, v, h->ph, U, h->gradC.ptr,
-                                           h->gradA.ptr);
+                    hipLaunchKernelGGL(h->species_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
                 });
         }
         if (h->strategy == ABL_FV_STRATEGY_TILE) {
@@ -3036,12 +3062,7 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
         });
         if (L.ns)
             h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
-                if (L.dim == 3)
-                    hipLaunchKernelGGL((abl::k_species_gradients<double, 3>), grid, block, 0, st, v, h->ph, U,
-                                       h->gradC.ptr, h->gradA.ptr);
-                else
-                    hipLaunchKernelGGL((abl::k_species_gradients<double, 2>), grid, block, 0, st, v, h->ph, U,
-                                       h->gradC.ptr, h->gradA.ptr);
+                hipLaunchKernelGGL(h->species_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
             });
         abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM};
         h->timed(ABL_FV_PHASE_VJP, st, [&] {
