@@ -14,7 +14,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -42,8 +41,8 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=128, help="cells per direction of the adjoint workload")
     ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral NS + species leg")
     ap.add_argument("--tets-n", type=int, default=111, help="cubes per direction of the tetrahedral workload")
-    ap.add_argument("--cpu-n", type=int, default=48, help="cells per direction of the CPU-baseline sample")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget")
+    ap.add_argument("--cpu-n", type=int, default=32, help="cells per direction of each CPU-baseline process's sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time per phase")
     ap.add_argument("--copy-gib", type=float, default=2.0,
                     help="size of the plain device copy timed before the RHS (HBM reference and counter calibration; "
                          "0 = skip)")
@@ -95,38 +94,18 @@ def ns_processes(fv, mu, k):
 
 
 def cpu_baseline(n, seconds, mu, k, mach):
-    """Times the CPU oracle (a C restatement of the reference loops, single thread) on an n^3 sample of the same
-    workload; setup (LSQ weights, face stencils) excluded like the reference's lazily built interpolants."""
-    from ablate_amd import synthetic
-    from ablate_amd.mesh import BoxMesh
-    from oracle import binding as orc
+    """Times the CPU oracle (a C restatement of the reference loops) on the host cores: one process per core, each on
+    its own n^3 sample of the same workload (the reference runs one MPI rank per core), plus one process alone.  Runs
+    as a child process (oracle/cpu_baseline.py) so that its workers never share this process's GPU context."""
+    import subprocess
 
-    mesh = BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [True] * 3, ordering="natural")
-    U = synthetic.taylor_green(mesh, mach)
-    phys = orc.physics(1.4, 287.0, 0, "ausm", "leastsquares", "barthjespersen", 1e10, k, mu, 0.0)
-    ctx = orc.Context(mesh.arrays, phys)
-    X = np.ascontiguousarray(U.T)
-    F = np.zeros_like(X)
-    ctx.rhs_aos(X, F)  # warm-up
-    calls, t0 = 0, time.perf_counter()
-    while True:
-        F[:] = 0.0
-        ctx.rhs_aos(X, F)
-        calls += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or calls >= 200:
-            break
-    ctx.close()
-    rate = mesh.n_interior * calls / el / 1e6
-    return {
-        "value": round(rate, 4),
-        "unit": "Mcell/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"C oracle (reference loop structure, single thread) on the same NS/LSQ/BJ/AUSM workload at "
-                  f"{n}^3 = {mesh.n_interior} cells, {calls} RHS calls in {el:.1f} s on {platform.processor() or 'host'} "
-                  f"({os.cpu_count()} logical cores visible)",
-    }
+    root = os.path.dirname(os.path.abspath(__file__))
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--seconds", str(seconds), "--mu", repr(mu),
+           "--k", repr(k), "--mach", repr(mach)]
+    out = subprocess.run(cmd, cwd=root, capture_output=True, text=True, timeout=max(120.0, 6 * seconds + 60))
+    if out.returncode != 0:
+        return {"error": f"cpu baseline failed ({out.returncode}): {out.stderr[-400:]}"}
+    return json.loads(out.stdout.strip().splitlines()[-1])
 
 
 def per_field_rel(a, b):
