@@ -40,6 +40,7 @@ void set_global_error(const std::string& msg) { g_last_error = msg; }
 constexpr int kBlock = 256;
 constexpr int kTileMax = 256;  // largest tile (workgroup) of the tile strategy
 constexpr int kTileLdsMax = 64 * 1024;  // LDS budget of one tile workgroup
+constexpr int kCuLdsBytes = 160 * 1024;  // LDS of one gfx950 compute unit
 
 // ============================================================================================ kernels
 
@@ -751,8 +752,9 @@ __device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, cons
                                                                      ghost, x, nh, cl, cr, cd, w[1]);
 }
 
-// Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored
-// (euler components as a padded record [F][FS], species as [ns][F]).
+// Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored: euler components as a
+// padded record [F][FS] (one contiguous read per face in the gather), species as rows [ns][F] (coalesced stores;
+// species inside the record would make every store of a wave touch 64 cache lines, measured 2.5x slower).
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                       const double* __restrict__ gradC, const double* __restrict__ gradA,
@@ -1316,6 +1318,54 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
 #pragma unroll
         for (int k = 0; k < KM; k++)
             if (sg[k] != 0.0) acc += sg[k] * double(lf[p * nfm + off[k]]);
+        const double r = acc * iv;
+        R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+    }
+}
+
+// Mixed form, face-flux strategy, pass 1: one lane per face stores both sides' FP32 perturbations as rows
+// [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
+__global__ __launch_bounds__(kBlock) void k_face_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
+                                                            const float* __restrict__ rec, const float* __restrict__ gS,
+                                                            const float* __restrict__ gY, float* __restrict__ Fm,
+                                                            int nrow) {
+    constexpr int NE = DIM + 2;
+    const int64_t f = xcd_cell_index();
+    if (f >= m.F) return;
+    const int64_t F = m.F;
+    float dFL[NE], dFR[NE];
+    mixed_face<DIM, RECON, VISC, SPECIES>(m, ph, U, rec, gS, gY, f, dFL, dFR, Fm + NE * F + f,
+                                          Fm + (nrow + NE) * F + f, F);
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        Fm[p * F + f] = dFL[p];
+        Fm[(nrow + p) * F + f] = dFR[p];
+    }
+}
+
+// Mixed form, face-flux strategy, pass 2: each interior cell gathers its own side's perturbations in face-number
+// order, in FP64; the same sums in the same order as the tile gather, so both strategies give the same residual.
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_gather_mixed(MeshView<float> m, int nrow, const float* __restrict__ Fm,
+                                                         double* __restrict__ R, int accumulate) {
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI;
+    int64_t off[KM];
+    double sg[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
+        off[k] = e >= 0 ? (int64_t)(e >> 1) + ((e & 1) ? (int64_t)nrow * m.F : 0) : 0;
+        sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
+    }
+    const double iv = m.inv_vol[c];
+    for (int p = 0; p < nrow; p++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++)
+            if (sg[k] != 0.0) acc += sg[k] * double(Fm[p * m.F + off[k]]);
         const double r = acc * iv;
         R[p * N + c] = accumulate ? R[p * N + c] + r : r;
     }
@@ -2221,6 +2271,26 @@ template <int DIM, bool RECON, bool VISC>
 TileMixedKernel pick_tile_mixed2(bool species, int K) {
     return species ? pick_tile_mixed3<DIM, RECON, VISC, true>(K) : pick_tile_mixed3<DIM, RECON, VISC, false>(K);
 }
+using FaceFluxMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, const float*,
+                                     float*, int);
+template <int DIM, bool RECON, bool VISC>
+FaceFluxMixedKernel pick_face_flux_mixed2(bool species) {
+    return species ? k_face_flux_mixed<DIM, RECON, VISC, true> : k_face_flux_mixed<DIM, RECON, VISC, false>;
+}
+template <int DIM>
+FaceFluxMixedKernel pick_face_flux_mixed(bool recon, bool visc, bool species) {
+    if (recon) return visc ? pick_face_flux_mixed2<DIM, true, true>(species) : pick_face_flux_mixed2<DIM, true, false>(species);
+    return visc ? pick_face_flux_mixed2<DIM, false, true>(species) : pick_face_flux_mixed2<DIM, false, false>(species);
+}
+using GatherMixedKernel = void (*)(MeshView<float>, int, const float*, double*, int);
+template <int DIM>
+GatherMixedKernel pick_gather_mixed(int K) {
+    switch (slot_width(K)) {
+        case 4: return k_gather_mixed<DIM, 4>;
+        case 6: return k_gather_mixed<DIM, 6>;
+        default: return k_gather_mixed<DIM, 8>;
+    }
+}
 template <int DIM>
 TileMixedKernel pick_tile_mixed(bool recon, bool visc, bool species, int K) {
     if (recon)
@@ -2436,11 +2506,15 @@ struct abl_fv_solver {
     abl::DevBuf<double> fna, fdx, fwa, inv_vol, gradC, gradA, slot_state;
     abl::DevBuf<double> sw, swa, slim;  // slot-major gradient geometry (FP64 form)
     abl::DevBuf<double> rec;            // cell records [NI][RS] of the FP64 form
-    abl::DevBuf<double> face_flux_e, face_flux_s;  // face-flux records (allocated when that strategy is selected)
+    // face fluxes (allocated when that strategy is selected): FP64 euler records [F][FS] and species rows [ns][F];
+    // mixed form rows [2 (n_euler + n_species)][F]
+    abl::DevBuf<double> face_flux_e, face_flux_s;
+    abl::DevBuf<float> face_flux_m;
     abl::DevBuf<int32_t> tile_start, tile_faces, tile_slot;  // tile strategy (built when selected)
     int tile_T = 0, tile_nfm = 0;
     int64_t tile_n = 0;
     size_t tile_lds = 0;
+    double tile_redundancy = 0.0;  // face evaluations of the tile kernel per face
     abl::TileKernel tile_kernel = nullptr;
     // mixed form: FP32 geometry and gradients
     abl::DevBuf<float> fna_f, fdx_f, fwa_f;
@@ -2448,6 +2522,8 @@ struct abl_fv_solver {
     abl::GradKernelT<float> grad_kernel_f = nullptr;
     abl::MixedFluxKernel flux_kernel_mixed = nullptr;
     abl::TileMixedKernel tile_kernel_mixed = nullptr;
+    abl::FaceFluxMixedKernel face_flux_kernel_mixed = nullptr;
+    abl::GatherMixedKernel gather_kernel_mixed = nullptr;
     abl::DevBuf<float> recf;            // cell records [NI][RS] of the mixed form
     abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients [ns*DIM][N] (densityYi limited, Yi aux)
     // adjoint
@@ -2559,12 +2635,17 @@ struct abl_fv_solver {
         v.slim = slim_f.ptr;
         return v;
     }
-    // Face-flux records [F][FS] (FS = n_euler rounded up to even) and species fluxes [ns][F]; FP64 form only.
+    // Face-flux storage of strategy 2 (see the members).
     void ensure_face_flux() {
-        if (form != ABL_FV_FORM_FP64 || face_flux_e.ptr) return;
-        const int fs = ((host.dim + 2) + 1) & ~1;
-        face_flux_e.alloc((size_t)fs * host.F);
-        if (host.ns) face_flux_s.alloc((size_t)host.ns * host.F);
+        const int nrow = host.dim + 2 + host.ns;
+        if (form == ABL_FV_FORM_FP64) {
+            if (face_flux_e.ptr) return;
+            face_flux_e.alloc((size_t)(((host.dim + 2) + 1) & ~1) * host.F);
+            if (host.ns) face_flux_s.alloc((size_t)host.ns * host.F);
+        } else {
+            if (face_flux_m.ptr) return;
+            face_flux_m.alloc((size_t)2 * nrow * host.F);
+        }
     }
     // Tiles for strategy 3 (FP64 form): the largest tile (256, 128 or 64 cells) whose face fluxes fit the LDS
     // budget.  Needs the host cell -> face table, so it is built at creation.
@@ -2587,9 +2668,24 @@ struct abl_fv_solver {
         tile_n = tl.n;
         tile_nfm = tl.nf_max;
         tile_lds = (size_t)tl.nf_max * per_face;
+        tile_redundancy = (double)tl.faces.size() / (double)std::max<int64_t>(host.F, 1);
+    }
+    // Default strategy.  The tile kernel evaluates faces on tile borders twice and holds its face fluxes in LDS;
+    // the face-flux strategy evaluates every face once but writes and re-reads the fluxes through HBM.  Tiles win
+    // while the border overhead stays below ~1.8 evaluations per face (rows of a structured box: 1.2-1.7) and the
+    // LDS footprint leaves at least two waves per SIMD for latency hiding; cell orders with little locality within
+    // a tile (bandwidth-reducing orders of unstructured meshes: ~2 evaluations per face) and wide species LDS rows
+    // go to face flux.
+    int auto_strategy() const {
+        if (tile_n > 0) {
+            const double wgs_per_cu = std::floor((double)abl::kCuLdsBytes / (double)std::max<size_t>(tile_lds, 1));
+            const double waves_per_simd = wgs_per_cu * (tile_T / 64.0) / 4.0;
+            if (tile_redundancy <= 1.8 && waves_per_simd >= 2.0) return ABL_FV_STRATEGY_TILE;
+        }
+        return ABL_FV_STRATEGY_FACE_FLUX;
     }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
                          gradA.bytes() + rec.bytes() + adjA.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
@@ -2720,10 +2816,14 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
                 s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc);
                 s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, species, L.K);
+                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<3>(s->recon, s->visc, species);
+                s->gather_kernel_mixed = abl::pick_gather_mixed<3>(L.K);
             } else {
                 s->grad_kernel_f = abl::pick_gradients<float, 2>(s->recon, s->visc, lim, L.K);
                 s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc);
                 s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, species, L.K);
+                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<2>(s->recon, s->visc, species);
+                s->gather_kernel_mixed = abl::pick_gather_mixed<2>(L.K);
             }
         }
         std::vector<int32_t> bf, bs;
@@ -2804,12 +2904,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->cfl_out.alloc(1);
         }
         s->build_tiles();
-        if (s->strategy == ABL_FV_STRATEGY_TILE && s->tile_n == 0) {
-            if (form != ABL_FV_FORM_FP64 && L.ns > 0)
-                throw std::invalid_argument("the mixed form with species needs the tile strategy, and no tile size "
-                                            "fits the LDS budget");
-            s->strategy = form == ABL_FV_FORM_FP64 ? ABL_FV_STRATEGY_FACE_FLUX : ABL_FV_STRATEGY_RECOMPUTE;
-        }
+        s->strategy = s->auto_strategy();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
@@ -2921,7 +3016,17 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                 h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
                     hipLaunchKernelGGL(h->species_kernel_f, grid, block, 0, st, vf, h->ph, U, h->gS_f.ptr, h->gY_f.ptr);
                 });
-            if (h->strategy == ABL_FV_STRATEGY_TILE) {
+            if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
+                const int nrow = L.dim + 2 + L.ns;
+                h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                    hipLaunchKernelGGL(h->face_flux_kernel_mixed, dim3(grid_for(L.F)), block, 0, st, vf, h->ph, U,
+                                       h->recf.ptr, h->gS_f.ptr, h->gY_f.ptr, h->face_flux_m.ptr, nrow);
+                });
+                h->timed(ABL_FV_PHASE_GATHER, st, [&] {
+                    hipLaunchKernelGGL(h->gather_kernel_mixed, grid, block, 0, st, vf, nrow, h->face_flux_m.ptr, R,
+                                       accumulate);
+                });
+            } else if (h->strategy == ABL_FV_STRATEGY_TILE) {
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                     hipLaunchKernelGGL(h->tile_kernel_mixed, dim3((unsigned)h->tile_n), dim3(h->tile_T), h->tile_lds, st,
                                        vf, h->ph, U, h->recf.ptr, h->gS_f.ptr, h->gY_f.ptr, h->tile_start.ptr,
@@ -2979,22 +3084,28 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
 
 extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
     if (!h) return fail(h, "abl_fv_set_strategy: null handle");
-    if (strategy != ABL_FV_STRATEGY_RECOMPUTE && strategy != ABL_FV_STRATEGY_FACE_FLUX &&
-        strategy != ABL_FV_STRATEGY_TILE)
+    if (strategy != ABL_FV_STRATEGY_AUTO && strategy != ABL_FV_STRATEGY_RECOMPUTE &&
+        strategy != ABL_FV_STRATEGY_FACE_FLUX && strategy != ABL_FV_STRATEGY_TILE)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
+    if (strategy == ABL_FV_STRATEGY_AUTO) strategy = h->auto_strategy();
     if (strategy == ABL_FV_STRATEGY_TILE && h->tile_n == 0)
         return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (no tile size fits the "
                        "LDS budget)");
-    if (strategy == ABL_FV_STRATEGY_FACE_FLUX && h->form != ABL_FV_FORM_FP64)
-        return fail(h, "abl_fv_set_strategy: the mixed form has the recompute and tile strategies only");
     if (strategy == ABL_FV_STRATEGY_RECOMPUTE && h->form != ABL_FV_FORM_FP64 && h->host.ns > 0)
-        return fail(h, "abl_fv_set_strategy: the mixed form transports species with the tile strategy only");
+        return fail(h, "abl_fv_set_strategy: the mixed form transports species with the tile and face-flux "
+                       "strategies only");
     try {
         if (strategy == ABL_FV_STRATEGY_FACE_FLUX) h->ensure_face_flux();
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_set_strategy: ") + e.what());
     }
     h->strategy = strategy;
+    return 0;
+}
+
+extern "C" int abl_fv_get_strategy(abl_fv_handle h, int32_t* strategy) {
+    if (!h || !strategy) return fail(h, "abl_fv_get_strategy: null argument");
+    *strategy = h->strategy;
     return 0;
 }
 

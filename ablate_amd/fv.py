@@ -251,9 +251,16 @@ class FiniteVolumeSolver:
         return theta.reshape(16, 3)
 
     def set_strategy(self, strategy: str) -> None:
-        """"tile" (default: a workgroup per tile of cells evaluates the tile's faces into LDS and gathers its cells),
-        "face_flux" (stored once in HBM, gathered per cell) or "recompute" (each face flux evaluated by both cells)."""
+        """"auto" (the default at creation: tile or face_flux from the mesh's tile statistics), "tile" (a workgroup
+        per tile of cells evaluates the tile's faces into LDS and gathers its cells), "face_flux" (stored once in HBM,
+        gathered per cell) or "recompute" (each face flux evaluated by both cells)."""
         _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]), self._h, "abl_fv_set_strategy")
+
+    def strategy(self) -> str:
+        """The strategy compute_rhs runs ("auto" resolved)."""
+        v = ctypes.c_int32()
+        _lib.check(_lib.lib().abl_fv_get_strategy(self._h, ctypes.byref(v)), self._h, "abl_fv_get_strategy")
+        return {code: name for name, code in _lib.STRATEGY.items()}[v.value]
 
     def set_profiling(self, enable: bool) -> None:
         _lib.check(_lib.lib().abl_fv_set_profiling(self._h, int(bool(enable))), self._h, "abl_fv_set_profiling")

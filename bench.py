@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (default: the workload's 256)")
-    ap.add_argument("--strategy", default="tile", choices=["recompute", "face_flux", "tile"])
+    ap.add_argument("--strategy", default="auto", choices=["auto", "recompute", "face_flux", "tile"],
+                    help="evaluation strategy of the headline leg (auto: the library's default choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
@@ -149,6 +150,7 @@ def mixed_leg(fv, synthetic, mesh, mu, k, R64, U, mach, args, fp64_ms):
         "unit": "Mcell/s",
         "ms_per_step": round(ms, 4),
         "speedup_vs_fp64": round(fp64_ms / ms, 3),
+        "strategy": solver.strategy(),
         "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in phases.items() if c},
         "rel_err_vs_fp64_mach0.3": [float(f"{e:.3e}") for e in err],
         "rel_err_vs_fp64_mach0.01": [float(f"{e:.3e}") for e in err_low],
@@ -181,6 +183,7 @@ def tets_leg(fv, synthetic, args):
     R = torch.zeros_like(U)
     setup_s = time.perf_counter() - t0
     info = solver.info()
+    strategy = solver.strategy()
     for _ in range(args.warmup):
         solver.compute_rhs(0.0, U, R)
     torch.cuda.synchronize()
@@ -199,6 +202,7 @@ def tets_leg(fv, synthetic, args):
     # mixed form on the same workload, error per field against the FP64 residual
     sm = fv.FiniteVolumeSolver(mesh, procs, {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"},
                                bcs, form="mixed")
+    strategy_mixed = sm.strategy()
     Rm = torch.zeros_like(U)
     for _ in range(args.warmup):
         sm.compute_rhs(0.0, U, Rm)
@@ -243,6 +247,7 @@ def tets_leg(fv, synthetic, args):
         "workload": f"ns_10species_fp64_tets{n}_lsq_bj_rusanov",
         "cells": cells,
         "faces": info.n_faces,
+        "strategy": strategy,
         "value": round(cells / (ms * 1e-3) / 1e6, 3),
         "unit": "Mcell/s",
         "ms_per_step": round(ms, 4),
@@ -254,6 +259,7 @@ def tets_leg(fv, synthetic, args):
         "finite": bool(torch.isfinite(R).all().item()),
         "mixed": {"value": round(cells / (ms_mixed * 1e-3) / 1e6, 3), "unit": "Mcell/s",
                   "ms_per_step": round(ms_mixed, 4), "speedup_vs_fp64": round(ms / ms_mixed, 3),
+                  "strategy": strategy_mixed,
                   "rel_err_vs_fp64_max": float(f"{max(err):.3e}"),
                   "error_norm": "max over the 15 conserved fields of ||R_mixed - R_fp64|| / ||R_fp64||"},
         "adjoint": {"value": round(cells / (ms_adj * 1e-3) / 1e6, 3), "unit": "Mcell/s", "ms_per_step": round(ms_adj, 4),
@@ -359,6 +365,7 @@ def main():
                                    {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"},
                                    form=cfg["form"])
     solver.set_strategy(args.strategy)
+    strategy = solver.strategy()
     U = torch.tensor(synthetic.taylor_green(mesh, cfg["mach"]), dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     info = solver.info()
@@ -409,7 +416,7 @@ def main():
             "faces": info.n_faces,
             "mesh": f"{n}^3 periodic hex, brick-ordered",
             "form": cfg["form"],
-            "strategy": args.strategy,
+            "strategy": strategy,
             "physics": "NavierStokesTransport(AUSM) + Constant(mu=1.8e-5, Pr=0.72), LSQ + Barth-Jespersen, Mach "
                        f"{cfg['mach']}",
         },
@@ -419,8 +426,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(cfg, n, args.strategy),
-            "kernel": "one RHS launch sequence (" + " + ".join(KERNELS[args.strategy]) + "), event-timed per "
+            "traffic": measured_traffic(cfg, n, strategy),
+            "kernel": "one RHS launch sequence (" + " + ".join(KERNELS[strategy]) + "), event-timed per "
                       f"launch on its stream; algorithmic bytes {info.alg_bytes_per_cell:.1f} B/cell x {cells} cells",
         },
         "phases_ms": {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in phases.items() if cnt},

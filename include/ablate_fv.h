@@ -197,17 +197,24 @@ int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* U, abl_fv_s
 int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max_time, double dt0, double safety,
                     int64_t max_steps, double* t_out, int64_t* steps_out, abl_fv_stream stream);
 
-/* ---- evaluation strategy (both deterministic, no float atomics) ----------------------------------------------- */
+/* ---- evaluation strategy (all deterministic, no float atomics).  The reference has one CPU loop order; these
+ *      have no counterpart there and all give the residual of ComputeRHSFunction. ------------------------------ */
 enum {
-    ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice */
+    ABL_FV_STRATEGY_AUTO = 0,      /* pick TILE or FACE_FLUX from the mesh's tile statistics (the default) */
+    ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice
+                                      (mixed form: no species) */
     ABL_FV_STRATEGY_FACE_FLUX = 2, /* one lane per face stores its flux once; a second pass gathers per cell
-                                      (holds 8 * (n_euler rounded up to even + n_species) bytes per face) */
-    ABL_FV_STRATEGY_TILE = 3       /* default: one workgroup per tile of consecutive cells evaluates the tile's faces
-                                      once into LDS and gathers its cells; bitwise equal to FACE_FLUX */
+                                      (FP64: 8 * (n_euler + n_species rounded up to even) bytes per face; mixed:
+                                      8 * (n_euler + n_species)) */
+    ABL_FV_STRATEGY_TILE = 3       /* one workgroup per tile of consecutive cells evaluates the tile's faces once into
+                                      LDS and gathers its cells; bitwise equal to FACE_FLUX */
 };
-/* Applies to the FP64 form (the mixed form has one cell-centric strategy).  Selecting FACE_FLUX allocates its
- * face-flux storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
+/* AUTO chooses TILE when the tile kernel evaluates at most ~1.8 faces per face (border faces twice) and its LDS
+ * footprint leaves two or more waves per SIMD, FACE_FLUX otherwise.  Selecting FACE_FLUX allocates its face-flux
+ * storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
 int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);
+/* The strategy abl_fv_rhs runs (AUTO resolved). */
+int abl_fv_get_strategy(abl_fv_handle h, int32_t* strategy);
 
 /* ---- per-phase timing (replaces the Loggable / PetscLogEvent brackets around each RHS sub-phase,
  *      finiteVolumeSolver.cpp:239-288, src/utilities/loggable.hpp:26-48) ------------------------------------- */

@@ -52,7 +52,8 @@ def mixed_species_cases():
 
 CASES = mixed_cases()
 SPECIES_CASES = mixed_species_cases()
-STRATEGIES = ["tile", "recompute"]
+STRATEGIES = ["tile", "face_flux", "recompute"]
+SPECIES_STRATEGIES = ["tile", "face_flux"]
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
@@ -75,15 +76,17 @@ def test_mixed_matches_fp64_oracle(case, strategy, torch_cuda):
     assert err.max() <= TOL_MIXED_COARSE, f"{case.name}: {err}"
 
 
+@pytest.mark.parametrize("strategy", SPECIES_STRATEGIES)
 @pytest.mark.parametrize("case", SPECIES_CASES, ids=[c.name for c in SPECIES_CASES])
-def test_mixed_species_matches_fp64_oracle(case, torch_cuda):
-    """Species transport in the mixed form (tile strategy): densityYi advection as a perturbation of the left
-    baseline rhoY u_n, species diffusion as a full FP32 flux."""
+def test_mixed_species_matches_fp64_oracle(case, strategy, torch_cuda):
+    """Species transport in the mixed form (tile and face-flux strategies): densityYi advection as a perturbation of
+    the left baseline rhoY u_n, species diffusion as a full FP32 flux."""
     torch = torch_cuda
-    print(f"[mixed] {case.name} cells={case.mesh.n_interior} species={case.n_species}", flush=True)
+    print(f"[mixed] {case.name} {strategy} cells={case.mesh.n_interior} species={case.n_species}", flush=True)
     U0 = case.initial_state()
     Rref, _ = case.oracle_rhs(U0.copy())
     solver = case.solver(form="mixed")
+    solver.set_strategy(strategy)
     U = torch.tensor(U0, dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -95,7 +98,7 @@ def test_mixed_species_matches_fp64_oracle(case, torch_cuda):
     assert err.max() <= TOL_MIXED_COARSE, f"{case.name}: {err}"
 
 
-def test_mixed_species_needs_tile_strategy(torch_cuda):
+def test_mixed_species_rejects_recompute_strategy(torch_cuda):
     sm = SPECIES_CASES[0].solver(form="mixed")
     with pytest.raises(RuntimeError):
         sm.set_strategy("recompute")
@@ -154,8 +157,18 @@ def test_mixed_strategies_agree(torch_cuda):
     assert torch.all((Ra - Rb).abs() <= 1e-13 * scale)
 
 
-def test_mixed_rejects_face_flux_strategy(torch_cuda):
-    case = CASES[2]
+@pytest.mark.parametrize("case", [CASES[0], CASES[6], SPECIES_CASES[1], SPECIES_CASES[2], SPECIES_CASES[3]],
+                         ids=lambda c: c.name)
+def test_mixed_face_flux_equals_tile_bitwise(case, torch_cuda):
+    """The face-flux strategy stores the same FP32 perturbations the tile kernel keeps in LDS and gathers them in the
+    same order, so the mixed residuals are identical."""
+    torch = torch_cuda
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Ra, Rb = torch.zeros_like(U), torch.zeros_like(U)
     sm = case.solver(form="mixed")
-    with pytest.raises(RuntimeError):
-        sm.set_strategy("face_flux")
+    sm.set_strategy("face_flux")
+    sm.compute_rhs(0.0, U, Ra)
+    sm.set_strategy("tile")
+    sm.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    assert torch.equal(Ra, Rb), (Ra - Rb).abs().max().item()

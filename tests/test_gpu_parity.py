@@ -3,7 +3,7 @@ C ABI, plus determinism, accumulate semantics and AoS packing."""
 import numpy as np
 import pytest
 
-from tests.fvcases import rel_err_per_field, standard_cases
+from tests.fvcases import Case, all_outflow, inflow_x, rel_err_per_field, standard_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +104,30 @@ def test_tile_equals_face_flux_bitwise(case, torch_cuda):
     solver.compute_rhs(0.0, U, Rb)
     torch.cuda.synchronize()
     assert torch.equal(Ra, Rb), (Ra - Rb).abs().max().item()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_auto_strategy_choice(form, torch_cuda):
+    """The default strategy: tiles on a structured box (row tiles evaluate <= ~1.7 faces per face), face flux on a
+    tetrahedral mesh in bandwidth-reducing order (~2 evaluations per face) and where species rows fill the LDS."""
+    from ablate_amd.mesh import BoxMesh, TetBoxMesh
+
+    box = Case("auto_box", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [True] * 3), recon="leastsquares",
+               limiter="barthjespersen", mu=1.8e-5, k=0.0251125)
+    # 48000 tets in reverse Cuthill-McKee order: a tile of consecutive cells shares almost no faces (1.9 evaluations
+    # per face)
+    tets = Case("auto_tets", TetBoxMesh(20, 0.2), recon="leastsquares", limiter="barthjespersen", mu=0.05, k=0.5,
+                bcs=all_outflow(6))
+    species = Case("auto_species", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [False, True, True]), n_species=10,
+                   bcs=inflow_x(10))
+    assert box.solver(form=form).strategy() == "tile"
+    assert tets.solver(form=form).strategy() == "face_flux"
+    assert species.solver(form=form).strategy() == "face_flux"
+    s = box.solver(form=form)
+    s.set_strategy("face_flux")
+    assert s.strategy() == "face_flux"
+    s.set_strategy("auto")
+    assert s.strategy() == "tile"
 
 
 def test_pack_unpack_roundtrip(torch_cuda):
