@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libablate_fv.so"
 
 # ---- enums (values are part of the ABI) ----
 FORM_FP64, FORM_MIXED = 0, 1
-FLUX = {"none": 0, "ausm": 1, "average": 2, "off": 3, "rusanov": 4}
+FLUX = {"none": 0, "ausm": 1, "average": 2, "off": 3, "rusanov": 4, "ausmpup": 5}
 RECON = {"upwind": 0, "none": 0, "leastsquares": 1}
 LIMITER = {
     "none": 0, "zero": 1, "minmod": 2, "vanleer": 3, "vanalbada": 4, "superbee": 5, "mc": 6, "sin": 7,
@@ -55,6 +55,16 @@ class PhysicsDesc(ctypes.Structure):
         ("mu", c_double),
         ("diff", c_double),
         ("cfl", c_double),
+        ("ausmpup_minf", c_double),
+    ]
+
+
+class PgsDesc(ctypes.Structure):
+    _fields_ = [
+        ("alpha_init", c_double),
+        ("domain_length", c_double),
+        ("max_alpha_allowed", c_double),
+        ("max_delta_pressure_fac", c_double),
     ]
 
 
@@ -102,6 +112,10 @@ SIGNATURES = {
     "abl_fv_axpy": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "abl_fv_ts_euler": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double, c_int64, _dp,
                                   POINTER(c_int64), c_void_p]),
+    "abl_fv_set_pgs": (c_int32, [c_void_p, POINTER(PgsDesc)]),
+    "abl_fv_pgs_update": (c_int32, [c_void_p, c_void_p, c_double, _dp, _dp, c_void_p]),
+    "abl_fv_get_pgs_alpha": (c_int32, [c_void_p, _dp]),
+    "abl_fv_set_pgs_alpha": (c_int32, [c_void_p, c_double]),
     "abl_fv_set_profiling": (c_int32, [c_void_p, c_int32]),
     "abl_fv_set_strategy": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_strategy": (c_int32, [c_void_p, _ip]),

@@ -14,6 +14,8 @@ enum : int { kRho = 0, kRhoE = 1, kRhoU = 2 };
 // Physics constants passed by value to every kernel.
 struct Phys {
     double gamma, R, cv, icv, mu, k, diff, max_lim_grad;
+    double minf;       // AusmpUp reference Mach number (0: fa = 1)
+    double pgs_alpha;  // pressure-gradient-scaling factor seen by AusmpUp (1 without PGS)
     int flux, recon, limiter, ns;
     int visc;   // Navier-Stokes diffusion flux active (k or mu nonzero, or any transport active)
     int sdiff;  // species diffusion active
@@ -154,10 +156,63 @@ __device__ inline int ausm(T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR, T
     return dir;
 }
 
-// Flux calculators behind FluxCalculatorFunction: Ausm, AverageFlux (averageFlux.cpp:2-10), OffFlux.
+// Liou's AUSM+-up (J. Comput. Phys. 214 (2006) 137-170) as fluxCalculator::AusmpUp (ausmpUp.cpp:5-97; constants
+// beta 1/8, Kp 1/4, Ku 3/4, sigma 1/4 from ausmpUp.hpp:21-24).  s = +1 / -1 selects the split polynomial:
+// M1 = 1/2 (m + s|m|), M2 = s/4 (m + s)^2, M4 = M1 (|m| >= 1) or M2 (1 - 16 s beta M2(-s)),
+// P5 = M1 / (m + 1e-30) (|m| >= 1) or M2 ((2s - m) - 16 s a m M2(-s)), a = 3/16 (5 fa^2 - 4).
 template <typename T>
-__device__ inline int flux_calculator(int type, T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR, T& mdot, T& p12) {
+__device__ inline T aup_m1(T m, T s) { return T(0.5) * (m + s * fabs(m)); }
+template <typename T>
+__device__ inline T aup_m2(T m, T s) {
+    const T q = m + s;
+    return s * T(0.25) * (q * q);
+}
+template <typename T>
+__device__ inline T aup_m4(T m, T s) {
+    if (fabs(m) >= T(1)) return aup_m1(m, s);
+    return aup_m2(m, s) * (T(1) - s * T(2) * aup_m2(m, -s));  // 16 beta = 2
+}
+template <typename T>
+__device__ inline T aup_p5(T m, T s, T fa) {
+    if (fabs(m) >= T(1)) return aup_m1(m, s) / (m + T(1e-30));
+    const T a = T(3.0 / 16.0) * (T(-4) + T(5) * fa * fa);
+    return aup_m2(m, s) * ((s * T(2) - m) - s * T(16) * a * m * aup_m2(m, -s));
+}
+template <typename T>
+__device__ inline int ausmpup(T minf, T alpha, T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR, T& massFlux,
+                              T& p12) {
+    const T rho12 = T(0.5) * (rhoL + rhoR);
+    const T a12 = T(0.5) * (aL + aR) / alpha;
+    const T mL = uL / a12, mR = uR / a12;
+    const T mBar2 = (uL * uL + uR * uR) / (T(2) * a12 * a12);
+    T fa = T(1);
+    if (minf > T(0)) {
+        const T mO = sqrt(pmin(T(1), pmax(mBar2, minf * minf)));
+        fa = mO * (T(2) - mO);
+    }
+    // products associate left to right as in ausmpUp.cpp:38,53
+    const T m12 = aup_m4(mL, T(1)) + aup_m4(mR, T(-1)) -
+                  (T(0.25) / fa) * pmax(T(1) - T(0.25) * mBar2, T(0)) * (pR - pL) / (rho12 * a12 * a12 * alpha * alpha);
+    int dir;
+    if (m12 > T(0)) {
+        dir = kLeft;
+        massFlux = a12 * m12 * rhoL;
+    } else {
+        dir = kRight;
+        massFlux = a12 * m12 * rhoR;
+    }
+    const T pp = aup_p5(mL, T(1), fa), pm = aup_p5(mR, T(-1), fa);
+    p12 = (pp * pL + pm * pR - T(0.75) * pp * pm * rho12 * fa * a12 * a12 * alpha * alpha * (mR - mL)) / (alpha * alpha);
+    return dir;
+}
+
+// Flux calculators behind FluxCalculatorFunction: Ausm, AverageFlux (averageFlux.cpp:2-10), OffFlux, AusmpUp.
+template <typename T>
+__device__ inline int flux_calculator(int type, const Phys& ph, T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR,
+                                      T& mdot, T& p12) {
     if (type == 1) return ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, mdot, p12);
+    if (type == 5)
+        return ausmpup(T(ph.minf), T(ph.pgs_alpha), uL, aL, rhoL, pL, uR, aR, rhoR, pR, mdot, p12);
     if (type == 2) {
         mdot = T(0.5) * (uL * rhoL + uR * rhoR);
         p12 = T(0.5) * (pL + pR);
@@ -204,7 +259,7 @@ __device__ inline void advection_flux(int type, const T* uL, const T* uR, const 
     decode_side<T, DIM>(uL, nhat, ph, L);
     decode_side<T, DIM>(uR, nhat, ph, R);
     T massFlux, p12;
-    const int dir = flux_calculator<T>(type, L.un, L.a, L.rho, L.p, R.un, R.a, R.rho, R.p, massFlux, p12);
+    const int dir = flux_calculator<T>(type, ph, L.un, L.a, L.rho, L.p, R.un, R.a, R.rho, R.p, massFlux, p12);
     const T mA = massFlux * area;
     const T p12A = p12 * area;
     flux[kRho] = mA;

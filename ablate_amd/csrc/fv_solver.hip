@@ -2418,6 +2418,85 @@ __global__ __launch_bounds__(kBlock) void k_min_final(const double* __restrict__
     const double m = block_min(v);
     if (threadIdx.x == 0) out[0] = m;
 }
+// PressureGradientScaling::UpdatePreconditioner reductions over the interior cells (pressureGradientScaling.cpp:55-104,
+// PerfectGas T-route): r = {sum p, min p, max p, max a, max Mach}; a fixed reduction tree (deterministic sums),
+// NaN entries skipped by the min / max.
+constexpr int kPgsN = 5;
+__device__ inline void block_reduce_pgs(double* r) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        r[0] += __shfl_xor(r[0], o);
+        r[1] = fmin(r[1], __shfl_xor(r[1], o));
+        r[2] = fmax(r[2], __shfl_xor(r[2], o));
+        r[3] = fmax(r[3], __shfl_xor(r[3], o));
+        r[4] = fmax(r[4], __shfl_xor(r[4], o));
+    }
+    __shared__ double wr[kPgsN][kBlock / 64];
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < kPgsN; i++) wr[i][threadIdx.x >> 6] = r[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPgsN; i++) r[i] = wr[i][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+        r[0] += wr[0][w];
+        r[1] = fmin(r[1], wr[1][w]);
+        r[2] = fmax(r[2], wr[2][w]);
+        r[3] = fmax(r[3], wr[3][w]);
+        r[4] = fmax(r[4], wr[4][w]);
+    }
+}
+__device__ inline void pgs_identity(double* r) {
+    r[0] = 0.0;
+    r[1] = DBL_MAX;
+    r[2] = 0.0;
+    r[3] = 0.0;
+    r[4] = 0.0;
+}
+__global__ __launch_bounds__(kBlock) void k_pgs_partial(const double* __restrict__ U, int64_t N, int64_t NI, int dim,
+                                                        Phys ph, double* __restrict__ partial) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double r[kPgsN];
+    pgs_identity(r);
+    if (c < NI) {
+        const double rho = U[c];
+        double vel2 = 0.0;
+        for (int d = 0; d < dim; d++) {
+            const double v = U[(kRhoU + d) * N + c] / rho;
+            vel2 += v * v;
+        }
+        const double T = (U[kRhoE * N + c] / rho - 0.5 * vel2) / ph.cv;
+        const double p = ph.R * rho * T;
+        const double a = sqrt(ph.gamma * p / rho);
+        r[0] = p;
+        r[1] = p;
+        r[2] = p;
+        r[3] = a;
+        r[4] = sqrt(vel2) / a;
+    }
+    block_reduce_pgs(r);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < kPgsN; i++) partial[(int64_t)blockIdx.x * kPgsN + i] = r[i];
+}
+__global__ __launch_bounds__(kBlock) void k_pgs_final(const double* __restrict__ partial, int64_t n,
+                                                      double* __restrict__ out) {
+    double r[kPgsN];
+    pgs_identity(r);
+    for (int64_t b = threadIdx.x; b < n; b += blockDim.x) {
+        const double* q = partial + b * kPgsN;
+        r[0] += q[0];
+        r[1] = fmin(r[1], q[1]);
+        r[2] = fmax(r[2], q[2]);
+        r[3] = fmax(r[3], q[3]);
+        r[4] = fmax(r[4], q[4]);
+    }
+    block_reduce_pgs(r);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < kPgsN; i++) out[i] = r[i];
+}
+
 // Forward-Euler stage update of the interior rows in FP64: U[p][c] += dt * R[p][c] (blockIdx.y = component).
 __global__ __launch_bounds__(kBlock) void k_axpy(double dt, const double* __restrict__ R, double* __restrict__ U,
                                                  int64_t N, int64_t NI) {
@@ -2547,6 +2626,11 @@ struct abl_fv_solver {
     int strategy = ABL_FV_STRATEGY_TILE;
     abl::DevBuf<unsigned long long> counter;
     abl::DevBuf<double> rmin, cfl_partial, cfl_out;  // MIN_CELL_RADIUS per interior cell; CFL reduction scratch
+    // PressureGradientScaling attached by abl_fv_set_pgs: its state (alpha in ph.pgs_alpha) and reduction scratch
+    bool pgs_on = false;
+    abl_fv_pgs_desc pgs{};
+    double pgs_max_mach = 0.0;
+    abl::DevBuf<double> pgs_partial, pgs_out;
     std::vector<int32_t> h_slot_kind;
     std::vector<double> h_slot_state;
     std::vector<std::array<double, 3>> h_slot_theta;
@@ -2715,7 +2799,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
     if (form != ABL_FV_FORM_FP64 && form != ABL_FV_FORM_MIXED) return fail(nullptr, "abl_fv_create: unknown form");
     const abl_fv_physics_desc& p = *physics;
     if (!(p.gamma > 1.0) || !(p.Rgas > 0.0)) return fail(nullptr, "abl_fv_create: PerfectGas needs gamma > 1 and Rgas > 0");
-    if (p.flux_calculator < ABL_FV_FLUX_NONE || p.flux_calculator > ABL_FV_FLUX_RUSANOV)
+    if (p.flux_calculator < ABL_FV_FLUX_NONE || p.flux_calculator > ABL_FV_FLUX_AUSMPUP)
         return fail(nullptr, "abl_fv_create: unknown flux calculator");
     if (p.reconstruction != ABL_FV_RECON_NONE && p.reconstruction != ABL_FV_RECON_LEASTSQUARES)
         return fail(nullptr, "abl_fv_create: unknown reconstruction");
@@ -2739,6 +2823,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         ph.diff = p.diff;
         ph.max_lim_grad = p.max_lim_grad;
         ph.flux = p.flux_calculator;
+        ph.minf = p.ausmpup_minf;
+        ph.pgs_alpha = 1.0;
         ph.recon = s->recon;
         ph.limiter = p.limiter;
         ph.ns = p.n_species;
@@ -3318,6 +3404,87 @@ extern "C" int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* 
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_axpy: ") + e.what());
     }
+    return 0;
+}
+
+extern "C" int abl_fv_set_pgs(abl_fv_handle h, const abl_fv_pgs_desc* pgs) {
+    if (!h) return fail(h, "abl_fv_set_pgs: null handle");
+    if (!pgs) {
+        h->pgs_on = false;
+        h->ph.pgs_alpha = 1.0;
+        return 0;
+    }
+    try {
+        h->pgs = *pgs;
+        // constructor defaults, pressureGradientScaling.cpp:10-11
+        h->pgs.max_alpha_allowed = pgs->max_alpha_allowed >= 1.0 ? pgs->max_alpha_allowed : 100.0;
+        h->pgs.max_delta_pressure_fac = pgs->max_delta_pressure_fac > 0.0 ? pgs->max_delta_pressure_fac : 0.05;
+        if (!h->pgs_out.ptr) {
+            h->pgs_partial.alloc((size_t)std::max<int64_t>(grid_for(h->host.NI), 1) * abl::kPgsN);
+            h->pgs_out.alloc(abl::kPgsN);
+        }
+        h->ph.pgs_alpha = pgs->alpha_init;
+        h->pgs_max_mach = 0.0;
+        h->pgs_on = true;
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_set_pgs: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_pgs_update(abl_fv_handle h, const double* U, double dt, double* alpha, double* max_mach,
+                                 abl_fv_stream stream) {
+    if (!h || !U) return fail(h, "abl_fv_pgs_update: null argument");
+    if (!h->pgs_on) return fail(h, "abl_fv_pgs_update: no PressureGradientScaling attached (abl_fv_set_pgs)");
+    try {
+        const abl::Layout& L = h->host;
+        hipStream_t st = (hipStream_t)stream;
+        const unsigned nb = grid_for(L.NI);
+        hipLaunchKernelGGL(abl::k_pgs_partial, dim3(nb), dim3(abl::kBlock), 0, st, U, L.N, L.NI, L.dim, h->ph,
+                           h->pgs_partial.ptr);
+        hipLaunchKernelGGL(abl::k_pgs_final, dim3(1), dim3(abl::kBlock), 0, st, h->pgs_partial.ptr, (int64_t)nb,
+                           h->pgs_out.ptr);
+        HIP_CHECK(hipGetLastError());
+        double r[abl::kPgsN];
+        HIP_CHECK(hipMemcpyAsync(r, h->pgs_out.ptr, sizeof(r), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        // the alpha update of pressureGradientScaling.cpp:106-131 (PetscMin / PetscMax comparison order kept)
+        auto pmin = [](double a, double b) { return a < b ? a : b; };
+        auto pmax = [](double a, double b) { return a < b ? b : a; };
+        const double kMaxMachAllowed = 0.7, kMaxAlphaChange = 0.10;  // pressureGradientScaling.hpp:32,37
+        const double pRef = r[0] / (double)L.NI;
+        const double maxDeltaP = pmax(std::fabs(r[2] - pRef), std::fabs(r[1] - pRef));
+        const double cMax = r[3];
+        const double machMax = r[4];
+        // min over cells of maxMachAllowed / Mach == maxMachAllowed / max Mach (division is monotone)
+        const double alphaMax = machMax > 0.0 ? pmin(DBL_MAX, kMaxMachAllowed / machMax) : DBL_MAX;
+        const double alphaOld = h->ph.pgs_alpha;
+        const double term1 = 0.5e+0 * dt * cMax / h->pgs.domain_length;
+        double a = alphaOld + term1 * (std::sqrt(h->pgs.max_delta_pressure_fac * pRef / (maxDeltaP + 1E-30)) - 1.0);
+        a = pmin(a, (1. + kMaxAlphaChange) * alphaOld);
+        a = pmin(a, alphaMax);
+        a = pmin(a, h->pgs.max_alpha_allowed);
+        a = pmax(a, 1.e+0);
+        h->ph.pgs_alpha = a;
+        h->pgs_max_mach = machMax;
+        if (alpha) *alpha = a;
+        if (max_mach) *max_mach = machMax;
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_pgs_update: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_get_pgs_alpha(abl_fv_handle h, double* alpha) {
+    if (!h || !alpha) return fail(h, "abl_fv_get_pgs_alpha: null argument");
+    *alpha = h->ph.pgs_alpha;
+    return 0;
+}
+
+extern "C" int abl_fv_set_pgs_alpha(abl_fv_handle h, double alpha) {
+    if (!h) return fail(h, "abl_fv_set_pgs_alpha: null handle");
+    if (!h->pgs_on) return fail(h, "abl_fv_set_pgs_alpha: no PressureGradientScaling attached (abl_fv_set_pgs)");
+    h->ph.pgs_alpha = alpha;
     return 0;
 }
 

@@ -3,7 +3,9 @@
 Class and parameter names follow the reference's YAML surface:
   ablate::eos::PerfectGas{parameters{gamma, Rgas}, species}                 (src/eos/perfectGas.cpp:4-7,328-329)
   ablate::eos::transport::Constant{k, mu, diff}                             (src/eos/transport/constant.cpp:74-75)
-  ablate::finiteVolume::fluxCalculator::{Ausm, AverageFlux, OffFlux}        (fluxCalculator/*.cpp)
+  ablate::finiteVolume::fluxCalculator::{Ausm, AverageFlux, OffFlux, AusmpUp{mInf, pgs}}   (fluxCalculator/*.cpp)
+  ablate::finiteVolume::processes::PressureGradientScaling{eos, alphaInit, domainLength, maxAlphaAllowed,
+                                                           maxDeltaPressureFac}  (processes/pressureGradientScaling.cpp)
   ablate::finiteVolume::processes::NavierStokesTransport{parameters{cfl}, eos, fluxCalculator, transport}
                                                                             (processes/navierStokesTransport.cpp:569-574)
   ablate::finiteVolume::processes::SpeciesTransport{eos, fluxCalculator, transport}   (speciesTransport.cpp:474-478)
@@ -15,6 +17,7 @@ State and residual are CUDA (HIP) float64 tensors in structure-of-arrays layout 
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -64,6 +67,17 @@ class OffFlux(FluxCalculator):
     kind = "off"
 
 
+class AusmpUp(FluxCalculator):
+    """AUSM+-up, Liou 2006 (fluxCalculator/ausmpUp.cpp:5-97): reference Mach number mInf (0 = none) and an optional
+    PressureGradientScaling whose alpha scales the acoustic speed."""
+
+    kind = "ausmpup"
+
+    def __init__(self, mInf: float = 0.0, pgs: "PressureGradientScaling | None" = None):
+        self.mInf = float(mInf)
+        self.pgs = pgs
+
+
 class Rusanov(FluxCalculator):
     """Local Lax-Friedrichs flux (build-defined, full-vector)."""
 
@@ -78,6 +92,50 @@ class NavierStokesTransport:
     fluxCalculator: FluxCalculator | None = None
     transport: Constant | None = None
     pgs: object = None
+
+
+class PressureGradientScaling:
+    """processes::PressureGradientScaling (pressureGradientScaling.cpp:7-137): alpha rescales the acoustic speed
+    seen by AusmpUp; UpdatePreconditioner(dt) is the pre-step update (device reductions over the interior cells).
+    Attached to the solver it is listed in; GetAlpha / GetMaxMach read its state."""
+
+    def __init__(self, eos=None, alphaInit: float = 1.0, domainLength: float = float("nan"),
+                 maxAlphaAllowed: float | None = None, maxDeltaPressureFac: float | None = None):
+        self.eos = eos
+        self.alphaInit = float(alphaInit)
+        self.domainLength = float(domainLength)
+        self.maxAlphaAllowed = 0.0 if maxAlphaAllowed is None else float(maxAlphaAllowed)
+        self.maxDeltaPressureFac = 0.0 if maxDeltaPressureFac is None else float(maxDeltaPressureFac)
+        self._solver = lambda: None  # weak reference to the solver it is attached to
+        self.maxMach = 0.0
+
+    def desc(self) -> _lib.PgsDesc:
+        return _lib.PgsDesc(alpha_init=self.alphaInit, domain_length=self.domainLength,
+                            max_alpha_allowed=self.maxAlphaAllowed, max_delta_pressure_fac=self.maxDeltaPressureFac)
+
+    def GetAlpha(self) -> float:
+        sv = self._solver()
+        if sv is None:
+            return self.alphaInit
+        a = ctypes.c_double()
+        _lib.check(_lib.lib().abl_fv_get_pgs_alpha(sv.handle, ctypes.byref(a)), sv.handle, "abl_fv_get_pgs_alpha")
+        return float(a.value)
+
+    def GetMaxMach(self) -> float:
+        return self.maxMach
+
+    def UpdatePreconditioner(self, dt: float, U, stream=None) -> float:
+        """The pre-step alpha update from the state U (device, SoA); returns the new alpha."""
+        sv = self._solver()
+        if sv is None:
+            raise RuntimeError("PressureGradientScaling is not attached to a FiniteVolumeSolver")
+        sv._check_tensor(U, "U")
+        a, m = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().abl_fv_pgs_update(sv.handle, ctypes.c_void_p(U.data_ptr()), float(dt), ctypes.byref(a),
+                                                ctypes.byref(m), sv._stream_ptr(stream)), sv.handle,
+                   "abl_fv_pgs_update")
+        self.maxMach = float(m.value)
+        return float(a.value)
 
 
 @dataclass
@@ -150,7 +208,15 @@ class FiniteVolumeSolver:
             mu=float(tr.mu),
             diff=float(tr.diff),
             cfl=float(nst.parameters.get("cfl", 0.5)),
+            ausmpup_minf=float(getattr(nst.fluxCalculator, "mInf", 0.0)),
         )
+        pgs_proc = [p for p in processes if isinstance(p, PressureGradientScaling)]
+        flux_pgs = getattr(nst.fluxCalculator, "pgs", None)
+        if len(pgs_proc) > 1:
+            raise ValueError("at most one PressureGradientScaling process")
+        if pgs_proc and flux == "ausmpup" and flux_pgs is not pgs_proc[0]:
+            raise ValueError("AusmpUp must reference the PressureGradientScaling process of the solver (pgs: *pgs)")
+        pgs = pgs_proc[0] if pgs_proc else flux_pgs
         self.mesh = mesh
         self.phys = phys
         self.form = form
@@ -163,6 +229,10 @@ class FiniteVolumeSolver:
                    "abl_fv_create")
         self._h = h
         self._theta_slots = []
+        self.pgs = pgs
+        if pgs is not None:
+            _lib.check(_lib.lib().abl_fv_set_pgs(h, ctypes.byref(pgs.desc())), h, "abl_fv_set_pgs")
+            pgs._solver = weakref.ref(self)
         for bc in boundaryConditions:
             for slot in bc.labelIds:
                 self.set_boundary(slot, bc)

@@ -40,8 +40,16 @@ typedef void* abl_fv_stream;
 enum { ABL_FV_FORM_FP64 = 0, ABL_FV_FORM_MIXED = 1 };
 
 /* Flux calculators: fluxCalculator::Ausm (ausm.cpp:2-43), AverageFlux (averageFlux.cpp:2-10),
- * OffFlux (offFlux.cpp:2-8); RUSANOV is build-defined (absent from the reference). */
-enum { ABL_FV_FLUX_NONE = 0, ABL_FV_FLUX_AUSM = 1, ABL_FV_FLUX_AVERAGE = 2, ABL_FV_FLUX_OFF = 3, ABL_FV_FLUX_RUSANOV = 4 };
+ * OffFlux (offFlux.cpp:2-8), AusmpUp (ausmpUp.cpp:5-97; mInf from the physics descriptor, the pressure-gradient
+ * scaling alpha from abl_fv_set_pgs); RUSANOV is build-defined (absent from the reference). */
+enum {
+    ABL_FV_FLUX_NONE = 0,
+    ABL_FV_FLUX_AUSM = 1,
+    ABL_FV_FLUX_AVERAGE = 2,
+    ABL_FV_FLUX_OFF = 3,
+    ABL_FV_FLUX_RUSANOV = 4,
+    ABL_FV_FLUX_AUSMPUP = 5
+};
 
 /* Cell reconstruction (PETSc petscfv_type): upwind = first order, leastsquares = LSQ gradients. */
 enum { ABL_FV_RECON_NONE = 0, ABL_FV_RECON_LEASTSQUARES = 1 };
@@ -103,6 +111,7 @@ typedef struct abl_fv_physics_desc {
     double mu;               /* dynamic viscosity */
     double diff;             /* species diffusivity */
     double cfl;              /* advection CFL (default 0.5) */
+    double ausmpup_minf;     /* AusmpUp{mInf} reference Mach number (ausmpUp.cpp:3; 0 = none) */
 } abl_fv_physics_desc;
 
 /* One boundary-condition slot. */
@@ -178,6 +187,29 @@ int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_st
 /* Number of interior cells whose state holds a non-finite value or a non-positive density
  * (mirrors the verboseSourceCheck scan, timeStepper.cpp:385-389). */
 int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_bad, abl_fv_stream stream);
+
+/* ---- pressure-gradient scaling (processes::PressureGradientScaling, pressureGradientScaling.cpp:7-137) ------- */
+
+/* Constructor arguments of PressureGradientScaling{alphaInit, domainLength, maxAlphaAllowed, maxDeltaPressureFac}
+ * (pressureGradientScaling.cpp:7-14: maxAlphaAllowed < 1 means 100, maxDeltaPressureFac <= 0 means 0.05). */
+typedef struct abl_fv_pgs_desc {
+    double alpha_init;
+    double domain_length;
+    double max_alpha_allowed;
+    double max_delta_pressure_fac;
+} abl_fv_pgs_desc;
+/* Attaches a PressureGradientScaling to the handle (NULL detaches): alpha starts at alpha_init, and the AusmpUp flux
+ * calculator uses it (AusmpUp{pgs}, ausmpUp.cpp:9). */
+int abl_fv_set_pgs(abl_fv_handle h, const abl_fv_pgs_desc* pgs);
+/* PressureGradientScaling::UpdatePreconditioner (pressureGradientScaling.cpp:16-137), the pre-step update: pressure
+ * mean / extremes, maximum sound speed and Mach number over the interior cells of U (device reductions), then the
+ * reference's alpha update with the step size dt.  Returns the new alpha and the maximum Mach number (host values;
+ * synchronises the stream).  Fails when no PressureGradientScaling is attached. */
+int abl_fv_pgs_update(abl_fv_handle h, const double* U, double dt, double* alpha, double* max_mach,
+                      abl_fv_stream stream);
+/* GetAlpha / restore of the saved alpha (pressureGradientScaling.cpp:154-164). */
+int abl_fv_get_pgs_alpha(abl_fv_handle h, double* alpha);
+int abl_fv_set_pgs_alpha(abl_fv_handle h, double alpha);
 
 /* ---- time step (the reference's CFL step size and forward-Euler stage, on the device) ------------------------- */
 

@@ -14,7 +14,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle_fv.so"
 
-FLUX = {"none": 0, "ausm": 1, "average": 2, "off": 3, "rusanov": 4}
+FLUX = {"none": 0, "ausm": 1, "average": 2, "off": 3, "rusanov": 4, "ausmpup": 5}
 RECON = {"upwind": 0, "none": 0, "leastsquares": 1}
 LIMITER = {"none": 0, "zero": 1, "minmod": 2, "vanleer": 3, "vanalbada": 4, "superbee": 5, "mc": 6, "sin": 7,
            "barthjespersen": 8}
@@ -51,6 +51,8 @@ class OrcPhysics(ctypes.Structure):
         ("k", c_double),
         ("mu", c_double),
         ("diff", c_double),
+        ("ausmpup_minf", c_double),
+        ("pgs_alpha", c_double),
     ]
 
 
@@ -76,6 +78,8 @@ def lib() -> ctypes.CDLL:
         for name in ("orc_ausm", "orc_average_flux", "orc_off_flux"):
             getattr(L, name).argtypes = [d] * 8 + [_dp, _dp]
             getattr(L, name).restype = c_int
+        L.orc_ausmpup.argtypes = [d] * 10 + [_dp, _dp]
+        L.orc_ausmpup.restype = c_int
         for name, args in {
             "orc_pg_pressure": [_dp, c_int, d, d],
             "orc_pg_temperature": [_dp, c_int, d, d],
@@ -111,6 +115,8 @@ def lib() -> ctypes.CDLL:
         L.orc_min_cell_radius.restype = None
         L.orc_cfl_dt.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), d, _dp, _dp]
         L.orc_cfl_dt.restype = d
+        L.orc_pgs_update.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, d, d, d, d, _dp, _dp]
+        L.orc_pgs_update.restype = None
         _lib = L
     return _lib
 
@@ -144,9 +150,10 @@ def _mesh_struct(m):
 
 
 def physics(gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", recon="upwind", limiter="none", max_lim_grad=1e10,
-            k=0.0, mu=0.0, diff=0.0) -> OrcPhysics:
+            k=0.0, mu=0.0, diff=0.0, ausmpup_minf=0.0, pgs_alpha=1.0) -> OrcPhysics:
     return OrcPhysics(gamma=gamma, Rgas=Rgas, n_species=n_species, flux=FLUX[flux], recon=RECON[recon],
-                      limiter=LIMITER[limiter], max_lim_grad=max_lim_grad, k=k, mu=mu, diff=diff)
+                      limiter=LIMITER[limiter], max_lim_grad=max_lim_grad, k=k, mu=mu, diff=diff,
+                      ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha)
 
 
 def rhs(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, bcs=None, time: float = 0.0):
@@ -227,6 +234,12 @@ def average_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR):
     return d, m.value, p.value
 
 
+def ausmpup(uL, aL, rhoL, pL, uR, aR, rhoR, pR, mInf=0.0, pgs_alpha=1.0):
+    m, p = c_double(), c_double()
+    d = lib().orc_ausmpup(mInf, pgs_alpha, uL, aL, rhoL, pL, uR, aR, rhoR, pR, ctypes.byref(m), ctypes.byref(p))
+    return d, m.value, p.value
+
+
 def off_flux(*args):
     m, p = c_double(), c_double()
     d = lib().orc_off_flux(*args, ctypes.byref(m), ctypes.byref(p))
@@ -301,6 +314,21 @@ def cfl_dt(mesh_arrays, phys: OrcPhysics, cfl: float, r: np.ndarray, U_soa: np.n
     X = np.ascontiguousarray(U_soa.T)
     rr = np.ascontiguousarray(r, dtype=np.float64)
     return float(lib().orc_cfl_dt(ctypes.byref(ms), ctypes.byref(phys), cfl, _ptr(rr), _ptr(X)))
+
+
+def pgs_update(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, dt: float, alpha: float, domain_length: float,
+               max_alpha_allowed: float = 0.0, max_delta_pressure_fac: float = 0.0):
+    """PressureGradientScaling::UpdatePreconditioner (pressureGradientScaling.cpp:16-137) on a SoA state; the
+    constructor defaults (:10-11) apply to max_alpha_allowed < 1 and max_delta_pressure_fac <= 0.
+    Returns (alpha, max_mach)."""
+    ms, keep = _mesh_struct(mesh_arrays)
+    X = np.ascontiguousarray(U_soa.T)
+    a, mm = c_double(alpha), c_double()
+    maa = max_alpha_allowed if max_alpha_allowed >= 1.0 else 100.0
+    fac = max_delta_pressure_fac if max_delta_pressure_fac > 0.0 else 0.05
+    lib().orc_pgs_update(ctypes.byref(ms), ctypes.byref(phys), _ptr(X), dt, domain_length, maa, fac,
+                         ctypes.byref(a), ctypes.byref(mm))
+    return a.value, mm.value
 
 
 def petsc_euler_solve(rhs_fn, dt_fn, U0: np.ndarray, max_time: float, interior: int, dt0: float = 0.1,

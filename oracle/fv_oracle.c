@@ -6,7 +6,8 @@
  * call this file; the product library never links it.
  *
  * Restated reference files (paths relative to the ABLATE tree):
- *   src/finiteVolume/fluxCalculator/ausm.cpp:2-43, averageFlux.cpp:2-10, offFlux.cpp:2-8
+ *   src/finiteVolume/fluxCalculator/ausm.cpp:2-43, averageFlux.cpp:2-10, offFlux.cpp:2-8, ausmpUp.cpp:5-97
+ *   src/finiteVolume/processes/pressureGradientScaling.cpp:16-137 (UpdatePreconditioner)
  *   src/eos/perfectGas.cpp:49-197
  *   src/finiteVolume/processes/navierStokesTransport.cpp:99-207 (AdvectionFlux), 454-532 (DiffusionFlux, stress)
  *   src/finiteVolume/processes/speciesTransport.cpp:98-110 (aux Yi), 196-228 (species diffusion), 263-341
@@ -90,10 +91,66 @@ int orc_off_flux(double uL, double aL, double rhoL, double pL, double uR, double
     return ORC_NA;
 }
 
-static int call_flux_calculator(int type, double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR,
-                                double pR, double* massFlux, double* p12) {
+/* AUSM+-up (Liou, J. Comput. Phys. 214 (2006) 137-170) as fluxCalculator::AusmpUp, ausmpUp.cpp:5-97, with the
+ * constants of ausmpUp.hpp:21-24.  Split Mach polynomials of degree 1, 2 and 4 and the degree-5 pressure split:
+ *   M1+-(m) = 1/2 (m +- |m|),  M2+-(m) = +-1/4 (m +- 1)^2,
+ *   M4+-(m) = M1+-(m) for |m| >= 1, else M2+-(m) (1 -+ 16 beta M2-+(m)),
+ *   P5+-(m) = M1+-(m) / (m + 1e-30) for |m| >= 1, else M2+-(m) ((+-2 - m) -+ 16 alpha m M2-+(m)),
+ *             alpha = 3/16 (-4 + 5 fa^2).
+ * pgs_alpha is the PressureGradientScaling factor (1 without one); mInf <= 0 leaves fa = 1. */
+static const double kAusmpUpBeta = 1.0 / 8.0, kAusmpUpKp = 0.25, kAusmpUpKu = 0.75, kAusmpUpSigma = 0.25;
+static double m1_split(double m, double s) { return 0.5 * (m + s * fabs(m)); }
+static double m2_split(double m, double s) { return s * 0.25 * psqr(m + s); }
+static double m4_split(double m, double s) {
+    if (fabs(m) >= 1.0) return m1_split(m, s);
+    return m2_split(m, s) * (1.0 - s * 16.0 * kAusmpUpBeta * m2_split(m, -s));
+}
+static double p5_split(double m, double s, double fa) {
+    if (fabs(m) >= 1.0) return m1_split(m, s) / (m + 1e-30);
+    const double alpha = 3.0 / 16.0 * (-4.0 + 5 * fa * fa);
+    return m2_split(m, s) * ((s * 2.0 - m) - s * 16. * alpha * m * m2_split(m, -s));
+}
+int orc_ausmpup(double mInf, double pgs_alpha, double uL, double aL, double rhoL, double pL, double uR, double aR,
+                double rhoR, double pR, double* massFlux, double* p12) {
+    const double rho12 = 0.5 * (rhoL + rhoR);
+    const double a12 = 0.5 * (aL + aR) / pgs_alpha;
+    const double mL = uL / a12, mR = uR / a12;
+    const double mBar2 = (psqr(uL) + psqr(uR)) / (2.0 * a12 * a12);
+    double fa = 1.0;
+    if (mInf > 0) {
+        const double mO = sqrt(pmin(1.0, pmax(mBar2, psqr(mInf))));
+        fa = mO * (2.0 - mO);
+    }
+    const double m12 = m4_split(mL, 1.0) + m4_split(mR, -1.0) - (kAusmpUpKp / fa) * pmax(1.0 - (kAusmpUpSigma * mBar2), 0) *
+                                                                    (pR - pL) / (rho12 * a12 * a12 * pgs_alpha * pgs_alpha);
+    int dir;
+    if (m12 > 0) {
+        dir = ORC_LEFT;
+        *massFlux = a12 * m12 * rhoL;
+    } else {
+        dir = ORC_RIGHT;
+        *massFlux = a12 * m12 * rhoR;
+    }
+    if (p12) {
+        const double pp = p5_split(mL, 1.0, fa), pm = p5_split(mR, -1.0, fa);
+        *p12 = pp * pL + pm * pR - kAusmpUpKu * pp * pm * rho12 * fa * a12 * a12 * pgs_alpha * pgs_alpha * (mR - mL);
+        *p12 /= psqr(pgs_alpha);
+    }
+    return dir;
+}
+
+/* Parameters of a flux calculator beyond the two states (AusmpUp's mInf and PGS alpha). */
+typedef struct {
+    double minf, pgs_alpha;
+} flux_ctx;
+static const flux_ctx kNoFluxCtx = {0.0, 1.0};
+
+static int call_flux_calculator(int type, const flux_ctx* fc, double uL, double aL, double rhoL, double pL, double uR,
+                                double aR, double rhoR, double pR, double* massFlux, double* p12) {
     switch (type) {
         case ORC_FLUX_AUSM: return orc_ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
+        case ORC_FLUX_AUSMPUP:
+            return orc_ausmpup(fc->minf, fc->pgs_alpha, uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
         case ORC_FLUX_AVERAGE: return orc_average_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
         default: return orc_off_flux(uL, aL, rhoL, pL, uR, aR, rhoR, pR, massFlux, p12);
     }
@@ -179,8 +236,8 @@ static void decode_side(int dim, const double* field, const double* norm, double
 }
 
 /* NavierStokesTransport::AdvectionFlux, navierStokesTransport.cpp:99-207 */
-int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, const double* fieldR, double gamma,
-                          double R, int flux_calculator, double* flux) {
+static int ns_advection_flux(int dim, const double* normal, const double* fieldL, const double* fieldR, double gamma,
+                             double R, int flux_calculator, const flux_ctx* fc, double* flux) {
     double norm[3];
     norm_vector(dim, normal, norm);
     const double areaMag = mag_vector(dim, normal);
@@ -188,7 +245,8 @@ int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, c
     decode_side(dim, fieldL, norm, gamma, R, &L);
     decode_side(dim, fieldR, norm, gamma, R, &Rs);
     double massFlux, p12;
-    int direction = call_flux_calculator(flux_calculator, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, &p12);
+    int direction = call_flux_calculator(flux_calculator, fc, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux,
+                                         &p12);
     if (direction == ORC_LEFT) {
         flux[RHO] = massFlux * areaMag;
         double velMagL = mag_vector(dim, L.vel);
@@ -212,13 +270,17 @@ int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, c
     }
     return 0;
 }
+int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, const double* fieldR, double gamma,
+                          double R, int flux_calculator, double* flux) {
+    return ns_advection_flux(dim, normal, fieldL, fieldR, gamma, R, flux_calculator, &kNoFluxCtx, flux);
+}
 
 /* SpeciesTransport::AdvectionFlux, speciesTransport.cpp:263-341. The calculator is called with p12 = NULL and any
  * non-LEFT direction (including NA) takes the right state. fieldL/R point at the euler field; yiL/yiR at
  * densityYi. */
 static void species_advection_flux(int dim, const double* normal, const double* fieldL, const double* yiL,
                                    const double* fieldR, const double* yiR, double gamma, double R, int flux_calculator,
-                                   int ns, double* flux) {
+                                   const flux_ctx* fc, int ns, double* flux) {
     double norm[3];
     norm_vector(dim, normal, norm);
     const double areaMag = mag_vector(dim, normal);
@@ -227,7 +289,8 @@ static void species_advection_flux(int dim, const double* normal, const double* 
     decode_side(dim, fieldL, norm, gamma, R, &L);
     decode_side(dim, fieldR, norm, gamma, R, &Rs);
     double massFlux;
-    if (call_flux_calculator(flux_calculator, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, NULL) == ORC_LEFT) {
+    if (call_flux_calculator(flux_calculator, fc, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, NULL) ==
+        ORC_LEFT) {
         for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiL[sp] / L.rho) * areaMag;
     } else {
         for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiR[sp] / Rs.rho) * areaMag;
@@ -845,6 +908,7 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
             if (ns) field_gradients(&t, ph, X, ctx->fgrad, dim + 2, ns, grad); /* densityYi field */
         }
         double uL[64], uR[64], flux[64];
+        const flux_ctx fc = {ph->ausmpup_minf, ph->pgs_alpha > 0 ? ph->pgs_alpha : 1.0};
         /* ComputeFluxSourceTerms, cellInterpolant.cpp:561-631 */
         for (int64_t f = 0; f < t.F; f++) {
             int32_t cl = mesh->face_cells[2 * f], cr = mesh->face_cells[2 * f + 1];
@@ -865,7 +929,7 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
             }
             /* NavierStokesTransport::AdvectionFlux on the euler field */
             for (int k = 0; k < nc; k++) flux[k] = 0.0;
-            orc_ns_advection_flux(dim, normal, uL, uR, ph->gamma, ph->Rgas, ph->flux, flux);
+            ns_advection_flux(dim, normal, uL, uR, ph->gamma, ph->Rgas, ph->flux, &fc, flux);
             for (int d = 0; d < dim + 2; ++d) {
                 if (fLp) fLp[d] -= flux[d] / vL;
                 if (fRp) fRp[d] += flux[d] / vR;
@@ -873,7 +937,8 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
             /* SpeciesTransport::AdvectionFlux on densityYi */
             if (ns) {
                 for (int k = 0; k < nc; k++) flux[k] = 0.0;
-                species_advection_flux(dim, normal, uL, uL + dim + 2, uR, uR + dim + 2, ph->gamma, ph->Rgas, ph->flux, ns, flux);
+                species_advection_flux(dim, normal, uL, uL + dim + 2, uR, uR + dim + 2, ph->gamma, ph->Rgas, ph->flux, &fc, ns,
+                                       flux);
                 for (int d = 0; d < ns; ++d) {
                     if (fLp) fLp[dim + 2 + d] -= flux[d] / vL;
                     if (fRp) fRp[dim + 2 + d] += flux[d] / vR;
@@ -989,4 +1054,47 @@ double orc_cfl_dt(const orc_mesh* m, const orc_physics* ph, double cfl, const do
         dtMin = dt < dtMin ? dt : dtMin;
     }
     return dtMin;
+}
+
+/* PressureGradientScaling::UpdatePreconditioner, pressureGradientScaling.cpp:16-137: pressure mean and extremes,
+ * maximum sound speed and Mach number over the interior cells (PerfectGas T route), then the alpha update with the
+ * step dt.  Constants of pressureGradientScaling.hpp:32,37; constructor defaults of :10-11 applied by the caller
+ * (max_alpha_allowed >= 1, fac > 0).  Updates *alpha, writes the maximum Mach number. */
+void orc_pgs_update(const orc_mesh* m, const orc_physics* ph, const double* X, double dt, double domain_length,
+                    double max_alpha_allowed, double max_delta_pressure_fac, double* alpha, double* max_mach) {
+    const int dim = m->dim, nc = dim + 2 + ph->n_species;
+    const double maxMachAllowed = 0.7, maxAlphaChange = 0.10;
+    double pAvgLocal = 0.e+0, pMinLocal = PETSC_MAX_REAL, pMaxLocal = 0.e+0, cMaxLocal = 0.e+0;
+    double machMaxLocal = 0.e+0, alphaMaxLocal = PETSC_MAX_REAL;
+    int64_t countLocal = 0;
+    for (int64_t c = 0; c < m->n_interior; ++c) {
+        const double* conserved = X + c * nc;
+        const double temperature = orc_pg_temperature(conserved, dim, ph->gamma, ph->Rgas);
+        const double p = orc_pg_pressure_T(conserved, temperature, ph->Rgas);
+        const double a = orc_pg_speed_of_sound_T(conserved, temperature, ph->gamma, ph->Rgas);
+        const double density = conserved[RHO];
+        double velMag = 0.0;
+        for (int d = 0; d < dim; d++) velMag += psqr(conserved[RHOU + d] / density);
+        const double mach = sqrt(velMag) / a;
+        countLocal++;
+        pAvgLocal += p;
+        pMinLocal = pmin(pMinLocal, p);
+        pMaxLocal = pmax(pMaxLocal, p);
+        cMaxLocal = pmax(cMaxLocal, a);
+        machMaxLocal = pmax(machMaxLocal, mach);
+        alphaMaxLocal = pmin(alphaMaxLocal, maxMachAllowed / mach);
+    }
+    const double alphaMax = alphaMaxLocal;
+    const double cMax = cMaxLocal;
+    const double pRef = pAvgLocal / (double)countLocal;
+    const double maxDeltaP = pmax(fabs(pMaxLocal - pRef), fabs(pMinLocal - pRef));
+    *max_mach = machMaxLocal;
+    const double alphaOld = *alpha;
+    const double term1 = 0.5e+0 * dt * cMax / domain_length;
+    double a = (*alpha + term1 * (sqrt(max_delta_pressure_fac * pRef / (maxDeltaP + 1E-30)) - 1.0));
+    a = pmin(a, (1. + maxAlphaChange) * alphaOld);
+    a = pmin(a, alphaMax);
+    a = pmin(a, max_alpha_allowed);
+    a = pmax(a, 1.e+0);
+    *alpha = a;
 }

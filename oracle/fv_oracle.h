@@ -20,7 +20,8 @@ extern "C" {
 /* Flux calculator direction, fluxCalculator.hpp:12 */
 enum { ORC_NA = 0, ORC_LEFT = 1, ORC_RIGHT = 2 };
 /* Same numeric values as the product ABI enums (checked by the tests). */
-enum { ORC_FLUX_NONE = 0, ORC_FLUX_AUSM = 1, ORC_FLUX_AVERAGE = 2, ORC_FLUX_OFF = 3, ORC_FLUX_RUSANOV = 4 };
+enum { ORC_FLUX_NONE = 0, ORC_FLUX_AUSM = 1, ORC_FLUX_AVERAGE = 2, ORC_FLUX_OFF = 3, ORC_FLUX_RUSANOV = 4,
+       ORC_FLUX_AUSMPUP = 5 };
 enum { ORC_RECON_NONE = 0, ORC_RECON_LSQ = 1 };
 enum {
     ORC_LIM_NONE = 0, ORC_LIM_ZERO, ORC_LIM_MINMOD, ORC_LIM_VANLEER, ORC_LIM_VANALBADA,
@@ -48,6 +49,8 @@ typedef struct {
     int limiter;
     double max_lim_grad;
     double k, mu, diff;
+    double ausmpup_minf; /* AusmpUp mInf (0 = none) */
+    double pgs_alpha;    /* PressureGradientScaling alpha seen by AusmpUp (<= 0: no PGS, alpha 1) */
 } orc_physics;
 
 typedef struct {
@@ -63,6 +66,9 @@ int orc_average_flux(double uL, double aL, double rhoL, double pL, double uR, do
                      double* massFlux, double* p12);
 int orc_off_flux(double uL, double aL, double rhoL, double pL, double uR, double aR, double rhoR, double pR,
                  double* massFlux, double* p12);
+/* AusmpUp (ausmpUp.cpp:5-97); pgs_alpha = 1 without a PressureGradientScaling. */
+int orc_ausmpup(double mInf, double pgs_alpha, double uL, double aL, double rhoL, double pL, double uR, double aR,
+                double rhoR, double pR, double* massFlux, double* p12);
 
 /* PerfectGas property functions on an euler field of `dim` momentum components at offset 0 (perfectGas.cpp). */
 double orc_pg_pressure(const double* euler, int dim, double gamma, double R);
@@ -111,6 +117,11 @@ void orc_min_cell_radius(const orc_mesh* mesh, double* r);
  * min over interior cells of cfl * 2 r_min / (a + sum_d |rhoU_d / rho|), a from the PerfectGas T route.
  * locX is AoS [n_cells][dim+2+ns]. */
 double orc_cfl_dt(const orc_mesh* mesh, const orc_physics* phys, double cfl, const double* r, const double* locX);
+
+/* PressureGradientScaling::UpdatePreconditioner (pressureGradientScaling.cpp:16-137) over the interior cells of the
+ * AoS state locX: updates *alpha with step dt, writes the maximum Mach number. */
+void orc_pgs_update(const orc_mesh* mesh, const orc_physics* phys, const double* locX, double dt, double domain_length,
+                    double max_alpha_allowed, double max_delta_pressure_fac, double* alpha, double* max_mach);
 
 #ifdef __cplusplus
 }

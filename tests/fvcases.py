@@ -28,12 +28,18 @@ class Case:
     Rgas: float = 287.0
     max_lim_grad: float = 1e10
     phase: tuple = (0.0, 0.0, 0.0)
+    minf: float = 0.0  # AusmpUp mInf
+    pgs_alpha: float | None = None  # AusmpUp with a PressureGradientScaling held at this alpha
 
     def processes(self):
         species = [f"S{i}" for i in range(self.n_species)]
         eos = fv.PerfectGas({"gamma": self.gamma, "Rgas": self.Rgas}, species)
-        fc = {"ausm": fv.Ausm, "average": fv.AverageFlux, "off": fv.OffFlux, "rusanov": fv.Rusanov}.get(self.flux)
-        flux = fc() if fc else None
+        if self.flux == "ausmpup":
+            pgs = fv.PressureGradientScaling(eos, alphaInit=self.pgs_alpha, domainLength=1.0) if self.pgs_alpha else None
+            flux = fv.AusmpUp(self.minf, pgs)
+        else:
+            fc = {"ausm": fv.Ausm, "average": fv.AverageFlux, "off": fv.OffFlux, "rusanov": fv.Rusanov}.get(self.flux)
+            flux = fc() if fc else None
         tr = fv.Constant(k=self.k, mu=self.mu, diff=self.diff) if (self.k or self.mu or self.diff) else None
         procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, flux, tr)]
         if self.n_species:
@@ -69,7 +75,8 @@ class Case:
 
     def oracle_physics(self):
         return orc.physics(self.gamma, self.Rgas, self.n_species, self.flux, self.recon, self.limiter,
-                           self.max_lim_grad, self.k, self.mu, self.diff)
+                           self.max_lim_grad, self.k, self.mu, self.diff, ausmpup_minf=self.minf,
+                           pgs_alpha=self.pgs_alpha or 1.0)
 
     def oracle_bcs(self):
         kinds = {"inflow": 1, "outflow": 2}
@@ -137,4 +144,11 @@ def standard_cases():
                       limiter="barthjespersen", mu=0.05, k=0.5, diff=2e-2, bcs=all_outflow(6)))
     cases.append(Case("tets_ausm_vanleer", TetBoxMesh(3, 0.15, ordering="natural"), recon="leastsquares",
                       limiter="vanleer", mu=0.05, k=0.5, bcs=all_outflow(6)))
+    # AUSM+up (ausmpUp.cpp): plain, with mInf and a pressure-gradient-scaling alpha, low-Mach 2-D
+    cases.append(Case("ausmpup_ns_lsq_bj", box3(8), flux="ausmpup", recon="leastsquares", limiter="barthjespersen",
+                      mu=1.8e-5, k=0.0251125))
+    cases.append(Case("ausmpup_minf_pgs_species", chan(6), flux="ausmpup", minf=0.3, pgs_alpha=3.0, n_species=2,
+                      recon="leastsquares", limiter="minmod", mu=0.5, k=5.0, diff=0.2, bcs=inflow_x(2)))
+    cases.append(Case("ausmpup_lowmach_2d", BoxMesh([12, 10], [0, 0], [1, 1], [True, True]), flux="ausmpup",
+                      minf=0.05, pgs_alpha=10.0, mach=0.02))
     return cases

@@ -3,6 +3,8 @@
 Run in a container that has the ABLATE source tree (default /root/reference); the fixtures it writes are data
 (inputs and expected outputs), committed next to this script:
   flux_calculators.json   fluxCalculatorTests.cpp: Ausm, AverageFlux and OffFlux tables
+  ausmpup.json            fluxCalculatorTests.cpp: AusmpUp tables with their constructor arguments (mInf, and the
+                          initial alpha of the PressureGradientScaling object when one is attached)
   ns_advection_flux.json  navierStokesTransportTests.cpp: AdvectionFlux cases
   stress_tensor.json      navierStokesTransportTests.cpp: CompressibleFlowComputeStressTensor cases
   perfect_gas.json        perfectGasTests.cpp: PerfectGas thermodynamic property cases
@@ -72,6 +74,27 @@ def flux_calculators():
         if name not in ("Ausm", "AverageFlux", "OffFlux"):
             continue
         rec = {"source": f"tests/unitTests/finiteVolume/fluxCalculator/fluxCalculatorTests.cpp:{line_of(text, m.start())}"}
+        for key in ("uL", "aL", "rhoL", "pL", "uR", "aR", "rhoR", "pR", "expectedMassFlux", "expectedInterfacePressure"):
+            rec[key] = [num(x) for x in field_list(block, key)]
+        rec["expectedDirection"] = [x.strip() for x in field_list(block, "expectedDirection")]
+        rec["tolerance"] = {"massFlux": 1e-2, "interfacePressure": 1.0}
+        out[name] = rec
+    return out
+
+
+def ausmpup_tables():
+    path = UT / "finiteVolume" / "fluxCalculator" / "fluxCalculatorTests.cpp"
+    text = path.read_text()
+    out = {}
+    for m in re.finditer(r"\(FluxCalculatorTestParameters\)\s*\{", text):
+        block, _ = braces(text, m.end() - 1)
+        name = re.search(r'\.testName\s*=\s*"([^"]+)"', block).group(1)
+        if not name.startswith("AusmpUp"):
+            continue
+        ctor = re.search(r"AusmpUp>\(\s*([^,)]+)", block)
+        pgs = re.search(r"PressureGradientScaling>\(\s*[^,]+,\s*([^,]+),", block)
+        rec = {"source": f"tests/unitTests/finiteVolume/fluxCalculator/fluxCalculatorTests.cpp:{line_of(text, m.start())}",
+               "mInf": num(ctor.group(1)), "pgsAlpha": num(pgs.group(1)) if pgs else None}
         for key in ("uL", "aL", "rhoL", "pL", "uR", "aR", "rhoR", "pR", "expectedMassFlux", "expectedInterfacePressure"):
             rec[key] = [num(x) for x in field_list(block, key)]
         rec["expectedDirection"] = [x.strip() for x in field_list(block, "expectedDirection")]
@@ -178,6 +201,7 @@ def main():
     adv, stress = ns_cases()
     pgt = perfect_gas()
     (OUT / "flux_calculators.json").write_text(json.dumps(fc, indent=1))
+    (OUT / "ausmpup.json").write_text(json.dumps(ausmpup_tables(), indent=1))
     (OUT / "ns_advection_flux.json").write_text(json.dumps(adv, indent=1))
     (OUT / "stress_tensor.json").write_text(json.dumps(stress, indent=1))
     (OUT / "perfect_gas.json").write_text(json.dumps(pgt, indent=1, allow_nan=True))

@@ -40,6 +40,7 @@ def test_struct_layouts_match_header(tmp_path):
         "abl_fv_physics_desc": [f[0] for f in _lib.PhysicsDesc._fields_],
         "abl_fv_bc_desc": [f[0] for f in _lib.BcDesc._fields_],
         "abl_fv_info": [f[0] for f in _lib.Info._fields_],
+        "abl_fv_pgs_desc": [f[0] for f in _lib.PgsDesc._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
@@ -52,7 +53,7 @@ def test_struct_layouts_match_header(tmp_path):
     subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
     got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n") if l)
     for st, cls in (("abl_fv_mesh_desc", _lib.MeshDesc), ("abl_fv_physics_desc", _lib.PhysicsDesc),
-                    ("abl_fv_bc_desc", _lib.BcDesc), ("abl_fv_info", _lib.Info)):
+                    ("abl_fv_bc_desc", _lib.BcDesc), ("abl_fv_info", _lib.Info), ("abl_fv_pgs_desc", _lib.PgsDesc)):
         assert int(got[st]) == ctypes.sizeof(cls), st
         for f in fields[st]:
             assert int(got[f"{st}.{f}"]) == getattr(cls, f).offset, f"{st}.{f}"

@@ -1,0 +1,115 @@
+"""AUSM+up and the pressure-gradient scaling on the GPU against the oracle: the FP64 RHS parity of the AusmpUp
+cases runs with the standard cases (test_gpu_parity.py); here the PGS pre-step update, the alpha it feeds back into
+the flux, the mixed form and the adjoint's refusal."""
+import numpy as np
+import pytest
+
+from ablate_amd import fv
+from ablate_amd.mesh import BoxMesh
+from oracle import binding as orc
+from tests.fvcases import Case, rel_err_per_field
+
+pytestmark = pytest.mark.gpu
+
+
+def _pgs_case(mach=0.05):
+    return Case("pgs_tg", BoxMesh([12, 12, 12], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup", minf=0.1,
+                recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125, mach=mach)
+
+
+def _solver_with_pgs(case, alpha_init, form="fp64", **kw):
+    eos = fv.PerfectGas({"gamma": case.gamma, "Rgas": case.Rgas})
+    pgs = fv.PressureGradientScaling(eos, alphaInit=alpha_init, **kw)
+    flux = fv.AusmpUp(case.minf, pgs)
+    procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, flux, fv.Constant(k=case.k, mu=case.mu)), pgs]
+    s = fv.FiniteVolumeSolver(case.mesh, procs, {"petscfv_type": case.recon, "petsclimiter_type": case.limiter},
+                              form=form)
+    return s, pgs
+
+
+@pytest.mark.parametrize("alpha0,dt,L", [(1.0, 1e-4, 0.5), (20.0, 3e-5, 1.0), (100.0, 1e-3, float("nan"))])
+def test_pgs_update_matches_oracle(alpha0, dt, L, torch_cuda):
+    torch = torch_cuda
+    case = _pgs_case()
+    print(f"[ausmpup] pgs update alpha0={alpha0} dt={dt} L={L}", flush=True)
+    U0 = case.initial_state()
+    s, pgs = _solver_with_pgs(case, alpha0, domainLength=L, maxAlphaAllowed=60.0)
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    alpha = pgs.UpdatePreconditioner(dt, U)
+    want, mach = orc.pgs_update(case.mesh.arrays, case.oracle_physics(), U0, dt, alpha0, L, max_alpha_allowed=60.0)
+    assert alpha == pytest.approx(want, rel=1e-12)
+    assert pgs.GetMaxMach() == pytest.approx(mach, rel=1e-13)
+    assert pgs.GetAlpha() == alpha
+    # the next RHS sees the updated alpha
+    R = torch.zeros_like(U)
+    s.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    case.pgs_alpha = alpha
+    Rref, _ = case.oracle_rhs(U0.copy())
+    err = rel_err_per_field(R.cpu().numpy(), Rref, case.mesh.n_interior)
+    print(f"[ausmpup] alpha {alpha} rhs error {err}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_pgs_alpha_changes_the_flux(torch_cuda):
+    """alpha enters the interface speed of sound and pressure: a different alpha gives a different residual."""
+    torch = torch_cuda
+    case = _pgs_case()
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    s, pgs = _solver_with_pgs(case, 1.0, domainLength=1.0)
+    R1, R2 = torch.zeros_like(U), torch.zeros_like(U)
+    s.compute_rhs(0.0, U, R1)
+    from ablate_amd import _lib
+    _lib.check(_lib.lib().abl_fv_set_pgs_alpha(s.handle, 5.0), s.handle, "abl_fv_set_pgs_alpha")
+    assert pgs.GetAlpha() == 5.0
+    s.compute_rhs(0.0, U, R2)
+    torch.cuda.synchronize()
+    assert not torch.equal(R1, R2)
+
+
+def test_pgs_update_requires_attachment(torch_cuda):
+    torch = torch_cuda
+    case = Case("no_pgs", BoxMesh([4, 4, 4], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup")
+    s = case.solver()
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    import ctypes
+    from ablate_amd import _lib
+    a, m = ctypes.c_double(), ctypes.c_double()
+    rc = _lib.lib().abl_fv_pgs_update(s.handle, ctypes.c_void_p(U.data_ptr()), 1e-3, ctypes.byref(a), ctypes.byref(m),
+                                      None)
+    assert rc != 0 and b"PressureGradientScaling" in _lib.lib().abl_fv_last_error(s.handle)
+
+
+@pytest.mark.parametrize("minf,alpha,tol", [(0.0, None, 5e-6), (0.2, 2.0, 3e-5)])
+@pytest.mark.parametrize("strategy", ["tile", "face_flux", "recompute"])
+def test_ausmpup_mixed_form(strategy, minf, alpha, tol, torch_cuda):
+    """The mixed form evaluates AUSM+up faces on its FP64 route (full flux of the FP32-reconstructed states minus the
+    FP64 baseline), so the FP32 rounding of the stored gradients enters at full size; on this coarse mesh that is
+    the 5e-6 of the other coarse mixed cases.  With a pressure-gradient scaling alpha the interface pressure is
+    divided by alpha^2 (ausmpUp.cpp:94) while the rounding of the other terms is not, so the relative error grows
+    by up to alpha^2 in the momentum rows."""
+    torch = torch_cuda
+    case = Case("ausmpup_mixed", BoxMesh([10, 10, 10], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup", minf=minf,
+                pgs_alpha=alpha, recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125)
+    U0 = case.initial_state()
+    Rref, _ = case.oracle_rhs(U0.copy())
+    s = case.solver(form="mixed")
+    s.set_strategy(strategy)
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    R = torch.zeros_like(U)
+    s.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    err = rel_err_per_field(R.cpu().numpy(), Rref, case.mesh.n_interior)
+    print(f"[ausmpup] mixed {strategy} error {err}", flush=True)
+    assert np.all(np.isfinite(R.cpu().numpy())) and err.max() <= tol
+
+
+def test_ausmpup_adjoint_is_refused(torch_cuda):
+    torch = torch_cuda
+    case = Case("ausmpup_vjp", BoxMesh([4, 4, 4], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup")
+    s = case.solver()
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Rb = torch.ones_like(U)
+    Ub = torch.empty_like(U)
+    with pytest.raises(RuntimeError):
+        s.rhs_vjp(0.0, U, Rb, Ub)
