@@ -207,11 +207,13 @@ __device__ inline int ausmpup(T minf, T alpha, T uL, T aL, T rhoL, T pL, T uR, T
 }
 
 // Flux calculators behind FluxCalculatorFunction: Ausm, AverageFlux (averageFlux.cpp:2-10), OffFlux, AusmpUp.
-template <typename T>
+// AUP = false compiles AusmpUp out: its temporaries would otherwise raise the register count of kernels that never
+// run it (the mixed form's FP64 route: 156 -> 178 VGPRs, one wave per SIMD less).
+template <typename T, bool AUP = true>
 __device__ inline int flux_calculator(int type, const Phys& ph, T uL, T aL, T rhoL, T pL, T uR, T aR, T rhoR, T pR,
                                       T& mdot, T& p12) {
     if (type == 1) return ausm(uL, aL, rhoL, pL, uR, aR, rhoR, pR, mdot, p12);
-    if (type == 5)
+    if (AUP && type == 5)
         return ausmpup(T(ph.minf), T(ph.pgs_alpha), uL, aL, rhoL, pL, uR, aR, rhoR, pR, mdot, p12);
     if (type == 2) {
         mdot = T(0.5) * (uL * rhoL + uR * rhoR);
@@ -252,14 +254,14 @@ __device__ inline void decode_side(const T* u, const T* nhat, const Phys& ph, Si
 // NavierStokesTransport::AdvectionFlux (navierStokesTransport.cpp:99-207) for euler components, plus the mass
 // flux per area and upwind side needed by SpeciesTransport::AdvectionFlux (speciesTransport.cpp:325-337: any
 // non-LEFT direction takes the right state).  flux[] receives dim+2 values.
-template <typename T, int DIM>
+template <typename T, int DIM, bool AUP = true>
 __device__ inline void advection_flux(int type, const T* uL, const T* uR, const T* nhat, T area, const Phys& ph, T* flux,
                                       T& mdotA, int& species_right, T& rhoL_out, T& rhoR_out) {
     SideState<T, DIM> L, R;
     decode_side<T, DIM>(uL, nhat, ph, L);
     decode_side<T, DIM>(uR, nhat, ph, R);
     T massFlux, p12;
-    const int dir = flux_calculator<T>(type, ph, L.un, L.a, L.rho, L.p, R.un, R.a, R.rho, R.p, massFlux, p12);
+    const int dir = flux_calculator<T, AUP>(type, ph, L.un, L.a, L.rho, L.p, R.un, R.a, R.rho, R.p, massFlux, p12);
     const T mA = massFlux * area;
     const T p12A = p12 * area;
     flux[kRho] = mA;

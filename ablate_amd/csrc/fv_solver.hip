@@ -1116,7 +1116,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
         } else {
             double mA, rlo, rro;
             int right;
-            advection_flux<double, DIM>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rlo, rro);
+            advection_flux<double, DIM, false>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rlo, rro);
             if (right)
                 cro = mA / rro;
             else
@@ -2805,6 +2805,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         return fail(nullptr, "abl_fv_create: unknown reconstruction");
     if (p.limiter < ABL_FV_LIMITER_NONE || p.limiter > ABL_FV_LIMITER_BARTH_JESPERSEN)
         return fail(nullptr, "abl_fv_create: unknown limiter");
+    // the mixed form has no perturbation form of AusmpUp: such handles are created in the FP64 form
+    if (form == ABL_FV_FORM_MIXED && p.flux_calculator == ABL_FV_FLUX_AUSMPUP) form = ABL_FV_FORM_FP64;
     std::unique_ptr<abl_fv_solver> s(new abl_fv_solver());
     try {
         s->phys = p;

@@ -228,6 +228,7 @@ class FiniteVolumeSolver:
         _lib.check(_lib.lib().abl_fv_create(ctypes.byref(mesh.desc), ctypes.byref(phys), fm, ctypes.byref(h)), None,
                    "abl_fv_create")
         self._h = h
+        self.form = "fp64" if self.info().form == _lib.FORM_FP64 else "mixed"  # AusmpUp runs the FP64 form
         self._theta_slots = []
         self.pgs = pgs
         if pgs is not None:

@@ -36,7 +36,8 @@ typedef void* abl_fv_stream;
 
 /* ---- enumerations (values are part of the ABI) ------------------------------------------------------------ */
 
-/* Arithmetic form of the RHS. */
+/* Arithmetic form of the RHS.  The mixed form has no perturbation form of the AusmpUp calculator: a mixed handle
+ * with AusmpUp is created in the FP64 form (abl_fv_info.form reports it). */
 enum { ABL_FV_FORM_FP64 = 0, ABL_FV_FORM_MIXED = 1 };
 
 /* Flux calculators: fluxCalculator::Ausm (ausm.cpp:2-43), AverageFlux (averageFlux.cpp:2-10),
