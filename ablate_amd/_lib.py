@@ -112,6 +112,7 @@ SIGNATURES = {
     "abl_fv_axpy": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "abl_fv_ts_euler": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double, c_int64, _dp,
                                   POINTER(c_int64), c_void_p]),
+    "abl_fv_mesh_gmsh": (c_int32, [c_char_p, c_int32, POINTER(c_void_p)]),
     "abl_fv_set_pgs": (c_int32, [c_void_p, POINTER(PgsDesc)]),
     "abl_fv_pgs_update": (c_int32, [c_void_p, c_void_p, c_double, _dp, _dp, c_void_p]),
     "abl_fv_get_pgs_alpha": (c_int32, [c_void_p, _dp]),

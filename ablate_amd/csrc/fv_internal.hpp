@@ -23,6 +23,10 @@ struct HostMesh {
     bool has_bc = false;
 };
 
+// Ghost cell `ghost` of boundary face f: the centroid of `interior` reflected across the face plane and its volume
+// (the DMPlexComputeGeometryFVM ghost-cell convention).
+void finish_ghost(HostMesh& m, int64_t f, int32_t interior, int32_t ghost);
+
 // Sets the calling thread's global error message (returned by abl_fv_last_error(NULL)).
 void set_global_error(const std::string& msg);
 

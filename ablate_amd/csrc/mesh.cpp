@@ -95,6 +95,8 @@ std::vector<int64_t> box_order(int dim, const int32_t* n, int32_t ordering) {
     return rcm_order(N, off, adj);
 }
 
+}  // namespace
+
 void finish_ghost(HostMesh& m, int64_t f, int32_t interior, int32_t ghost) {
     // reflect the interior centroid across the face plane (DMPlexComputeGeometryFVM ghost convention)
     const double* n = &m.face_normal[3 * f];
@@ -110,8 +112,6 @@ void finish_ghost(HostMesh& m, int64_t f, int32_t interior, int32_t ghost) {
     for (int d = 0; d < 3; d++) m.cell_centroid[3 * ghost + d] = xi[d] + 2.0 * a * n[d];
     m.cell_volume[ghost] = m.cell_volume[interior];
 }
-
-}  // namespace
 
 std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& off, const std::vector<int32_t>& adj) {
     std::vector<int64_t> perm;

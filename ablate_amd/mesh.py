@@ -1,7 +1,8 @@
 """Generated finite-volume meshes (DMPlex-style FV geometry) exposed as numpy views.
 
 ``BoxMesh`` mirrors ablate::domain::BoxMesh + GhostBoundaryCells (src/domain/boxMesh.cpp:17-43,
-src/domain/modifiers/ghostBoundaryCells.cpp:4-8); ``TetBoxMesh`` is the jittered tetrahedral unit cube.
+src/domain/modifiers/ghostBoundaryCells.cpp:4-8); ``TetBoxMesh`` is the jittered tetrahedral unit cube;
+``GmshMesh`` reads a Gmsh file as ablate::domain::MeshFile + GhostBoundaryCells do (src/domain/meshFile.cpp).
 Arrays are views of memory owned by the native mesh object and live as long as the Python object.
 """
 from __future__ import annotations
@@ -29,12 +30,19 @@ class MeshArrays:
     face_bc: np.ndarray | None  # [F] int32
 
 
-def _view(ptr, shape, dtype):
+def _view(ptr, shape, dtype, owner):
+    """numpy view of native memory; the view's base keeps `owner` (the native mesh) alive, so arrays taken from a
+    temporary mesh object stay valid."""
     if not ptr:
         return None
     count = int(np.prod(shape))
     ctype = ctypes.c_double if dtype == np.float64 else ctypes.c_int32
-    buf = ctypes.cast(ptr, ctypes.POINTER(ctype * count)).contents
+
+    class _Buf(ctype * count):  # a subclass can carry the owner reference
+        pass
+
+    buf = _Buf.from_address(ctypes.cast(ptr, ctypes.c_void_p).value)
+    buf._owner = owner
     return np.frombuffer(buf, dtype=dtype).reshape(shape)
 
 
@@ -49,13 +57,13 @@ class _NativeMesh:
             n_cells=N,
             n_interior=desc.n_interior,
             n_faces=F,
-            face_cells=_view(desc.face_cells, (F, 2), np.int32),
-            face_normal=_view(desc.face_normal, (F, 3), np.float64),
-            face_centroid=_view(desc.face_centroid, (F, 3), np.float64),
-            face_shift=_view(desc.face_shift, (F, 3), np.float64),
-            cell_centroid=_view(desc.cell_centroid, (N, 3), np.float64),
-            cell_volume=_view(desc.cell_volume, (N,), np.float64),
-            face_bc=_view(desc.face_bc, (F,), np.int32),
+            face_cells=_view(desc.face_cells, (F, 2), np.int32, self),
+            face_normal=_view(desc.face_normal, (F, 3), np.float64, self),
+            face_centroid=_view(desc.face_centroid, (F, 3), np.float64, self),
+            face_shift=_view(desc.face_shift, (F, 3), np.float64, self),
+            cell_centroid=_view(desc.cell_centroid, (N, 3), np.float64, self),
+            cell_volume=_view(desc.cell_volume, (N,), np.float64, self),
+            face_bc=_view(desc.face_bc, (F,), np.int32, self),
         )
         self.desc = desc
 
@@ -118,3 +126,30 @@ class TetBoxMesh(_NativeMesh):
         )
         self.n, self.jitter, self.seed = n, jitter, seed
         super().__init__(h)
+
+
+def gmsh_physical_names(path) -> dict:
+    """The $PhysicalNames section of a Gmsh file: name -> (dimension, tag)."""
+    names, lines = {}, open(path, encoding="utf-8", errors="replace").read().splitlines()
+    if "$PhysicalNames" not in lines:
+        return names
+    i = lines.index("$PhysicalNames")
+    for line in lines[i + 2 : i + 2 + int(lines[i + 1])]:
+        dim, tag, name = line.split(maxsplit=2)
+        names[name.strip().strip('"')] = (int(dim), int(tag))
+    return names
+
+
+class GmshMesh(_NativeMesh):
+    """Gmsh MSH 4.1 ASCII mesh with one ghost cell per boundary face; boundary faces carry their physical tag as
+    boundary-condition slot (the label value of a boundary condition's labelIds).  ``labels`` maps the physical
+    names of the boundary groups to those slots."""
+
+    def __init__(self, path, ordering: str = "natural"):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().abl_fv_mesh_gmsh(str(path).encode(), _lib.ORDER[ordering], ctypes.byref(h)), None,
+                   "abl_fv_mesh_gmsh")
+        self.path = str(path)
+        super().__init__(h)
+        bdim = self.arrays.dim - 1
+        self.labels = {n: t for n, (d, t) in gmsh_physical_names(path).items() if d == bdim}

@@ -149,6 +149,12 @@ int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* lower, cons
 /* Unit-cube tetrahedral mesh: n^3 cubes split into 6 tets each, interior nodes jittered by up to
  * jitter * h (uniform, seeded), boundary faces with ghost cells (face_bc = 0..5 by cube side). */
 int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int32_t ordering, abl_fv_mesh_handle* out);
+/* Mesh file ingestion for `!ablate::domain::MeshFile` inputs in Gmsh format (PETSc's DMPlexCreateGmshFromFile
+ * + GhostBoundaryCells + DMPlexComputeGeometryFVM, src/domain/meshFile.cpp): MSH 4.1 ASCII, first-order
+ * triangles / quadrilaterals (2-D) or tetrahedra / hexahedra / prisms / pyramids (3-D); boundary faces carry the
+ * physical tag of the element on them as face_bc (the "Face Sets" label value; tags must be below 16) and one
+ * ghost cell each.  ordering: ABL_FV_ORDER_NATURAL (file order) or ABL_FV_ORDER_RCM. */
+int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_handle* out);
 /* Descriptor view of a generated mesh (pointers stay valid until abl_fv_mesh_free). */
 int abl_fv_mesh_describe(abl_fv_mesh_handle mesh, abl_fv_mesh_desc* desc);
 void abl_fv_mesh_free(abl_fv_mesh_handle mesh);
@@ -176,8 +182,9 @@ int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accum
 /* Vector-Jacobian product of the RHS (build-defined; the reference has no adjoint):
  *   Ubar = Rbar^T dR/dU   (interior rows; ghost rows receive zero)
  *   theta_bar[3 * slot + {0,1,2}] += Rbar^T dR/d(rho_in, u_in, T_in) for every INFLOW slot (host array sized
- *   3 * number of slots, or NULL).  Fluxes are recomputed, not stored.  Supported for first-order
- *   reconstruction without diffusion. */
+ *   3 * number of slots, or NULL).  Fluxes are recomputed, not stored.  Supported for the FP64 form with the
+ *   Ausm, Rusanov or Off calculators, first order or least squares with the Barth-Jespersen limiter (or none),
+ *   viscous and species fluxes; the first second-order call allocates its workspace. */
 int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar, double* theta_bar,
                    abl_fv_stream stream);
 

@@ -10,6 +10,8 @@ Run in a container that has the ABLATE source tree (default /root/reference); th
   perfect_gas.json        perfectGasTests.cpp: PerfectGas thermodynamic property cases
   shock_tube.json         compressibleShockTubeTests.cpp: first-order AUSM shock-tube cases (forward Euler, CFL
                           time step) with the reference's x, rho, u, e profiles
+  pipeFlow.msh            tests/integrationTests/inputs/compressibleFlow/gmshPipeFlow/pipeFlow.msh (data, copied)
+  mixedCells2D.msh        tests/integrationTests/inputs/compressibleFlow/mixedCellTypeTest/mixedCells2D.msh (copied)
 Each record keeps the source file and line of its table so that a reader can find it.
 """
 from __future__ import annotations
@@ -17,6 +19,7 @@ from __future__ import annotations
 import json
 import math
 import re
+import shutil
 import sys
 from pathlib import Path
 
@@ -205,6 +208,8 @@ def main():
     (OUT / "ns_advection_flux.json").write_text(json.dumps(adv, indent=1))
     (OUT / "stress_tensor.json").write_text(json.dumps(stress, indent=1))
     (OUT / "perfect_gas.json").write_text(json.dumps(pgt, indent=1, allow_nan=True))
+    for rel in ("compressibleFlow/gmshPipeFlow/pipeFlow.msh", "compressibleFlow/mixedCellTypeTest/mixedCells2D.msh"):
+        shutil.copyfile(REF / "tests" / "integrationTests" / "inputs" / rel, OUT / Path(rel).name)
     print({k: len(v["uL"]) for k, v in fc.items()}, len(adv), len(stress), len(pgt), len(st))
 
 
