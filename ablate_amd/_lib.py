@@ -20,7 +20,7 @@ LIMITER = {
     "barthjespersen": 8,
 }
 BC_NONE, BC_INFLOW, BC_OUTFLOW = 0, 1, 2
-ORDER = {"natural": 0, "brick": 1, "rcm": 2}
+ORDER = {"natural": 0, "brick": 1, "rcm": 2, "morton": 3}
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)

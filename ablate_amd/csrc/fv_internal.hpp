@@ -30,6 +30,9 @@ void finish_ghost(HostMesh& m, int64_t f, int32_t interior, int32_t ghost);
 // Sets the calling thread's global error message (returned by abl_fv_last_error(NULL)).
 void set_global_error(const std::string& msg);
 
+// Morton (Z-order) order of n points xc[3 n] (21 bits per axis over their bounding cube); returns perm[new] = old.
+std::vector<int64_t> morton_order(const std::vector<double>& xc, int64_t n);
+
 // Reverse Cuthill-McKee order of a cell graph given as CSR adjacency; returns perm[new] = old.
 std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& off, const std::vector<int32_t>& adj);
 

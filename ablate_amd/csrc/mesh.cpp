@@ -8,7 +8,8 @@
 //     across the face plane and the interior volume (the DMPlex ghost-cell convention);
 //   - periodic directions produce wrap faces whose right cell is translated into the left cell's frame by
 //     face_shift (the localized-coordinate image).
-// Cell orderings: natural (x fastest), brick (8^3 / 16^2 bricks in Morton order of the bricks) and reverse
+// Cell orderings: natural (x fastest), brick (8^3 / 16^2 bricks in Morton order of the bricks), Morton order of the
+// cell centroids (unstructured meshes) and reverse
 // Cuthill-McKee.
 #include <algorithm>
 #include <cmath>
@@ -113,6 +114,26 @@ void finish_ghost(HostMesh& m, int64_t f, int32_t interior, int32_t ghost) {
     m.cell_volume[ghost] = m.cell_volume[interior];
 }
 
+std::vector<int64_t> morton_order(const std::vector<double>& xc, int64_t n) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t c = 0; c < n; c++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], xc[3 * c + d]);
+            hi[d] = std::max(hi[d], xc[3 * c + d]);
+        }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    std::vector<std::pair<uint64_t, int64_t>> key(n);
+    for (int64_t c = 0; c < n; c++) {
+        uint32_t q[3];
+        for (int d = 0; d < 3; d++) q[d] = (uint32_t)std::min(2097151.0, (xc[3 * c + d] - lo[d]) / span * 2097151.0);
+        key[c] = {morton3(q[0], q[1], q[2]), c};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int64_t> order(n);
+    for (int64_t c = 0; c < n; c++) order[c] = key[c].second;
+    return order;
+}
+
 std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& off, const std::vector<int32_t>& adj) {
     std::vector<int64_t> perm;
     perm.reserve(n);
@@ -180,6 +201,8 @@ extern "C" int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* 
     try {
         if (!out || !cells || !lower || !upper) throw std::invalid_argument("abl_fv_mesh_box: null argument");
         if (dim != 2 && dim != 3) throw std::invalid_argument("abl_fv_mesh_box: dim must be 2 or 3");
+        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_BRICK && ordering != ABL_FV_ORDER_RCM)
+            throw std::invalid_argument("abl_fv_mesh_box: ordering must be natural, brick or RCM");
         int32_t n[3] = {1, 1, 1};
         double h[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, len[3] = {1, 1, 1};
         int per[3] = {0, 0, 0};
@@ -290,6 +313,8 @@ extern "C" int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int3
         if (!out) throw std::invalid_argument("abl_fv_mesh_tet_box: null argument");
         if (n < 1 || n > 400) throw std::invalid_argument("abl_fv_mesh_tet_box: n must be in [1, 400]");
         if (jitter < 0 || jitter >= 0.5) throw std::invalid_argument("abl_fv_mesh_tet_box: jitter must be in [0, 0.5)");
+        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_RCM && ordering != ABL_FV_ORDER_MORTON)
+            throw std::invalid_argument("abl_fv_mesh_tet_box: ordering must be natural, RCM or Morton");
         const int64_t nn = n + 1;
         const double h = 1.0 / n;
         std::vector<double> X(3 * nn * nn * nn);
@@ -362,7 +387,16 @@ extern "C" int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int3
         recs.shrink_to_fit();
         // ordering of the tets
         std::vector<int64_t> order(NI);
-        if (ordering == ABL_FV_ORDER_RCM) {
+        if (ordering == ABL_FV_ORDER_MORTON) {
+            std::vector<double> xc(3 * NI);
+            for (int64_t t = 0; t < NI; t++)
+                for (int d = 0; d < 3; d++) {
+                    double s = 0.0;
+                    for (int q = 0; q < 4; q++) s += X[3 * tet[4 * t + q] + d];
+                    xc[3 * t + d] = 0.25 * s;
+                }
+            order = abl::morton_order(xc, NI);
+        } else if (ordering == ABL_FV_ORDER_RCM) {
             std::vector<int64_t> off(NI + 1, 0);
             for (auto& p : fp)
                 if (p.b >= 0) {

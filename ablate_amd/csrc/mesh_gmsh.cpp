@@ -157,8 +157,8 @@ extern "C" int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_
     using namespace abl;
     try {
         if (!path || !out) throw std::invalid_argument("abl_fv_mesh_gmsh: null argument");
-        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_RCM)
-            throw std::invalid_argument("abl_fv_mesh_gmsh: ordering must be natural or RCM");
+        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_RCM && ordering != ABL_FV_ORDER_MORTON)
+            throw std::invalid_argument("abl_fv_mesh_gmsh: ordering must be natural, RCM or Morton");
         std::ifstream f(path, std::ios::binary);
         if (!f) throw std::runtime_error(std::string("abl_fv_mesh_gmsh: cannot open ") + path);
         std::stringstream ss;
@@ -303,7 +303,15 @@ extern "C" int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_
         // cell order
         std::vector<int64_t> order(NI);
         std::iota(order.begin(), order.end(), 0);
-        if (ordering == ABL_FV_ORDER_RCM) {
+        if (ordering == ABL_FV_ORDER_MORTON) {
+            std::vector<double> xc(3 * NI, 0.0);  // vertex means as sort keys
+            for (int64_t c = 0; c < NI; c++) {
+                const Elem& e = elems[cell_elem[c]];
+                for (auto n : e.nodes)
+                    for (int d = 0; d < 3; d++) xc[3 * c + d] += X[3 * n + d] / (double)e.nodes.size();
+            }
+            order = morton_order(xc, NI);
+        } else if (ordering == ABL_FV_ORDER_RCM) {
             std::vector<int64_t> off(NI + 1, 0);
             for (auto& p : fp)
                 if (p.b >= 0) {

@@ -139,7 +139,12 @@ typedef struct abl_fv_info {
  *      modifiers/ghostBoundaryCells.cpp:4-8, and DMPlex RCM reordering) --------------------------------------- */
 
 /* Ordering of generated cells. */
-enum { ABL_FV_ORDER_NATURAL = 0, ABL_FV_ORDER_BRICK = 1, ABL_FV_ORDER_RCM = 2 };
+enum {
+    ABL_FV_ORDER_NATURAL = 0,
+    ABL_FV_ORDER_BRICK = 1,  /* boxes: 8^3 / 16^2 bricks in Morton order of the bricks */
+    ABL_FV_ORDER_RCM = 2,    /* reverse Cuthill-McKee of the cell graph */
+    ABL_FV_ORDER_MORTON = 3  /* Morton (Z-order) curve of the cell centroids (tet and Gmsh meshes) */
+};
 
 /* Uniform quad (dim 2) / hex (dim 3) box: cells[d] cells per direction on [lower, upper], periodic[d] != 0 for
  * periodic directions.  Non-periodic directions get one ghost cell per boundary face; boundary faces carry
@@ -147,13 +152,14 @@ enum { ABL_FV_ORDER_NATURAL = 0, ABL_FV_ORDER_BRICK = 1, ABL_FV_ORDER_RCM = 2 };
 int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* lower, const double* upper, const int32_t* periodic,
                     int32_t ordering, abl_fv_mesh_handle* out);
 /* Unit-cube tetrahedral mesh: n^3 cubes split into 6 tets each, interior nodes jittered by up to
- * jitter * h (uniform, seeded), boundary faces with ghost cells (face_bc = 0..5 by cube side). */
+ * jitter * h (uniform, seeded), boundary faces with ghost cells (face_bc = 0..5 by cube side).  ordering: natural,
+ * RCM or Morton. */
 int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int32_t ordering, abl_fv_mesh_handle* out);
 /* Mesh file ingestion for `!ablate::domain::MeshFile` inputs in Gmsh format (PETSc's DMPlexCreateGmshFromFile
  * + GhostBoundaryCells + DMPlexComputeGeometryFVM, src/domain/meshFile.cpp): MSH 4.1 ASCII, first-order
  * triangles / quadrilaterals (2-D) or tetrahedra / hexahedra / prisms / pyramids (3-D); boundary faces carry the
  * physical tag of the element on them as face_bc (the "Face Sets" label value; tags must be below 16) and one
- * ghost cell each.  ordering: ABL_FV_ORDER_NATURAL (file order) or ABL_FV_ORDER_RCM. */
+ * ghost cell each.  ordering: ABL_FV_ORDER_NATURAL (file order), ABL_FV_ORDER_RCM or ABL_FV_ORDER_MORTON. */
 int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_handle* out);
 /* Descriptor view of a generated mesh (pointers stay valid until abl_fv_mesh_free). */
 int abl_fv_mesh_describe(abl_fv_mesh_handle mesh, abl_fv_mesh_desc* desc);

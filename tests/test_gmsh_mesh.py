@@ -25,7 +25,7 @@ def label_normal_sum(a, tag):
     return a.face_normal[a.face_bc == tag].sum(axis=0)
 
 
-@pytest.mark.parametrize("ordering", ["natural", "rcm"])
+@pytest.mark.parametrize("ordering", ["natural", "rcm", "morton"])
 def test_reference_pipe_mesh(ordering):
     """0.5 x 0.1 channel: inlet at x = 0, outlet at x = 0.5, walls at y = 0 and 0.1."""
     m = GmshMesh(PIPE, ordering)
@@ -193,3 +193,18 @@ def test_rejects_unsupported_files(tmp_path):
         GmshMesh(p)
     with pytest.raises(RuntimeError, match="cannot open"):
         GmshMesh(tmp_path / "missing.msh")
+
+
+def test_morton_order_is_a_local_permutation():
+    """Morton order of a tet mesh: the same cells (volumes and centroids as multisets), faces numbered by their left
+    cell, and neighbours close in index (the point of the order: median |R - L| far below the RCM order's)."""
+    from ablate_amd.mesh import TetBoxMesh
+
+    a, b = TetBoxMesh(12, 0.2, ordering="rcm").arrays, TetBoxMesh(12, 0.2, ordering="morton").arrays
+    NI = a.n_interior
+    assert np.allclose(np.sort(a.cell_volume[:NI]), np.sort(b.cell_volume[:NI]), rtol=0, atol=0)
+    key = lambda x: np.lexsort(x[:NI].T)
+    assert np.array_equal(a.cell_centroid[:NI][key(a.cell_centroid)], b.cell_centroid[:NI][key(b.cell_centroid)])
+    assert np.all(np.diff(b.face_cells[:, 0]) >= 0)
+    gap = lambda m: np.median(np.abs(np.diff(m.face_cells[m.face_cells[:, 1] < NI], axis=1)))
+    assert gap(b) * 20 < gap(a)
