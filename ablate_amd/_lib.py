@@ -120,6 +120,7 @@ SIGNATURES = {
     "abl_fv_set_profiling": (c_int32, [c_void_p, c_int32]),
     "abl_fv_set_strategy": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_strategy": (c_int32, [c_void_p, _ip]),
+    "abl_fv_set_tile_size": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_phase_times": (c_int32, [c_void_p, _dp, POINTER(c_int64), c_int32]),
 }
 PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]

@@ -38,8 +38,10 @@ thread_local std::string g_last_error;
 void set_global_error(const std::string& msg) { g_last_error = msg; }
 
 constexpr int kBlock = 256;
-constexpr int kTileMax = 256;  // largest tile (workgroup) of the tile strategy
-constexpr int kTileLdsMax = 64 * 1024;  // LDS budget of one tile workgroup
+constexpr int kTileMax = 512;           // largest tile (workgroup) of the tile strategy
+constexpr int kTileDefault = 256;       // largest tile the default choice takes
+constexpr int kTileLdsMax = 160 * 1024;  // LDS of one workgroup (gfx950: the whole 160 KB of a CU)
+constexpr int kTileLdsDefault = 64 * 1024;  // LDS budget of the default choice (two or more workgroups per CU)
 constexpr int kCuLdsBytes = 160 * 1024;  // LDS of one gfx950 compute unit
 
 // ============================================================================================ kernels
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys p
 // cell's faces from LDS in face-number order.  No face-flux round trip through HBM; the face fluxes and the
 // gather order are those of strategy 2, so the residual is bitwise the same.
 template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM>
-__global__ __launch_bounds__(kTileMax, 2) void k_tile_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+__global__ __launch_bounds__(kTileMax) void k_tile_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                         const double* __restrict__ gradC,
                                                         const double* __restrict__ gradA,
                                                         const double* __restrict__ rec,
@@ -2733,18 +2735,28 @@ struct abl_fv_solver {
     }
     // Tiles for strategy 3 (FP64 form): the largest tile (256, 128 or 64 cells) whose face fluxes fit the LDS
     // budget.  Needs the host cell -> face table, so it is built at creation.
-    void build_tiles() {
-        const int KM = abl::slot_width(host.K);
+    // only_T = 0: the largest of 256/128/64 cells within the default LDS budget; otherwise exactly only_T cells
+    // within the workgroup LDS limit.  Returns false when no size fits (the strategy then stays unavailable).
+    bool build_tiles(const abl::Layout& lay, int only_T = 0) {
+        const int KM = abl::slot_width(lay.K);
         // LDS per face: FP64 euler + species fluxes, or the mixed form's two FP32 perturbation records
-        const int per_face = form == ABL_FV_FORM_FP64 ? (host.dim + 2 + host.ns) * (int)sizeof(double)
-                                                      : 2 * (host.dim + 2 + host.ns) * (int)sizeof(float);
+        const int per_face = form == ABL_FV_FORM_FP64 ? (lay.dim + 2 + lay.ns) * (int)sizeof(double)
+                                                      : 2 * (lay.dim + 2 + lay.ns) * (int)sizeof(float);
         abl::Tiles tl;
-        for (int T = abl::kTileMax; T >= 64; T /= 2) {
-            abl::build_tiles(host, KM, T, tl);
-            if ((size_t)tl.nf_max * per_face <= (size_t)abl::kTileLdsMax) break;
+        const size_t budget = only_T ? (size_t)abl::kTileLdsMax : (size_t)abl::kTileLdsDefault;
+        for (int T = only_T ? only_T : abl::kTileDefault; T >= 64; T /= 2) {
+            abl::build_tiles(lay, KM, T, tl);
+            if ((size_t)tl.nf_max * per_face <= budget) break;
             tl.n = 0;
+            if (only_T) break;
         }
-        if (tl.n == 0) return;  // no tile size fits: the strategy stays unavailable
+        if (tl.n == 0) return false;
+        if (form == ABL_FV_FORM_FP64 && tile_kernel)
+            HIP_CHECK(hipFuncSetAttribute((const void*)tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)((size_t)tl.nf_max * per_face)));
+        if (form != ABL_FV_FORM_FP64 && tile_kernel_mixed)
+            HIP_CHECK(hipFuncSetAttribute((const void*)tile_kernel_mixed, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)((size_t)tl.nf_max * per_face)));
         tile_start.upload(tl.start);
         tile_faces.upload(tl.faces);
         tile_slot.upload(tl.slot);
@@ -2753,6 +2765,7 @@ struct abl_fv_solver {
         tile_nfm = tl.nf_max;
         tile_lds = (size_t)tl.nf_max * per_face;
         tile_redundancy = (double)tl.faces.size() / (double)std::max<int64_t>(host.F, 1);
+        return true;
     }
     // Default strategy.  The tile kernel evaluates faces on tile borders twice and holds its face fluxes in LDS;
     // the face-flux strategy evaluates every face once but writes and re-reads the fluxes through HBM.  Tiles win
@@ -2991,7 +3004,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->cfl_partial.alloc((size_t)std::max<int64_t>(grid_for(L.NI), 1));
             s->cfl_out.alloc(1);
         }
-        s->build_tiles();
+        s->build_tiles(L);
         s->strategy = s->auto_strategy();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
@@ -3188,6 +3201,25 @@ extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
         return fail(h, std::string("abl_fv_set_strategy: ") + e.what());
     }
     h->strategy = strategy;
+    return 0;
+}
+
+extern "C" int abl_fv_set_tile_size(abl_fv_handle h, int32_t cells) {
+    if (!h) return fail(h, "abl_fv_set_tile_size: null handle");
+    if (cells != 0 && cells != 64 && cells != 128 && cells != 256 && cells != 512)
+        return fail(h, "abl_fv_set_tile_size: cells must be 0 (default), 64, 128, 256 or 512");
+    try {
+        // the host copies of the cell -> face table and the face cells were released at creation
+        abl::Layout lay = h->host;
+        lay.ell.resize(h->ell.n);
+        lay.fcell.resize(h->fcell.n);
+        HIP_CHECK(hipMemcpy(lay.ell.data(), h->ell.ptr, h->ell.bytes(), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(lay.fcell.data(), h->fcell.ptr, h->fcell.bytes(), hipMemcpyDeviceToHost));
+        if (!h->build_tiles(lay, cells))
+            return fail(h, "abl_fv_set_tile_size: the tile's face fluxes exceed the workgroup LDS");
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_set_tile_size: ") + e.what());
+    }
     return 0;
 }
 

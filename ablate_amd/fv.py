@@ -327,6 +327,10 @@ class FiniteVolumeSolver:
         gathered per cell) or "recompute" (each face flux evaluated by both cells)."""
         _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]), self._h, "abl_fv_set_strategy")
 
+    def set_tile_size(self, cells: int = 0) -> None:
+        """Cells per tile of the tile strategy (0 = default choice; 64, 128, 256 or 512)."""
+        _lib.check(_lib.lib().abl_fv_set_tile_size(self._h, int(cells)), self._h, "abl_fv_set_tile_size")
+
     def strategy(self) -> str:
         """The strategy compute_rhs runs ("auto" resolved)."""
         v = ctypes.c_int32()

@@ -259,6 +259,10 @@ enum {
  * footprint leaves two or more waves per SIMD, FACE_FLUX otherwise.  Selecting FACE_FLUX allocates its face-flux
  * storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
 int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);
+/* Cells per tile of the tile strategy: 0 = the default choice (the largest of 256/128/64 whose face fluxes fit
+ * 64 KB of LDS), or exactly 64, 128, 256 or 512 (up to the 160 KB a gfx950 workgroup may hold).  Fails when the
+ * tile's face fluxes do not fit.  The residual does not depend on it (bitwise). */
+int abl_fv_set_tile_size(abl_fv_handle h, int32_t cells);
 /* The strategy abl_fv_rhs runs (AUTO resolved). */
 int abl_fv_get_strategy(abl_fv_handle h, int32_t* strategy);
 

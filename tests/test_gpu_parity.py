@@ -130,6 +130,29 @@ def test_auto_strategy_choice(form, torch_cuda):
     assert s.strategy() == "tile"
 
 
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_tile_size_does_not_change_the_residual(form, torch_cuda):
+    """Tiles of 64 to 512 cells (512: more than 64 KB of LDS per workgroup) give bitwise the face-flux residual."""
+    from ablate_amd.mesh import BoxMesh
+
+    torch = torch_cuda
+    case = Case("tiles_box", BoxMesh([24, 24, 24], [0, 0, 0], [1, 1, 1], [True] * 3), recon="leastsquares",
+                limiter="barthjespersen", mu=1.8e-5, k=0.0251125)
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    s = case.solver(form=form)
+    ref = torch.zeros_like(U)
+    s.set_strategy("face_flux")
+    s.compute_rhs(0.0, U, ref)
+    s.set_strategy("tile")
+    for cells in (64, 128, 512, 0):
+        print(f"[parity] tile size {cells} {form}", flush=True)
+        s.set_tile_size(cells)
+        R = torch.zeros_like(U)
+        s.compute_rhs(0.0, U, R)
+        torch.cuda.synchronize()
+        assert torch.equal(R, ref), cells
+
+
 def test_pack_unpack_roundtrip(torch_cuda):
     torch = torch_cuda
     case = CASES[0]
