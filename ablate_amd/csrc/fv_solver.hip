@@ -1554,6 +1554,7 @@ struct AdjBufs {
     double* contrib;        // [KM][nc][NI]
     const int32_t* mirror;  // [KM][NI] slot of the same face in the neighbour's list, -1 when none / ghost
     int km;                 // slot width of contrib / mirror
+    int lds_species;        // k_adj_faces keeps the species accumulators in dynamic LDS (7 ns doubles per lane)
 };
 
 template <int DIM, bool RUS, int KM>
@@ -1585,13 +1586,22 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
     for (int d = 0; d < DIM; d++) velb[d] = 0.0;
 #pragma unroll
     for (int q = 0; q <= DIM; q++) auxb[q] = 0.0;
-    for (int sp = 0; sp < ns; sp++) {
-        A[(int64_t)(NE + sp) * NI + c] = 0.0;
-        for (int d = 0; d < DIM; d++) {
-            A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = 0.0;
-            A[(oY + (int64_t)sp * DIM + d) * NI + c] = 0.0;
-        }
-    }
+    // Species adjoint accumulators of this cell, field q: densityYi (q < ns), its limited-gradient adjoint
+    // (ns + sp * DIM + d) and the Yi aux-gradient adjoint (ns + ns * DIM + sp * DIM + d).  In LDS (one column per
+    // lane) when the launch provides 7 ns doubles per lane, else in place in the A record: the face loop adds to
+    // them once per face and species, which through HBM would be 2 (1 + 2 DIM) ns read-modify-writes per face.
+    extern __shared__ double sacc[];
+    const bool in_lds = b.lds_species != 0;
+    const int nq = ns * (1 + 2 * DIM);
+    auto spa = [&](int q) -> double& {
+        if (in_lds) return sacc[(int64_t)q * blockDim.x + threadIdx.x];
+        int64_t row;
+        if (q < ns) row = NE + q;
+        else if (q < ns + ns * DIM) row = oG + (int64_t)NE * DIM + (q - ns);
+        else row = oY + (q - ns - ns * DIM);
+        return A[row * NI + c];
+    };
+    for (int q = 0; q < nq; q++) spa(q) = 0.0;
     double uc[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
@@ -1752,9 +1762,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
             const int64_t comp = NE + sp;
             const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
             const double yb = cS * gsp;
-            A[comp * NI + c] += yb;
+            spa(sp) += yb;
             if (recon)
-                for (int d = 0; d < DIM; d++) A[(oG + comp * DIM + d) * NI + c] += yb * x[s][d];
+                for (int d = 0; d < DIM; d++) spa(ns + sp * DIM + d) += yb * x[s][d];
             double ygb = ghost ? cR * gsp : 0.0, rgb = 0.0;
             if (sdiff) {
                 const double rl = uL[kRho], rr = uR[kRho];
@@ -1770,7 +1780,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
                 const double cdf = -ph.diff * (0.5 * rl + 0.5 * rr) * 0.5 * area;
                 const double drho = gsp * (-ph.diff * 0.25 * area) * dot;
                 rhob += drho;  // rho_L and rho_R enter alike; this cell's side
-                for (int d = 0; d < DIM; d++) A[(oY + (int64_t)sp * DIM + d) * NI + c] += gsp * cdf * nh[d];
+                for (int d = 0; d < DIM; d++) spa(ns + ns * DIM + sp * DIM + d) += gsp * cdf * nh[d];
                 if (ghost) {
                     // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (this cell) and Y_G
                     double t = 0.0;
@@ -1778,7 +1788,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
                     for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
                     double yb_c = 0.0, rb_c = 0.0;
                     massfrac_vjp(rl, yl, t, yb_c, rb_c);
-                    A[comp * NI + c] += yb_c;
+                    spa(sp) += yb_c;
                     rhob += rb_c;
                     massfrac_vjp(rr, yr, -t, ygb, rgb);
                     rgb += drho;  // rho_G in the face density
@@ -1801,6 +1811,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
         aux_vjp<DIM>(uc, ph, ab, ub);
     }
     ub[kRho] += rhob;
+    if (in_lds)
+        for (int sp = 0; sp < ns; sp++) {
+            A[(int64_t)(NE + sp) * NI + c] = spa(sp);
+            for (int d = 0; d < DIM; d++) {
+                A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = spa(ns + sp * DIM + d);
+                A[(oY + (int64_t)sp * DIM + d) * NI + c] = spa(ns + ns * DIM + sp * DIM + d);
+            }
+        }
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         A[(int64_t)p * NI + c] = ub[p];
@@ -3295,10 +3313,16 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
             h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
                 hipLaunchKernelGGL(h->species_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
             });
-        abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM};
+        const size_t sp_lds = (size_t)L.ns * (1 + 2 * L.dim) * sizeof(double) * abl::kBlock;
+        const bool use_lds = L.ns > 0 && sp_lds <= (size_t)abl::kTileLdsMax;
+        if (use_lds)
+            HIP_CHECK(hipFuncSetAttribute((const void*)h->adj_faces, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)sp_lds));
+        abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM,
+                       use_lds ? 1 : 0};
         h->timed(ABL_FV_PHASE_VJP, st, [&] {
-            hipLaunchKernelGGL(h->adj_faces, grid, block, 0, st, v, h->ph, U, Rbar, h->rec.ptr, h->gradC.ptr,
-                               h->gradA.ptr, b);
+            hipLaunchKernelGGL(h->adj_faces, grid, block, use_lds ? sp_lds : 0, st, v, h->ph, U, Rbar, h->rec.ptr,
+                               h->gradC.ptr, h->gradA.ptr, b);
             hipLaunchKernelGGL(h->adj_cells, grid, block, 0, st, v, h->ph, U, h->ghost_slot.ptr, h->slot_kind.ptr, b);
             hipLaunchKernelGGL(h->adj_gather, grid, block, 0, st, v, h->ph, h->ghost_slot.ptr, h->slot_kind.ptr, b,
                                Ubar);
