@@ -1889,12 +1889,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
     }
     // ghost contributions are added to the ghost's adjoint (only this cell touches its ghosts)
     auto to_ghost = [&](int64_t g, int comp, double v) { b.Ughost[(int64_t)comp * NG + (g - NI)] += v; };
-    for (int sp = 0; sp < ns; sp++)
-#pragma unroll
-        for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + NE + sp) * NI + c] = 0.0;
 
     if (recon) {
-        for (int p = 0; p < NE; p++) {
+#pragma unroll
+        for (int p = 0; p < NE; p++) {  // unrolled: ce, ubc and uc stay in registers
             double G[DIM], Gb[DIM], gbl[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
@@ -1950,12 +1948,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
                 const int kd = mx ? lim.kpos : lim.kneg;
                 const double ub_ext = phib / den;
                 ubc[p] -= ub_ext;
-                if (ext < 0)
-                    ubc[p] += ub_ext;
-                else if (bnd[ext])
-                    to_ghost(nb[ext], p, ub_ext);
-                else
-                    ce[ext][p] += ub_ext;
+                if (ext < 0) ubc[p] += ub_ext;
+#pragma unroll
+                for (int k = 0; k < KM; k++)  // select, not a runtime register index (which would spill ce)
+                    if (k == ext) {
+                        if (bnd[k])
+                            to_ghost(nb[k], p, ub_ext);
+                        else
+                            ce[k][p] += ub_ext;
+                    }
                 const double den_b = -phi * phib / den;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
@@ -2018,6 +2019,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
         const int64_t comp = NE + sp;
         const double ycs = U[comp * N + c];
         double usb = A[comp * NI + c];
+        double cs[KM];  // this species' contributions to the neighbours, stored once per slot
+#pragma unroll
+        for (int k = 0; k < KM; k++) cs[k] = 0.0;
         if (recon) {
             double G[DIM], Gb[DIM];
 #pragma unroll
@@ -2071,12 +2075,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
                 const int kd = mx ? lim.kpos : lim.kneg;
                 const double ub_ext = phib / den;
                 usb -= ub_ext;
-                if (ext < 0)
-                    usb += ub_ext;
-                else if (bnd[ext])
-                    to_ghost(nb[ext], (int)comp, ub_ext);
-                else
-                    b.contrib[((int64_t)ext * nc + comp) * NI + c] += ub_ext;
+                if (ext < 0) usb += ub_ext;
+#pragma unroll
+                for (int k = 0; k < KM; k++)
+                    if (k == ext) {
+                        if (bnd[k])
+                            to_ghost(nb[k], (int)comp, ub_ext);
+                        else
+                            cs[k] += ub_ext;
+                    }
                 const double den_b = -lim.phi * phib / den;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
@@ -2087,7 +2094,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
                 double t = 0.0;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) t += m.sw[(k * DIM + d) * NI + c] * Gb[d];
-                b.contrib[((int64_t)k * nc + comp) * NI + c] += t;
+                cs[k] += t;
                 usb -= t;
             }
         }
@@ -2108,7 +2115,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
                     to_ghost(nb[k], (int)comp, yb_n);
                     to_ghost(nb[k], kRho, rb_n);
                 } else {
-                    b.contrib[((int64_t)k * nc + comp) * NI + c] += yb_n;
+                    cs[k] += yb_n;
                     ce[k][kRho] += rb_n;
                 }
             }
@@ -2117,6 +2124,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
             ubc[kRho] += rb_c;
         }
         b.Uself[comp * NI + c] = usb;
+#pragma unroll
+        for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + comp) * NI + c] = cs[k];
     }
 #pragma unroll
     for (int p = 0; p < NE; p++) b.Uself[(int64_t)p * NI + c] = ubc[p];
