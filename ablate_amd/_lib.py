@@ -56,6 +56,7 @@ class PhysicsDesc(ctypes.Structure):
         ("diff", c_double),
         ("cfl", c_double),
         ("ausmpup_minf", c_double),
+        ("compact_species", c_int32),
     ]
 
 

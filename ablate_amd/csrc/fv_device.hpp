@@ -16,6 +16,7 @@ struct Phys {
     double gamma, R, cv, icv, mu, k, diff, max_lim_grad;
     double minf;       // AusmpUp reference Mach number (0: fa = 1)
     double pgs_alpha;  // pressure-gradient-scaling factor seen by AusmpUp (1 without PGS)
+    int compact;       // CompactCompressibleNSSpeciesTransport: NA directions average the species
     int flux, recon, limiter, ns;
     int visc;   // Navier-Stokes diffusion flux active (k or mu nonzero, or any transport active)
     int sdiff;  // species diffusion active
@@ -252,11 +253,11 @@ __device__ inline void decode_side(const T* u, const T* nhat, const Phys& ph, Si
 }
 
 // NavierStokesTransport::AdvectionFlux (navierStokesTransport.cpp:99-207) for euler components, plus the mass
-// flux per area and upwind side needed by SpeciesTransport::AdvectionFlux (speciesTransport.cpp:325-337: any
-// non-LEFT direction takes the right state).  flux[] receives dim+2 values.
+// flux per area and the calculator's direction (kLeft / kRight / kNA) for the species fluxes.  flux[] receives
+// dim+2 values.
 template <typename T, int DIM, bool AUP = true>
 __device__ inline void advection_flux(int type, const T* uL, const T* uR, const T* nhat, T area, const Phys& ph, T* flux,
-                                      T& mdotA, int& species_right, T& rhoL_out, T& rhoR_out) {
+                                      T& mdotA, int& direction, T& rhoL_out, T& rhoR_out) {
     SideState<T, DIM> L, R;
     decode_side<T, DIM>(uL, nhat, ph, L);
     decode_side<T, DIM>(uR, nhat, ph, R);
@@ -289,9 +290,27 @@ __device__ inline void advection_flux(int type, const T* uL, const T* uR, const 
         for (int d = 0; d < DIM; d++) flux[kRhoU + d] = T(0.5) * (L.vel[d] + R.vel[d]) * mA + p12A * nhat[d];
     }
     mdotA = mA;
-    species_right = dir != kLeft;
+    direction = dir;
     rhoL_out = L.rho;
     rhoR_out = R.rho;
+}
+
+// Species advection coefficients F_sp = cl rhoY_L + cr rhoY_R from the mass flux mA (area included):
+// SpeciesTransport::AdvectionFlux takes the left state for LEFT and the right one otherwise
+// (speciesTransport.cpp:325-337); CompactCompressibleNSSpeciesTransport::AdvectionFlux averages rhoY and rho for
+// NA (compactCompressibleNSSpeciesTransport.cpp:254-265).
+template <typename T>
+__device__ inline void species_coefficients(int dir, bool compact, T mA, T rl, T rr, T& cl, T& cr) {
+    cl = T(0);
+    cr = T(0);
+    if (dir == kLeft) {
+        cl = mA / rl;
+    } else if (dir == kRight || !compact) {
+        cr = mA / rr;
+    } else {
+        cl = mA * T(0.5) / (T(0.5) * (rl + rr));
+        cr = cl;
+    }
 }
 
 // Rusanov full-vector flux for the euler components (build-defined, absent from the reference):

@@ -588,12 +588,9 @@ __device__ inline void face_flux(const Phys& ph, const double* nhat, double area
         cr = area * (0.5 * unR - 0.5 * lam);
     } else if (ph.flux != ABL_FV_FLUX_NONE) {
         double mA, rl, rr;
-        int right;
-        advection_flux<double, DIM>(ph.flux, L.uf, R.uf, nhat, area, ph, flux, mA, right, rl, rr);
-        if (right)
-            cr = mA / rr;
-        else
-            cl = mA / rl;
+        int dir;
+        advection_flux<double, DIM>(ph.flux, L.uf, R.uf, nhat, area, ph, flux, mA, dir, rl, rr);
+        species_coefficients<double>(dir, ph.compact != 0, mA, rl, rr, cl, cr);
     }
     if (VISC) {
         double vel[DIM], tq[ViscTerms<DIM>::NT], normal[DIM];
@@ -1117,12 +1114,9 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
             cro = ad * (0.5 * unR - 0.5 * lam);
         } else {
             double mA, rlo, rro;
-            int right;
-            advection_flux<double, DIM, false>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, right, rlo, rro);
-            if (right)
-                cro = mA / rro;
-            else
-                clo = mA / rlo;
+            int dir;
+            advection_flux<double, DIM, false>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, dir, rlo, rro);
+            species_coefficients<double>(dir, ph.compact != 0, mA, rlo, rro, clo, cro);
         }
         double und = 0.0;
 #pragma unroll
@@ -2420,7 +2414,7 @@ __device__ inline double block_min(double v) {
 }
 __global__ __launch_bounds__(kBlock) void k_cfl_partial(const double* __restrict__ U, const double* __restrict__ rmin,
                                                         int64_t N, int64_t NI, int dim, Phys ph, double cfl,
-                                                        double* __restrict__ partial) {
+                                                        double alpha, double* __restrict__ partial) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dt = 1e10;  // utilities::Constants::large, the reference's starting minimum
     if (c < NI) {
@@ -2435,7 +2429,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl_partial(const double* __restrict
         const double T = (U[kRhoE * N + c] / rho - ke) / ph.cv;
         const double p = ph.R * rho * T;
         const double a = sqrt(ph.gamma * p / rho);
-        dt = cfl * (2.0 * rmin[c]) / (a / 1.0 + velSum);
+        dt = cfl * (2.0 * rmin[c]) / (a / alpha + velSum);
     }
     const double m = block_min(dt);
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
@@ -2866,6 +2860,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         ph.max_lim_grad = p.max_lim_grad;
         ph.flux = p.flux_calculator;
         ph.minf = p.ausmpup_minf;
+        ph.compact = p.compact_species != 0;
         ph.pgs_alpha = 1.0;
         ph.recon = s->recon;
         ph.limiter = p.limiter;
@@ -3436,8 +3431,11 @@ namespace {
 double cfl_dt_impl(abl_fv_handle h, const double* U, hipStream_t st) {
     const abl::Layout& L = h->host;
     const unsigned nb = grid_for(L.NI);
+    // NavierStokesTransport's CFL uses pgsAlpha = 1; the compact process divides a by the PGS alpha
+    // (compactCompressibleNSSpeciesTransport.cpp:305-339)
+    const double alpha = h->ph.compact ? h->ph.pgs_alpha : 1.0;
     hipLaunchKernelGGL(abl::k_cfl_partial, dim3(nb), dim3(abl::kBlock), 0, st, U, h->rmin.ptr, L.N, L.NI, L.dim, h->ph,
-                       h->phys.cfl, h->cfl_partial.ptr);
+                       h->phys.cfl, alpha, h->cfl_partial.ptr);
     hipLaunchKernelGGL(abl::k_min_final, dim3(1), dim3(abl::kBlock), 0, st, h->cfl_partial.ptr, (int64_t)nb,
                        h->cfl_out.ptr);
     HIP_CHECK(hipGetLastError());

@@ -139,6 +139,20 @@ class PressureGradientScaling:
 
 
 @dataclass
+class CompactCompressibleNSSpeciesTransport:
+    """processes::CompactCompressibleNSSpeciesTransport (compactCompressibleNSSpeciesTransport.cpp): euler and
+    densityYi in one process.  Same fluxes as NavierStokesTransport + SpeciesTransport except that a flux
+    calculator's NA direction advects the averaged species and the CFL step divides the sound speed by the `pgs`
+    alpha; the species-enthalpy diffusion term is zero for PerfectGas (species sensible enthalpies 0)."""
+
+    parameters: dict = field(default_factory=dict)
+    eos: PerfectGas = field(default_factory=PerfectGas)
+    fluxCalculator: FluxCalculator | None = None
+    transport: Constant | None = None
+    pgs: object = None
+
+
+@dataclass
 class SpeciesTransport:
     eos: PerfectGas = field(default_factory=PerfectGas)
     fluxCalculator: FluxCalculator | None = None
@@ -176,11 +190,15 @@ class FiniteVolumeSolver:
     def __init__(self, mesh: _NativeMesh, processes, options: dict | None = None, boundaryConditions=(),
                  maxLimGrad: float = 1e10, form: str = "fp64"):
         options = options or {}
-        ns_proc = [p for p in processes if isinstance(p, NavierStokesTransport)]
+        ns_proc = [p for p in processes if isinstance(p, (NavierStokesTransport, CompactCompressibleNSSpeciesTransport))]
         sp_proc = [p for p in processes if isinstance(p, SpeciesTransport)]
         if len(ns_proc) != 1:
-            raise ValueError("exactly one NavierStokesTransport process is required")
+            raise ValueError("exactly one NavierStokesTransport or CompactCompressibleNSSpeciesTransport process is "
+                             "required")
         nst = ns_proc[0]
+        compact = isinstance(nst, CompactCompressibleNSSpeciesTransport)
+        if compact and sp_proc:
+            raise ValueError("CompactCompressibleNSSpeciesTransport transports the species itself")
         eos = nst.eos
         n_species = len(eos.species)
         flux = nst.fluxCalculator.kind if nst.fluxCalculator else "none"
@@ -191,7 +209,7 @@ class FiniteVolumeSolver:
                 raise ValueError("SpeciesTransport and NavierStokesTransport must use the same flux calculator")
             if (spt.transport or Constant()) != (transport or Constant()):
                 raise ValueError("SpeciesTransport and NavierStokesTransport must use the same transport model")
-        elif n_species:
+        elif n_species and not compact:
             raise ValueError("species are declared in the EOS but no SpeciesTransport process was given")
         tr = transport or Constant()
         recon = options.get("petscfv_type", "upwind")
@@ -209,6 +227,7 @@ class FiniteVolumeSolver:
             diff=float(tr.diff),
             cfl=float(nst.parameters.get("cfl", 0.5)),
             ausmpup_minf=float(getattr(nst.fluxCalculator, "mInf", 0.0)),
+            compact_species=int(compact),
         )
         pgs_proc = [p for p in processes if isinstance(p, PressureGradientScaling)]
         flux_pgs = getattr(nst.fluxCalculator, "pgs", None)
@@ -217,6 +236,10 @@ class FiniteVolumeSolver:
         if pgs_proc and flux == "ausmpup" and flux_pgs is not pgs_proc[0]:
             raise ValueError("AusmpUp must reference the PressureGradientScaling process of the solver (pgs: *pgs)")
         pgs = pgs_proc[0] if pgs_proc else flux_pgs
+        if compact and nst.pgs is not None:
+            if pgs is not None and pgs is not nst.pgs:
+                raise ValueError("the compact process and AusmpUp must share one PressureGradientScaling")
+            pgs = nst.pgs
         self.mesh = mesh
         self.phys = phys
         self.form = form

@@ -113,6 +113,10 @@ typedef struct abl_fv_physics_desc {
     double diff;             /* species diffusivity */
     double cfl;              /* advection CFL (default 0.5) */
     double ausmpup_minf;     /* AusmpUp{mInf} reference Mach number (ausmpUp.cpp:3; 0 = none) */
+    int32_t compact_species; /* 1: CompactCompressibleNSSpeciesTransport (compactCompressibleNSSpeciesTransport.cpp)
+                                instead of NavierStokesTransport + SpeciesTransport: NA flux-calculator directions
+                                advect the averaged species, the CFL step divides a by the PGS alpha; with PerfectGas
+                                (species enthalpies 0) the species-enthalpy diffusion term is zero */
 } abl_fv_physics_desc;
 
 /* One boundary-condition slot. */
@@ -227,7 +231,8 @@ int abl_fv_set_pgs_alpha(abl_fv_handle h, double alpha);
 
 /* ---- time step (the reference's CFL step size and forward-Euler stage, on the device) ------------------------- */
 
-/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285, pgsAlpha = 1): the minimum over
+/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285, pgsAlpha = 1; with compact_species
+ * CompactCompressibleNSSpeciesTransport::ComputeCflTimeStep, a divided by the attached PGS alpha): the minimum over
  * interior cells of cfl * 2 r_min / (a + sum_d |rhoU_d / rho|), r_min the MIN_CELL_RADIUS mesh characteristic
  * (finiteVolumeSolver.cpp:185-215), cfl from the physics descriptor.  *dt is a host value (synchronises the
  * stream).  Cells with a NaN step are skipped. */

@@ -53,6 +53,7 @@ class OrcPhysics(ctypes.Structure):
         ("diff", c_double),
         ("ausmpup_minf", c_double),
         ("pgs_alpha", c_double),
+        ("compact", c_int),
     ]
 
 
@@ -150,10 +151,10 @@ def _mesh_struct(m):
 
 
 def physics(gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", recon="upwind", limiter="none", max_lim_grad=1e10,
-            k=0.0, mu=0.0, diff=0.0, ausmpup_minf=0.0, pgs_alpha=1.0) -> OrcPhysics:
+            k=0.0, mu=0.0, diff=0.0, ausmpup_minf=0.0, pgs_alpha=1.0, compact=False) -> OrcPhysics:
     return OrcPhysics(gamma=gamma, Rgas=Rgas, n_species=n_species, flux=FLUX[flux], recon=RECON[recon],
                       limiter=LIMITER[limiter], max_lim_grad=max_lim_grad, k=k, mu=mu, diff=diff,
-                      ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha)
+                      ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha, compact=int(bool(compact)))
 
 
 def rhs(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, bcs=None, time: float = 0.0):

@@ -8,6 +8,7 @@
  * Restated reference files (paths relative to the ABLATE tree):
  *   src/finiteVolume/fluxCalculator/ausm.cpp:2-43, averageFlux.cpp:2-10, offFlux.cpp:2-8, ausmpUp.cpp:5-97
  *   src/finiteVolume/processes/pressureGradientScaling.cpp:16-137 (UpdatePreconditioner)
+ *   src/finiteVolume/processes/compactCompressibleNSSpeciesTransport.cpp:162-270 (species advection), 273-345 (CFL)
  *   src/eos/perfectGas.cpp:49-197
  *   src/finiteVolume/processes/navierStokesTransport.cpp:99-207 (AdvectionFlux), 454-532 (DiffusionFlux, stress)
  *   src/finiteVolume/processes/speciesTransport.cpp:98-110 (aux Yi), 196-228 (species diffusion), 263-341
@@ -280,7 +281,7 @@ int orc_ns_advection_flux(int dim, const double* normal, const double* fieldL, c
  * densityYi. */
 static void species_advection_flux(int dim, const double* normal, const double* fieldL, const double* yiL,
                                    const double* fieldR, const double* yiR, double gamma, double R, int flux_calculator,
-                                   const flux_ctx* fc, int ns, double* flux) {
+                                   const flux_ctx* fc, int ns, int compact, double* flux) {
     double norm[3];
     norm_vector(dim, normal, norm);
     const double areaMag = mag_vector(dim, normal);
@@ -289,11 +290,17 @@ static void species_advection_flux(int dim, const double* normal, const double* 
     decode_side(dim, fieldL, norm, gamma, R, &L);
     decode_side(dim, fieldR, norm, gamma, R, &Rs);
     double massFlux;
-    if (call_flux_calculator(flux_calculator, fc, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux, NULL) ==
-        ORC_LEFT) {
+    const int dir = call_flux_calculator(flux_calculator, fc, L.un, L.a, L.rho, L.p, Rs.un, Rs.a, Rs.rho, Rs.p, &massFlux,
+                                         NULL);
+    if (dir == ORC_LEFT) {
         for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiL[sp] / L.rho) * areaMag;
-    } else {
+    } else if (dir == ORC_RIGHT || !compact) {
         for (int sp = 0; sp < ns; sp++) flux[sp] = (massFlux * yiR[sp] / Rs.rho) * areaMag;
+    } else {
+        /* the compact process averages species and density for NA (compactCompressibleNSSpeciesTransport.cpp:
+         * 264-265) */
+        for (int sp = 0; sp < ns; sp++)
+            flux[sp] = massFlux * 0.5 * (yiR[sp] + yiL[sp]) / (0.5 * (L.rho + Rs.rho)) * areaMag;
     }
 }
 
@@ -938,7 +945,7 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
             if (ns) {
                 for (int k = 0; k < nc; k++) flux[k] = 0.0;
                 species_advection_flux(dim, normal, uL, uL + dim + 2, uR, uR + dim + 2, ph->gamma, ph->Rgas, ph->flux, &fc, ns,
-                                       flux);
+                                       ph->compact, flux);
                 for (int d = 0; d < ns; ++d) {
                     if (fLp) fLp[dim + 2 + d] -= flux[d] / vL;
                     if (fRp) fRp[dim + 2 + d] += flux[d] / vR;
@@ -1050,7 +1057,8 @@ double orc_cfl_dt(const orc_mesh* m, const orc_physics* ph, double cfl, const do
         const double dx = 2.0 * r[c];
         double velSum = 0.0;
         for (int d = 0; d < dim; d++) velSum += fabs(euler[2 + d]) / rho;
-        const double dt = cfl * dx / (a / 1.0 + velSum);
+        const double alpha = ph->compact ? (ph->pgs_alpha > 0 ? ph->pgs_alpha : 1.0) : 1.0;
+        const double dt = cfl * dx / (a / alpha + velSum);
         dtMin = dt < dtMin ? dt : dtMin;
     }
     return dtMin;

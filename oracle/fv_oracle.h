@@ -51,6 +51,7 @@ typedef struct {
     double k, mu, diff;
     double ausmpup_minf; /* AusmpUp mInf (0 = none) */
     double pgs_alpha;    /* PressureGradientScaling alpha seen by AusmpUp (<= 0: no PGS, alpha 1) */
+    int compact;         /* CompactCompressibleNSSpeciesTransport instead of NavierStokes + SpeciesTransport */
 } orc_physics;
 
 typedef struct {
@@ -113,7 +114,9 @@ int orc_cell_gradients(const orc_mesh* mesh, const orc_physics* phys, const doub
  * characteristic, finiteVolumeSolver.cpp:185-215; every face with two supporting cells counts).  r[n_cells]. */
 void orc_min_cell_radius(const orc_mesh* mesh, double* r);
 
-/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285) with pgsAlpha = 1:
+/* NavierStokesTransport::ComputeCflTimeStep (navierStokesTransport.cpp:209-285) with pgsAlpha = 1, or with
+ * phys->compact CompactCompressibleNSSpeciesTransport::ComputeCflTimeStep (compactCompressibleNSSpeciesTransport.cpp:
+ * 273-345, a divided by the PGS alpha):
  * min over interior cells of cfl * 2 r_min / (a + sum_d |rhoU_d / rho|), a from the PerfectGas T route.
  * locX is AoS [n_cells][dim+2+ns]. */
 double orc_cfl_dt(const orc_mesh* mesh, const orc_physics* phys, double cfl, const double* r, const double* locX);

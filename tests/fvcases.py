@@ -30,17 +30,20 @@ class Case:
     phase: tuple = (0.0, 0.0, 0.0)
     minf: float = 0.0  # AusmpUp mInf
     pgs_alpha: float | None = None  # AusmpUp with a PressureGradientScaling held at this alpha
+    compact: bool = False  # CompactCompressibleNSSpeciesTransport instead of NavierStokes + SpeciesTransport
 
     def processes(self):
         species = [f"S{i}" for i in range(self.n_species)]
         eos = fv.PerfectGas({"gamma": self.gamma, "Rgas": self.Rgas}, species)
+        pgs = fv.PressureGradientScaling(eos, alphaInit=self.pgs_alpha, domainLength=1.0) if self.pgs_alpha else None
         if self.flux == "ausmpup":
-            pgs = fv.PressureGradientScaling(eos, alphaInit=self.pgs_alpha, domainLength=1.0) if self.pgs_alpha else None
             flux = fv.AusmpUp(self.minf, pgs)
         else:
             fc = {"ausm": fv.Ausm, "average": fv.AverageFlux, "off": fv.OffFlux, "rusanov": fv.Rusanov}.get(self.flux)
             flux = fc() if fc else None
         tr = fv.Constant(k=self.k, mu=self.mu, diff=self.diff) if (self.k or self.mu or self.diff) else None
+        if self.compact:
+            return [fv.CompactCompressibleNSSpeciesTransport({"cfl": 0.5}, eos, flux, tr, pgs)]
         procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, flux, tr)]
         if self.n_species:
             procs.append(fv.SpeciesTransport(eos, flux, tr))
@@ -76,7 +79,7 @@ class Case:
     def oracle_physics(self):
         return orc.physics(self.gamma, self.Rgas, self.n_species, self.flux, self.recon, self.limiter,
                            self.max_lim_grad, self.k, self.mu, self.diff, ausmpup_minf=self.minf,
-                           pgs_alpha=self.pgs_alpha or 1.0)
+                           pgs_alpha=self.pgs_alpha or 1.0, compact=self.compact)
 
     def oracle_bcs(self):
         kinds = {"inflow": 1, "outflow": 2}
@@ -151,4 +154,10 @@ def standard_cases():
                       recon="leastsquares", limiter="minmod", mu=0.5, k=5.0, diff=0.2, bcs=inflow_x(2)))
     cases.append(Case("ausmpup_lowmach_2d", BoxMesh([12, 10], [0, 0], [1, 1], [True, True]), flux="ausmpup",
                       minf=0.05, pgs_alpha=10.0, mach=0.02))
+    # CompactCompressibleNSSpeciesTransport: AverageFlux (NA direction: averaged species) and AUSM+up with PGS
+    cases.append(Case("compact_average_species", chan(6), flux="average", compact=True, n_species=3,
+                      recon="leastsquares", limiter="barthjespersen", mu=0.5, k=5.0, diff=0.3, bcs=inflow_x(3)))
+    cases.append(Case("compact_ausmpup_pgs_tets", TetBoxMesh(3, 0.2), flux="ausmpup", minf=0.2, pgs_alpha=2.0,
+                      compact=True, n_species=2, recon="leastsquares", limiter="barthjespersen", mu=0.05, k=0.5,
+                      diff=2e-2, bcs=all_outflow(6)))
     return cases

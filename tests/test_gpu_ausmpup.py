@@ -109,3 +109,19 @@ def test_ausmpup_adjoint_is_refused(torch_cuda):
     Ub = torch.empty_like(U)
     with pytest.raises(RuntimeError):
         s.rhs_vjp(0.0, U, Rb, Ub)
+
+
+def test_compact_process_cfl_uses_pgs_alpha(torch_cuda):
+    """CompactCompressibleNSSpeciesTransport::ComputeCflTimeStep divides a by the PGS alpha (after an update)."""
+    torch = torch_cuda
+    case = Case("compact_cfl", BoxMesh([10, 10, 10], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup", minf=0.1,
+                compact=True, n_species=2, pgs_alpha=4.0)
+    U0 = case.initial_state()
+    s = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    dt = s.compute_cfl_dt(U)
+    r = orc.min_cell_radius(case.mesh.arrays)[: case.mesh.n_interior]
+    want = orc.cfl_dt(case.mesh.arrays, case.oracle_physics(), 0.5, r, U0)
+    assert dt == pytest.approx(want, rel=1e-14)
+    plain = orc.cfl_dt(case.mesh.arrays, orc.physics(n_species=2), 0.5, r, U0)
+    assert dt > plain  # a / 4 < a
