@@ -114,6 +114,9 @@ SIGNATURES = {
     "abl_fv_ts_euler": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double, c_int64, _dp,
                                   POINTER(c_int64), c_void_p]),
     "abl_fv_mesh_gmsh": (c_int32, [c_char_p, c_int32, POINTER(c_void_p)]),
+    "abl_fv_mesh_dmplex": (c_int32, [c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                     POINTER(c_void_p)]),
+    "abl_fv_mesh_source_cells": (c_int32, [c_void_p, c_void_p]),
     "abl_fv_set_pgs": (c_int32, [c_void_p, POINTER(PgsDesc)]),
     "abl_fv_pgs_update": (c_int32, [c_void_p, c_void_p, c_double, _dp, _dp, c_void_p]),
     "abl_fv_get_pgs_alpha": (c_int32, [c_void_p, _dp]),
@@ -161,4 +164,7 @@ class NativeError(RuntimeError):
 def check(status: int, handle=None, what: str = "") -> None:
     if status != 0:
         msg = lib().abl_fv_last_error(handle)
-        raise NativeError(f"{what}: {msg.decode() if msg else 'unknown error'} (status {status})")
+        text = msg.decode() if msg else "unknown error"
+        if not text.startswith(what + ":"):  # the library's messages mostly name their entry point already
+            text = f"{what}: {text}"
+        raise NativeError(f"{text} (status {status})")

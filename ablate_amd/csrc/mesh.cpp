@@ -224,6 +224,7 @@ extern "C" int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* 
         std::vector<int64_t> order = abl::box_order(dim, n, ordering);
         std::vector<int32_t> inv(NI);
         for (int64_t k = 0; k < NI; k++) inv[order[k]] = (int32_t)k;
+        m.source = order;  // lexicographic index (x fastest) of each cell
         int64_t nghost = 0;
         for (int d = 0; d < dim; d++)
             if (!per[d]) nghost += 2 * NI / n[d];
@@ -440,6 +441,7 @@ extern "C" int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int3
         auto* mesh = new abl_fv_mesh();
         HostMesh& m = mesh->m;
         m.dim = 3;
+        m.source = order;  // index of each tetrahedron in generation order
         const int64_t NF = (int64_t)fn.size();
         int64_t nghost = 0;
         for (auto& x : fn) nghost += x.R < 0;
@@ -545,6 +547,20 @@ extern "C" int abl_fv_mesh_describe(abl_fv_mesh_handle mesh, abl_fv_mesh_desc* d
     desc->cell_centroid = m.cell_centroid.data();
     desc->cell_volume = m.cell_volume.data();
     desc->face_bc = m.face_bc.empty() ? nullptr : m.face_bc.data();
+    return 0;
+}
+
+extern "C" int abl_fv_mesh_source_cells(abl_fv_mesh_handle mesh, int64_t* source) {
+    if (!mesh || !source) {
+        abl::set_global_error("abl_fv_mesh_source_cells: null argument");
+        return 1;
+    }
+    const HostMesh& m = mesh->m;
+    if ((int64_t)m.source.size() != m.n_interior) {
+        abl::set_global_error("abl_fv_mesh_source_cells: the mesh records no source numbering");
+        return 1;
+    }
+    std::copy(m.source.begin(), m.source.end(), source);
     return 0;
 }
 

@@ -11,7 +11,8 @@
 //   - geometry is exact for straight-sided cells with planar faces: polygon area / centroid, polyhedron volume and
 //     centroid from a tetrahedral fan about the vertex mean, face normals from Newell's formula (area vector);
 //     normals point from the left cell (support[0]) to the right one.
-// Faces are numbered by their left cell (the lower cell number); cell order natural (file order) or RCM.
+// Faces are numbered by their left cell (the lower cell number); cell order natural (file order), RCM or Morton
+// (numbering and face geometry: assemble_polytope_mesh).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -101,51 +102,6 @@ std::string section(const std::string& text, const std::string& name, bool requi
     size_t b = text.find(close, a);
     if (b == std::string::npos) throw std::runtime_error("abl_fv_mesh_gmsh: unterminated $" + name + " section");
     return text.substr(a, b - a);
-}
-
-// Polygon (2-D) area and centroid: shoelace formulas about the first vertex.
-void polygon_geometry(const std::vector<const double*>& v, double& area, double* c) {
-    double A = 0.0, cx = 0.0, cy = 0.0;
-    const double x0 = v[0][0], y0 = v[0][1];
-    for (size_t i = 1; i + 1 < v.size(); i++) {
-        const double ax = v[i][0] - x0, ay = v[i][1] - y0, bx = v[i + 1][0] - x0, by = v[i + 1][1] - y0;
-        const double t = 0.5 * (ax * by - ay * bx);  // signed triangle (v0, vi, vi+1)
-        A += t;
-        cx += t * (ax + bx) / 3.0;
-        cy += t * (ay + by) / 3.0;
-    }
-    area = std::fabs(A);
-    c[0] = x0 + cx / A;
-    c[1] = y0 + cy / A;
-    c[2] = 0.0;
-}
-
-// Planar polygon face in 3-D: Newell area vector and area-weighted centroid of the fan about vertex 0.
-void face_geometry3(const std::vector<const double*>& v, double* n, double* c) {
-    n[0] = n[1] = n[2] = 0.0;
-    double A = 0.0, cs[3] = {0, 0, 0};
-    for (size_t i = 0; i < v.size(); i++) {
-        const double* a = v[i];
-        const double* b = v[(i + 1) % v.size()];
-        n[0] += 0.5 * (a[1] * b[2] - a[2] * b[1]);
-        n[1] += 0.5 * (a[2] * b[0] - a[0] * b[2]);
-        n[2] += 0.5 * (a[0] * b[1] - a[1] * b[0]);
-    }
-    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    for (size_t i = 1; i + 1 < v.size(); i++) {
-        double u[3], w[3], x[3];
-        for (int d = 0; d < 3; d++) {
-            u[d] = v[i][d] - v[0][d];
-            w[d] = v[i + 1][d] - v[0][d];
-        }
-        x[0] = u[1] * w[2] - u[2] * w[1];
-        x[1] = u[2] * w[0] - u[0] * w[2];
-        x[2] = u[0] * w[1] - u[1] * w[0];
-        const double t = 0.5 * (x[0] * n[0] + x[1] * n[1] + x[2] * n[2]) / nn;  // signed triangle area
-        A += t;
-        for (int d = 0; d < 3; d++) cs[d] += t * (v[0][d] + v[i][d] + v[i + 1][d]) / 3.0;
-    }
-    for (int d = 0; d < 3; d++) c[d] = cs[d] / A;
 }
 
 }  // namespace
@@ -300,54 +256,17 @@ extern "C" int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_
                 r += 1;
             }
         }
-        // cell order
-        std::vector<int64_t> order(NI);
-        std::iota(order.begin(), order.end(), 0);
-        if (ordering == ABL_FV_ORDER_MORTON) {
-            std::vector<double> xc(3 * NI, 0.0);  // vertex means as sort keys
-            for (int64_t c = 0; c < NI; c++) {
-                const Elem& e = elems[cell_elem[c]];
-                for (auto n : e.nodes)
-                    for (int d = 0; d < 3; d++) xc[3 * c + d] += X[3 * n + d] / (double)e.nodes.size();
-            }
-            order = morton_order(xc, NI);
-        } else if (ordering == ABL_FV_ORDER_RCM) {
-            std::vector<int64_t> off(NI + 1, 0);
-            for (auto& p : fp)
-                if (p.b >= 0) {
-                    off[p.a + 1]++;
-                    off[p.b + 1]++;
-                }
-            for (int64_t c = 0; c < NI; c++) off[c + 1] += off[c];
-            std::vector<int32_t> adj(off[NI]);
-            std::vector<int64_t> fill(off.begin(), off.end() - 1);
-            for (auto& p : fp)
-                if (p.b >= 0) {
-                    adj[fill[p.a]++] = p.b;
-                    adj[fill[p.b]++] = p.a;
-                }
-            order = rcm_order(NI, off, adj);
-        }
-        std::vector<int32_t> inv(NI);
-        for (int64_t k = 0; k < NI; k++) inv[order[k]] = (int32_t)k;
-
-        auto* mesh = new abl_fv_mesh();
-        HostMesh& m = mesh->m;
-        m.dim = dim;
-        // cell geometry (new numbering)
-        int64_t nghost = 0;
-        for (auto& p : fp) nghost += p.b < 0;
-        m.n_interior = NI;
-        m.n_cells = NI + nghost;
-        m.cell_centroid.assign(3 * m.n_cells, 0.0);
-        m.cell_volume.assign(m.n_cells, 0.0);
-        for (int64_t k = 0; k < NI; k++) {
-            const Elem& e = elems[cell_elem[order[k]]];
-            double* xc = &m.cell_centroid[3 * k];
+        // cell geometry (file order)
+        std::vector<double> vol(NI), cent(3 * NI, 0.0);
+        std::vector<int64_t> source(NI);
+        for (int64_t c = 0; c < NI; c++) {
+            const Elem& e = elems[cell_elem[c]];
+            double* xc = &cent[3 * c];
+            source[c] = c;
             if (dim == 2) {
                 std::vector<const double*> v;
                 for (auto n : e.nodes) v.push_back(&X[3 * n]);
-                polygon_geometry(v, m.cell_volume[k], xc);
+                polygon_geometry(v, vol[c], xc);
             } else {
                 // tetrahedral fan about the vertex mean p over the triangulated faces (outward by construction
                 // of the signed volumes' sum)
@@ -360,95 +279,45 @@ extern "C" int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_
                     const double* v[4];
                     for (int q = 0; q < 4 && lf[q] >= 0; q++) v[k2++] = &X[3 * e.nodes[lf[q]]];
                     for (int i = 1; i + 1 < k2; i++) {
-                        double a[3], b[3], c[3];
+                        double a[3], b[3], cc[3];
                         for (int d = 0; d < 3; d++) {
                             a[d] = v[0][d] - p[d];
                             b[d] = v[i][d] - p[d];
-                            c[d] = v[i + 1][d] - p[d];
+                            cc[d] = v[i + 1][d] - p[d];
                         }
-                        const double t = (a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
-                                          a[2] * (b[0] * c[1] - b[1] * c[0])) / 6.0;
+                        const double t = (a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) +
+                                          a[2] * (b[0] * cc[1] - b[1] * cc[0])) / 6.0;
                         V += t;
                         for (int d = 0; d < 3; d++) cs[d] += t * (p[d] + v[0][d] + v[i][d] + v[i + 1][d]) / 4.0;
                     }
                 }
-                m.cell_volume[k] = std::fabs(V);
+                vol[c] = std::fabs(V);
                 for (int d = 0; d < 3; d++) xc[d] = cs[d] / V;
             }
-            if (!(m.cell_volume[k] > 0.0)) {
-                delete mesh;
-                throw std::runtime_error("abl_fv_mesh_gmsh: degenerate cell");
-            }
+            if (!(vol[c] > 0.0)) throw std::runtime_error("abl_fv_mesh_gmsh: degenerate cell");
         }
-        // faces numbered by (left, right) in the new numbering, boundary faces after the interior ones of a cell
-        struct FaceNew {
-            int32_t L, R;
-            int64_t idx;
-        };
-        std::vector<FaceNew> fn(fp.size());
-        for (size_t i = 0; i < fp.size(); i++) {
-            const int32_t a = inv[fp[i].a];
-            if (fp[i].b >= 0) {
-                const int32_t b = inv[fp[i].b];
-                fn[i] = {std::min(a, b), std::max(a, b), (int64_t)i};
-            } else {
-                fn[i] = {a, -1, (int64_t)i};
-            }
-        }
-        std::sort(fn.begin(), fn.end(), [](const FaceNew& x, const FaceNew& y) {
-            if (x.L != y.L) return x.L < y.L;
-            return (uint32_t)x.R < (uint32_t)y.R;
-        });
-        const int64_t NF = (int64_t)fn.size();
-        m.n_faces = NF;
-        m.face_cells.resize(2 * NF);
-        m.face_normal.assign(3 * NF, 0.0);
-        m.face_centroid.assign(3 * NF, 0.0);
-        m.face_bc.assign(NF, -1);
-        m.has_bc = nghost > 0;
-        int64_t g = NI;
-        for (int64_t fi = 0; fi < NF; fi++) {
-            const FacePair& p = fp[fn[fi].idx];
+        // faces: the node loop of the left cell's local face; boundary faces take their physical tag
+        PolyFaces pf;
+        for (const auto& p : fp) {
             const Elem& e = elems[cell_elem[p.a]];
             const auto& lf = cell_faces(e.type)[p.la];
-            std::vector<const double*> v;
-            for (int q = 0; q < 4 && lf[q] >= 0; q++) v.push_back(&X[3 * e.nodes[lf[q]]]);
-            double nrm[3] = {0, 0, 0}, xf[3] = {0, 0, 0};
-            if (dim == 2) {
-                for (int d = 0; d < 2; d++) xf[d] = 0.5 * (v[0][d] + v[1][d]);
-                nrm[0] = v[1][1] - v[0][1];
-                nrm[1] = -(v[1][0] - v[0][0]);
-            } else {
-                face_geometry3(v, nrm, xf);
-            }
-            const int32_t L = fn[fi].L, R = fn[fi].R;
-            const double* xl = &m.cell_centroid[3 * L];
-            double s = 0.0;
-            for (int d = 0; d < 3; d++) s += nrm[d] * (xf[d] - xl[d]);  // outward from the left cell
-            const double sign = s < 0 ? -1.0 : 1.0;
-            for (int d = 0; d < 3; d++) {
-                m.face_normal[3 * fi + d] = sign * nrm[d];
-                m.face_centroid[3 * fi + d] = xf[d];
-            }
-            m.face_cells[2 * fi] = L;
-            if (R >= 0) {
-                m.face_cells[2 * fi + 1] = R;
-            } else {
+            int64_t nodes[4];
+            int k = 0;
+            for (int q = 0; q < 4 && lf[q] >= 0; q++) nodes[k++] = e.nodes[lf[q]];
+            int32_t slot = -1;
+            if (p.b < 0) {
                 auto it = bnd_phys.find(p.key);
-                if (it == bnd_phys.end()) {
-                    delete mesh;
-                    throw std::runtime_error("abl_fv_mesh_gmsh: a boundary face has no physical group");
-                }
-                if (it->second >= kMaxBcSlots) {
-                    delete mesh;
-                    throw std::runtime_error("abl_fv_mesh_gmsh: boundary physical tags must be below " +
-                                             std::to_string(kMaxBcSlots));
-                }
-                m.face_cells[2 * fi + 1] = (int32_t)g;
-                m.face_bc[fi] = it->second;
-                finish_ghost(m, fi, L, (int32_t)g);
-                g++;
+                if (it == bnd_phys.end()) throw std::runtime_error("abl_fv_mesh_gmsh: a boundary face has no physical group");
+                slot = it->second;
             }
+            pf.add(p.a, p.b, slot, nodes, k);
+        }
+        auto* mesh = new abl_fv_mesh();
+        try {
+            assemble_polytope_mesh(dim, X, vol, cent, pf, ordering, source, "abl_fv_mesh_gmsh", mesh->m);
+        } catch (...) {
+            delete mesh;
+            throw;
         }
         *out = mesh;
         return 0;

@@ -153,3 +153,85 @@ class GmshMesh(_NativeMesh):
         super().__init__(h)
         bdim = self.arrays.dim - 1
         self.labels = {n: t for n, (d, t) in gmsh_physical_names(path).items() if d == bdim}
+
+
+
+class DMPlexMesh(_NativeMesh):
+    """Mesh from a DMPlex topology (abl_fv_mesh_dmplex): ``cones`` the cone of every point (a list of point lists, or
+    ``(cone_offsets, cone_points)``), ``vertex_coords`` [nv][dim] the depth-0 points' coordinates in point order,
+    ``face_sets`` {value: face points} of the "Face Sets" label."""
+
+    def __init__(self, dim: int, cones, vertex_coords, face_sets=None, ordering: str = "natural"):
+        if isinstance(cones, tuple):
+            off, pts = (np.ascontiguousarray(x, dtype=np.int64) for x in cones)
+        else:
+            off = np.concatenate([[0], np.cumsum([len(c) for c in cones])]).astype(np.int64)
+            pts = np.array([q for c in cones for q in c], dtype=np.int64)
+        n = off.size - 1
+        X = np.ascontiguousarray(vertex_coords, dtype=np.float64)
+        fs = None
+        if face_sets:
+            fs = np.full(n, -1, dtype=np.int32)
+            for v, idx in face_sets.items():
+                fs[np.asarray(idx, dtype=np.int64)] = int(v)
+        h = ctypes.c_void_p()
+        _lib.check(
+            _lib.lib().abl_fv_mesh_dmplex(int(dim), n, off.ctypes.data, pts.ctypes.data, X.ctypes.data,
+                                          None if fs is None else fs.ctypes.data, _lib.ORDER[ordering],
+                                          ctypes.byref(h)),
+            None,
+            "abl_fv_mesh_dmplex",
+        )
+        super().__init__(h)
+        self.labels = {int(v): int(v) for v in (face_sets or {})}
+
+    def source_cells(self) -> np.ndarray:
+        """DMPlex point number of every interior cell, in this mesh's cell order."""
+        out = np.empty(self.n_interior, dtype=np.int64)
+        _lib.check(_lib.lib().abl_fv_mesh_source_cells(self._handle, out.ctypes.data), None, "abl_fv_mesh_source_cells")
+        return out
+
+
+class Hdf5Mesh(DMPlexMesh):
+    """The mesh of an ABLATE Hdf5Serializer output file (src/io/hdf5Serializer.cpp:23-160, PETSc's
+    DMPlexView_HDF5): /topology/order (global point number of each stored point), /topology/cones (cone sizes),
+    /topology/cells (the cones, concatenated, in global point numbers), /geometry/vertices (rows in vertex point
+    order) and /labels/<label>/<value>/indices.  ``fields(name)`` returns a saved field of the last time step as
+    [n_comp, n_interior] in this mesh's cell order."""
+
+    def __init__(self, path, ordering: str = "natural", label: str = "Face Sets"):
+        from .hdf5 import H5File
+
+        h = H5File(path)
+        order = h.read("/topology/order").ravel().astype(np.int64)
+        csize = h.read("/topology/cones").ravel().astype(np.int64)
+        flat = h.read("/topology/cells").ravel().astype(np.int64)
+        n = order.size
+        if np.any(np.sort(order) != np.arange(n)):
+            raise ValueError(f"{path}: /topology/order is not a permutation of the points")
+        # re-index the stored cones by global point number
+        start = np.concatenate([[0], np.cumsum(csize)])
+        size_g = np.empty(n, dtype=np.int64)
+        size_g[order] = csize
+        off = np.concatenate([[0], np.cumsum(size_g)]).astype(np.int64)
+        pts = np.empty(off[-1], dtype=np.int64)
+        for g, p in zip(order, range(n)):
+            pts[off[g] : off[g + 1]] = flat[start[p] : start[p + 1]]
+        X = h.read("/geometry/vertices")
+        dim = X.shape[1]
+        face_sets = {}
+        if h.has(f"/labels/{label}"):
+            for v in h.keys(f"/labels/{label}"):
+                face_sets[int(v)] = h.read(f"/labels/{label}/{v}/indices").ravel().astype(np.int64)
+        self.h5, self.path = h, str(path)
+        super().__init__(dim, (off, pts), X, face_sets, ordering)
+        self._cstart = int(self.source_cells().min())
+
+    def fields(self, name: str = "solution", group: str = "fields") -> np.ndarray:
+        """Field `name` of the last saved time step as [n_comp, n_interior] in this mesh's cell order (the file
+        stores it per cell in point order from the first cell point on)."""
+        data = self.h5.read(f"/{group}/{name}")
+        last = data[-1] if data.ndim > 1 else data
+        last = last.reshape(last.shape[0], -1)
+        rows = self.source_cells() - self._cstart
+        return np.ascontiguousarray(last[rows].T)

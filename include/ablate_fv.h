@@ -165,6 +165,24 @@ int abl_fv_mesh_tet_box(int32_t n, double jitter, uint64_t seed, int32_t orderin
  * physical tag of the element on them as face_bc (the "Face Sets" label value; tags must be below 16) and one
  * ghost cell each.  ordering: ABL_FV_ORDER_NATURAL (file order), ABL_FV_ORDER_RCM or ABL_FV_ORDER_MORTON. */
 int abl_fv_mesh_gmsh(const char* path, int32_t ordering, abl_fv_mesh_handle* out);
+/* Mesh from a DMPlex topology (an interpolated plex, points 0..n_points-1), as an ABLATE domain holds it after
+ * DMPlexInterpolate (DMPlexGetCone / DMGetCoordinatesLocal / DMGetLabel("Face Sets"), src/domain/domain.cpp) and as
+ * DMPlexView_HDF5 stores it in the Hdf5Serializer output (src/io/hdf5Serializer.cpp:23-160):
+ *   cone_offsets[n_points+1], cones   the cone of point p is cones[cone_offsets[p] .. cone_offsets[p+1]);
+ *   vertex_coords [nv][dim]           coordinates of the depth-0 points in increasing point number;
+ *   face_sets [n_points]              "Face Sets" value of each point (-1: none; NULL when no face is on the boundary).
+ * Cells are the points of depth dim (ghost cells of DMPlexConstructGhostCells, with a one-face cone, are dropped and
+ * rebuilt); boundary faces take their "Face Sets" value as face_bc (below 16) and one ghost cell each; geometry as
+ * DMPlexComputeGeometryFVM gives it for straight-sided convex cells with planar faces (src/solver/cellSolver.cpp:
+ * 193-196).  ordering: natural (point order), RCM or Morton. */
+int abl_fv_mesh_dmplex(int32_t dim, int64_t n_points, const int64_t* cone_offsets, const int64_t* cones,
+                       const double* vertex_coords, const int32_t* face_sets, int32_t ordering,
+                       abl_fv_mesh_handle* out);
+/* Producer's index of every interior cell (n_interior entries, in the mesh's cell order): the DMPlex point number
+ * (abl_fv_mesh_dmplex), the cell element's ordinal in the file (abl_fv_mesh_gmsh), the lexicographic index with x
+ * fastest (abl_fv_mesh_box) or the generation index (abl_fv_mesh_tet_box).  Maps fields stored in the producer's
+ * cell order onto the mesh. */
+int abl_fv_mesh_source_cells(abl_fv_mesh_handle mesh, int64_t* source);
 /* Descriptor view of a generated mesh (pointers stay valid until abl_fv_mesh_free). */
 int abl_fv_mesh_describe(abl_fv_mesh_handle mesh, abl_fv_mesh_desc* desc);
 void abl_fv_mesh_free(abl_fv_mesh_handle mesh);

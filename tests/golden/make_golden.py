@@ -12,6 +12,12 @@ Run in a container that has the ABLATE source tree (default /root/reference); th
                           time step) with the reference's x, rho, u, e profiles
   pipeFlow.msh            tests/integrationTests/inputs/compressibleFlow/gmshPipeFlow/pipeFlow.msh (data, copied)
   mixedCells2D.msh        tests/integrationTests/inputs/compressibleFlow/mixedCellTypeTest/mixedCells2D.msh (copied)
+  2D_Circle.hdf5, 2D_Circle_Tri.hdf5
+                          tests/integrationTests/inputs/levelSet/ (Hdf5Serializer output of an 80 x 80 box on
+                          [-2, 2]^2, quadrilaterals / triangles, with the volume fraction of the unit circle; copied)
+  initializer.2D.hdf5, initializer.3D.hdf5
+                          tests/unitTests/inputs/domain/ (copied)
+  hdf5_initializer.json   hdf5InitializerTests.cpp: the points and expected field values of its test table
 Each record keeps the source file and line of its table so that a reader can find it.
 """
 from __future__ import annotations
@@ -197,6 +203,26 @@ def shock_tube():
     return out
 
 
+def hdf5_initializer_points():
+    """hdf5InitializerTests.cpp INSTANTIATE_TEST_SUITE_P table: file -> field -> [(point, expected values)]."""
+    src = UT / "domain" / "hdf5InitializerTests.cpp"
+    text = src.read_text()
+    start = text.index("INSTANTIATE_TEST_SUITE_P")
+    line0 = text[:start].count("\n") + 1
+    out = {"source": f"tests/unitTests/domain/hdf5InitializerTests.cpp:{line0}", "files": {}}
+    chunks = text[start:].split('.hdf5File = "')[1:]
+    for ch in chunks:
+        fname = ch[: ch.index('"')].split("/")[-1]
+        fields = {}
+        parts = re.split(r'\{"(\w+)",', ch)
+        for name, body in zip(parts[1::2], parts[2::2]):
+            pts = re.findall(r"\.point = \{([^}]*)\}, \.expectedValue = \{([^}]*)\}", body)
+            fields[name] = [{"point": [num(x) for x in pnt.split(",")], "expected": [num(x) for x in ev.split(",")]}
+                            for pnt, ev in pts]
+        out["files"][fname] = fields
+    return out
+
+
 def main():
     st = shock_tube()
     (OUT / "shock_tube.json").write_text(json.dumps(st, indent=1))
@@ -210,6 +236,10 @@ def main():
     (OUT / "perfect_gas.json").write_text(json.dumps(pgt, indent=1, allow_nan=True))
     for rel in ("compressibleFlow/gmshPipeFlow/pipeFlow.msh", "compressibleFlow/mixedCellTypeTest/mixedCells2D.msh"):
         shutil.copyfile(REF / "tests" / "integrationTests" / "inputs" / rel, OUT / Path(rel).name)
+    for rel in ("integrationTests/inputs/levelSet/2D_Circle.hdf5", "integrationTests/inputs/levelSet/2D_Circle_Tri.hdf5",
+                "unitTests/inputs/domain/initializer.2D.hdf5", "unitTests/inputs/domain/initializer.3D.hdf5"):
+        shutil.copyfile(REF / "tests" / rel, OUT / Path(rel).name)
+    (OUT / "hdf5_initializer.json").write_text(json.dumps(hdf5_initializer_points(), indent=1))
     print({k: len(v["uL"]) for k, v in fc.items()}, len(adv), len(stress), len(pgt), len(st))
 
 
