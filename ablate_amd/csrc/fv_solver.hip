@@ -180,9 +180,8 @@ __device__ inline void store_records(const uint64_t* wd, uint64_t* __restrict__ 
 }
 
 template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
-                                                      G* __restrict__ gradC, G* __restrict__ gradA,
-                                                      void* __restrict__ rec) {
+__device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                               G* __restrict__ gradC, G* __restrict__ gradA, void* __restrict__ rec) {
     constexpr int NE = DIM + 2;
     constexpr int NA = DIM + 1;
     const int64_t c_lane = xcd_cell_index();
@@ -366,6 +365,21 @@ __global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, co
         for (int p = 0; p < NE; p++) wd[CR::U0 / 2 + p] = (uint64_t)__double_as_longlong(uc[p]);
         store_records<RSW>(wd, reinterpret_cast<uint64_t*>(rec), c_lane, active, NI);
     }
+}
+
+template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                                      G* __restrict__ gradC, G* __restrict__ gradA,
+                                                      void* __restrict__ rec) {
+    gradients_body<G, DIM, RECON, VISC, LIM, KM>(m, ph, U, gradC, gradA, rec);
+}
+// The same kernel held to 128 VGPRs (four waves per SIMD instead of three: more loads in flight).  Picked for the
+// FP64 3-D forms without the PETSc-family limiter, which fit without spilling (that limiter's would spill).
+template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_gradients_w4(
+    MeshView<G> m, Phys ph, const double* __restrict__ U, G* __restrict__ gradC, G* __restrict__ gradA,
+    void* __restrict__ rec) {
+    gradients_body<G, DIM, RECON, VISC, LIM, KM>(m, ph, U, gradC, gradA, rec);
 }
 
 // densityYi gradients (limited, one field) into gS [ns*DIM][N] and Yi aux gradients (unlimited) into gY [ns*DIM][N];
@@ -2228,6 +2242,14 @@ using GatherKernel = void (*)(MeshView<double>, int, const double*, const double
 template <typename G, int DIM, int KM>
 GradKernelT<G> pick_gradients_k(bool recon, bool visc, int lim) {
     if (!recon) return visc ? k_gradients<G, DIM, false, true, kLimNone, KM> : nullptr;
+    if constexpr (sizeof(G) == 8 && DIM == 3) {
+        if (lim != kLimPetsc) {
+            if (visc) return lim == kLimBJ ? k_gradients_w4<G, DIM, true, true, kLimBJ, KM>
+                                           : k_gradients_w4<G, DIM, true, true, kLimNone, KM>;
+            return lim == kLimBJ ? k_gradients_w4<G, DIM, true, false, kLimBJ, KM>
+                                 : k_gradients_w4<G, DIM, true, false, kLimNone, KM>;
+        }
+    }
     if (visc) {
         if (lim == kLimBJ) return k_gradients<G, DIM, true, true, kLimBJ, KM>;
         if (lim == kLimPetsc) return k_gradients<G, DIM, true, true, kLimPetsc, KM>;
