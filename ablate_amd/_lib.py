@@ -126,6 +126,7 @@ SIGNATURES = {
     "abl_fv_get_strategy": (c_int32, [c_void_p, _ip]),
     "abl_fv_set_tile_size": (c_int32, [c_void_p, c_int32]),
     "abl_fv_get_phase_times": (c_int32, [c_void_p, _dp, POINTER(c_int64), c_int32]),
+    "abl_hbm_probe": (c_int32, [c_int32, c_int64, c_int32, _dp, _dp]),
 }
 PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]
 STRATEGY = {"auto": 0, "recompute": 1, "face_flux": 2, "tile": 3}

@@ -17,7 +17,7 @@ LIB_PATH = HERE / "libablate_fv.so"
 ARCH = "gfx950"
 
 HOST_SOURCES = ["mesh.cpp", "mesh_polytope.cpp", "mesh_gmsh.cpp", "mesh_dmplex.cpp", "fv_setup.cpp"]
-HIP_SOURCES = ["fv_solver.hip"]
+HIP_SOURCES = ["fv_solver.hip", "hbm_probe.hip"]
 
 # Packed-FP32 instructions are disabled for all device code (the compiler would otherwise form them from
 # ordinary float arithmetic).

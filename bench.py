@@ -52,8 +52,9 @@ def parse():
 
 def copy_probe(gib, reps=5):
     """Times a plain device-to-device copy of `gib` GiB (torch copy_, one read + one write per byte) on the current
-    stream.  It is the achievable-bandwidth reference for the RHS and, under a hardware-counter profiler, the
-    calibration kernel for FETCH_SIZE / WRITE_SIZE (its algorithmic traffic is exactly `gib` read + `gib` written)."""
+    stream.  Under a hardware-counter profiler it is the calibration kernel for FETCH_SIZE / WRITE_SIZE (its
+    algorithmic traffic is exactly `gib` read + `gib` written); its rate is below what the memory system delivers
+    (see stream_probe)."""
     import torch
 
     n = int(gib * (1 << 30)) // 8
@@ -76,8 +77,24 @@ def copy_probe(gib, reps=5):
             "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-KERNELS = {"face_flux": ["k_gradients", "k_face_flux", "k_gather"], "recompute": ["k_gradients", "k_flux"],
-           "tile": ["k_gradients", "k_tile_flux"]}
+def stream_probe(gib, reps=10):
+    """Achievable HBM bandwidth: the library's streaming copy and streaming read (abl_hbm_probe, 16-byte loads per
+    lane) over `gib` GiB, after warm-up passes."""
+    import ctypes
+
+    from ablate_amd import _lib
+
+    out = {"bytes": int(gib * (1 << 30))}
+    for kind, name in ((0, "copy"), (1, "read")):
+        gbs, ms = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().abl_hbm_probe(kind, out["bytes"], reps, ctypes.byref(gbs), ctypes.byref(ms)), None,
+                   "abl_hbm_probe")
+        out[f"{name}_GB_per_s"] = round(gbs.value, 1)
+    return out
+
+
+KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recompute": ["k_gradients_w4", "k_flux"],
+           "tile": ["k_gradients_w4", "k_tile_flux"]}
 
 # HBM bytes per RHS from hardware counters (FETCH_SIZE + WRITE_SIZE summed over the RHS kernels of one step, each
 # scaled by the copy-kernel calibration factor), measured with the commands recorded in
@@ -359,6 +376,7 @@ def main():
     from ablate_amd.mesh import BoxMesh
 
     copy = copy_probe(args.copy_gib) if args.copy_gib > 0 else None
+    sprobe = stream_probe(4.0) if args.copy_gib > 0 else None
     mu = 1.8e-5
     cp = 1.4 * 287.0 / 0.4
     k = mu * cp / 0.72
@@ -437,6 +455,7 @@ def main():
         "event_ms_per_step": round(event_ms, 4),
         "setup_s": round(setup_s, 2),
         "hbm_copy_probe": copy,
+        "hbm_stream_probe": sprobe,
         "device_bytes": info.device_bytes,
         "finite": finite,
     }

@@ -305,6 +305,13 @@ int abl_fv_set_profiling(abl_fv_handle h, int32_t enable);
 /* Accumulated device milliseconds and launch counts per phase since the last reset (waits for the events). */
 int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* launches, int32_t reset);
 
+/* ---- measurement utility (no counterpart in the reference) ---------------------------------------------- */
+
+/* Achievable-bandwidth reference: times `reps` streaming passes over a `bytes`-sized device buffer (kind 0: copy
+ * into a second buffer, counted as 2 x bytes moved; kind 1: read only) with 16-byte loads per lane, after two
+ * warm-up passes; returns GB/s and the milliseconds per pass.  Allocates and frees its buffers. */
+int abl_hbm_probe(int32_t kind, int64_t bytes, int32_t reps, double* gb_per_s, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ HEADER = ROOT / "include" / "ablate_fv.h"
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(abl_fv_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(abl_(?:fv|hbm)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), f"{n} declared in ablate_fv.h but not exported"
     assert set(names) == set(_lib.SIGNATURES), "binding signature table out of sync with the header"
     out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\b(abl_fv_\w+)$", out, flags=re.M))
+    exported = set(re.findall(r"\b(abl_(?:fv|hbm)_\w+)$", out, flags=re.M))
     assert set(names) <= exported
 
 
