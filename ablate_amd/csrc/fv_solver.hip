@@ -79,18 +79,24 @@ __device__ inline void aux_of(const double* u, const Phys& ph, double* a) {
 enum : int { kLimNone = 0, kLimPetsc = 1, kLimBJ = 2 };
 
 // Cell record of the FP64 form, written by the gradient pass for every interior cell and read by the face
-// fluxes: [0, NE) the state, [G0, G0 + NE*DIM) the limited euler gradients (p * DIM + d), [T0, T0 + NT) the
-// viscous terms (zero where unused), padded to RS doubles (256 bytes in 3-D) so that a neighbour's whole record
-// arrives in whole cache lines instead of one line per structure-of-arrays row.
+// fluxes: words [0, NE) the state, [G0, G0 + NE*DIM) the limited euler gradients (p * DIM + d), [T0, T0 + NT) the
+// viscous terms.  Records are stored in blocks of 64 consecutive cells, word-major inside a block: word j of cell
+// c at rec[rec_base(c) + 64 j].  The 64 lanes of a wave (64 consecutive cells) store each word with one coalesced
+// store, and the faces of a wave, whose cells cluster, read each word from a few cache lines (a per-cell record
+// read by 16-byte loads made every load of the wave touch up to 64 lines).
 template <int DIM>
 struct CellRec {
     static constexpr int NE = DIM + 2;
     static constexpr int G0 = NE;
     static constexpr int T0 = NE + NE * DIM;
     static constexpr int END = T0 + ViscTerms<DIM>::NT;
-    static constexpr int RS = DIM == 3 ? 32 : 18;
-    static_assert(END <= RS && RS % 2 == 0, "record layout");
+    static constexpr int RS = END;  // words per cell
+    static constexpr int BLOCK = 64;
 };
+template <int DIM>
+__device__ inline int64_t rec_base(int64_t c) {
+    return (c >> 6) * (int64_t)(CellRec<DIM>::BLOCK * CellRec<DIM>::RS) + (c & 63);
+}
 
 // Cell record of the mixed form: [0, NE*DIM) FP32 limited gradients, [TF, TF + NT) FP32 viscous terms, then the
 // FP64 state at float offset U0 (16-byte aligned); RS floats per cell (144 bytes in 3-D).
@@ -326,20 +332,17 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
     if (VISC) stress_heat<G, DIM>(G(ph.mu), G(ph.k), ga, tq);
     if (sizeof(G) == 8) {
         using CR = CellRec<DIM>;
-        uint64_t wd[CR::RS];
+        if (active) {
+            double* __restrict__ r = reinterpret_cast<double*>(rec) + rec_base<DIM>(c_lane);
 #pragma unroll
-        for (int i = 0; i < CR::RS; i++) wd[i] = 0;
+            for (int p = 0; p < NE; p++) {
+                r[CR::BLOCK * p] = uc[p];
 #pragma unroll
-        for (int p = 0; p < NE; p++) {
-            wd[p] = (uint64_t)__double_as_longlong(uc[p]);
+                for (int d = 0; d < DIM; d++) r[CR::BLOCK * (CR::G0 + p * DIM + d)] = RECON ? double(g[p][d]) : 0.0;
+            }
 #pragma unroll
-            for (int d = 0; d < DIM; d++)
-                if (RECON) wd[CR::G0 + p * DIM + d] = (uint64_t)__double_as_longlong(double(g[p][d]));
+            for (int q = 0; q < ViscTerms<DIM>::NT; q++) r[CR::BLOCK * (CR::T0 + q)] = VISC ? double(tq[q]) : 0.0;
         }
-#pragma unroll
-        for (int q = 0; q < ViscTerms<DIM>::NT; q++)
-            if (VISC) wd[CR::T0 + q] = (uint64_t)__double_as_longlong(double(tq[q]));
-        store_records<CR::RS>(wd, reinterpret_cast<uint64_t*>(rec), c_lane, active, NI);
     } else {
         // FP32 record: float pairs packed little-endian into 64-bit words, then the FP64 state
         using CR = CellRecF<DIM>;
@@ -532,15 +535,11 @@ __device__ inline void load_side(const double* __restrict__ U, const double* __r
     using CR = CellRec<DIM>;
     constexpr int NE = DIM + 2;
     if ((RECON || VISC) && !ghost) {
-        constexpr int NL = ((VISC ? CR::END : CR::T0) + 1) & ~1;
+        constexpr int NL = VISC ? CR::END : CR::T0;
         double r[NL];
-        const double2* p = reinterpret_cast<const double2*>(rec + c * CR::RS);
+        const double* __restrict__ rb = rec + rec_base<DIM>(c);
 #pragma unroll
-        for (int i = 0; i < NL / 2; i++) {
-            const double2 t = p[i];
-            r[2 * i] = t.x;
-            r[2 * i + 1] = t.y;
-        }
+        for (int i = 0; i < NL; i++) r[i] = (RECON || i < CR::G0 || i >= CR::T0) ? rb[CR::BLOCK * i] : 0.0;
 #pragma unroll
         for (int p2 = 0; p2 < NE; p2++) {
             double a = 0.0;
@@ -1637,18 +1636,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
             double* fu = sd == 0 ? fL : fR;
             double* tq = sd == 0 ? tqL : tqR;
             if (useRec && !(sd == 1 && ghost)) {
-                const double* r = rec + cc * CR::RS;
+                const double* r = rec + rec_base<DIM>(cc);
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
-                    u[p] = r[p];
+                    u[p] = r[CR::BLOCK * p];
                     double a = 0.0;
                     if (recon)
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) a += r[CR::G0 + p * DIM + d] * x[sd][d];
+                        for (int d = 0; d < DIM; d++) a += r[CR::BLOCK * (CR::G0 + p * DIM + d)] * x[sd][d];
                     fu[p] = u[p] + a;
                 }
 #pragma unroll
-                for (int q = 0; q < NT; q++) tq[q] = r[CR::T0 + q];
+                for (int q = 0; q < NT; q++) tq[q] = r[CR::BLOCK * (CR::T0 + q)];
             } else {
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
@@ -2926,7 +2925,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->slim.upload(sg.lim);
             s->fwa.upload(L.fwa);
             if (s->recon || s->visc)
-                s->rec.alloc((size_t)(L.dim == 3 ? abl::CellRec<3>::RS : abl::CellRec<2>::RS) * L.NI);
+                s->rec.alloc((size_t)(L.dim == 3 ? abl::CellRec<3>::RS : abl::CellRec<2>::RS) * ((L.NI + 63) / 64 * 64));
             // species gradients: structure of arrays [ns*DIM][N], limited densityYi (gradC) and Yi aux (gradA)
             if (s->recon && L.ns) {
                 s->gradC.alloc((size_t)L.ns * L.dim * N);
