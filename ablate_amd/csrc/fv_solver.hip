@@ -98,15 +98,21 @@ __device__ inline int64_t rec_base(int64_t c) {
     return (c >> 6) * (int64_t)(CellRec<DIM>::BLOCK * CellRec<DIM>::RS) + (c & 63);
 }
 
-// Cell record of the mixed form: [0, NE*DIM) FP32 limited gradients, [TF, TF + NT) FP32 viscous terms, then the
-// FP64 state at float offset U0 (16-byte aligned); RS floats per cell (144 bytes in 3-D).
+// Cell record of the mixed form, in 64-bit words: [0, U0/2) pairs of FP32 values (the limited gradients at float
+// [0, NE*DIM), the viscous terms at float [TF, TF + NT)), then the NE FP64 state words.  Stored in blocks of 64
+// cells, word-major inside a block like the FP64 record (recf_base).
 template <int DIM>
 struct CellRecF {
     static constexpr int NE = DIM + 2;
     static constexpr int TF = NE * DIM;
-    static constexpr int U0 = ((TF + ViscTerms<DIM>::NT) + 3) & ~3;
-    static constexpr int RS = (U0 + 2 * NE + 3) & ~3;
+    static constexpr int U0 = ((TF + ViscTerms<DIM>::NT) + 1) & ~1;  // float offset of the state (even)
+    static constexpr int RSW = U0 / 2 + NE;                          // 64-bit words per cell
+    static constexpr int RS = 2 * RSW;                                // floats per cell
 };
+template <int DIM>
+__device__ inline int64_t recf_base(int64_t c) {
+    return (c >> 6) * (int64_t)(64 * CellRecF<DIM>::RSW) + (c & 63);
+}
 
 template <int DIM, bool RECON, bool VISC>
 struct RecF {
@@ -119,16 +125,14 @@ template <int DIM, bool RECON, bool VISC>
 __device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF<DIM, RECON, VISC>& r) {
     using CR = CellRecF<DIM>;
     constexpr int NE = DIM + 2;
-    const float* base = rec + c * CR::RS;
+    const uint64_t* __restrict__ b = reinterpret_cast<const uint64_t*>(rec) + recf_base<DIM>(c);
     constexpr int NF = VISC ? CR::TF + ViscTerms<DIM>::NT : (RECON ? CR::TF : 0);
-    float t[NF > 0 ? ((NF + 3) & ~3) : 4];
+    float t[NF > 0 ? ((NF + 1) & ~1) : 2];
 #pragma unroll
-    for (int i = 0; i < (NF + 3) / 4; i++) {
-        const float4 v = reinterpret_cast<const float4*>(base)[i];
-        t[4 * i] = v.x;
-        t[4 * i + 1] = v.y;
-        t[4 * i + 2] = v.z;
-        t[4 * i + 3] = v.w;
+    for (int i = 0; i < (NF + 1) / 2; i++) {
+        const uint64_t w = b[64 * i];
+        t[2 * i] = __uint_as_float((uint32_t)w);
+        t[2 * i + 1] = __uint_as_float((uint32_t)(w >> 32));
     }
     if (RECON)
 #pragma unroll
@@ -136,55 +140,14 @@ __device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF
     if (VISC)
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++) r.tq[q] = t[CR::TF + q];
-    const double* ud = reinterpret_cast<const double*>(base + CR::U0);
 #pragma unroll
-    for (int i = 0; i < NE / 2; i++) {
-        const double2 v = reinterpret_cast<const double2*>(ud)[i];
-        r.u[2 * i] = v.x;
-        r.u[2 * i + 1] = v.y;
-    }
-    if (NE & 1) r.u[NE - 1] = ud[NE - 1];
+    for (int p = 0; p < NE; p++) r.u[p] = __longlong_as_double((long long)b[64 * (CR::U0 / 2 + p)]);
 }
 
 // Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
 // G is the storage/arithmetic type of weights and gradients (double: FP64 form, float: mixed form); state
-// differences are always taken in FP64 before rounding to G.  The FP64 form writes the cell record `rec`; the
-// mixed form writes structure-of-arrays gradC [NE*DIM][N] and gradA [NT][N].
-// Stores each lane's cell record (RSW 64-bit words) at out + c * RSW.  The 64 records of a wave are contiguous, so
-// they are staged through LDS (in two halves) and written as consecutive 8-byte words by consecutive lanes: every
-// store instruction covers whole cache lines instead of 64 scattered 16-byte pieces.  Every thread of the block
-// must call it (it synchronises the block); lanes with active == false store nothing.
-template <int RSW>
-__device__ inline void store_records(const uint64_t* wd, uint64_t* __restrict__ out, int64_t c, bool active,
-                                     int64_t NI) {
-    constexpr int HALF = (RSW + 1) / 2;
-    constexpr int PITCH = HALF + 1;  // odd pitch: the lanes' staging writes spread over the banks
-    __shared__ uint64_t stage[kBlock / 64][64 * PITCH];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t cb = c - lane;  // first cell of the wave
-    (void)active;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        constexpr int H0 = 0;
-        const int lo = h * HALF;
-        const int hw = h == 0 ? HALF : RSW - HALF;
-        if (h == 1) __syncthreads();
-#pragma unroll
-        for (int i = 0; i < HALF; i++)
-            if (i < hw) stage[w][lane * PITCH + i] = wd[lo + i];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < HALF; j++) {
-            if (j < hw) {
-                const int idx = j * 64 + lane;
-                const int r = idx / hw, word = idx - r * hw;
-                if (cb + r < NI) out[(cb + r) * RSW + lo + word] = stage[w][r * PITCH + word];
-            }
-        }
-        (void)H0;
-    }
-}
-
+// differences are always taken in FP64 before rounding to G.  Each lane writes its cell's record: CellRec (FP64
+// form) or CellRecF (mixed form), word by word into the 64-cell blocks.
 template <typename G, int DIM, bool RECON, bool VISC, int LIM, int KM>
 __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const double* __restrict__ U,
                                                G* __restrict__ gradC, G* __restrict__ gradA, void* __restrict__ rec) {
@@ -357,16 +320,14 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++)
             if (VISC) t[CR::TF + q] = float(tq[q]);
-        constexpr int RSW = CR::RS / 2;
-        uint64_t wd[RSW];
+        if (active) {
+            uint64_t* __restrict__ w = reinterpret_cast<uint64_t*>(rec) + recf_base<DIM>(c_lane);
 #pragma unroll
-        for (int i = 0; i < RSW; i++) wd[i] = 0;
+            for (int i = 0; i < CR::U0 / 2; i++)
+                w[64 * i] = (uint64_t)__float_as_uint(t[2 * i]) | ((uint64_t)__float_as_uint(t[2 * i + 1]) << 32);
 #pragma unroll
-        for (int i = 0; i < CR::U0 / 2; i++)
-            wd[i] = (uint64_t)__float_as_uint(t[2 * i]) | ((uint64_t)__float_as_uint(t[2 * i + 1]) << 32);
-#pragma unroll
-        for (int p = 0; p < NE; p++) wd[CR::U0 / 2 + p] = (uint64_t)__double_as_longlong(uc[p]);
-        store_records<RSW>(wd, reinterpret_cast<uint64_t*>(rec), c_lane, active, NI);
+            for (int p = 0; p < NE; p++) w[64 * (CR::U0 / 2 + p)] = (uint64_t)__double_as_longlong(uc[p]);
+        }
     }
 }
 
@@ -803,10 +764,13 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux(MeshView<double> m, Phys
     double* ls = lds + NE * nfm;   // [ns][nfm]
     const int64_t t = xcd_block_index();
     const int32_t f0 = tstart[t], nf = tstart[t + 1] - f0;
+    // the face number of a lane's next face is loaded one round ahead
+    int32_t fnext = (int)threadIdx.x < nf ? tfaces[f0 + threadIdx.x] : 0;
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         double flux[NE];
-        eval_face<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, (int64_t)tfaces[f0 + i], flux, ls + i,
-                                             nfm);
+        const int32_t fcur = fnext;
+        if (i + (int)blockDim.x < nf) fnext = tfaces[f0 + i + blockDim.x];
+        eval_face<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, (int64_t)fcur, flux, ls + i, nfm);
 #pragma unroll
         for (int p = 0; p < NE; p++) le[p * nfm + i] = flux[p];
     }
@@ -2647,7 +2611,7 @@ struct abl_fv_solver {
     abl::TileMixedKernel tile_kernel_mixed = nullptr;
     abl::FaceFluxMixedKernel face_flux_kernel_mixed = nullptr;
     abl::GatherMixedKernel gather_kernel_mixed = nullptr;
-    abl::DevBuf<float> recf;            // cell records [NI][RS] of the mixed form
+    abl::DevBuf<float> recf;            // cell records of the mixed form (recf_base blocks)
     abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients [ns*DIM][N] (densityYi limited, Yi aux)
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
@@ -2947,7 +2911,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->slim_f.upload(sg.lim);
             s->fwa_f.upload(to_f(L.fwa));
             if (s->recon || s->visc)
-                s->recf.alloc((size_t)(L.dim == 3 ? abl::CellRecF<3>::RS : abl::CellRecF<2>::RS) * L.NI);
+                s->recf.alloc((size_t)(L.dim == 3 ? abl::CellRecF<3>::RS : abl::CellRecF<2>::RS) * ((L.NI + 63) / 64 * 64));
             if (s->recon && L.ns) {
                 s->gS_f.alloc((size_t)L.ns * L.dim * N);
                 HIP_CHECK(hipMemset(s->gS_f.ptr, 0, s->gS_f.bytes()));
