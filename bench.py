@@ -97,12 +97,12 @@ KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recomput
            "tile": ["k_gradients_w4", "k_tile_flux"]}
 
 # HBM bytes per RHS from hardware counters (FETCH_SIZE + WRITE_SIZE summed over the RHS kernels of one step, each
-# scaled by the copy-kernel calibration factor), measured with the commands recorded in
+# scaled by the calibration factor the same counters give for the streaming copy of abl_hbm_probe), measured with the commands recorded in
 # profiles/rhs_ns_fp64_256cube_tile.md, rhs_ns_fp64_256cube_face_flux.md and rhs_ns_mixed_256cube_tile.md.  Keyed by
 # (workload, cells per direction, strategy).
-TRAFFIC_BYTES = {("ns_fp64_256cube_lsq_bj_ausm", 256, "tile"): 26359366612,
-                 ("ns_fp64_256cube_lsq_bj_ausm", 256, "face_flux"): 26464258488,
-                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 15458133159}
+TRAFFIC_BYTES = {("ns_fp64_256cube_lsq_bj_ausm", 256, "tile"): 29272677881,
+                 ("ns_fp64_256cube_lsq_bj_ausm", 256, "face_flux"): 27052399285,
+                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 16359357774}
 
 
 def measured_traffic(cfg, n, strategy):
