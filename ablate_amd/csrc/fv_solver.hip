@@ -2140,6 +2140,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<double> m, Phys 
     }
 }
 
+// Per-slot sums of the per-boundary-face theta_bar contributions: block 3 s + j sums component j over the faces of
+// slot s, each lane over a fixed stride of faces, then a fixed-order tree (deterministic).
+__global__ __launch_bounds__(kBlock) void k_theta_reduce(int64_t nb, const int32_t* __restrict__ bslot,
+                                                         const double* __restrict__ contrib, double* __restrict__ out) {
+    const int s = blockIdx.x / 3, j = blockIdx.x % 3;
+    double a = 0.0;
+    for (int64_t i = threadIdx.x; i < nb; i += kBlock)
+        if (bslot[i] == s) a += contrib[3 * i + j];
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // theta_bar contributions of each inflow boundary face from its ghost's adjoint: dU_ghost/dtheta^T Ubar_ghost.
 __global__ __launch_bounds__(kBlock) void k_adj_theta(int64_t nb, int dim, int ns, int64_t N, int64_t NI,
                                                       const int32_t* __restrict__ bface, const int32_t* __restrict__ bslot,
@@ -2616,6 +2634,8 @@ struct abl_fv_solver {
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
+    abl::DevBuf<double> theta_sum;    // [3 kMaxBcSlots] per-slot sums (k_theta_reduce)
+    std::vector<int> theta_slots;     // slots that have boundary faces (read once from bslot)
     std::vector<double> h_slot_param, h_slot_Y;
     abl::VjpKernel vjp_kernel = nullptr;
     // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
@@ -3261,16 +3281,27 @@ extern "C" int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* laun
 }
 
 namespace {
-// theta_bar: fixed-order host reduction of per-boundary-face contributions (deterministic)
+// theta_bar: per-slot sums of the per-boundary-face contributions reduced on the device in a fixed order
+// (deterministic, k_theta_reduce); only the 3 sums of each slot come back, and are added into the caller's array
+// for the slots that have boundary faces.
 void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
-    std::vector<double> contrib((size_t)3 * h->n_bfaces);
-    std::vector<int32_t> slots(h->n_bfaces);
-    HIP_CHECK(hipMemcpyAsync(contrib.data(), h->theta_contrib.ptr, contrib.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (h->theta_slots.empty()) {
+        std::vector<int32_t> slots(h->n_bfaces);
+        HIP_CHECK(hipMemcpy(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        std::vector<char> seen(abl::kMaxBcSlots, 0);
+        for (int32_t v : slots) seen[v] = 1;
+        for (int v = 0; v < abl::kMaxBcSlots; v++)
+            if (seen[v]) h->theta_slots.push_back(v);
+    }
+    if (!h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
+    hipLaunchKernelGGL(abl::k_theta_reduce, dim3(3 * abl::kMaxBcSlots), dim3(abl::kBlock), 0, st, h->n_bfaces,
+                       h->bslot.ptr, h->theta_contrib.ptr, h->theta_sum.ptr);
+    HIP_CHECK(hipGetLastError());
+    double sums[3 * abl::kMaxBcSlots];
+    HIP_CHECK(hipMemcpyAsync(sums, h->theta_sum.ptr, sizeof(sums), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    for (int64_t i = 0; i < h->n_bfaces; i++)
-        for (int j = 0; j < 3; j++) theta_bar[3 * slots[i] + j] += contrib[3 * i + j];
+    for (int v : h->theta_slots)
+        for (int j = 0; j < 3; j++) theta_bar[3 * v + j] += sums[3 * v + j];
 }
 
 int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, hipStream_t st) {
@@ -3378,16 +3409,7 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
                                h->n_bfaces, h->bface.ptr, h->bslot.ptr, h->slot_kind.ptr, h->slot_param.ptr,
                                h->slot_Y.ptr, h->theta_contrib.ptr);
             HIP_CHECK(hipGetLastError());
-            std::vector<double> contrib((size_t)3 * h->n_bfaces);
-            std::vector<int32_t> slots(h->n_bfaces);
-            HIP_CHECK(hipMemcpyAsync(contrib.data(), h->theta_contrib.ptr, contrib.size() * sizeof(double),
-                                     hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                     st));
-            HIP_CHECK(hipStreamSynchronize(st));
-            // fixed-order host reduction per slot (deterministic)
-            for (int64_t i = 0; i < h->n_bfaces; i++)
-                for (int j = 0; j < 3; j++) theta_bar[3 * slots[i] + j] += contrib[3 * i + j];
+            reduce_theta(h, st, theta_bar);
         }
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
