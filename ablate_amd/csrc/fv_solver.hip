@@ -1365,7 +1365,8 @@ __device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, cons
     QL = 0.0;
     QR = 0.0;
     QdL = 0.0;
-    for (int sp = 0; sp < ph.ns; sp++) {
+#pragma unroll 2
+    for (int sp = 0; sp < ph.ns; sp++) {  // two species' loads in flight
         const int64_t comp = NE + sp;
         const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
         const double yl = U[comp * N + L], yr = U[comp * N + Rc];
@@ -1432,16 +1433,28 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
     }
 #pragma unroll
     for (int p = 0; p < NE; p++) Ub[p * N + c] = ub[p];
+    if (ph.ns == 0) return;
+    // species: the faces' cells and inverse volumes are loaded once for all species
+    int64_t sL[KM], sR[KM];
+    double ivL[KM], ivR[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const Slot sl = face_slot(m, k, c);
+        sL[k] = m.fcell[sl.f];
+        sR[k] = m.fcell[F + sl.f];
+        ivL[k] = m.inv_vol[sL[k]];
+        ivR[k] = sR[k] >= NI ? 0.0 : m.inv_vol[sR[k]];
+        if (!sl.valid) coef[k] = 0.0;
+    }
+#pragma unroll 2
     for (int sp = 0; sp < ph.ns; sp++) {
         const int64_t comp = NE + sp;
         double a = 0.0;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
-            const Slot sl = face_slot(m, k, c);
-            const int64_t L = m.fcell[sl.f], Rc = m.fcell[F + sl.f];
-            const bool ghost = Rc >= NI;
-            const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * m.inv_vol[Rc]) - Rb[comp * N + L] * m.inv_vol[L];
-            if (sl.valid) a += coef[k] * g;
+            const bool ghost = sR[k] >= NI;
+            const double g = (ghost ? 0.0 : Rb[comp * N + sR[k]] * ivR[k]) - Rb[comp * N + sL[k]] * ivL[k];
+            a += coef[k] * g;
         }
         Ub[comp * N + c] = a;
     }
