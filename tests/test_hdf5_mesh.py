@@ -204,3 +204,17 @@ def test_plex_errors():
         bad = [list(c) for c in cones]
         bad[8] = [2, 4]  # the diagonal edge no longer joins the triangles' vertices consistently
         DMPlexMesh(2, bad, X, {7: [6], 8: [7], 9: [9], 10: [10]})
+
+
+def test_plex_ghost_cells_are_rebuilt():
+    """Ghost cells of DMPlexConstructGhostCells (points of depth dim whose cone is a single boundary face) are dropped
+    and made again: the mesh equals the one built from the same plex without them."""
+    cones, X = two_triangles()
+    labels = {7: [6], 8: [7], 9: [9], 10: [10]}
+    base = DMPlexMesh(2, cones, X, labels).arrays
+    # append one ghost cell per boundary edge (points 11-14), cones of one face each
+    with_ghosts = cones + [[6], [7], [9], [10]]
+    g = DMPlexMesh(2, with_ghosts, X, labels).arrays
+    assert (g.n_interior, g.n_cells, g.n_faces) == (base.n_interior, base.n_cells, base.n_faces)
+    for name in ("face_cells", "face_normal", "face_centroid", "cell_centroid", "cell_volume", "face_bc"):
+        assert np.array_equal(getattr(g, name), getattr(base, name)), name
