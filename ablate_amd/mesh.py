@@ -214,9 +214,10 @@ class Hdf5Mesh(DMPlexMesh):
         size_g = np.empty(n, dtype=np.int64)
         size_g[order] = csize
         off = np.concatenate([[0], np.cumsum(size_g)]).astype(np.int64)
+        # element i of stored cone p lands at off[order[p]] + i
+        dst = np.repeat(off[order] - start[:-1], csize) + np.arange(flat.size, dtype=np.int64)
         pts = np.empty(off[-1], dtype=np.int64)
-        for g, p in zip(order, range(n)):
-            pts[off[g] : off[g + 1]] = flat[start[p] : start[p + 1]]
+        pts[dst] = flat
         X = h.read("/geometry/vertices")
         dim = X.shape[1]
         face_sets = {}

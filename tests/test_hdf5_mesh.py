@@ -218,3 +218,42 @@ def test_plex_ghost_cells_are_rebuilt():
     assert (g.n_interior, g.n_cells, g.n_faces) == (base.n_interior, base.n_cells, base.n_faces)
     for name in ("face_cells", "face_normal", "face_centroid", "cell_centroid", "cell_volume", "face_bc"):
         assert np.array_equal(getattr(g, name), getattr(base, name)), name
+
+
+def test_stored_point_order_is_undone(monkeypatch):
+    """/topology/order gives the global number of each stored point: a file that stores the points of the
+    two-triangle plex in a shuffled order yields the same mesh as the plex itself."""
+    import ablate_amd.hdf5 as h5mod
+
+    cones, X = two_triangles()
+    perm = np.random.default_rng(3).permutation(len(cones))  # stored position p holds global point perm[p]
+    data = {
+        "/topology/order": perm.astype(np.int32)[:, None],
+        "/topology/cones": np.array([len(cones[g]) for g in perm], dtype=np.int32)[:, None],
+        "/topology/cells": np.array([q for g in perm for q in cones[g]], dtype=np.int32)[:, None],
+        "/geometry/vertices": np.array(X, dtype=np.float64),
+        "/labels/Face Sets/7/indices": np.array([[6]], dtype=np.int32),
+        "/labels/Face Sets/8/indices": np.array([[7]], dtype=np.int32),
+        "/labels/Face Sets/9/indices": np.array([[9]], dtype=np.int32),
+        "/labels/Face Sets/10/indices": np.array([[10]], dtype=np.int32),
+    }
+
+    class FakeFile:
+        def __init__(self, path):
+            self.path = str(path)
+
+        def read(self, path):
+            return data[path]
+
+        def has(self, path):
+            return any(k.startswith(path) for k in data)
+
+        def keys(self, path):
+            pre = path.rstrip("/") + "/"
+            return sorted({k[len(pre):].split("/")[0] for k in data if k.startswith(pre)})
+
+    monkeypatch.setattr(h5mod, "H5File", FakeFile)
+    a = Hdf5Mesh("shuffled.h5").arrays
+    b = DMPlexMesh(2, cones, X, {7: [6], 8: [7], 9: [9], 10: [10]}).arrays
+    for name in ("face_cells", "face_normal", "face_centroid", "cell_centroid", "cell_volume", "face_bc"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
