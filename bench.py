@@ -365,8 +365,8 @@ def adjoint_leg(fv, synthetic, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU (halo-exchange) sharding is designed but "
-                                   "not built, so --gpus must be 1"}))
+        print(json.dumps({"error": "the RHS runs on one GPU; multi-GPU sharding (halo exchange between domain "
+                                   "partitions) is designed but not implemented, so --gpus must be 1"}))
         sys.exit(2)
     cfg = CONFIGS[args.config]
     n = args.n or cfg["n"]

@@ -1,4 +1,4 @@
-"""The reference's 1-D shock-tube cases (compressibleShockTubeTests.cpp:209-314) on this build's meshes.
+"""The reference's 1-D shock-tube cases (compressibleShockTubeTests.cpp:209-314) on generated box meshes.
 
 The reference runs a 1-D DMPlex box of nx cells with ghost cells at both ends, Ghost boundary conditions that
 hold the initial left / right states, PerfectGas (gamma, default Rgas), AUSM, first order, forward Euler with the
