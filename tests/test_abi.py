@@ -110,3 +110,13 @@ def test_null_arguments_return_status():
     assert b"null" in L.abl_fv_last_error(None)
     assert L.abl_fv_get_info(None, None) != 0
     assert L.abl_fv_mesh_box(3, None, None, None, None, 0, None) != 0
+
+
+def test_hbm_probe_rejects_bad_arguments():
+    """The bandwidth probe validates its arguments before touching the device."""
+    lib = _lib.lib()
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    assert lib.abl_hbm_probe(5, 1 << 30, 1, ctypes.byref(g), ctypes.byref(ms)) != 0
+    assert "kind" in lib.abl_fv_last_error(None).decode()
+    assert lib.abl_hbm_probe(0, 1024, 1, ctypes.byref(g), ctypes.byref(ms)) != 0  # below 1 MiB
+    assert lib.abl_hbm_probe(1, 1 << 30, 0, ctypes.byref(g), ctypes.byref(ms)) != 0  # no repetitions
