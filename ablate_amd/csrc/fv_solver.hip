@@ -80,22 +80,29 @@ enum : int { kLimNone = 0, kLimPetsc = 1, kLimBJ = 2 };
 
 // Cell record of the FP64 form, written by the gradient pass for every interior cell and read by the face
 // fluxes: words [0, NE) the state, [G0, G0 + NE*DIM) the limited euler gradients (p * DIM + d), [T0, T0 + NT) the
-// viscous terms.  Records are stored in blocks of 64 consecutive cells, word-major inside a block: word j of cell
-// c at rec[rec_base(c) + 64 j].  The 64 lanes of a wave (64 consecutive cells) store each word with one coalesced
-// store, and the faces of a wave, whose cells cluster, read each word from a few cache lines (a per-cell record
-// read by 16-byte loads made every load of the wave touch up to 64 lines).
+// viscous terms.  Records are stored in blocks of 64 consecutive cells, word-pair-major inside a block: words
+// (2q, 2q + 1) of cell c form the 16-byte pair at double2 index rec_pair(c) + 64 q.  The 64 lanes of a wave (64
+// consecutive cells) store each pair with one coalesced 1 KB store, and the faces of a wave, whose cells cluster,
+// read each pair from a few cache lines with one 16-byte load (a per-cell record made every load of the wave touch
+// up to 64 lines; single words per load took twice the load instructions).
 template <int DIM>
 struct CellRec {
     static constexpr int NE = DIM + 2;
     static constexpr int G0 = NE;
     static constexpr int T0 = NE + NE * DIM;
     static constexpr int END = T0 + ViscTerms<DIM>::NT;
-    static constexpr int RS = END;  // words per cell
+    static constexpr int NP = (END + 1) / 2;  // word pairs per cell
+    static constexpr int RS = 2 * NP;         // words per cell (the last may be padding)
     static constexpr int BLOCK = 64;
 };
 template <int DIM>
-__device__ inline int64_t rec_base(int64_t c) {
-    return (c >> 6) * (int64_t)(CellRec<DIM>::BLOCK * CellRec<DIM>::RS) + (c & 63);
+__device__ inline int64_t rec_pair(int64_t c) {
+    return (c >> 6) * (int64_t)(CellRec<DIM>::BLOCK * CellRec<DIM>::NP) + (c & 63);
+}
+// Word i of cell c's record.
+template <int DIM>
+__device__ inline double rec_word(const double* __restrict__ rec, int64_t c, int i) {
+    return rec[2 * (rec_pair<DIM>(c) + (int64_t)CellRec<DIM>::BLOCK * (i >> 1)) + (i & 1)];
 }
 
 // Cell record of the mixed form, in 64-bit words: [0, U0/2) pairs of FP32 values (the limited gradients at float
@@ -296,15 +303,20 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
     if (sizeof(G) == 8) {
         using CR = CellRec<DIM>;
         if (active) {
-            double* __restrict__ r = reinterpret_cast<double*>(rec) + rec_base<DIM>(c_lane);
+            double w[CR::RS];
 #pragma unroll
             for (int p = 0; p < NE; p++) {
-                r[CR::BLOCK * p] = uc[p];
+                w[p] = uc[p];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) r[CR::BLOCK * (CR::G0 + p * DIM + d)] = RECON ? double(g[p][d]) : 0.0;
+                for (int d = 0; d < DIM; d++) w[CR::G0 + p * DIM + d] = RECON ? double(g[p][d]) : 0.0;
             }
 #pragma unroll
-            for (int q = 0; q < ViscTerms<DIM>::NT; q++) r[CR::BLOCK * (CR::T0 + q)] = VISC ? double(tq[q]) : 0.0;
+            for (int q = 0; q < ViscTerms<DIM>::NT; q++) w[CR::T0 + q] = VISC ? double(tq[q]) : 0.0;
+#pragma unroll
+            for (int i = CR::END; i < CR::RS; i++) w[i] = 0.0;
+            double2* __restrict__ r = reinterpret_cast<double2*>(rec) + rec_pair<DIM>(c_lane);
+#pragma unroll
+            for (int q = 0; q < CR::NP; q++) r[CR::BLOCK * q] = make_double2(w[2 * q], w[2 * q + 1]);
         }
     } else {
         // FP32 record: float pairs packed little-endian into 64-bit words, then the FP64 state
@@ -497,10 +509,16 @@ __device__ inline void load_side(const double* __restrict__ U, const double* __r
     constexpr int NE = DIM + 2;
     if ((RECON || VISC) && !ghost) {
         constexpr int NL = VISC ? CR::END : CR::T0;
-        double r[NL];
-        const double* __restrict__ rb = rec + rec_base<DIM>(c);
+        double r[2 * ((NL + 1) / 2)];
+        const double2* __restrict__ rb = reinterpret_cast<const double2*>(rec) + rec_pair<DIM>(c);
 #pragma unroll
-        for (int i = 0; i < NL; i++) r[i] = (RECON || i < CR::G0 || i >= CR::T0) ? rb[CR::BLOCK * i] : 0.0;
+        for (int q = 0; q < (NL + 1) / 2; q++) {
+            // without reconstruction the pairs holding only gradient words are not read
+            const bool need = RECON || 2 * q < CR::G0 || 2 * q + 1 >= CR::T0;
+            const double2 v = need ? rb[CR::BLOCK * q] : make_double2(0.0, 0.0);
+            r[2 * q] = v.x;
+            r[2 * q + 1] = v.y;
+        }
 #pragma unroll
         for (int p2 = 0; p2 < NE; p2++) {
             double a = 0.0;
@@ -1613,18 +1631,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
             double* fu = sd == 0 ? fL : fR;
             double* tq = sd == 0 ? tqL : tqR;
             if (useRec && !(sd == 1 && ghost)) {
-                const double* r = rec + rec_base<DIM>(cc);
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
-                    u[p] = r[CR::BLOCK * p];
+                    u[p] = rec_word<DIM>(rec, cc, p);
                     double a = 0.0;
                     if (recon)
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) a += r[CR::BLOCK * (CR::G0 + p * DIM + d)] * x[sd][d];
+                        for (int d = 0; d < DIM; d++) a += rec_word<DIM>(rec, cc, CR::G0 + p * DIM + d) * x[sd][d];
                     fu[p] = u[p] + a;
                 }
 #pragma unroll
-                for (int q = 0; q < NT; q++) tq[q] = r[CR::BLOCK * (CR::T0 + q)];
+                for (int q = 0; q < NT; q++) tq[q] = rec_word<DIM>(rec, cc, CR::T0 + q);
             } else {
 #pragma unroll
                 for (int p = 0; p < NE; p++) {
