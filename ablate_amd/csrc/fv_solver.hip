@@ -107,18 +107,19 @@ __device__ inline double rec_word(const double* __restrict__ rec, int64_t c, int
 
 // Cell record of the mixed form, in 64-bit words: [0, U0/2) pairs of FP32 values (the limited gradients at float
 // [0, NE*DIM), the viscous terms at float [TF, TF + NT)), then the NE FP64 state words.  Stored in blocks of 64
-// cells, word-major inside a block like the FP64 record (recf_base).
+// cells in word pairs like the FP64 record: words (2q, 2q + 1) of cell c at 16-byte index recf_pair(c) + 64 q.
 template <int DIM>
 struct CellRecF {
     static constexpr int NE = DIM + 2;
     static constexpr int TF = NE * DIM;
     static constexpr int U0 = ((TF + ViscTerms<DIM>::NT) + 1) & ~1;  // float offset of the state (even)
-    static constexpr int RSW = U0 / 2 + NE;                          // 64-bit words per cell
-    static constexpr int RS = 2 * RSW;                                // floats per cell
+    static constexpr int RSW = U0 / 2 + NE;                          // 64-bit words in use per cell
+    static constexpr int NP = (RSW + 1) / 2;                          // word pairs per cell
+    static constexpr int RS = 4 * NP;                                 // floats per cell (with padding)
 };
 template <int DIM>
-__device__ inline int64_t recf_base(int64_t c) {
-    return (c >> 6) * (int64_t)(64 * CellRecF<DIM>::RSW) + (c & 63);
+__device__ inline int64_t recf_pair(int64_t c) {
+    return (c >> 6) * (int64_t)(64 * CellRecF<DIM>::NP) + (c & 63);
 }
 
 template <int DIM, bool RECON, bool VISC>
@@ -132,14 +133,22 @@ template <int DIM, bool RECON, bool VISC>
 __device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF<DIM, RECON, VISC>& r) {
     using CR = CellRecF<DIM>;
     constexpr int NE = DIM + 2;
-    const uint64_t* __restrict__ b = reinterpret_cast<const uint64_t*>(rec) + recf_base<DIM>(c);
+    const ulonglong2* __restrict__ b = reinterpret_cast<const ulonglong2*>(rec) + recf_pair<DIM>(c);
     constexpr int NF = VISC ? CR::TF + ViscTerms<DIM>::NT : (RECON ? CR::TF : 0);
+    constexpr int NFW = (NF + 1) / 2;  // words holding the FP32 values needed
+    uint64_t wd[2 * CR::NP];
+#pragma unroll
+    for (int q = 0; q < CR::NP; q++) {
+        const bool need = 2 * q < NFW || (2 * q + 1 >= CR::U0 / 2 && 2 * q < CR::U0 / 2 + NE);
+        const ulonglong2 v = need ? b[64 * q] : make_ulonglong2(0ull, 0ull);
+        wd[2 * q] = v.x;
+        wd[2 * q + 1] = v.y;
+    }
     float t[NF > 0 ? ((NF + 1) & ~1) : 2];
 #pragma unroll
-    for (int i = 0; i < (NF + 1) / 2; i++) {
-        const uint64_t w = b[64 * i];
-        t[2 * i] = __uint_as_float((uint32_t)w);
-        t[2 * i + 1] = __uint_as_float((uint32_t)(w >> 32));
+    for (int i = 0; i < NFW; i++) {
+        t[2 * i] = __uint_as_float((uint32_t)wd[i]);
+        t[2 * i + 1] = __uint_as_float((uint32_t)(wd[i] >> 32));
     }
     if (RECON)
 #pragma unroll
@@ -148,7 +157,7 @@ __device__ inline void load_rec_f(const float* __restrict__ rec, int64_t c, RecF
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++) r.tq[q] = t[CR::TF + q];
 #pragma unroll
-    for (int p = 0; p < NE; p++) r.u[p] = __longlong_as_double((long long)b[64 * (CR::U0 / 2 + p)]);
+    for (int p = 0; p < NE; p++) r.u[p] = __longlong_as_double((long long)wd[CR::U0 / 2 + p]);
 }
 
 // Gradients of the euler field (limited) and of the aux T/velocity (unlimited), one interior cell per lane.
@@ -333,12 +342,17 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
         for (int q = 0; q < ViscTerms<DIM>::NT; q++)
             if (VISC) t[CR::TF + q] = float(tq[q]);
         if (active) {
-            uint64_t* __restrict__ w = reinterpret_cast<uint64_t*>(rec) + recf_base<DIM>(c_lane);
+            uint64_t wd[2 * CR::NP];
 #pragma unroll
             for (int i = 0; i < CR::U0 / 2; i++)
-                w[64 * i] = (uint64_t)__float_as_uint(t[2 * i]) | ((uint64_t)__float_as_uint(t[2 * i + 1]) << 32);
+                wd[i] = (uint64_t)__float_as_uint(t[2 * i]) | ((uint64_t)__float_as_uint(t[2 * i + 1]) << 32);
 #pragma unroll
-            for (int p = 0; p < NE; p++) w[64 * (CR::U0 / 2 + p)] = (uint64_t)__double_as_longlong(uc[p]);
+            for (int p = 0; p < NE; p++) wd[CR::U0 / 2 + p] = (uint64_t)__double_as_longlong(uc[p]);
+#pragma unroll
+            for (int i = CR::RSW; i < 2 * CR::NP; i++) wd[i] = 0;
+            ulonglong2* __restrict__ w = reinterpret_cast<ulonglong2*>(rec) + recf_pair<DIM>(c_lane);
+#pragma unroll
+            for (int q = 0; q < CR::NP; q++) w[64 * q] = make_ulonglong2(wd[2 * q], wd[2 * q + 1]);
         }
     }
 }
@@ -2659,7 +2673,7 @@ struct abl_fv_solver {
     abl::TileMixedKernel tile_kernel_mixed = nullptr;
     abl::FaceFluxMixedKernel face_flux_kernel_mixed = nullptr;
     abl::GatherMixedKernel gather_kernel_mixed = nullptr;
-    abl::DevBuf<float> recf;            // cell records of the mixed form (recf_base blocks)
+    abl::DevBuf<float> recf;            // cell records of the mixed form (recf_pair blocks)
     abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients [ns*DIM][N] (densityYi limited, Yi aux)
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
