@@ -100,9 +100,9 @@ KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recomput
 # scaled by the calibration factor the same counters give for the streaming copy of abl_hbm_probe), measured with the commands recorded in
 # profiles/rhs_ns_fp64_256cube_tile.md, rhs_ns_fp64_256cube_face_flux.md and rhs_ns_mixed_256cube_tile.md.  Keyed by
 # (workload, cells per direction, strategy).
-TRAFFIC_BYTES = {("ns_fp64_256cube_lsq_bj_ausm", 256, "tile"): 29272677881,
-                 ("ns_fp64_256cube_lsq_bj_ausm", 256, "face_flux"): 27052399285,
-                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 16359357774}
+TRAFFIC_BYTES = {("ns_fp64_256cube_lsq_bj_ausm", 256, "tile"): 29266830114,
+                 ("ns_fp64_256cube_lsq_bj_ausm", 256, "face_flux"): 27296793657,
+                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 16652114611}
 
 
 def measured_traffic(cfg, n, strategy):
