@@ -105,6 +105,15 @@ __device__ inline double rec_word(const double* __restrict__ rec, int64_t c, int
     return rec[2 * (rec_pair<DIM>(c) + (int64_t)CellRec<DIM>::BLOCK * (i >> 1)) + (i & 1)];
 }
 
+// FP64 species gradients: per species and cell 2 DIM words, [0, DIM) the limited densityYi gradient and
+// [DIM, 2 DIM) the Yi aux gradient, stored as DIM word pairs; pair j of species sp of cell c is the double2 at
+// (sp DIM + j) N + c.  Lanes of consecutive cells move each pair with one coalesced 16-byte access, and a face's
+// right cell (scattered) costs DIM cache lines per species instead of 2 DIM.
+template <int DIM>
+__device__ inline int64_t sgp_word(int64_t N, int sp, int64_t c, int w) {
+    return (((int64_t)sp * DIM + (w >> 1)) * N + c) * 2 + (w & 1);
+}
+
 // Cell record of the mixed form, in 64-bit words: [0, U0/2) pairs of FP32 values (the limited gradients at float
 // [0, NE*DIM), the viscous terms at float [TF, TF + NT)), then the NE FP64 state words.  Stored in blocks of 64
 // cells in word pairs like the FP64 record: words (2q, 2q + 1) of cell c at 16-byte index recf_pair(c) + 64 q.
@@ -470,16 +479,33 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phy
                 for (int d = 0; d < DIM; d++)
                     if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradC[(sp * DIM + d) * N + c] = G(g[d] * phi);
+            for (int d = 0; d < DIM; d++) {
+                if constexpr (sizeof(G) == 8) g[d] *= phi;  // stored with the aux gradient below
+                else gradC[(sp * DIM + d) * N + c] = G(g[d] * phi);
+            }
         }
-        if (aux)
+        if constexpr (sizeof(G) == 8) {
+            double v[2 * DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                v[d] = recon ? g[d] : 0.0;
+                v[DIM + d] = aux ? ga[d] : 0.0;
+            }
+            double2* __restrict__ o = reinterpret_cast<double2*>(gradC);
+#pragma unroll
+            for (int j = 0; j < DIM; j++) o[((int64_t)sp * DIM + j) * N + c] = make_double2(v[2 * j], v[2 * j + 1]);
+        } else if (aux) {
 #pragma unroll
             for (int d = 0; d < DIM; d++) gradA[(sp * DIM + d) * N + c] = G(ga[d]);
+        }
     }
     if (cancel)  // the maxLimGrad rule zeroes the whole densityYi field of the cell (cellInterpolant.cpp:459-481)
         for (int sp = 0; sp < ph.ns; sp++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gradC[(sp * DIM + d) * N + c] = G(0);
+            for (int d = 0; d < DIM; d++) {
+                if constexpr (sizeof(G) == 8) gradC[sgp_word<DIM>(N, sp, c, d)] = 0.0;
+                else gradC[(sp * DIM + d) * N + c] = G(0);
+            }
 }
 
 // Everything a face flux needs from one side: the state projected to the face (ProjectToFace,
@@ -624,13 +650,26 @@ __device__ inline double species_face_flux(const MeshView<double>& m, const doub
     const int64_t N = m.N;
     double yL = U[comp * N + L], yR = U[comp * N + R];
     const double rL = U[kRho * N + L], rR = U[kRho * N + R];
+    // both sides' gradient words (sgp_word pairs); a pair is read only if it holds a word this face uses
+    double vL[2 * DIM], vR[2 * DIM];
+    const double2* __restrict__ sg = reinterpret_cast<const double2*>(gradC);
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+        const bool need = (RECON && 2 * j < DIM) || (VISC && 2 * j + 1 >= DIM);
+        const double2 a = need ? sg[((int64_t)sp * DIM + j) * N + L] : make_double2(0.0, 0.0);
+        const double2 b = (need && !ghost) ? sg[((int64_t)sp * DIM + j) * N + R] : make_double2(0.0, 0.0);
+        vL[2 * j] = a.x;
+        vL[2 * j + 1] = a.y;
+        vR[2 * j] = b.x;
+        vR[2 * j + 1] = b.y;
+    }
     double flux = 0.0;
     if (RECON) {
         double pl = 0.0, pr = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            pl += gradC[(sp * DIM + d) * N + L] * x[0][d];
-            if (!ghost) pr += gradC[(sp * DIM + d) * N + R] * x[1][d];
+            pl += vL[d] * x[0][d];
+            if (!ghost) pr += vR[d] * x[1][d];
         }
         flux = cl * (yL + pl) + cr * (yR + pr);
     } else {
@@ -638,15 +677,15 @@ __device__ inline double species_face_flux(const MeshView<double>& m, const doub
     }
     if (VISC) {
         double dot = 0.0;
-        const int64_t base = (int64_t)sp * DIM;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            const double gl = gradA[(base + d) * N + L];
-            const double gr = ghost ? wghost[d] * (yL / rL - yR / rR) : gradA[(base + d) * N + R];
+            const double gl = vL[DIM + d];
+            const double gr = ghost ? wghost[d] * (yL / rL - yR / rR) : vR[DIM + d];
             dot += nhat[d] * (gl + gr);
         }
         flux += cd * dot;
     }
+    (void)gradA;
     return flux;
 }
 
@@ -1675,8 +1714,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
             if (recon)
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    yl += gS[((int64_t)sp * DIM + d) * N + L] * x[0][d];
-                    if (!ghost) yr += gS[((int64_t)sp * DIM + d) * N + Rc] * x[1][d];
+                    yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                    if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
                 }
             QL += gsp * yl;
             QR += gsp * yr;
@@ -1788,8 +1827,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
                 double GL[DIM], GR[DIM], dot = 0.0;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    GL[d] = gY[((int64_t)sp * DIM + d) * N + L];
-                    GR[d] = ghost ? wg[1][d] * (YL - YR) : gY[((int64_t)sp * DIM + d) * N + Rc];
+                    GL[d] = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
+                    GR[d] = ghost ? wg[1][d] * (YL - YR) : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
                     dot += nh[d] * (GL[d] + GR[d]);
                 }
                 const double cdf = -ph.diff * (0.5 * rl + 0.5 * rr) * 0.5 * area;
@@ -2954,14 +2993,11 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->fwa.upload(L.fwa);
             if (s->recon || s->visc)
                 s->rec.alloc((size_t)(L.dim == 3 ? abl::CellRec<3>::RS : abl::CellRec<2>::RS) * ((L.NI + 63) / 64 * 64));
-            // species gradients: structure of arrays [ns*DIM][N], limited densityYi (gradC) and Yi aux (gradA)
-            if (s->recon && L.ns) {
-                s->gradC.alloc((size_t)L.ns * L.dim * N);
-                HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost rows stay zero
-            }
-            if (s->visc && L.ns) {
-                s->gradA.alloc((size_t)L.ns * L.dim * N);
-                HIP_CHECK(hipMemset(s->gradA.ptr, 0, s->gradA.bytes()));
+            // species gradients: limited densityYi and Yi aux together, DIM word pairs per species and cell
+            // (sgp_word); gradA stays unallocated in the FP64 form
+            if ((s->recon || s->visc) && L.ns) {
+                s->gradC.alloc((size_t)L.ns * 2 * L.dim * N);
+                HIP_CHECK(hipMemset(s->gradC.ptr, 0, s->gradC.bytes()));  // ghost cells stay zero
             }
         } else {
             auto to_f = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
