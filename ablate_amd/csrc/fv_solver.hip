@@ -479,33 +479,31 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phy
                 for (int d = 0; d < DIM; d++)
                     if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                if constexpr (sizeof(G) == 8) g[d] *= phi;  // stored with the aux gradient below
-                else gradC[(sp * DIM + d) * N + c] = G(g[d] * phi);
-            }
+            for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
         }
-        if constexpr (sizeof(G) == 8) {
-            double v[2 * DIM];
+        {
+            using V2 = typename Vec2T<G>::type;
+            G v[2 * DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                v[d] = recon ? g[d] : 0.0;
-                v[DIM + d] = aux ? ga[d] : 0.0;
+                v[d] = recon ? G(g[d]) : G(0);
+                v[DIM + d] = aux ? G(ga[d]) : G(0);
             }
-            double2* __restrict__ o = reinterpret_cast<double2*>(gradC);
+            V2* __restrict__ o = reinterpret_cast<V2*>(gradC);
 #pragma unroll
-            for (int j = 0; j < DIM; j++) o[((int64_t)sp * DIM + j) * N + c] = make_double2(v[2 * j], v[2 * j + 1]);
-        } else if (aux) {
-#pragma unroll
-            for (int d = 0; d < DIM; d++) gradA[(sp * DIM + d) * N + c] = G(ga[d]);
+            for (int j = 0; j < DIM; j++) {
+                V2 t;
+                t.x = v[2 * j];
+                t.y = v[2 * j + 1];
+                o[((int64_t)sp * DIM + j) * N + c] = t;
+            }
         }
     }
     if (cancel)  // the maxLimGrad rule zeroes the whole densityYi field of the cell (cellInterpolant.cpp:459-481)
         for (int sp = 0; sp < ph.ns; sp++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                if constexpr (sizeof(G) == 8) gradC[sgp_word<DIM>(N, sp, c, d)] = 0.0;
-                else gradC[(sp * DIM + d) * N + c] = G(0);
-            }
+            for (int d = 0; d < DIM; d++) gradC[sgp_word<DIM>(N, sp, c, d)] = G(0);
+    (void)gradA;
 }
 
 // Everything a face flux needs from one side: the state projected to the face (ProjectToFace,
@@ -1130,8 +1128,8 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
                 if (RECON)
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
-                        dsl += gS[(sp * DIM + d) * N + L] * x[0][d];
-                        if (!ghost) dsr += gS[(sp * DIM + d) * N + Rc] * x[1][d];
+                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
                     }
                 const float dsu = mm < 0.f ? dsr : dsl;
                 const float dYu = (dsu - YL * up.drho) / up.rho;
@@ -1205,8 +1203,8 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
                 if (RECON)
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
-                        dsl += gS[(sp * DIM + d) * N + L] * x[0][d];
-                        if (!ghost) dsr += gS[(sp * DIM + d) * N + Rc] * x[1][d];
+                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
                     }
                 const double fsp = clo * (yl + double(dsl)) + cro * (yl + double(dsr)) - yl * und * ad;
                 sFL[sp * s_stride] = float(fsp);
@@ -1265,8 +1263,8 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
                 }
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    const float gl = gY[(sp * DIM + d) * N + L];
-                    const float gr = ghost ? w[1][d] * gyg : gY[(sp * DIM + d) * N + Rc];
+                    const float gl = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
+                    const float gr = ghost ? w[1][d] * gyg : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
                     dot += nhat[d] * (gl + gr);
                 }
                 sFL[sp * s_stride] += cd * dot;
@@ -2713,7 +2711,7 @@ struct abl_fv_solver {
     abl::FaceFluxMixedKernel face_flux_kernel_mixed = nullptr;
     abl::GatherMixedKernel gather_kernel_mixed = nullptr;
     abl::DevBuf<float> recf;            // cell records of the mixed form (recf_pair blocks)
-    abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients [ns*DIM][N] (densityYi limited, Yi aux)
+    abl::DevBuf<float> gS_f, gY_f;      // mixed form species gradients (sgp_word pairs in gS_f; gY_f unused)
     // adjoint
     abl::DevBuf<int32_t> ghost_slot;
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
@@ -3012,13 +3010,11 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->fwa_f.upload(to_f(L.fwa));
             if (s->recon || s->visc)
                 s->recf.alloc((size_t)(L.dim == 3 ? abl::CellRecF<3>::RS : abl::CellRecF<2>::RS) * ((L.NI + 63) / 64 * 64));
-            if (s->recon && L.ns) {
-                s->gS_f.alloc((size_t)L.ns * L.dim * N);
+            // species gradients: limited densityYi and Yi aux together, DIM FP32 pairs per species and cell
+            // (sgp_word); gY_f stays unallocated
+            if ((s->recon || s->visc) && L.ns) {
+                s->gS_f.alloc((size_t)L.ns * 2 * L.dim * N);
                 HIP_CHECK(hipMemset(s->gS_f.ptr, 0, s->gS_f.bytes()));
-            }
-            if (s->visc && L.ns) {
-                s->gY_f.alloc((size_t)L.ns * L.dim * N);
-                HIP_CHECK(hipMemset(s->gY_f.ptr, 0, s->gY_f.bytes()));
             }
             if (mesh->dim == 3) {
                 s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
