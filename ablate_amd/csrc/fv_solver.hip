@@ -767,7 +767,10 @@ __global__ __launch_bounds__(kBlock) void k_flux(MeshView<double> m, Phys ph, co
 
 // The flux through face f with the reference orientation: euler components into fe[0, NE), species into
 // fs[sp * fs_stride] (shared by the face-flux and tile strategies, which therefore agree bitwise).
-template <int DIM, bool RECON, bool VISC, bool SPECIES>
+// Species flux sp goes to fs[sp * fs_stride], or with PAIRS to fs[(sp / 2) * fs_stride + sp % 2] (species pairs
+// interleaved: the face-flux strategy's rows, written and read by the gather two species per 16-byte access; fs
+// 16-byte aligned).
+template <int DIM, bool RECON, bool VISC, bool SPECIES, bool PAIRS = false>
 __device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U,
                                  const double* __restrict__ gradC, const double* __restrict__ gradA,
                                  const double* __restrict__ rec, int64_t f, double* fe, double* fs,
@@ -788,10 +791,24 @@ __device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, cons
     }
     double cl, cr, cd;
     face_flux<DIM, VISC>(ph, nh, area, sl, sr, fe, cl, cr, cd);
-    if (SPECIES)
+    if (!SPECIES) return;
+    if constexpr (PAIRS) {
+        // two species per 16-byte store
+        for (int q = 0; 2 * q < ph.ns; q++) {
+            const int sp = 2 * q;
+            const double a = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, DIM + 2 + sp, sp, f, L, Rc, ghost,
+                                                                 x, nh, cl, cr, cd, w[1]);
+            const double b = sp + 1 < ph.ns ? species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, DIM + 3 + sp,
+                                                                                 sp + 1, f, L, Rc, ghost, x, nh, cl,
+                                                                                 cr, cd, w[1])
+                                            : 0.0;
+            *reinterpret_cast<double2*>(fs + q * fs_stride) = make_double2(a, b);
+        }
+    } else {
         for (int sp = 0; sp < ph.ns; sp++)
             fs[sp * fs_stride] = species_face_flux<DIM, RECON, VISC>(m, U, gradC, gradA, DIM + 2 + sp, sp, f, L, Rc,
                                                                      ghost, x, nh, cl, cr, cd, w[1]);
+    }
 }
 
 // Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored: euler components as a
@@ -807,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys p
     const int64_t f = xcd_cell_index();
     if (f >= m.F) return;
     double flux[FS];
-    eval_face<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, f, flux, Fs + f, m.F);
+    eval_face<DIM, RECON, VISC, SPECIES, true>(m, ph, U, gradC, gradA, rec, f, flux, Fs + 2 * f, 2 * m.F);
     if (FS > NE) flux[FS - 1] = 0.0;
     double2* out = reinterpret_cast<double2*>(Fe + FS * f);
 #pragma unroll
@@ -920,16 +937,25 @@ __global__ __launch_bounds__(kBlock) void k_gather(MeshView<double> m, int ns, c
         R[p * N + c] = accumulate ? R[p * N + c] + r : r;
     }
     if (!SPECIES) return;
-    for (int sp = 0; sp < ns; sp++) {
-        double a = 0.0;
+    // species pairs (sp, sp + 1) interleaved per face: one 16-byte load per face and pair
+    const double2* __restrict__ Fs2 = reinterpret_cast<const double2*>(Fs);
+    for (int q = 0; 2 * q < ns; q++) {
+        double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
-            const double fv = Fs[(int64_t)sp * F + fk[k]];
-            if (sg[k] != 0.0) a += sg[k] * fv;
+            const double2 fv = Fs2[(int64_t)q * F + fk[k]];
+            if (sg[k] != 0.0) {
+                a0 += sg[k] * fv.x;
+                a1 += sg[k] * fv.y;
+            }
         }
-        const int64_t comp = NE + sp;
-        const double r = a * iv;
-        R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
+        for (int h = 0; h < 2 && 2 * q + h < ns; h++) {
+            const int sp = 2 * q + h;
+            const double a = h ? a1 : a0;
+            const int64_t comp = NE + sp;
+            const double r = a * iv;
+            R[comp * N + c] = accumulate ? R[comp * N + c] + r : r;
+        }
     }
 }
 
@@ -2834,7 +2860,10 @@ struct abl_fv_solver {
         if (form == ABL_FV_FORM_FP64) {
             if (face_flux_e.ptr) return;
             face_flux_e.alloc((size_t)(((host.dim + 2) + 1) & ~1) * host.F);
-            if (host.ns) face_flux_s.alloc((size_t)host.ns * host.F);
+            if (host.ns) {  // species pairs interleaved per face (k_gather); an odd count's last slot stays zero
+                face_flux_s.alloc((size_t)(host.ns + 1) / 2 * 2 * host.F);
+                HIP_CHECK(hipMemset(face_flux_s.ptr, 0, face_flux_s.bytes()));
+            }
         } else {
             if (face_flux_m.ptr) return;
             face_flux_m.alloc((size_t)2 * nrow * host.F);
