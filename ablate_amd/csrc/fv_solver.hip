@@ -812,8 +812,9 @@ __device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, cons
 }
 
 // Strategy 2 (face flux + gather), pass 1: one lane per face, flux evaluated once and stored: euler components as a
-// padded record [F][FS] (one contiguous read per face in the gather), species as rows [ns][F] (coalesced stores;
-// species inside the record would make every store of a wave touch 64 cache lines, measured 2.5x slower).
+// padded record [F][FS] (one contiguous read per face in the gather), species as interleaved pairs [ns/2][F][2]
+// (coalesced 16-byte stores; species inside the record would make every store of a wave touch 64 cache lines,
+// measured 2.5x slower).
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
 __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                       const double* __restrict__ gradC, const double* __restrict__ gradA,
@@ -2717,8 +2718,9 @@ struct abl_fv_solver {
     abl::DevBuf<int32_t> ell, fcell, bface, bslot, slot_kind, snbr;
     abl::DevBuf<double> fna, fdx, fwa, inv_vol, gradC, gradA, slot_state;
     abl::DevBuf<double> sw, swa, slim;  // slot-major gradient geometry (FP64 form)
-    abl::DevBuf<double> rec;            // cell records [NI][RS] of the FP64 form
-    // face fluxes (allocated when that strategy is selected): FP64 euler records [F][FS] and species rows [ns][F];
+    abl::DevBuf<double> rec;            // cell records of the FP64 form (rec_pair blocks)
+    // face fluxes (allocated when that strategy is selected): FP64 euler records [F][FS] and species pairs
+    // [ns/2][F][2];
     // mixed form rows [2 (n_euler + n_species)][F]
     abl::DevBuf<double> face_flux_e, face_flux_s;
     abl::DevBuf<float> face_flux_m;
