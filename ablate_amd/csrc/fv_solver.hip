@@ -385,8 +385,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // FP64 arithmetic, stored as G (double: FP64 form, float: mixed form).  The cell's slot geometry (neighbours,
 // weights, limiter vectors, neighbour densities) is loaded once for all species.
 template <typename G, int DIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
-                                                              G* __restrict__ gradC, G* __restrict__ gradA) {
+__device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                             G* __restrict__ gradC, G* __restrict__ gradA) {
     constexpr int NE = DIM + 2;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
@@ -504,6 +504,18 @@ __global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phy
 #pragma unroll
             for (int d = 0; d < DIM; d++) gradC[sgp_word<DIM>(N, sp, c, d)] = G(0);
     (void)gradA;
+}
+
+template <typename G, int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_species_gradients(MeshView<G> m, Phys ph, const double* __restrict__ U,
+                                                              G* __restrict__ gradC, G* __restrict__ gradA) {
+    species_body<G, DIM, KM>(m, ph, U, gradC, gradA);
+}
+// The same kernel held to three waves per SIMD (168 VGPRs), picked for 4-slot cells (triangles, tetrahedra).
+template <typename G, int DIM, int KM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_species_gradients_w3(
+    MeshView<G> m, Phys ph, const double* __restrict__ U, G* __restrict__ gradC, G* __restrict__ gradA) {
+    species_body<G, DIM, KM>(m, ph, U, gradC, gradA);
 }
 
 // Everything a face flux needs from one side: the state projected to the face (ProjectToFace,
@@ -2317,7 +2329,7 @@ using SpeciesGradKernelT = void (*)(MeshView<G>, Phys, const double*, G*, G*);
 template <typename G, int DIM>
 SpeciesGradKernelT<G> pick_species_gradients(int K) {
     switch (K <= 4 ? 4 : (K <= 6 ? 6 : 8)) {
-        case 4: return k_species_gradients<G, DIM, 4>;
+        case 4: return k_species_gradients_w3<G, DIM, 4>;
         case 6: return k_species_gradients<G, DIM, 6>;
         default: return k_species_gradients<G, DIM, 8>;
     }
