@@ -102,7 +102,7 @@ KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recomput
 # (workload, cells per direction, strategy).
 TRAFFIC_BYTES = {("ns_fp64_256cube_lsq_bj_ausm", 256, "tile"): 29266830114,
                  ("ns_fp64_256cube_lsq_bj_ausm", 256, "face_flux"): 27296793657,
-                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 16638664388}
+                 ("ns_mixed_256cube_lsq_bj_ausm", 256, "tile"): 16644691203}
 
 
 def measured_traffic(cfg, n, strategy):
