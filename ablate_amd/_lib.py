@@ -19,7 +19,7 @@ LIMITER = {
     "none": 0, "zero": 1, "minmod": 2, "vanleer": 3, "vanalbada": 4, "superbee": 5, "mc": 6, "sin": 7,
     "barthjespersen": 8,
 }
-BC_NONE, BC_INFLOW, BC_OUTFLOW = 0, 1, 2
+BC_NONE, BC_INFLOW, BC_OUTFLOW, BC_GHOST = 0, 1, 2, 3
 ORDER = {"natural": 0, "brick": 1, "rcm": 2, "morton": 3}
 
 _dp = POINTER(c_double)
@@ -76,6 +76,7 @@ class BcDesc(ctypes.Structure):
         ("vel", c_double * 3),
         ("T", c_double),
         ("Yi", _dp),
+        ("enforce_at_face", c_int32),
     ]
 
 

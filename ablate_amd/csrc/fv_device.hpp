@@ -11,6 +11,10 @@ namespace abl {
 
 enum : int { kRho = 0, kRhoE = 1, kRhoU = 2 };
 
+// Device code of an INFLOW slot with enforceAtFace (essentialGhost.cpp:15-21: ghost = 2 state - interior); the other
+// slot kinds keep their ABL_FV_BC_* values on the device.
+constexpr int kBcInflowAtFace = 16;
+
 // Physics constants passed by value to every kernel.
 struct Phys {
     double gamma, R, cv, icv, mu, k, diff, max_lim_grad;

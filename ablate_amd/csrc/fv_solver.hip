@@ -55,8 +55,12 @@ __global__ __launch_bounds__(kBlock) void k_ghost_fill(int64_t nb, const int32_t
     const int32_t f = bface[i], s = bslot[i];
     const int32_t interior = fcell[f], ghost = fcell[F + f];
     const int kind = slot_kind[s];
+    if (kind == ABL_FV_BC_GHOST) return;  // caller-filled: the ghost rows are inputs
     for (int p = 0; p < nc; p++) {
-        double v = kind == ABL_FV_BC_INFLOW ? slot_state[s * nc + p] : U[(int64_t)p * N + interior];
+        const double ui = U[(int64_t)p * N + interior];
+        double v = ui;  // OUTFLOW: zero-gradient extrapolation
+        if (kind == ABL_FV_BC_INFLOW) v = slot_state[s * nc + p];
+        if (kind == kBcInflowAtFace) v = 2.0 * slot_state[s * nc + p] - ui;  // essentialGhost.cpp:17-20
         U[(int64_t)p * N + ghost] = v;
     }
 }
@@ -1502,7 +1506,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, F = m.F, NI = m.NI;
-    double ub[NE], coef[KM];
+    double ub[NE], coef[KM], cghost[KM];  // cghost: species multiplier of a caller-filled ghost across slot k
 #pragma unroll
     for (int p = 0; p < NE; p++) ub[p] = 0.0;
 #pragma unroll
@@ -1511,6 +1515,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
         const int64_t f = sl.f;
         const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
         const bool ghost = Rc >= NI;
+        cghost[k] = 0.0;
         double nh[3], area, uL[NE], uR[NE], fb[NE], QL, QR, QdL;
         load_face_na(m.fna, f, nh, area);
 #pragma unroll
@@ -1531,11 +1536,21 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
         double cf = sl.s == 0 ? cL : cR;
 #pragma unroll
         for (int p = 0; p < NE; p++) ub[p] += sl.s == 0 ? ubL[p] : ubR[p];
-        // an outflow ghost copies this cell's state: its adjoint flows back here
-        if (ghost && slot_kind[ghost_slot[Rc - NI]] == ABL_FV_BC_OUTFLOW) {
+        if (ghost) {
+            const int gk = slot_kind[ghost_slot[Rc - NI]];
+            if (gk == ABL_FV_BC_OUTFLOW) {  // the ghost copies this cell's state: its adjoint flows back here
 #pragma unroll
-            for (int p = 0; p < NE; p++) ub[p] += ubR[p];
-            cf += cR;
+                for (int p = 0; p < NE; p++) ub[p] += ubR[p];
+                cf += cR;
+            } else if (gk == kBcInflowAtFace) {  // ghost = 2 state - this cell
+#pragma unroll
+                for (int p = 0; p < NE; p++) ub[p] -= ubR[p];
+                cf -= cR;
+            } else if (gk == ABL_FV_BC_GHOST) {  // caller-filled ghost: its own adjoint row (one face per ghost)
+#pragma unroll
+                for (int p = 0; p < NE; p++) Ub[p * N + Rc] = ubR[p];
+                cghost[k] = cR;
+            }
         }
         coef[k] = cf;
     }
@@ -1563,6 +1578,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
             const bool ghost = sR[k] >= NI;
             const double g = (ghost ? 0.0 : Rb[comp * N + sR[k]] * ivR[k]) - Rb[comp * N + sL[k]] * ivL[k];
             a += coef[k] * g;
+            if (cghost[k] != 0.0) Ub[comp * N + sR[k]] = cghost[k] * g;
         }
         Ub[comp * N + c] = a;
     }
@@ -1583,7 +1599,8 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
     const int64_t f = bface[i];
     const int slot = bslot[i];
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (slot_kind[slot] == ABL_FV_BC_INFLOW) {
+    const int kind = slot_kind[slot];
+    if (kind == ABL_FV_BC_INFLOW || kind == kBcInflowAtFace) {
         const int64_t N = m.N, F = m.F;
         const int64_t L = m.fcell[f], G = m.fcell[F + f];
         double nh[3], area, uL[NE], uG[NE], fb[NE], QL, QR, QdL;
@@ -1616,6 +1633,11 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
             const int64_t comp = NE + sp;
             const double g = -Rb[comp * N + L] * m.inv_vol[L];
             t0 += cR * g * slot_Y[slot * ph.ns + sp];
+        }
+        if (kind == kBcInflowAtFace) {  // ghost = 2 state(theta) - interior
+            t0 *= 2.0;
+            t1 *= 2.0;
+            t2 *= 2.0;
         }
     }
     contrib[3 * i + 0] = t0;
@@ -2238,13 +2260,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<double> m, Phys 
     const int nc = NE + ph.ns;
     int64_t nb[KM];
     int mk[KM];
-    bool from_ghost[KM];
+    double from_ghost[KM];  // dU_ghost / dU_cell: +1 (OUTFLOW copy), -1 (enforceAtFace inflow), 0 otherwise
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int32_t e = m.snbr[k * NI + c];
         nb[k] = e;
         mk[k] = b.mirror[k * NI + c];
-        from_ghost[k] = e >= NI && slot_kind[ghost_slot[e - NI]] == ABL_FV_BC_OUTFLOW;
+        const int gk = e >= NI ? slot_kind[ghost_slot[e - NI]] : ABL_FV_BC_NONE;
+        from_ghost[k] = gk == ABL_FV_BC_OUTFLOW ? 1.0 : (gk == kBcInflowAtFace ? -1.0 : 0.0);
     }
     for (int p = 0; p < nc; p++) {
         double acc = b.Uself[(int64_t)p * NI + c];
@@ -2253,11 +2276,22 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<double> m, Phys 
             if (nb[k] < 0) continue;
             if (nb[k] < NI)
                 acc += b.contrib[((int64_t)mk[k] * nc + p) * NI + nb[k]];
-            else if (from_ghost[k])
-                acc += b.Ughost[(int64_t)p * NG + (nb[k] - NI)];
+            else if (from_ghost[k] != 0.0)
+                acc += from_ghost[k] * b.Ughost[(int64_t)p * NG + (nb[k] - NI)];
         }
         Ub[(int64_t)p * N + c] = acc;
     }
+}
+
+// Adjoint rows of caller-filled ghost cells (ABL_FV_BC_GHOST): their values are inputs of the RHS, so their
+// accumulated adjoint is returned in the ghost rows of Ubar (the other ghost rows were zeroed).
+__global__ __launch_bounds__(kBlock) void k_ghost_adjoint_out(int64_t NG, int64_t N, int64_t NI, int nc,
+                                                              const int32_t* __restrict__ ghost_slot,
+                                                              const int32_t* __restrict__ slot_kind,
+                                                              const double* __restrict__ Ughost, double* __restrict__ Ub) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NG || slot_kind[ghost_slot[g]] != ABL_FV_BC_GHOST) return;
+    for (int p = 0; p < nc; p++) Ub[(int64_t)p * N + NI + g] = Ughost[(int64_t)p * NG + g];
 }
 
 // Per-slot sums of the per-boundary-face theta_bar contributions: block 3 s + j sums component j over the faces of
@@ -2290,7 +2324,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_theta(int64_t nb, int dim, int n
     if (i >= nb) return;
     const int slot = bslot[i];
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    if (slot_kind[slot] == ABL_FV_BC_INFLOW) {
+    const int kind = slot_kind[slot];
+    if (kind == ABL_FV_BC_INFLOW || kind == kBcInflowAtFace) {
         const int64_t g = fcell[F + bface[i]] - NI, NG = N - NI;
         const double* sp_ = slot_param + 6 * slot;
         const double rho = sp_[0], um = sp_[1], T = sp_[2];
@@ -2304,6 +2339,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_theta(int64_t nb, int dim, int n
             t1 += ubm * rho * sp_[3 + d];
         }
         for (int sp = 0; sp < ns; sp++) t0 += Ughost[(dim + 2 + sp) * NG + g] * slot_Y[slot * ns + sp];
+        if (kind == kBcInflowAtFace) {  // ghost = 2 state(theta) - interior
+            t0 *= 2.0;
+            t1 *= 2.0;
+            t2 *= 2.0;
+        }
     }
     contrib[3 * i + 0] = t0;
     contrib[3 * i + 1] = t1;
@@ -2756,7 +2796,6 @@ struct abl_fv_solver {
     abl::DevBuf<int32_t> ghost_slot;
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
     abl::DevBuf<double> theta_sum;    // [3 kMaxBcSlots] per-slot sums (k_theta_reduce)
-    std::vector<int> theta_slots;     // slots that have boundary faces (read once from bslot)
     std::vector<double> h_slot_param, h_slot_Y;
     abl::VjpKernel vjp_kernel = nullptr;
     // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
@@ -2780,7 +2819,9 @@ struct abl_fv_solver {
     abl_fv_pgs_desc pgs{};
     double pgs_max_mach = 0.0;
     abl::DevBuf<double> pgs_partial, pgs_out;
-    std::vector<int32_t> h_slot_kind;
+    std::vector<int32_t> h_slot_kind;        // ABL_FV_BC_* per slot (as set)
+    std::vector<char> h_slot_at_face;        // INFLOW slots with enforceAtFace
+    std::vector<int> bc_slots;               // slots that have boundary faces (set at creation)
     std::vector<double> h_slot_state;
     std::vector<std::array<double, 3>> h_slot_theta;
     int64_t n_bfaces = 0;
@@ -2948,6 +2989,16 @@ int fail(abl_fv_handle h, const std::string& msg) {
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + abl::kBlock - 1) / abl::kBlock); }
 
+// Every slot that has boundary faces needs a condition: an unset slot would leave its ghost cells undefined.
+// Returns an empty string when all are set.
+std::string unset_boundary(abl_fv_handle h) {
+    for (int v : h->bc_slots)
+        if (h->h_slot_kind[v] == ABL_FV_BC_NONE)
+            return "boundary slot " + std::to_string(v) +
+                   " has boundary faces but no boundary condition (abl_fv_set_boundary)";
+    return std::string();
+}
+
 }  // namespace
 
 extern "C" const char* abl_fv_last_error(abl_fv_handle h) {
@@ -2968,8 +3019,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         return fail(nullptr, "abl_fv_create: unknown reconstruction");
     if (p.limiter < ABL_FV_LIMITER_NONE || p.limiter > ABL_FV_LIMITER_BARTH_JESPERSEN)
         return fail(nullptr, "abl_fv_create: unknown limiter");
-    // the mixed form has no perturbation form of AusmpUp: such handles are created in the FP64 form
-    if (form == ABL_FV_FORM_MIXED && p.flux_calculator == ABL_FV_FLUX_AUSMPUP) form = ABL_FV_FORM_FP64;
+    if (form == ABL_FV_FORM_MIXED && p.flux_calculator == ABL_FV_FLUX_AUSMPUP)
+        return fail(nullptr, "abl_fv_create: the mixed form has no AusmpUp flux calculator (create the FP64 form)");
     std::unique_ptr<abl_fv_solver> s(new abl_fv_solver());
     try {
         s->phys = p;
@@ -3074,9 +3125,15 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             }
         }
         std::vector<int32_t> bf, bs;
-        for (auto& b : L.bfaces) {
-            bf.push_back(b.face);
-            bs.push_back(b.slot);
+        {
+            std::vector<char> seen(abl::kMaxBcSlots, 0);
+            for (auto& b : L.bfaces) {
+                bf.push_back(b.face);
+                bs.push_back(b.slot);
+                seen[b.slot] = 1;
+            }
+            for (int v = 0; v < abl::kMaxBcSlots; v++)
+                if (seen[v]) s->bc_slots.push_back(v);
         }
         s->n_bfaces = (int64_t)bf.size();
         {
@@ -3121,6 +3178,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         s->bface.upload(bf);
         s->bslot.upload(bs);
         s->h_slot_kind.assign(abl::kMaxBcSlots, ABL_FV_BC_NONE);
+        s->h_slot_at_face.assign(abl::kMaxBcSlots, 0);
         s->h_slot_state.assign((size_t)abl::kMaxBcSlots * L.nc, 0.0);
         s->h_slot_theta.assign(abl::kMaxBcSlots, {0.0, 0.0, 0.0});
         s->slot_kind.upload(s->h_slot_kind);
@@ -3197,8 +3255,11 @@ extern "C" int abl_fv_get_info(abl_fv_handle h, abl_fv_info* info) {
 extern "C" int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_bc_desc* bc) {
     if (!h || !bc) return fail(h, "abl_fv_set_boundary: null argument");
     if (slot < 0 || slot >= abl::kMaxBcSlots) return fail(h, "abl_fv_set_boundary: slot out of range");
-    if (bc->kind != ABL_FV_BC_INFLOW && bc->kind != ABL_FV_BC_OUTFLOW && bc->kind != ABL_FV_BC_NONE)
+    if (bc->kind != ABL_FV_BC_INFLOW && bc->kind != ABL_FV_BC_OUTFLOW && bc->kind != ABL_FV_BC_NONE &&
+        bc->kind != ABL_FV_BC_GHOST)
         return fail(h, "abl_fv_set_boundary: unknown boundary kind");
+    if (bc->enforce_at_face && bc->kind != ABL_FV_BC_INFLOW)
+        return fail(h, "abl_fv_set_boundary: enforce_at_face applies to INFLOW slots only");
     const abl::Layout& L = h->host;
     const int nc = L.nc, dim = L.dim;
     double* st = &h->h_slot_state[(size_t)slot * nc];
@@ -3213,6 +3274,10 @@ extern "C" int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_b
         for (int sp = 0; sp < L.ns; sp++) st[dim + 2 + sp] = bc->rho * (bc->Yi ? bc->Yi[sp] : 0.0);
     }
     h->h_slot_kind[slot] = bc->kind;
+    h->h_slot_at_face[slot] = bc->kind == ABL_FV_BC_INFLOW && bc->enforce_at_face != 0;
+    std::vector<int32_t> dev_kind(h->h_slot_kind);  // device codes: enforceAtFace inflow has its own
+    for (int v = 0; v < abl::kMaxBcSlots; v++)
+        if (dev_kind[v] == ABL_FV_BC_INFLOW && h->h_slot_at_face[v]) dev_kind[v] = abl::kBcInflowAtFace;
     {
         double* sp = &h->h_slot_param[(size_t)slot * 6];
         for (int i = 0; i < 6; i++) sp[i] = 0.0;
@@ -3230,7 +3295,7 @@ extern "C" int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_b
     try {
         HIP_CHECK(hipMemcpy(h->slot_param.ptr, h->h_slot_param.data(), h->slot_param.bytes(), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(h->slot_Y.ptr, h->h_slot_Y.data(), h->slot_Y.bytes(), hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(h->slot_kind.ptr, h->h_slot_kind.data(), h->slot_kind.bytes(), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->slot_kind.ptr, dev_kind.data(), h->slot_kind.bytes(), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(h->slot_state.ptr, h->h_slot_state.data(), h->slot_state.bytes(), hipMemcpyHostToDevice));
     } catch (const std::exception& e) {
         return fail(h, e.what());
@@ -3239,8 +3304,12 @@ extern "C" int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_b
 }
 
 extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accumulate, abl_fv_stream stream) {
-    (void)time;
+    (void)time;  // constant-state conditions do not depend on it; caller-filled ghosts were evaluated at it
     if (!h || !U || !R) return fail(h, "abl_fv_rhs: null argument");
+    {
+        const std::string bad = unset_boundary(h);
+        if (!bad.empty()) return fail(h, "abl_fv_rhs: " + bad);
+    }
     try {
         hipStream_t st = (hipStream_t)stream;
         const abl::Layout& L = h->host;
@@ -3404,14 +3473,6 @@ namespace {
 // (deterministic, k_theta_reduce); only the 3 sums of each slot come back, and are added into the caller's array
 // for the slots that have boundary faces.
 void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
-    if (h->theta_slots.empty()) {
-        std::vector<int32_t> slots(h->n_bfaces);
-        HIP_CHECK(hipMemcpy(slots.data(), h->bslot.ptr, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-        std::vector<char> seen(abl::kMaxBcSlots, 0);
-        for (int32_t v : slots) seen[v] = 1;
-        for (int v = 0; v < abl::kMaxBcSlots; v++)
-            if (seen[v]) h->theta_slots.push_back(v);
-    }
     if (!h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
     hipLaunchKernelGGL(abl::k_theta_reduce, dim3(3 * abl::kMaxBcSlots), dim3(abl::kBlock), 0, st, h->n_bfaces,
                        h->bslot.ptr, h->theta_contrib.ptr, h->theta_sum.ptr);
@@ -3419,7 +3480,7 @@ void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
     double sums[3 * abl::kMaxBcSlots];
     HIP_CHECK(hipMemcpyAsync(sums, h->theta_sum.ptr, sizeof(sums), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    for (int v : h->theta_slots)
+    for (int v : h->bc_slots)
         for (int j = 0; j < 3; j++) theta_bar[3 * v + j] += sums[3 * v + j];
 }
 
@@ -3466,9 +3527,11 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
             hipLaunchKernelGGL(h->adj_gather, grid, block, 0, st, v, h->ph, h->ghost_slot.ptr, h->slot_kind.ptr, b,
                                Ubar);
         });
-        if (L.N > L.NI)
-            hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
-                               L.nc);
+        if (NG > 0) {
+            hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for(NG * L.nc)), block, 0, st, Ubar, L.N, L.NI, L.nc);
+            hipLaunchKernelGGL(abl::k_ghost_adjoint_out, dim3(grid_for(NG)), block, 0, st, NG, L.N, L.NI, L.nc,
+                               h->ghost_slot.ptr, h->slot_kind.ptr, h->adjUghost.ptr, Ubar);
+        }
         HIP_CHECK(hipGetLastError());
         if (theta_bar && h->n_bfaces) {
             if (!h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
@@ -3489,6 +3552,10 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
                               double* theta_bar, abl_fv_stream stream) {
     (void)time;
     if (!h || !U || !Rbar || !Ubar) return fail(h, "abl_fv_rhs_vjp: null argument");
+    {
+        const std::string bad = unset_boundary(h);
+        if (!bad.empty()) return fail(h, "abl_fv_rhs_vjp: " + bad);
+    }
     const abl::Layout& L = h->host;
     const int fl = h->phys.flux_calculator;
     if (h->form != ABL_FV_FORM_FP64 ||
@@ -3512,13 +3579,14 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
             HIP_CHECK(hipMemsetAsync(Ubar, 0, sizeof(double) * (size_t)L.nc * L.N, st));
         } else {
             const auto v = h->view();
+            // ghost rows first: the VJP kernel then writes those of caller-filled ghosts
+            if (L.N > L.NI)
+                hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
+                                   L.nc);
             h->timed(ABL_FV_PHASE_VJP, st, [&] {
                 hipLaunchKernelGGL(h->vjp_kernel, grid, block, 0, st, v, h->ph, U, Rbar, h->ghost_slot.ptr,
                                    h->slot_kind.ptr, Ubar);
             });
-            if (L.N > L.NI)
-                hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
-                                   L.nc);
         }
         HIP_CHECK(hipGetLastError());
         if (theta_bar && h->n_bfaces && !noflux) {
@@ -3685,6 +3753,10 @@ extern "C" int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0,
     if (!h || !U || !R) return fail(h, "abl_fv_ts_euler: null argument");
     if (!(h->phys.cfl > 0.0) || !(safety > 0.0) || !(dt0 > 0.0))
         return fail(h, "abl_fv_ts_euler: cfl, safety and dt0 must be positive");
+    {
+        const std::string bad = unset_boundary(h);
+        if (!bad.empty()) return fail(h, "abl_fv_ts_euler: " + bad);
+    }
     hipStream_t st = (hipStream_t)stream;
     double t = t0;
     int64_t steps = 0;
@@ -3692,6 +3764,10 @@ extern "C" int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0,
         double dt = std::min(dt0, safety * cfl_dt_impl(h, U, st));  // AdaptPhysics::AdaptInitializer
         while (t < max_time && steps < max_steps) {
             if (!(dt > 0.0)) throw std::runtime_error("non-positive or NaN step size at t = " + std::to_string(t));
+            // TS pre-step: PressureGradientScaling::UpdatePreconditioner with the step about to be taken
+            // (registered by PressureGradientScaling::Setup, pressureGradientScaling.cpp:144-146)
+            if (h->pgs_on)
+                if (int rc = abl_fv_pgs_update(h, U, dt, nullptr, nullptr, stream)) return rc;
             if (int rc = abl_fv_rhs(h, t, U, R, 0, stream)) return rc;
             // TSStep_Euler asks the adaptor for the next step while the solution still holds U_n
             double next = safety * cfl_dt_impl(h, U, st);

@@ -163,13 +163,31 @@ class SpeciesTransport:
 # ----------------------------------------------------------------------------------------------- boundaries
 @dataclass
 class InflowGhost:
-    """EssentialGhost with a constant (rho, velocity, T, Yi) state (boundaryConditions/essentialGhost.cpp)."""
+    """EssentialGhost with a constant (rho, velocity, T, Yi) state (boundaryConditions/essentialGhost.cpp);
+    ``enforceAtFace`` sets the ghost to 2 state - interior so that the face average takes the state
+    (essentialGhost.cpp:15-21)."""
 
     labelIds: list
     rho: float
     velocity: list
     T: float
     Yi: list | None = None
+    enforceAtFace: bool = False
+
+
+class Ghost:
+    """boundaryConditions::Ghost (ghost.cpp:17-28): the ghost value of field `fieldName` on the faces of `labelIds`
+    is ``updateFunction(time, centroid, normal, xI) -> xG`` of the face centroid, the area-weighted normal (left to
+    right, i.e. out of the domain) and the interior cell's field values, all [n_faces, ...] numpy arrays.  The solver
+    evaluates it on the host before each RHS and writes the ghost rows (the reference's boundary-value insertion,
+    timeStepper.cpp:347-349); the device path treats those rows as caller-filled (ABL_FV_BC_GHOST)."""
+
+    def __init__(self, fieldName: str, boundaryName: str, labelIds, updateFunction):
+        if fieldName not in ("euler", "densityYi"):
+            raise ValueError("Ghost applies to the euler or densityYi field")
+        self.fieldName, self.boundaryName = fieldName, boundaryName
+        self.labelIds = list(labelIds)
+        self.updateFunction = updateFunction
 
 
 @dataclass
@@ -229,6 +247,8 @@ class FiniteVolumeSolver:
             ausmpup_minf=float(getattr(nst.fluxCalculator, "mInf", 0.0)),
             compact_species=int(compact),
         )
+        if form == "mixed" and flux == "ausmpup":
+            raise ValueError("the mixed form has no AusmpUp flux calculator: use form='fp64'")
         pgs_proc = [p for p in processes if isinstance(p, PressureGradientScaling)]
         flux_pgs = getattr(nst.fluxCalculator, "pgs", None)
         if len(pgs_proc) > 1:
@@ -251,8 +271,8 @@ class FiniteVolumeSolver:
         _lib.check(_lib.lib().abl_fv_create(ctypes.byref(mesh.desc), ctypes.byref(phys), fm, ctypes.byref(h)), None,
                    "abl_fv_create")
         self._h = h
-        self.form = "fp64" if self.info().form == _lib.FORM_FP64 else "mixed"  # AusmpUp runs the FP64 form
         self._theta_slots = []
+        self._ghosts = []  # (Ghost, component slice, boundary faces, interior cells, ghost cells)
         self.pgs = pgs
         if pgs is not None:
             _lib.check(_lib.lib().abl_fv_set_pgs(h, ctypes.byref(pgs.desc())), h, "abl_fv_set_pgs")
@@ -287,16 +307,51 @@ class FiniteVolumeSolver:
             vel = list(bc.velocity) + [0.0] * (3 - len(bc.velocity))
             desc.vel = (ctypes.c_double * 3)(*vel)
             desc.T = float(bc.T)
+            desc.enforce_at_face = int(bool(bc.enforceAtFace))
             if bc.Yi is not None:
+                if len(bc.Yi) != self.n_species:
+                    raise ValueError(f"InflowGhost.Yi needs {self.n_species} mass fractions, got {len(bc.Yi)}")
                 yi_buf = (ctypes.c_double * len(bc.Yi))(*bc.Yi)
                 desc.Yi = ctypes.cast(yi_buf, ctypes.POINTER(ctypes.c_double))
             self._theta_slots.append(slot)
         elif isinstance(bc, OutflowGhost):
             desc.kind = _lib.BC_OUTFLOW
+        elif isinstance(bc, Ghost):
+            desc.kind = _lib.BC_GHOST
+            self._register_ghost(slot, bc)
         else:
             raise TypeError(f"unsupported boundary condition {bc!r}")
         _lib.check(_lib.lib().abl_fv_set_boundary(self._h, int(slot), ctypes.byref(desc)), self._h,
                    "abl_fv_set_boundary")
+
+    def _register_ghost(self, slot: int, bc: "Ghost") -> None:
+        a = self.mesh.arrays
+        if a.face_bc is None:
+            raise ValueError("the mesh has no boundary faces")
+        faces = np.nonzero(a.face_bc == slot)[0]
+        ne = self.dim + 2
+        comps = slice(0, ne) if bc.fieldName == "euler" else slice(ne, ne + self.n_species)
+        self._ghosts.append((bc, comps, faces, a.face_cells[faces, 0].astype(np.int64),
+                             a.face_cells[faces, 1].astype(np.int64)))
+
+    def insert_boundary_values(self, time: float, U) -> None:
+        """Writes the ghost rows of the Ghost conditions into the device state U (the reference's
+        DMPlexInsertBoundaryValues before ComputeRHSFunction, timeStepper.cpp:347-349)."""
+        if not self._ghosts:
+            return
+        import torch
+
+        a = self.mesh.arrays
+        for bc, comps, faces, interior, ghosts in self._ghosts:
+            if faces.size == 0:
+                continue
+            xI = U[comps][:, torch.as_tensor(interior, device=U.device)].T.cpu().numpy()
+            xG = np.asarray(bc.updateFunction(float(time), a.face_centroid[faces], a.face_normal[faces], xI),
+                            dtype=np.float64)
+            if xG.shape != xI.shape:
+                raise ValueError(f"Ghost update function returned shape {xG.shape}, expected {xI.shape}")
+            U[comps.start:comps.stop, torch.as_tensor(ghosts, device=U.device)] = torch.as_tensor(
+                xG.T.copy(), device=U.device)
 
     # ---- hot path
     @staticmethod
@@ -318,6 +373,7 @@ class FiniteVolumeSolver:
     def compute_rhs(self, time: float, U, R, accumulate: bool = False, stream=None) -> None:
         self._check_tensor(U, "U")
         self._check_tensor(R, "R")
+        self.insert_boundary_values(time, U)
         _lib.check(
             _lib.lib().abl_fv_rhs(self._h, float(time), ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(R.data_ptr()),
                                   int(bool(accumulate)), self._stream_ptr(stream)),
@@ -334,6 +390,7 @@ class FiniteVolumeSolver:
         self._check_tensor(U, "U")
         self._check_tensor(Rbar, "Rbar")
         self._check_tensor(Ubar, "Ubar")
+        self.insert_boundary_values(time, U)
         theta = np.zeros(3 * 16)
         _lib.check(
             _lib.lib().abl_fv_rhs_vjp(self._h, float(time), ctypes.c_void_p(U.data_ptr()),
@@ -394,6 +451,9 @@ class FiniteVolumeSolver:
         (the reference TimeStepper with ts_adapt_type physics).  Returns (final time, steps)."""
         import torch
 
+        if self._ghosts:
+            raise NotImplementedError("ts_euler runs on the device and cannot call the host Ghost update functions: "
+                                      "step with compute_rhs and axpy instead")
         self._check_tensor(U, "U")
         R = torch.empty_like(U) if R is None else R
         self._check_tensor(R, "R")

@@ -92,6 +92,49 @@ class _NativeMesh:
     def n_faces(self):
         return self.arrays.n_faces
 
+    def source_cells(self) -> np.ndarray:
+        """The producer's index of every interior cell, in this mesh's cell order (abl_fv_mesh_source_cells): the
+        DMPlex point number, the Gmsh element ordinal, the lexicographic box index (x fastest) or the tet generation
+        index.  The same physical cell has the same index under every cell ordering."""
+        out = np.empty(self.n_interior, dtype=np.int64)
+        _lib.check(_lib.lib().abl_fv_mesh_source_cells(self._handle, out.ctypes.data), None, "abl_fv_mesh_source_cells")
+        return out
+
+
+class ArrayMesh:
+    """A mesh given by the caller's own FV geometry arrays (abl_fv_mesh_desc: e.g. the PetscFVFaceGeom /
+    PetscFVCellGeom arrays of DMPlexComputeGeometryFVM and the face supports of an ABLATE domain).  The arrays are
+    held for the mesh's lifetime; ``face_bc`` may be None when there are no boundary faces."""
+
+    def __init__(self, arrays: MeshArrays, source=None):
+        a = arrays
+        i32 = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.int32)
+        f64 = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        self.arrays = MeshArrays(dim=int(a.dim), n_cells=int(a.n_cells), n_interior=int(a.n_interior),
+                                 n_faces=int(a.n_faces), face_cells=i32(a.face_cells), face_normal=f64(a.face_normal),
+                                 face_centroid=f64(a.face_centroid), face_shift=f64(a.face_shift),
+                                 cell_centroid=f64(a.cell_centroid), cell_volume=f64(a.cell_volume),
+                                 face_bc=i32(a.face_bc))
+        m = self.arrays
+        ptr = lambda x, t: None if x is None else x.ctypes.data_as(ctypes.POINTER(t))
+        self.desc = _lib.MeshDesc(dim=m.dim, n_cells=m.n_cells, n_interior=m.n_interior, n_faces=m.n_faces,
+                                  face_cells=ptr(m.face_cells, ctypes.c_int32),
+                                  face_normal=ptr(m.face_normal, ctypes.c_double),
+                                  face_centroid=ptr(m.face_centroid, ctypes.c_double),
+                                  face_shift=ptr(m.face_shift, ctypes.c_double),
+                                  cell_centroid=ptr(m.cell_centroid, ctypes.c_double),
+                                  cell_volume=ptr(m.cell_volume, ctypes.c_double),
+                                  face_bc=ptr(m.face_bc, ctypes.c_int32))
+        self._source = np.arange(m.n_interior, dtype=np.int64) if source is None else np.asarray(source, np.int64)
+
+    dim = property(lambda self: self.arrays.dim)
+    n_cells = property(lambda self: self.arrays.n_cells)
+    n_interior = property(lambda self: self.arrays.n_interior)
+    n_faces = property(lambda self: self.arrays.n_faces)
+
+    def source_cells(self) -> np.ndarray:
+        return self._source
+
 
 class BoxMesh(_NativeMesh):
     """Uniform quad/hex box. ``periodic`` per direction; non-periodic sides get ghost cells whose boundary faces
@@ -184,12 +227,6 @@ class DMPlexMesh(_NativeMesh):
         )
         super().__init__(h)
         self.labels = {int(v): int(v) for v in (face_sets or {})}
-
-    def source_cells(self) -> np.ndarray:
-        """DMPlex point number of every interior cell, in this mesh's cell order."""
-        out = np.empty(self.n_interior, dtype=np.int64)
-        _lib.check(_lib.lib().abl_fv_mesh_source_cells(self._handle, out.ctypes.data), None, "abl_fv_mesh_source_cells")
-        return out
 
 
 class Hdf5Mesh(DMPlexMesh):

@@ -235,7 +235,7 @@ def tets_leg(fv, synthetic, args):
     err = per_field_rel(Rm, R)
     del sm
     # FP64 adjoint on the same workload (second order: LSQ + Barth-Jespersen, viscous, species diffusion)
-    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior), device="cuda")
+    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior, cells=mesh.source_cells()), device="cuda")
     Ub = torch.empty_like(U)
     for _ in range(args.warmup):
         solver.rhs_vjp(0.0, U, Rb, Ub)
@@ -312,7 +312,7 @@ def adjoint_leg(fv, synthetic, args):
     solver = fv.FiniteVolumeSolver(mesh, procs, {"petscfv_type": "upwind"}, bcs)
     U0 = synthetic.taylor_green(mesh, 0.3, n_species=ns, phase=(0.37, 0.21, 0.53))
     U = torch.tensor(U0, dtype=torch.float64, device="cuda")
-    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior), device="cuda")
+    Rb = torch.tensor(synthetic.adjoint_seed(U.shape[0], mesh.n_cells, mesh.n_interior, cells=mesh.source_cells()), device="cuda")
     Ub = torch.empty_like(U)
     for _ in range(args.warmup):
         solver.rhs_vjp(0.0, U, Rb, Ub)

@@ -36,8 +36,8 @@ typedef void* abl_fv_stream;
 
 /* ---- enumerations (values are part of the ABI) ------------------------------------------------------------ */
 
-/* Arithmetic form of the RHS.  The mixed form has no perturbation form of the AusmpUp calculator: a mixed handle
- * with AusmpUp is created in the FP64 form (abl_fv_info.form reports it). */
+/* Arithmetic form of the RHS.  The mixed form has no perturbation form of the AusmpUp calculator: abl_fv_create
+ * rejects a mixed handle with AusmpUp (create the FP64 form instead). */
 enum { ABL_FV_FORM_FP64 = 0, ABL_FV_FORM_MIXED = 1 };
 
 /* Flux calculators: fluxCalculator::Ausm (ausm.cpp:2-43), AverageFlux (averageFlux.cpp:2-10),
@@ -69,11 +69,17 @@ enum {
     ABL_FV_LIMITER_BARTH_JESPERSEN = 8
 };
 
-/* Boundary (ghost-cell) conditions. */
+/* Boundary (ghost-cell) conditions.  Every slot that has boundary faces needs one: abl_fv_rhs, abl_fv_rhs_vjp and
+ * abl_fv_ts_euler fail while such a slot is still ABL_FV_BC_NONE (the reference would leave those ghost cells
+ * unset). */
 enum {
     ABL_FV_BC_NONE = 0,
     ABL_FV_BC_INFLOW = 1,  /* EssentialGhost (essentialGhost.cpp:5-23) with a constant (rho, u, T, Yi) state */
-    ABL_FV_BC_OUTFLOW = 2  /* zero-gradient extrapolation ghost: U_ghost = U_interior (build-defined) */
+    ABL_FV_BC_OUTFLOW = 2, /* zero-gradient extrapolation ghost: U_ghost = U_interior (build-defined) */
+    ABL_FV_BC_GHOST = 3    /* caller-filled ghost cells: the ghost rows of U are used exactly as the caller wrote them,
+                              as the reference's TS callback inserts boundary values into locX before
+                              ComputeRHSFunction (timeStepper.cpp:347-349) with any Ghost update function
+                              f(time, face centroid, normal, interior) (ghost.cpp:17-28) */
 };
 
 /* ---- descriptors ------------------------------------------------------------------------------------------ */
@@ -125,7 +131,10 @@ typedef struct abl_fv_bc_desc {
     double rho;              /* inflow density */
     double vel[3];           /* inflow velocity */
     double T;                /* inflow temperature */
-    const double* Yi;        /* [n_species] inflow mass fractions (host pointer, copied), NULL = zeros */
+    const double* Yi;        /* inflow mass fractions: exactly n_species values (host pointer, copied), or NULL =
+                                zeros */
+    int32_t enforce_at_face; /* EssentialGhost{enforceAtFace} (essentialGhost.cpp:15-21): the ghost is
+                                2 * state - interior, so that the face average equals the inflow state */
 } abl_fv_bc_desc;
 
 /* Read-only facts about a created solver. */
@@ -201,14 +210,19 @@ int abl_fv_set_boundary(abl_fv_handle h, int32_t slot, const abl_fv_bc_desc* bc)
 
 /* ---- the hot path ------------------------------------------------------------------------------------------- */
 
-/* One RHS evaluation: ghost fill (when the mesh has boundary faces, writing the ghost rows of U), aux decode,
- * gradients + limiter, face fluxes and per-cell residual gather.  With accumulate = 1 the residual is ADDED into
- * R (reference semantics: locF is zeroed by the caller, timeStepper.cpp:363, and ComputeRHSFunction adds into
- * it); with accumulate = 0, R = rhs.  Ghost rows of R are left untouched (accumulate = 1) or zeroed. */
+/* One RHS evaluation: ghost fill of the INFLOW / OUTFLOW slots (writing those ghost rows of U, the reference's
+ * DMPlexInsertBoundaryValues step; ghost rows of ABL_FV_BC_GHOST slots are read as the caller filled them and
+ * never written), aux decode, gradients + limiter, face fluxes and per-cell residual gather.  With accumulate = 1
+ * the residual is ADDED into R (reference semantics: locF is zeroed by the caller, timeStepper.cpp:363, and
+ * ComputeRHSFunction adds into it); with accumulate = 0, R = rhs.  Ghost rows of R are left untouched
+ * (accumulate = 1) or zeroed.  `time` is the time the caller's ghost values belong to; the constant-state
+ * conditions do not depend on it.  Fails before any device work when a boundary slot has no condition. */
 int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accumulate, abl_fv_stream stream);
 
 /* Vector-Jacobian product of the RHS (build-defined; the reference has no adjoint):
- *   Ubar = Rbar^T dR/dU   (interior rows; ghost rows receive zero)
+ *   Ubar = Rbar^T dR/dU   (interior rows; the ghost rows of ABL_FV_BC_GHOST slots receive the adjoint of the
+ *   caller's ghost values, which are inputs of the RHS; other ghost rows receive zero, their dependence on the
+ *   interior (OUTFLOW, enforce_at_face) and on theta being folded into Ubar and theta_bar)
  *   theta_bar[3 * slot + {0,1,2}] += Rbar^T dR/d(rho_in, u_in, T_in) for every INFLOW slot (host array sized
  *   3 * number of slots, or NULL).  Fluxes are recomputed, not stored.  Supported for the FP64 form with the
  *   Ausm, Rusanov or Off calculators, first order or least squares with the Barth-Jespersen limiter (or none),
@@ -262,7 +276,9 @@ int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* U, abl_fv_s
  * (timeStepper.cpp:80-130, adaptPhysics.cpp:3-59): first step min(dt0, safety * cfl_dt(U0)); each step evaluates
  * the RHS into the caller's workspace R [n_comp][n_cells], takes the next step size from U_n, updates U; steps
  * that would pass max_time are cut to it.  Stops at max_time or after max_steps; reports the final time and
- * the number of steps taken. */
+ * the number of steps taken.  With a PressureGradientScaling attached, its pre-step update (abl_fv_pgs_update with the
+ * step about to be taken, registered by PressureGradientScaling::Setup as a TS pre-step, pressureGradientScaling.cpp:
+ * 144-146) runs before every step. */
 int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max_time, double dt0, double safety,
                     int64_t max_steps, double* t_out, int64_t* steps_out, abl_fv_stream stream);
 

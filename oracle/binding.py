@@ -19,6 +19,7 @@ RECON = {"upwind": 0, "none": 0, "leastsquares": 1}
 LIMITER = {"none": 0, "zero": 1, "minmod": 2, "vanleer": 3, "vanalbada": 4, "superbee": 5, "mc": 6, "sin": 7,
            "barthjespersen": 8}
 LEFT, RIGHT, NA = 1, 2, 0
+BC_NONE, BC_INFLOW, BC_OUTFLOW, BC_GHOST = 0, 1, 2, 3
 
 _dp = POINTER(c_double)
 
@@ -58,10 +59,16 @@ class OrcPhysics(ctypes.Structure):
 
 
 class OrcBc(ctypes.Structure):
-    _fields_ = [("kind", c_int), ("rho", c_double), ("vel", c_double * 3), ("T", c_double), ("Yi", _dp)]
+    _fields_ = [("kind", c_int), ("rho", c_double), ("vel", c_double * 3), ("T", c_double), ("Yi", _dp),
+                ("enforce_at_face", c_int)]
 
 
 _lib = None
+
+# Planted-mistake variants of the oracle (ORC_PLANT in fv_oracle.c): each changes one formula so that a test can
+# show that its pin rejects the mistake.
+PLANTS = {"project_sign": 1, "rusanov_min_speed": 2, "bj_neighbor": 3, "minmod_as_mc": 4, "face_value_left": 5,
+          "conduction_sign": 6}
 
 
 def build() -> Path:
@@ -69,57 +76,89 @@ def build() -> Path:
     return LIB_PATH
 
 
+def _open(path: Path) -> ctypes.CDLL:
+    L = ctypes.CDLL(str(path))
+    _declare(L)
+    return L
+
+
+class planted:
+    """Context manager: inside it every oracle call goes to the variant with the named planted mistake."""
+
+    def __init__(self, name: str):
+        self.path = HERE / "_build" / f"liboracle_fv_plant{PLANTS[name]}.so"
+
+    def __enter__(self):
+        global _lib
+        subprocess.run(["make", "-s", "-C", str(HERE), str(self.path.relative_to(HERE))], check=True)
+        self._saved = lib()
+        _lib = _open(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+        return False
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = ctypes.CDLL(str(LIB_PATH))
-        d = c_double
-        for name in ("orc_ausm", "orc_average_flux", "orc_off_flux"):
-            getattr(L, name).argtypes = [d] * 8 + [_dp, _dp]
-            getattr(L, name).restype = c_int
-        L.orc_ausmpup.argtypes = [d] * 10 + [_dp, _dp]
-        L.orc_ausmpup.restype = c_int
-        for name, args in {
-            "orc_pg_pressure": [_dp, c_int, d, d],
-            "orc_pg_temperature": [_dp, c_int, d, d],
-            "orc_pg_internal_energy": [_dp, c_int],
-            "orc_pg_internal_energy_T": [d, d, d],
-            "orc_pg_speed_of_sound": [_dp, c_int, d, d],
-            "orc_pg_speed_of_sound_T": [_dp, d, d, d],
-            "orc_pg_pressure_T": [_dp, d, d],
-            "orc_pg_sensible_enthalpy_T": [d, d, d],
-            "orc_pg_cv": [d, d],
-            "orc_pg_cp": [d, d],
-            "orc_limiter": [c_int, d],
-        }.items():
-            getattr(L, name).argtypes = args
-            getattr(L, name).restype = d
-        L.orc_ns_advection_flux.argtypes = [c_int, _dp, _dp, _dp, d, d, c_int, _dp]
-        L.orc_ns_advection_flux.restype = c_int
-        L.orc_stress_tensor.argtypes = [c_int, d, _dp, _dp]
-        L.orc_stress_tensor.restype = None
-        L.orc_lsq_weights.argtypes = [c_int, c_int, _dp, _dp]
-        L.orc_lsq_weights.restype = c_int
-        L.orc_rhs.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(OrcBc), c_int, d, _dp, _dp]
-        L.orc_rhs.restype = c_int
-        L.orc_create.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(ctypes.c_void_p)]
-        L.orc_create.restype = c_int
-        L.orc_rhs_ctx.argtypes = [ctypes.c_void_p, POINTER(OrcBc), c_int, d, _dp, _dp]
-        L.orc_rhs_ctx.restype = c_int
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_destroy.restype = None
-        L.orc_cell_gradients.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp]
-        L.orc_cell_gradients.restype = c_int
-        L.orc_min_cell_radius.argtypes = [POINTER(OrcMesh), _dp]
-        L.orc_min_cell_radius.restype = None
-        L.orc_cfl_dt.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), d, _dp, _dp]
-        L.orc_cfl_dt.restype = d
-        L.orc_pgs_update.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, d, d, d, d, _dp, _dp]
-        L.orc_pgs_update.restype = None
-        _lib = L
+        _lib = _open(LIB_PATH)
     return _lib
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    """Argument and result types of the oracle's exported functions."""
+    d = c_double
+    for name in ("orc_ausm", "orc_average_flux", "orc_off_flux"):
+        getattr(L, name).argtypes = [d] * 8 + [_dp, _dp]
+        getattr(L, name).restype = c_int
+    L.orc_ausmpup.argtypes = [d] * 10 + [_dp, _dp]
+    L.orc_ausmpup.restype = c_int
+    for name, args in {
+        "orc_pg_pressure": [_dp, c_int, d, d],
+        "orc_pg_temperature": [_dp, c_int, d, d],
+        "orc_pg_internal_energy": [_dp, c_int],
+        "orc_pg_internal_energy_T": [d, d, d],
+        "orc_pg_speed_of_sound": [_dp, c_int, d, d],
+        "orc_pg_speed_of_sound_T": [_dp, d, d, d],
+        "orc_pg_pressure_T": [_dp, d, d],
+        "orc_pg_sensible_enthalpy_T": [d, d, d],
+        "orc_pg_cv": [d, d],
+        "orc_pg_cp": [d, d],
+        "orc_limiter": [c_int, d],
+    }.items():
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = d
+    L.orc_ns_advection_flux.argtypes = [c_int, _dp, _dp, _dp, d, d, c_int, _dp]
+    L.orc_ns_advection_flux.restype = c_int
+    L.orc_stress_tensor.argtypes = [c_int, d, _dp, _dp]
+    L.orc_stress_tensor.restype = None
+    L.orc_lsq_weights.argtypes = [c_int, c_int, _dp, _dp]
+    L.orc_lsq_weights.restype = c_int
+    L.orc_rhs.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(OrcBc), c_int, d, _dp, _dp]
+    L.orc_rhs.restype = c_int
+    L.orc_create.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), POINTER(ctypes.c_void_p)]
+    L.orc_create.restype = c_int
+    L.orc_rhs_ctx.argtypes = [ctypes.c_void_p, POINTER(OrcBc), c_int, d, _dp, _dp]
+    L.orc_rhs_ctx.restype = c_int
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_destroy.restype = None
+    L.orc_cell_gradients.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp]
+    L.orc_cell_gradients.restype = c_int
+    L.orc_min_cell_radius.argtypes = [POINTER(OrcMesh), _dp]
+    L.orc_min_cell_radius.restype = None
+    L.orc_cfl_dt.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), d, _dp, _dp]
+    L.orc_cfl_dt.restype = d
+    L.orc_pgs_update.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, d, d, d, d, _dp, _dp]
+    L.orc_pgs_update.restype = None
+    L.orc_rusanov_flux.argtypes = [c_int, _dp, _dp, _dp, d, d, c_int, _dp]
+    L.orc_rusanov_flux.restype = None
+    L.orc_face_interpolate.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp, _dp, _dp, _dp]
+    L.orc_face_interpolate.restype = c_int
 
 
 def _ptr(a):
@@ -175,6 +214,7 @@ def rhs(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, bcs=None, time: float 
             v = list(b.get("vel", [0.0, 0.0, 0.0])) + [0.0] * 3
             bc_arr[slot].vel = (c_double * 3)(*v[:3])
             bc_arr[slot].T = b.get("T", 0.0)
+            bc_arr[slot].enforce_at_face = int(bool(b.get("enforce_at_face", False)))
             if b.get("Yi") is not None:
                 y = np.ascontiguousarray(b["Yi"], dtype=np.float64)
                 ykeep.append(y)
@@ -255,6 +295,35 @@ def ns_advection_flux(normal, xL, xR, gamma=1.4, R=287.0, flux="ausm"):
     out = np.zeros(dim + 2)
     lib().orc_ns_advection_flux(dim, _ptr(n), _ptr(a), _ptr(b), gamma, R, FLUX[flux], _ptr(out))
     return out
+
+
+def rusanov_flux(normal, uL, uR, gamma=1.4, R=287.0, n_species=0):
+    """Build-defined Rusanov flux of one face (area-weighted normal): euler then species components."""
+    dim = len(normal)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    a = np.ascontiguousarray(uL, dtype=np.float64)
+    b = np.ascontiguousarray(uR, dtype=np.float64)
+    out = np.zeros(dim + 2 + n_species)
+    lib().orc_rusanov_flux(dim, _ptr(n), _ptr(a), _ptr(b), gamma, R, n_species, _ptr(out))
+    return out
+
+
+def face_interpolate(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray):
+    """FaceInterpolant::GetInterpolatedFaceVectors: (faceSol [F, nc], faceSolGrad [F, nc, dim], faceAux [F, naux],
+    faceAuxGrad [F, naux, dim]) with aux = (T, velocity, Yi); ghost rows of U are used as given."""
+    nc, N = U_soa.shape
+    dim = mesh_arrays.dim
+    F = mesh_arrays.n_faces
+    na = 1 + dim + phys.n_species
+    X = np.ascontiguousarray(U_soa.T)
+    fs, fsg = np.zeros((F, nc)), np.zeros((F, nc, dim))
+    fa, fag = np.zeros((F, na)), np.zeros((F, na, dim))
+    ms, keep = _mesh_struct(mesh_arrays)
+    rc = lib().orc_face_interpolate(ctypes.byref(ms), ctypes.byref(phys), _ptr(X), _ptr(fs), _ptr(fsg), _ptr(fa),
+                                    _ptr(fag))
+    if rc:
+        raise RuntimeError(f"orc_face_interpolate failed with status {rc}")
+    return fs, fsg, fa, fag
 
 
 def stress_tensor(dim, mu, grad_vel):

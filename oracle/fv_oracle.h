@@ -27,7 +27,9 @@ enum {
     ORC_LIM_NONE = 0, ORC_LIM_ZERO, ORC_LIM_MINMOD, ORC_LIM_VANLEER, ORC_LIM_VANALBADA,
     ORC_LIM_SUPERBEE, ORC_LIM_MC, ORC_LIM_SIN, ORC_LIM_BJ
 };
-enum { ORC_BC_NONE = 0, ORC_BC_INFLOW = 1, ORC_BC_OUTFLOW = 2 };
+/* ORC_BC_GHOST: ghost values supplied by the caller in locX (a Ghost update function evaluated before the RHS,
+ * ghost.cpp:17-28, timeStepper.cpp:347-349): left untouched. */
+enum { ORC_BC_NONE = 0, ORC_BC_INFLOW = 1, ORC_BC_OUTFLOW = 2, ORC_BC_GHOST = 3 };
 
 typedef struct {
     int dim;
@@ -58,6 +60,7 @@ typedef struct {
     int kind;
     double rho, vel[3], T;
     const double* Yi;
+    int enforce_at_face; /* EssentialGhost{enforceAtFace}: ghost = 2 state - interior (essentialGhost.cpp:15-21) */
 } orc_bc;
 
 /* ---- leaf functions (pinned by the reference's unit-test tables) ---- */
@@ -93,6 +96,9 @@ void orc_stress_tensor(int dim, double mu, const double* gradVel, double* tau);
 int orc_lsq_weights(int n, int dim, const double* dx, double* grad);
 /* Symmetric-form scalar limiter phi(f) (PETSc PetscLimiterLimit_*). */
 double orc_limiter(int type, double f);
+/* Build-defined Rusanov flux of one face (area-weighted normal), euler then ns species components. */
+void orc_rusanov_flux(int dim, const double* normal, const double* uL, const double* uR, double gamma, double R, int ns,
+                      double* flux);
 
 /* Full RHS: ghost fill of locX (boundary faces with a bc slot), aux update, discontinuous (cell interpolant) and
  * continuous (face interpolant) flux functions, accumulated into locF (AoS [n_cells][dim+2+ns]).
@@ -109,6 +115,13 @@ void orc_destroy(orc_ctx* ctx);
 /* Cell gradients (after limiting) of the euler+densityYi field as the oracle computes them, AoS
  * [n_cells][totDim][dim] (zero for ghost cells). For tests of the reconstruction stage. */
 int orc_cell_gradients(const orc_mesh* mesh, const orc_physics* phys, const double* locX, double* grad);
+
+/* FaceInterpolant::GetInterpolatedFaceVectors (faceInterpolant.cpp:109-249) with the LeastSquaresAverage stencils
+ * (stencils/leastSquaresAverage.cpp:5-155): per face, the interpolated solution values [F][nc], solution gradients
+ * [F][nc][dim], aux values (T, velocity, Yi) [F][naux] and aux gradients [F][naux][dim], naux = 1 + dim + ns.
+ * Ghost values are taken from locX as given (no boundary fill). */
+int orc_face_interpolate(const orc_mesh* mesh, const orc_physics* phys, const double* locX, double* faceSol,
+                         double* faceSolGrad, double* faceAux, double* faceAuxGrad);
 
 /* Minimum cell radius per cell: min over the cell's faces of |x_cell - x_face| (the MIN_CELL_RADIUS mesh
  * characteristic, finiteVolumeSolver.cpp:185-215; every face with two supporting cells counts).  r[n_cells]. */

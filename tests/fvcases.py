@@ -53,10 +53,26 @@ class Case:
         out = []
         for slot, b in self.bcs.items():
             if b["kind"] == "inflow":
-                out.append(fv.InflowGhost([slot], b["rho"], b["vel"], b["T"], b.get("Yi")))
+                out.append(fv.InflowGhost([slot], b["rho"], b["vel"], b["T"], b.get("Yi"),
+                                          enforceAtFace=b.get("enforce_at_face", False)))
+            elif b["kind"] == "ghost":
+                out.append(fv.Ghost("euler", f"slot{slot}", [slot], b["fn"]))
             else:
                 out.append(fv.OutflowGhost([slot]))
         return out
+
+    def fill_caller_ghosts(self, U, time=0.0):
+        """Host ghost fill of the "ghost" (caller-filled) slots into the numpy state U, as the solver's
+        insert_boundary_values does on the device: euler rows from the slot's update function."""
+        a = self.mesh.arrays
+        ne = self.mesh.dim + 2
+        for slot, b in self.bcs.items():
+            if b["kind"] != "ghost":
+                continue
+            f = np.nonzero(a.face_bc == slot)[0]
+            xI = U[:ne, a.face_cells[f, 0]].T
+            U[:ne, a.face_cells[f, 1]] = np.asarray(b["fn"](time, a.face_centroid[f], a.face_normal[f], xI)).T
+        return U
 
     def solver(self, form="fp64"):
         return fv.FiniteVolumeSolver(
@@ -66,15 +82,17 @@ class Case:
         )
 
     def initial_state(self, seed=synthetic.DEFAULT_SEED):
+        """Seeded state; the ghost rows of caller-filled ("ghost") slots hold their update function's values."""
         if self.state == "random2d":
             U = synthetic.random_euler_2d(self.mesh, self.gamma, seed)
             if self.n_species:
-                Y = synthetic.dirichlet_mass_fractions(self.n_species, self.mesh.n_interior, seed)
+                Y = synthetic.dirichlet_mass_fractions(self.n_species, self.mesh.source_cells(), seed)
                 U = np.concatenate([U, np.zeros((self.n_species, U.shape[1]))])
                 U[4:, : self.mesh.n_interior] = U[0, : self.mesh.n_interior] * Y
-            return U
-        return synthetic.taylor_green(self.mesh, self.mach, self.gamma, self.Rgas, seed=seed, n_species=self.n_species,
-                                      phase=self.phase)
+            return self.fill_caller_ghosts(U)
+        U = synthetic.taylor_green(self.mesh, self.mach, self.gamma, self.Rgas, seed=seed, n_species=self.n_species,
+                                   phase=self.phase)
+        return self.fill_caller_ghosts(U)
 
     def oracle_physics(self):
         return orc.physics(self.gamma, self.Rgas, self.n_species, self.flux, self.recon, self.limiter,
@@ -82,11 +100,13 @@ class Case:
                            pgs_alpha=self.pgs_alpha or 1.0, compact=self.compact)
 
     def oracle_bcs(self):
-        kinds = {"inflow": 1, "outflow": 2}
-        return {s: dict(b, kind=kinds[b["kind"]]) for s, b in self.bcs.items()} or None
+        kinds = {"inflow": 1, "outflow": 2, "ghost": 3}
+        return {s: {k: v for k, v in dict(b, kind=kinds[b["kind"]]).items() if k != "fn"}
+                for s, b in self.bcs.items()} or None
 
     def oracle_rhs(self, U):
-        return orc.rhs(self.mesh.arrays, self.oracle_physics(), U, self.oracle_bcs())
+        """The oracle's residual and the state with its ghost rows filled (caller-filled slots by the host)."""
+        return orc.rhs(self.mesh.arrays, self.oracle_physics(), self.fill_caller_ghosts(U), self.oracle_bcs())
 
 
 def rel_err_per_field(R, Rref, n_interior):
@@ -106,6 +126,28 @@ def all_outflow(nslots):
 def inflow_x(n_species=0):
     Yi = list(np.full(n_species, 1.0 / n_species)) if n_species else None
     return {0: {"kind": "inflow", "rho": 1.2, "vel": [100.0, 0.0, 0.0], "T": 300.0, "Yi": Yi}, 1: {"kind": "outflow"}}
+
+
+def smooth_ghost(time, c, n, xI):
+    """A caller-evaluated Ghost update function (ghost.cpp:17-28 signature, vectorised): a smooth state of the face
+    position one half-normal outside the face, independent of the interior values."""
+    x = c + 0.5 * n / np.linalg.norm(n, axis=1, keepdims=True) * np.cbrt(np.abs(n).sum(axis=1, keepdims=True))
+    dim = xI.shape[1] - 2
+    rho = 1.1 + 0.05 * np.sin(2.0 * x[:, 0]) * np.cos(x[:, 1])
+    vel = np.stack([60.0 + 10.0 * np.cos(3.0 * x[:, 1]), 5.0 * np.sin(x[:, 0] + x[:, 2]), -4.0 * np.cos(x[:, 2])])[:dim]
+    p = 1.0e5 * (1.0 + 0.02 * np.sin(x[:, 0] + 2.0 * x[:, 1]))
+    out = np.empty_like(xI)
+    out[:, 0] = rho
+    out[:, 1] = p / 0.4 + 0.5 * rho * np.sum(vel * vel, axis=0)
+    out[:, 2:] = (rho * vel).T
+    return out
+
+
+def ghost_and_at_face(n_species=0):
+    """Slot 0: caller-filled Ghost (smooth_ghost); slot 1: EssentialGhost inflow with enforceAtFace."""
+    Yi = list(np.linspace(1.0, 2.0, n_species) / np.linspace(1.0, 2.0, n_species).sum()) if n_species else None
+    return {0: {"kind": "ghost", "fn": smooth_ghost},
+            1: {"kind": "inflow", "rho": 1.15, "vel": [-40.0, 3.0, 0.0], "T": 310.0, "Yi": Yi, "enforce_at_face": True}}
 
 
 def standard_cases():
@@ -137,6 +179,10 @@ def standard_cases():
                       mu=0.5, k=5.0, diff=0.3, bcs=inflow_x(3)))
     cases.append(Case("rusanov_species_visc", chan(6, "rcm"), flux="rusanov", n_species=2, recon="leastsquares",
                       limiter="minmod", mu=0.5, k=5.0, diff=0.2, bcs=inflow_x(2)))
+    cases.append(Case("caller_ghost_at_face", chan(6), n_species=2, recon="leastsquares", limiter="barthjespersen",
+                      mu=0.5, k=5.0, diff=0.3, bcs=ghost_and_at_face(2)))
+    cases.append(Case("caller_ghost_at_face_2d_first", BoxMesh([9, 7], [0, 0], [1, 1], [False, True]),
+                      state="random2d", bcs=ghost_and_at_face()))
     cases.append(Case("box_all_walls", BoxMesh([5, 6, 7], [0, 0, 0], [1, 2, 3], [False, False, False]),
                       recon="leastsquares", limiter="vanleer", mu=0.3, k=2.0, bcs=all_outflow(6)))
     cases.append(Case("quad_walls_species", BoxMesh([10, 7], [0, 0], [2, 1], [False, True]), n_species=2,

@@ -1,6 +1,6 @@
 """AUSM+up and the pressure-gradient scaling on the GPU against the oracle: the FP64 RHS parity of the AusmpUp
 cases runs with the standard cases (test_gpu_parity.py); here the PGS pre-step update, the alpha it feeds back into
-the flux, a mixed-form request (created in the FP64 form) and the adjoint's refusal."""
+the flux, the refusal of a mixed-form request and of the adjoint."""
 import numpy as np
 import pytest
 
@@ -80,24 +80,22 @@ def test_pgs_update_requires_attachment(torch_cuda):
     assert rc != 0 and b"PressureGradientScaling" in _lib.lib().abl_fv_last_error(s.handle)
 
 
-@pytest.mark.parametrize("minf,alpha", [(0.0, None), (0.2, 2.0)])
-def test_ausmpup_mixed_request_runs_fp64(minf, alpha, torch_cuda):
-    """The mixed form has no perturbation form of AusmpUp: a mixed request is created in the FP64 form and matches
-    the oracle to the FP64 tolerance."""
-    torch = torch_cuda
-    case = Case("ausmpup_mixed", BoxMesh([10, 10, 10], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup", minf=minf,
-                pgs_alpha=alpha, recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125)
-    U0 = case.initial_state()
-    Rref, _ = case.oracle_rhs(U0.copy())
-    s = case.solver(form="mixed")
-    assert s.form == "fp64"
-    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
-    R = torch.zeros_like(U)
-    s.compute_rhs(0.0, U, R)
-    torch.cuda.synchronize()
-    err = rel_err_per_field(R.cpu().numpy(), Rref, case.mesh.n_interior)
-    print(f"[ausmpup] mixed request error {err}", flush=True)
-    assert err.max() <= 1e-12
+def test_ausmpup_mixed_request_is_refused(torch_cuda):
+    """The mixed form has no perturbation form of AusmpUp: a mixed request fails loudly, in the Python mirror and
+    at the C ABI, instead of running another form."""
+    import ctypes
+
+    from ablate_amd import _lib
+
+    case = Case("ausmpup_mixed", BoxMesh([6, 6, 6], [0, 0, 0], [1, 1, 1], [True] * 3), flux="ausmpup", minf=0.2,
+                recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125)
+    with pytest.raises(ValueError, match="AusmpUp"):
+        case.solver(form="mixed")
+    phys = case.solver().phys  # the FP64 form is created as usual
+    h = ctypes.c_void_p()
+    rc = _lib.lib().abl_fv_create(ctypes.byref(case.mesh.desc), ctypes.byref(phys), _lib.FORM_MIXED, ctypes.byref(h))
+    assert rc != 0 and not h.value
+    assert b"AusmpUp" in _lib.lib().abl_fv_last_error(None)
 
 
 def test_ausmpup_adjoint_is_refused(torch_cuda):

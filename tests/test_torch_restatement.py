@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from ablate_amd.mesh import BoxMesh, TetBoxMesh
-from tests.fvcases import Case, all_outflow, inflow_x, rel_err_per_field
+from tests.fvcases import Case, all_outflow, ghost_and_at_face, inflow_x, rel_err_per_field
 from tests.torch_rhs import TorchRHS
 
 
@@ -19,6 +19,7 @@ def _cases():
         Case("c4_like_rusanov", chan, flux="rusanov", n_species=2, bcs=inflow_x(2), phase=PHASE),
         Case("periodic_ausm", BoxMesh([5, 5, 5], [0, 0, 0], [1, 1, 1], [True] * 3), phase=PHASE),
         Case("tets_rusanov", TetBoxMesh(3, 0.2), flux="rusanov", n_species=2, bcs=all_outflow(6), phase=PHASE),
+        Case("caller_ghost_at_face", chan, n_species=2, bcs=ghost_and_at_face(2), phase=PHASE),
     ]
 
 
@@ -31,15 +32,23 @@ def thetas_of(case):
     return out
 
 
-def torch_model(case):
+def torch_bcs(case):
+    """Boundary slots of a case in the restatements' form (inflow direction / Yi / at-face, outflow, caller ghost)."""
     bcs = {}
     for slot, b in case.bcs.items():
         if b["kind"] == "inflow":
             um = float(np.linalg.norm(b["vel"]))
-            bcs[slot] = {"kind": "inflow", "dir": [v / um for v in b["vel"]], "Yi": b.get("Yi")}
+            bcs[slot] = {"kind": "inflow", "dir": [v / um for v in b["vel"]], "Yi": b.get("Yi"),
+                         "at_face": bool(b.get("enforce_at_face", False))}
+        elif b["kind"] == "ghost":
+            bcs[slot] = {"kind": "ghost"}
         else:
             bcs[slot] = {"kind": "outflow"}
-    return TorchRHS(case.mesh, case.gamma, case.Rgas, case.n_species, case.flux, bcs)
+    return bcs
+
+
+def torch_model(case):
+    return TorchRHS(case.mesh, case.gamma, case.Rgas, case.n_species, case.flux, torch_bcs(case))
 
 
 @pytest.mark.parametrize("case", _cases(), ids=lambda c: c.name)
@@ -90,21 +99,16 @@ def _cases2():
         Case("quad2d_species", BoxMesh([7, 6], [0, 0], [2, 1], [False, True]), n_species=2, recon="leastsquares",
              limiter="barthjespersen", mu=0.2, k=1.0, diff=0.1, bcs=inflow_x(2), phase=PHASE),
         Case("viscous_first_order", chan, n_species=2, mu=0.5, k=5.0, diff=0.3, bcs=inflow_x(2), phase=PHASE),
+        Case("caller_ghost_at_face_lsq_bj", chan, n_species=2, recon="leastsquares", limiter="barthjespersen", mu=0.5,
+             k=5.0, diff=0.3, bcs=ghost_and_at_face(2), phase=PHASE),
     ]
 
 
 def torch_model2(case):
     from tests.torch_rhs import TorchRHS2
 
-    bcs = {}
-    for slot, b in case.bcs.items():
-        if b["kind"] == "inflow":
-            um = float(np.linalg.norm(b["vel"]))
-            bcs[slot] = {"kind": "inflow", "dir": [v / um for v in b["vel"]], "Yi": b.get("Yi")}
-        else:
-            bcs[slot] = {"kind": "outflow"}
-    return TorchRHS2(case.mesh, case.gamma, case.Rgas, case.n_species, case.flux, bcs, case.limiter, case.mu, case.k,
-                     case.diff, recon=case.recon == "leastsquares")
+    return TorchRHS2(case.mesh, case.gamma, case.Rgas, case.n_species, case.flux, torch_bcs(case), case.limiter,
+                     case.mu, case.k, case.diff, recon=case.recon == "leastsquares")
 
 
 @pytest.mark.parametrize("case", _cases2(), ids=lambda c: c.name)

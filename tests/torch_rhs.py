@@ -28,13 +28,16 @@ class TorchRHS:
         self.device = device
 
     def ghost_state(self, U, thetas):
-        """Fills ghost rows: inflow from theta = (rho, |u|, T) and the slot's direction / Yi; outflow copies."""
+        """Fills ghost rows: inflow from theta = (rho, |u|, T) and the slot's direction / Yi (with "at_face": twice
+        that state minus the interior); outflow copies; "ghost" (caller-filled) rows are inputs and stay as given."""
         U = U.clone()
         for slot in np.unique(self.bslot):
             fsel = self.bface[self.bslot == slot]
             g = self.R[fsel]
             i = self.L[fsel]
             b = self.bcs[int(slot)]
+            if b["kind"] == "ghost":
+                continue
             if b["kind"] == "outflow":
                 U[:, g] = U[:, i]
             else:
@@ -46,6 +49,8 @@ class TorchRHS:
                 vals += [rho * Y[k] for k in range(self.ns)]
                 col = torch.stack(vals)
                 U[:, g] = col[:, None].expand(-1, g.numel())
+                if b.get("at_face"):
+                    U[:, g] = 2.0 * U[:, g] - U[:, i]
         return U
 
     def decode(self, u):
