@@ -48,18 +48,23 @@ def _run(cmd: list[str], verbose: bool) -> None:
 
 
 def build(verbose: bool = False) -> Path:
-    """Compile the native library for gfx950; returns its path."""
+    """Compile the native library for gfx950 (translation units in parallel); returns its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     BUILD_DIR.mkdir(exist_ok=True)
-    objs = []
+    hipcc = _hipcc()
+    steps, objs = [], []
+    for src in HIP_SOURCES:  # the long HIP compiles first
+        obj = BUILD_DIR / (src + ".o")
+        steps.append([hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
     for src in HOST_SOURCES:
         obj = BUILD_DIR / (src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        steps.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))
-    hipcc = _hipcc()
-    for src in HIP_SOURCES:
-        obj = BUILD_DIR / (src + ".o")
-        _run([hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
-        objs.append(str(obj))
+    with ThreadPoolExecutor(max_workers=min(8, len(steps))) as pool:
+        for fut in [pool.submit(_run, cmd, verbose) for cmd in steps]:
+            fut.result()
     tmp = LIB_PATH.with_name(LIB_PATH.name + ".tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs], verbose)
     os.replace(tmp, LIB_PATH)

@@ -1047,11 +1047,26 @@ __device__ inline void side_perturbation(const float* dU, const BaselineF<DIM>& 
     sp.dH = deps + R * sp.dT;
 }
 
+// Perturbation of the physical flux f.n = (m, m H, m u + p n), m = rho u_n, for a side perturbation sp of the
+// baseline (conserved perturbation already decoded by side_perturbation): dm = drho u_n + rho' du_n,
+// d(m H) = dm H + m' dH, d(m u + p n) = dm u + m' du + dp n.
+template <int DIM>
+__device__ inline void phys_flux_pert(const SidePert<DIM>& sp, const BaselineF<DIM>& b, float unc, float mdot_c,
+                                      const float* nhat, float* out) {
+    const float dm = sp.drho * unc + sp.rho * sp.dun;
+    const float mp = mdot_c + dm;
+    out[kRho] = dm;
+    out[kRhoE] = dm * b.H + mp * sp.dH;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) out[kRhoU + d] = dm * b.vel[d] + mp * sp.du[d] + sp.dp * nhat[d];
+}
+
 // Mixed-form flux of face f with both baselines: dFL = F - F(U_L).n A and dFR = F - F(U_R).n A (undefined for a
 // ghost right cell).  The face flux is formed as a perturbation of the left cell's state; the right baseline
 // differs by the physical-flux perturbation of the cell difference U_R - U_L, also formed in FP32 from exact
-// identities (or in FP64 on the FP64 route).
-template <int DIM, bool RECON, bool VISC, bool SPECIES>
+// identities (or in FP64 on the FP64 route).  RUS: the Rusanov calculator, always in FP32 perturbation form (it has
+// no supersonic branch), compiled without the FP64 route.
+template <int DIM, bool RECON, bool VISC, bool SPECIES, bool RUS>
 __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, const double* __restrict__ U,
                                   const float* __restrict__ rec, const float* __restrict__ gS,
                                   const float* __restrict__ gY, int64_t f, float* dFL, float* dFR, float* sFL,
@@ -1127,7 +1142,49 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
         dPhi[p] = 0.f;
     }
     const bool subsonic = fabsf(sL.M) <= 1.f && fabsf(sR.M) <= 1.f && fabsf(Mc) <= 1.f;
-    if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
+    if constexpr (RUS) {
+        // Rusanov (build-defined) as a perturbation of the left baseline: F - f(U_L).n A =
+        // A [1/2 (df(dL) + df(dR)) - 1/2 lambda (dR - dL)], lambda = max(|u_n| + a) of the face states
+        const float mdot_c = unc * bf.rho;
+        float dfl[NE], dfr[NE];
+        phys_flux_pert<DIM>(sL, bf, unc, mdot_c, nhat, dfl);
+        phys_flux_pert<DIM>(sR, bf, unc, mdot_c, nhat, dfr);
+        const float unL = unc + sL.dun, unR = unc + sR.dun;
+        const float spL = fabsf(unL) + sL.a, spR = fabsf(unR) + sR.a;
+        const float lam = spL < spR ? spR : spL;  // pmax order, as rusanov_flux
+#pragma unroll
+        for (int p = 0; p < NE; p++) dF[p] = area * (0.5f * (dfl[p] + dfr[p]) - 0.5f * lam * (dR[p] - dL[p]));
+        SidePert<DIM> sC;
+        if (!ghost) {
+            side_perturbation<DIM>(dcell, bf, nhat, unc, Mc, gR, Rg, icv, sC);
+            phys_flux_pert<DIM>(sC, bf, unc, mdot_c, nhat, dPhi);
+#pragma unroll
+            for (int p = 0; p < NE; p++) dPhi[p] *= area;
+        }
+        if (SPECIES) {
+            // F_sp = A [1/2 (rhoY_L u_n,L + rhoY_R u_n,R) - 1/2 lambda (rhoY_R - rhoY_L)] relative to rhoY_c u_n,c:
+            // A [rhoY_c 1/2 (du_n,L + du_n,R) + (1/2 u_n,L + 1/2 lambda) dsL + (1/2 u_n,R - 1/2 lambda) dsR]
+            const float cl = 0.5f * unL + 0.5f * lam, cr = 0.5f * unR - 0.5f * lam;
+            const float dun2 = 0.5f * (sL.dun + sR.dun);
+            for (int sp = 0; sp < ph.ns; sp++) {
+                const int64_t comp = NE + sp;
+                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+                const float dcs = float(yr - yl);
+                float dsl = 0.f, dsr = dcs;
+                if (RECON)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
+                    }
+                const float adv = area * (float(yl) * dun2 + cl * dsl + cr * dsr);
+                float shift = 0.f;
+                if (!ghost) shift = (dcs * unc + (float(yl) + dcs) * sC.dun) * area;
+                sFL[sp * s_stride] = adv;
+                sFR[sp * s_stride] = adv - shift;
+            }
+        }
+    } else if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
         const float dm = 0.25f * sL.dM * (sL.M + Mc + 2.f) - 0.25f * sR.dM * (sR.M + Mc - 2.f);
         const float mm = Mc + dm;
         const SidePert<DIM>& up = mm < 0.f ? sR : sL;
@@ -1149,12 +1206,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
             // F(U_R).n - F(U_L).n from the cell difference: m = rho u_n, F = (m, m H, m u + p n)
             SidePert<DIM> sC;
             side_perturbation<DIM>(dcell, bf, nhat, unc, Mc, gR, Rg, icv, sC);
-            const float dmc = sC.drho * unc + sC.rho * sC.dun;
-            const float mcp = mdot_c + dmc;
-            dPhi[kRho] = dmc;
-            dPhi[kRhoE] = dmc * bf.H + mcp * sC.dH;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) dPhi[kRhoU + d] = dmc * bf.vel[d] + mcp * sC.du[d] + sC.dp * nhat[d];
+            phys_flux_pert<DIM>(sC, bf, unc, mdot_c, nhat, dPhi);
 #pragma unroll
             for (int p = 0; p < NE; p++) dPhi[p] *= area;
         }
@@ -1183,7 +1235,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
                 sFR[sp * s_stride] = adv - shift;
             }
         }
-    } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF) {
+    } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF && ph.flux != ABL_FV_FLUX_RUSANOV) {
         // FP64 route: full face flux of the reconstructed states minus the FP64 baselines
         double uLd[NE], uRd[NE], nhd[DIM], fl[NE];
 #pragma unroll
@@ -1196,12 +1248,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
         for (int d = 0; d < DIM; d++) nhd[d] = double(nhat[d]);
         const double ad = double(area);
         double clo = 0.0, cro = 0.0;  // species flux coefficients: F_sp = clo rhoY_L,f + cro rhoY_R,f
-        if (ph.flux == ABL_FV_FLUX_RUSANOV) {
-            double unL, unR, lam;
-            rusanov_flux<double, DIM>(uLd, uRd, nhd, ad, ph, fl, unL, unR, lam);
-            clo = ad * (0.5 * unL + 0.5 * lam);
-            cro = ad * (0.5 * unR - 0.5 * lam);
-        } else {
+        {
             double mA, rlo, rro;
             int dir;
             advection_flux<double, DIM, false>(ph.flux, uLd, uRd, nhd, ad, ph, fl, mA, dir, rlo, rro);
@@ -1324,7 +1371,7 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 
 // Mixed form, cell-centric (strategy "recompute"): each cell evaluates its faces and keeps its own side's
 // perturbation; accumulation in FP64.  Euler components only (species need the tile strategy).
-template <int DIM, bool RECON, bool VISC>
+template <int DIM, bool RECON, bool VISC, bool RUS>
 __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                        const float* __restrict__ rec, double* __restrict__ R,
                                                        int accumulate) {
@@ -1339,7 +1386,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
         const int32_t e = m.ell[k * NI + c];
         if (e < 0) break;
         float dFL[NE], dFR[NE];
-        mixed_face<DIM, RECON, VISC, false>(m, ph, U, rec, nullptr, nullptr, (int64_t)(e >> 1), dFL, dFR, nullptr,
+        mixed_face<DIM, RECON, VISC, false, RUS>(m, ph, U, rec, nullptr, nullptr, (int64_t)(e >> 1), dFL, dFR, nullptr,
                                             nullptr, 0);
         if (e & 1) {
 #pragma unroll
@@ -1360,7 +1407,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_mixed(MeshView<float> m, Phys p
 // Mixed form, tile strategy: each face of a tile evaluated once into LDS (both baselines, FP32: rows [0, NE + ns)
 // left-baseline perturbations, rows [NE + ns, 2 (NE + ns)) right-baseline ones), gathered per cell in FP64 in
 // face-number order.
-template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM>
+template <int DIM, bool RECON, bool VISC, bool SPECIES, int KM, bool RUS>
 __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                               const float* __restrict__ rec,
                                                               const float* __restrict__ gS,
@@ -1377,7 +1424,7 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
     const int32_t f0 = tstart[t], nf = tstart[t + 1] - f0;
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         float dFL[NE], dFR[NE];
-        mixed_face<DIM, RECON, VISC, SPECIES>(m, ph, U, rec, gS, gY, (int64_t)tfaces[f0 + i], dFL, dFR,
+        mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, (int64_t)tfaces[f0 + i], dFL, dFR,
                                               lf + NE * nfm + i, lf + (nrow + NE) * nfm + i, nfm);
 #pragma unroll
         for (int p = 0; p < NE; p++) {
@@ -1410,7 +1457,7 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
 
 // Mixed form, face-flux strategy, pass 1: one lane per face stores both sides' FP32 perturbations as rows
 // [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.
-template <int DIM, bool RECON, bool VISC, bool SPECIES>
+template <int DIM, bool RECON, bool VISC, bool SPECIES, bool RUS>
 __global__ __launch_bounds__(kBlock) void k_face_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                             const float* __restrict__ rec, const float* __restrict__ gS,
                                                             const float* __restrict__ gY, float* __restrict__ Fm,
@@ -1420,7 +1467,7 @@ __global__ __launch_bounds__(kBlock) void k_face_flux_mixed(MeshView<float> m, P
     if (f >= m.F) return;
     const int64_t F = m.F;
     float dFL[NE], dFR[NE];
-    mixed_face<DIM, RECON, VISC, SPECIES>(m, ph, U, rec, gS, gY, f, dFL, dFR, Fm + NE * F + f,
+    mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, f, dFL, dFR, Fm + NE * F + f,
                                           Fm + (nrow + NE) * F + f, F);
 #pragma unroll
     for (int p = 0; p < NE; p++) {
@@ -2350,6 +2397,87 @@ __global__ __launch_bounds__(kBlock) void k_adj_theta(int64_t nb, int dim, int n
     contrib[3 * i + 2] = t2;
 }
 
+// FaceInterpolant::GetInterpolatedFaceVectors (faceInterpolant.cpp:109-249) with the LeastSquaresAverage stencils
+// (stencils/leastSquaresAverage.cpp:5-155), one lane per face: value 1/2 (q_L + q_R) and gradient 1/2 (G_L + G_R) of
+// every solution component and every aux field (T, velocity, Yi), G_c the unlimited least-squares gradient over all
+// face neighbours of c (the face-stencil weight set the viscous fluxes use) and, for a ghost, its one-neighbour
+// gradient w_g (q_interior - q_ghost).  Outputs [q][F] and [q * DIM + d][F]; null outputs are skipped.
+template <int DIM, int KM>
+__device__ inline double face_interp_value(const double* __restrict__ U, int64_t N, int nc, const Phys& ph, int q,
+                                           int64_t c) {
+    if (q < nc) return U[(int64_t)q * N + c];
+    const double rho = U[c], irho = 1.0 / rho;
+    const int a = q - nc;
+    if (a == 0) {  // PerfectGas temperature (perfectGas.cpp:76-86)
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double v = U[(int64_t)(kRhoU + d) * N + c] * irho;
+            ke += v * v;
+        }
+        return (U[(int64_t)kRhoE * N + c] * irho - 0.5 * ke) * ph.icv;
+    }
+    if (a <= DIM) return U[(int64_t)(kRhoU + a - 1) * N + c] * irho;
+    return U[(int64_t)(DIM + 2 + a - 1 - DIM) * N + c] * irho;
+}
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_face_interpolate(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                             int nc, double* __restrict__ fsol,
+                                                             double* __restrict__ fsolg, double* __restrict__ faux,
+                                                             double* __restrict__ fauxg) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    const int64_t N = m.N, NI = m.NI, F = m.F;
+    const int64_t L = m.fcell[f], R = m.fcell[F + f];
+    const bool ghost = R >= NI;
+    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+    const int nq = nc + 1 + DIM + ph.ns;
+    for (int q = 0; q < nq; q++) {
+        const double vL = face_interp_value<DIM, KM>(U, N, nc, ph, q, L);
+        const double vR = face_interp_value<DIM, KM>(U, N, nc, ph, q, R);
+        double g[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) g[d] = 0.0;
+        for (int sd = 0; sd < 2; sd++) {
+            const int64_t c = sd == 0 ? L : R;
+            const double vc = sd == 0 ? vL : vR;
+            if (sd == 1 && ghost) {
+#pragma unroll
+                for (int d = 0; d < DIM; d++) g[d] += 0.5 * (m.fwa[2 * DIM * f + DIM + d] * (vL - vR));
+                continue;
+            }
+            double gc[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) gc[d] = 0.0;
+            for (int k = 0; k < KM; k++) {
+                const int32_t e = m.snbr[k * NI + c];
+                if (e < 0) continue;
+                const double dv = face_interp_value<DIM, KM>(U, N, nc, ph, q, e) - vc;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gc[d] += wa_arr[(k * DIM + d) * NI + c] * dv;
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++) g[d] += 0.5 * gc[d];
+        }
+        double* val = q < nc ? fsol : faux;
+        double* grd = q < nc ? fsolg : fauxg;
+        const int qq = q < nc ? q : q - nc;
+        if (val) val[(int64_t)qq * F + f] = 0.5 * vL + 0.5 * vR;
+        if (grd)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) grd[((int64_t)qq * DIM + d) * F + f] = g[d];
+    }
+}
+using FaceInterpKernel = void (*)(MeshView<double>, Phys, const double*, int, double*, double*, double*, double*);
+template <int DIM>
+FaceInterpKernel pick_face_interpolate(int K) {
+    switch (K <= 4 ? 4 : (K <= 6 ? 6 : 8)) {
+        case 4: return k_face_interpolate<DIM, 4>;
+        case 6: return k_face_interpolate<DIM, 6>;
+        default: return k_face_interpolate<DIM, 8>;
+    }
+}
+
 // ---- compile-time variant selection
 using AdjFacesKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
                                 const double*, AdjBufs);
@@ -2438,35 +2566,47 @@ FaceFluxKernel pick_face_flux(bool recon, bool visc, bool species) {
     return recon ? pick_face_flux2<DIM, true>(visc, species) : pick_face_flux2<DIM, false>(visc, species);
 }
 using MixedFluxKernel = void (*)(MeshView<float>, Phys, const double*, const float*, double*, int);
+template <int DIM, bool RUS>
+MixedFluxKernel pick_flux_mixed2(bool recon, bool visc) {
+    if (recon) return visc ? k_flux_mixed<DIM, true, true, RUS> : k_flux_mixed<DIM, true, false, RUS>;
+    return visc ? k_flux_mixed<DIM, false, true, RUS> : k_flux_mixed<DIM, false, false, RUS>;
+}
 template <int DIM>
-MixedFluxKernel pick_flux_mixed(bool recon, bool visc) {
-    if (recon) return visc ? k_flux_mixed<DIM, true, true> : k_flux_mixed<DIM, true, false>;
-    return visc ? k_flux_mixed<DIM, false, true> : k_flux_mixed<DIM, false, false>;
+MixedFluxKernel pick_flux_mixed(bool recon, bool visc, bool rus) {
+    return rus ? pick_flux_mixed2<DIM, true>(recon, visc) : pick_flux_mixed2<DIM, false>(recon, visc);
 }
 using TileMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, const float*,
                                  const int32_t*, const int32_t*, const int32_t*, int, double*, int);
-template <int DIM, bool RECON, bool VISC, bool SPECIES>
+template <int DIM, bool RECON, bool VISC, bool SPECIES, bool RUS>
 TileMixedKernel pick_tile_mixed3(int K) {
     switch (slot_width(K)) {
-        case 4: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 4>;
-        case 6: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 6>;
-        default: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 8>;
+        case 4: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 4, RUS>;
+        case 6: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 6, RUS>;
+        default: return k_tile_flux_mixed<DIM, RECON, VISC, SPECIES, 8, RUS>;
     }
 }
-template <int DIM, bool RECON, bool VISC>
+template <int DIM, bool RECON, bool VISC, bool RUS>
 TileMixedKernel pick_tile_mixed2(bool species, int K) {
-    return species ? pick_tile_mixed3<DIM, RECON, VISC, true>(K) : pick_tile_mixed3<DIM, RECON, VISC, false>(K);
+    return species ? pick_tile_mixed3<DIM, RECON, VISC, true, RUS>(K) : pick_tile_mixed3<DIM, RECON, VISC, false, RUS>(K);
 }
 using FaceFluxMixedKernel = void (*)(MeshView<float>, Phys, const double*, const float*, const float*, const float*,
                                      float*, int);
-template <int DIM, bool RECON, bool VISC>
+template <int DIM, bool RECON, bool VISC, bool RUS>
 FaceFluxMixedKernel pick_face_flux_mixed2(bool species) {
-    return species ? k_face_flux_mixed<DIM, RECON, VISC, true> : k_face_flux_mixed<DIM, RECON, VISC, false>;
+    return species ? k_face_flux_mixed<DIM, RECON, VISC, true, RUS> : k_face_flux_mixed<DIM, RECON, VISC, false, RUS>;
+}
+template <int DIM, bool RUS>
+FaceFluxMixedKernel pick_face_flux_mixed1(bool recon, bool visc, bool species) {
+    if (recon)
+        return visc ? pick_face_flux_mixed2<DIM, true, true, RUS>(species)
+                    : pick_face_flux_mixed2<DIM, true, false, RUS>(species);
+    return visc ? pick_face_flux_mixed2<DIM, false, true, RUS>(species)
+                : pick_face_flux_mixed2<DIM, false, false, RUS>(species);
 }
 template <int DIM>
-FaceFluxMixedKernel pick_face_flux_mixed(bool recon, bool visc, bool species) {
-    if (recon) return visc ? pick_face_flux_mixed2<DIM, true, true>(species) : pick_face_flux_mixed2<DIM, true, false>(species);
-    return visc ? pick_face_flux_mixed2<DIM, false, true>(species) : pick_face_flux_mixed2<DIM, false, false>(species);
+FaceFluxMixedKernel pick_face_flux_mixed(bool recon, bool visc, bool species, bool rus) {
+    return rus ? pick_face_flux_mixed1<DIM, true>(recon, visc, species)
+               : pick_face_flux_mixed1<DIM, false>(recon, visc, species);
 }
 using GatherMixedKernel = void (*)(MeshView<float>, int, const float*, double*, int);
 template <int DIM>
@@ -2477,11 +2617,15 @@ GatherMixedKernel pick_gather_mixed(int K) {
         default: return k_gather_mixed<DIM, 8>;
     }
 }
-template <int DIM>
-TileMixedKernel pick_tile_mixed(bool recon, bool visc, bool species, int K) {
+template <int DIM, bool RUS>
+TileMixedKernel pick_tile_mixed1(bool recon, bool visc, bool species, int K) {
     if (recon)
-        return visc ? pick_tile_mixed2<DIM, true, true>(species, K) : pick_tile_mixed2<DIM, true, false>(species, K);
-    return visc ? pick_tile_mixed2<DIM, false, true>(species, K) : pick_tile_mixed2<DIM, false, false>(species, K);
+        return visc ? pick_tile_mixed2<DIM, true, true, RUS>(species, K) : pick_tile_mixed2<DIM, true, false, RUS>(species, K);
+    return visc ? pick_tile_mixed2<DIM, false, true, RUS>(species, K) : pick_tile_mixed2<DIM, false, false, RUS>(species, K);
+}
+template <int DIM>
+TileMixedKernel pick_tile_mixed(bool recon, bool visc, bool species, int K, bool rus) {
+    return rus ? pick_tile_mixed1<DIM, true>(recon, visc, species, K) : pick_tile_mixed1<DIM, false>(recon, visc, species, K);
 }
 using VjpKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const int32_t*, const int32_t*,
                           double*);
@@ -3110,17 +3254,18 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
                 s->gS_f.alloc((size_t)L.ns * 2 * L.dim * N);
                 HIP_CHECK(hipMemset(s->gS_f.ptr, 0, s->gS_f.bytes()));
             }
+            const bool rus_m = p.flux_calculator == ABL_FV_FLUX_RUSANOV;
             if (mesh->dim == 3) {
                 s->grad_kernel_f = abl::pick_gradients<float, 3>(s->recon, s->visc, lim, L.K);
-                s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc);
-                s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, species, L.K);
-                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<3>(s->recon, s->visc, species);
+                s->flux_kernel_mixed = abl::pick_flux_mixed<3>(s->recon, s->visc, rus_m);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<3>(s->recon, s->visc, species, L.K, rus_m);
+                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<3>(s->recon, s->visc, species, rus_m);
                 s->gather_kernel_mixed = abl::pick_gather_mixed<3>(L.K);
             } else {
                 s->grad_kernel_f = abl::pick_gradients<float, 2>(s->recon, s->visc, lim, L.K);
-                s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc);
-                s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, species, L.K);
-                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<2>(s->recon, s->visc, species);
+                s->flux_kernel_mixed = abl::pick_flux_mixed<2>(s->recon, s->visc, rus_m);
+                s->tile_kernel_mixed = abl::pick_tile_mixed<2>(s->recon, s->visc, species, L.K, rus_m);
+                s->face_flux_kernel_mixed = abl::pick_face_flux_mixed<2>(s->recon, s->visc, species, rus_m);
                 s->gather_kernel_mixed = abl::pick_gather_mixed<2>(L.K);
             }
         }
@@ -3469,19 +3614,22 @@ extern "C" int abl_fv_get_phase_times(abl_fv_handle h, double* ms, int64_t* laun
 }
 
 namespace {
-// theta_bar: per-slot sums of the per-boundary-face contributions reduced on the device in a fixed order
-// (deterministic, k_theta_reduce); only the 3 sums of each slot come back, and are added into the caller's array
-// for the slots that have boundary faces.
-void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
-    if (!h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
+// theta_bar: per-slot sums of the per-boundary-face contributions (contrib [3 nb]) reduced on the device in a fixed
+// order (deterministic, k_theta_reduce) into sum_dev [3 kMaxBcSlots]; only those sums come back, and are added into
+// the caller's array for the slots that have boundary faces.
+void reduce_theta_into(abl_fv_handle h, hipStream_t st, const double* contrib, double* sum_dev, double* theta_bar) {
     hipLaunchKernelGGL(abl::k_theta_reduce, dim3(3 * abl::kMaxBcSlots), dim3(abl::kBlock), 0, st, h->n_bfaces,
-                       h->bslot.ptr, h->theta_contrib.ptr, h->theta_sum.ptr);
+                       h->bslot.ptr, contrib, sum_dev);
     HIP_CHECK(hipGetLastError());
     double sums[3 * abl::kMaxBcSlots];
-    HIP_CHECK(hipMemcpyAsync(sums, h->theta_sum.ptr, sizeof(sums), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(sums, sum_dev, sizeof(sums), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     for (int v : h->bc_slots)
         for (int j = 0; j < 3; j++) theta_bar[3 * v + j] += sums[3 * v + j];
+}
+void reduce_theta(abl_fv_handle h, hipStream_t st, double* theta_bar) {
+    if (!h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
+    reduce_theta_into(h, st, h->theta_contrib.ptr, h->theta_sum.ptr, theta_bar);
 }
 
 int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, hipStream_t st) {
@@ -3548,6 +3696,11 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
 }
 }  // namespace
 
+namespace {
+int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, double* contrib,
+                    double* sums, hipStream_t st, const char* who);
+}  // namespace
+
 extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
                               double* theta_bar, abl_fv_stream stream) {
     (void)time;
@@ -3565,7 +3718,24 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
         return fail(h, "abl_fv_rhs_vjp: with reconstruction the adjoint supports the Barth-Jespersen limiter or none");
     if (h->recon || h->visc) return vjp_second_order(h, U, Rbar, Ubar, theta_bar, (hipStream_t)stream);
     try {
-        hipStream_t st = (hipStream_t)stream;
+        if (theta_bar && h->n_bfaces && !h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
+        if (theta_bar && h->n_bfaces && !h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
+    }
+    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, h->theta_contrib.ptr, h->theta_sum.ptr,
+                           (hipStream_t)stream, "abl_fv_rhs_vjp");
+}
+
+namespace {
+// First-order VJP (ghost fill, ghost rows of Ubar, the cell-centric VJP kernel, theta_bar) with the theta_bar
+// workspace supplied by the caller: contrib [3 n_bfaces] and sums [3 kMaxBcSlots] device doubles (written in
+// full before they are read).
+int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, double* contrib,
+                    double* sums, hipStream_t st, const char* who) {
+    const abl::Layout& L = h->host;
+    const int fl = h->phys.flux_calculator;
+    try {
         if (h->n_bfaces) {
             h->timed(ABL_FV_PHASE_GHOST, st, [&] {
                 hipLaunchKernelGGL(abl::k_ghost_fill, dim3(grid_for(h->n_bfaces)), dim3(abl::kBlock), 0, st, h->n_bfaces,
@@ -3590,16 +3760,37 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
         }
         HIP_CHECK(hipGetLastError());
         if (theta_bar && h->n_bfaces && !noflux) {
-            if (!h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
             const auto v = h->view();
             hipLaunchKernelGGL(h->vjp_theta_kernel, dim3(grid_for(h->n_bfaces)), block, 0, st, v, h->ph, U, Rbar,
                                h->n_bfaces, h->bface.ptr, h->bslot.ptr, h->slot_kind.ptr, h->slot_param.ptr,
-                               h->slot_Y.ptr, h->theta_contrib.ptr);
+                               h->slot_Y.ptr, contrib);
             HIP_CHECK(hipGetLastError());
-            reduce_theta(h, st, theta_bar);
+            reduce_theta_into(h, st, contrib, sums, theta_bar);
         }
     } catch (const std::exception& e) {
-        return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
+        return fail(h, std::string(who) + ": " + e.what());
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" int abl_fv_face_interpolate(abl_fv_handle h, const double* U, double* face_sol, double* face_sol_grad,
+                                       double* face_aux, double* face_aux_grad, abl_fv_stream stream) {
+    if (!h || !U) return fail(h, "abl_fv_face_interpolate: null argument");
+    if (h->form != ABL_FV_FORM_FP64) return fail(h, "abl_fv_face_interpolate: FP64 form only");
+    if (!h->visc)
+        return fail(h, "abl_fv_face_interpolate: the handle has no face interpolant (no diffusion terms: k, mu and "
+                       "diff are all zero)");
+    try {
+        const abl::Layout& L = h->host;
+        hipStream_t st = (hipStream_t)stream;
+        const auto v = h->view();
+        const auto kern = L.dim == 3 ? abl::pick_face_interpolate<3>(L.K) : abl::pick_face_interpolate<2>(L.K);
+        hipLaunchKernelGGL(kern, dim3(grid_for(L.F)), dim3(abl::kBlock), 0, st, v, h->ph, U, L.nc, face_sol,
+                           face_sol_grad, face_aux, face_aux_grad);
+        HIP_CHECK(hipGetLastError());
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_face_interpolate: ") + e.what());
     }
     return 0;
 }
@@ -3807,3 +3998,4 @@ extern "C" int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_b
     }
     return 0;
 }
+

@@ -182,7 +182,8 @@ class Ghost:
     evaluates it on the host before each RHS and writes the ghost rows (the reference's boundary-value insertion,
     timeStepper.cpp:347-349); the device path treats those rows as caller-filled (ABL_FV_BC_GHOST)."""
 
-    def __init__(self, fieldName: str, boundaryName: str, labelIds, updateFunction):
+    def __init__(self, fieldName: str, boundaryName: str, labelIds, updateFunction=None):
+        """updateFunction None: the caller writes the ghost rows of U itself before each RHS (the C contract)."""
         if fieldName not in ("euler", "densityYi"):
             raise ValueError("Ghost applies to the euler or densityYi field")
         self.fieldName, self.boundaryName = fieldName, boundaryName
@@ -331,6 +332,11 @@ class FiniteVolumeSolver:
         faces = np.nonzero(a.face_bc == slot)[0]
         ne = self.dim + 2
         comps = slice(0, ne) if bc.fieldName == "euler" else slice(ne, ne + self.n_species)
+        for i, (b, c, f, _, _) in enumerate(self._ghosts):
+            if b is bc:  # one update call per condition: the faces of all its label ids together
+                f = np.concatenate([f, faces])
+                self._ghosts[i] = (b, c, f, a.face_cells[f, 0].astype(np.int64), a.face_cells[f, 1].astype(np.int64))
+                return
         self._ghosts.append((bc, comps, faces, a.face_cells[faces, 0].astype(np.int64),
                              a.face_cells[faces, 1].astype(np.int64)))
 
@@ -343,7 +349,7 @@ class FiniteVolumeSolver:
 
         a = self.mesh.arrays
         for bc, comps, faces, interior, ghosts in self._ghosts:
-            if faces.size == 0:
+            if faces.size == 0 or bc.updateFunction is None:
                 continue
             xI = U[comps][:, torch.as_tensor(interior, device=U.device)].T.cpu().numpy()
             xG = np.asarray(bc.updateFunction(float(time), a.face_centroid[faces], a.face_normal[faces], xI),
@@ -401,6 +407,24 @@ class FiniteVolumeSolver:
         )
         return theta.reshape(16, 3)
 
+    def GetInterpolatedFaceVectors(self, U, stream=None):
+        """FaceInterpolant::GetInterpolatedFaceVectors (faceInterpolant.cpp:109-249) on the device state U: returns
+        (faceSolution [n_comp, F], faceSolutionGrad [n_comp, dim, F], faceAux [naux, F], faceAuxGrad [naux, dim, F])
+        with aux = (T, velocity, Yi)."""
+        import torch
+
+        self._check_tensor(U, "U")
+        F, nc, dim = self.mesh.n_faces, self.n_comp, self.dim
+        na = 1 + dim + self.n_species
+        out = [torch.empty((nc, F), dtype=torch.float64, device=U.device),
+               torch.empty((nc, dim, F), dtype=torch.float64, device=U.device),
+               torch.empty((na, F), dtype=torch.float64, device=U.device),
+               torch.empty((na, dim, F), dtype=torch.float64, device=U.device)]
+        _lib.check(_lib.lib().abl_fv_face_interpolate(self._h, ctypes.c_void_p(U.data_ptr()),
+                                                      *[ctypes.c_void_p(t.data_ptr()) for t in out],
+                                                      self._stream_ptr(stream)), self._h, "abl_fv_face_interpolate")
+        return tuple(out)
+
     def set_strategy(self, strategy: str) -> None:
         """"auto" (the default at creation: tile or face_flux from the mesh's tile statistics), "tile" (a workgroup
         per tile of cells evaluates the tile's faces into LDS and gathers its cells), "face_flux" (stored once in HBM,
@@ -451,7 +475,7 @@ class FiniteVolumeSolver:
         (the reference TimeStepper with ts_adapt_type physics).  Returns (final time, steps)."""
         import torch
 
-        if self._ghosts:
+        if any(g[0].updateFunction is not None for g in self._ghosts):
             raise NotImplementedError("ts_euler runs on the device and cannot call the host Ghost update functions: "
                                       "step with compute_rhs and axpy instead")
         self._check_tensor(U, "U")

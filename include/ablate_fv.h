@@ -230,6 +230,17 @@ int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accum
 int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar, double* theta_bar,
                    abl_fv_stream stream);
 
+/* FaceInterpolant::GetInterpolatedFaceVectors (faceInterpolant.cpp:109-249, the public method the reference's
+ * faceInterpolantTests.cpp:224-265 checks): per face f, the face values and gradients the continuous (viscous,
+ * conduction, species-diffusion) fluxes use, from the least-squares-average stencils
+ * (stencils/leastSquaresAverage.cpp:5-155): value 1/2 (q_L + q_R), gradient 1/2 (G_L + G_R) of unlimited
+ * least-squares cell gradients over all face neighbours (a ghost cell's: w (q_interior - q_ghost)).  Solution
+ * components q < n_comp into face_sol [n_comp][n_faces] and face_sol_grad [n_comp * dim + d][n_faces]; aux fields
+ * (temperature, velocity, Yi; 1 + dim + n_species) into face_aux / face_aux_grad likewise.  Device pointers; any
+ * output may be NULL.  Ghost rows of U are used as they stand.  FP64 handles with a transport term only. */
+int abl_fv_face_interpolate(abl_fv_handle h, const double* U, double* face_sol, double* face_sol_grad,
+                            double* face_aux, double* face_aux_grad, abl_fv_stream stream);
+
 /* DMPlex local-vector layout (AoS per cell, fields contiguous) <-> SoA. Device pointers. */
 int abl_fv_pack_aos(abl_fv_handle h, const double* aos, double* soa, abl_fv_stream stream);
 int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_stream stream);
