@@ -107,6 +107,7 @@ SIGNATURES = {
     "abl_fv_set_boundary": (c_int32, [c_void_p, c_int32, POINTER(BcDesc)]),
     "abl_fv_rhs": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_int32, c_void_p]),
     "abl_fv_rhs_vjp": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p, _dp, c_void_p]),
+    "abl_fv_rhs_vjp_noalloc": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p, _dp, c_void_p]),
     "abl_fv_face_interpolate": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_pack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_unpack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,6 +37,7 @@ namespace {
 thread_local std::string g_last_error;
 }
 void set_global_error(const std::string& msg) { g_last_error = msg; }
+void resolve_noalloc_scratch();
 
 constexpr int kBlock = 256;
 constexpr int kTileMax = 512;           // largest tile (workgroup) of the tile strategy
@@ -3355,6 +3357,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         }
         s->build_tiles(L);
         s->strategy = s->auto_strategy();
+        abl::resolve_noalloc_scratch();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
         // release the host copies of the big arrays
         L.ell.clear(); L.ell.shrink_to_fit();
@@ -3999,3 +4002,69 @@ extern "C" int abl_fv_check_state(abl_fv_handle h, const double* U, int64_t* n_b
     return 0;
 }
 
+// ============================================================================================ allocation-free VJP
+//
+// abl_fv_rhs_vjp_noalloc: the first-order VJP with its theta_bar workspace (per-boundary-face contributions
+// [3 nb] and per-slot sums [3 kMaxBcSlots]) in a file-scope device array instead of memory the library allocates,
+// for callers that cannot allocate (HIP graph capture, serving loops).  Each GPU holds its own copy of the array
+// until the process ends; calls on one GPU share it, so calls that overlap on one GPU race (serialise them).  Every
+// word of it that a call reads is written earlier in the same call.
+namespace {
+constexpr int64_t kNoAllocMaxBoundaryFaces = 1 << 18;
+constexpr int64_t kNoAllocScratchDoubles = 3 * kNoAllocMaxBoundaryFaces + 3 * abl::kMaxBcSlots;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_dev[kMaxDevices] = {nullptr};  // address of `scratch` on each device (hipGetSymbolAddress)
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+// The current device's address of `scratch`, resolved once per device with that device current.
+double* scratch_for_current_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
+namespace abl {
+// Called by abl_fv_create on the handle's device: resolves the scratch address at set-up.
+void resolve_noalloc_scratch() { (void)scratch_for_current_device(); }
+}  // namespace abl
+
+extern "C" int abl_fv_rhs_vjp_noalloc(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
+                                      double* theta_bar, abl_fv_stream stream) {
+    (void)time;
+    if (!h || !U || !Rbar || !Ubar) return fail(h, "abl_fv_rhs_vjp_noalloc: null argument");
+    {
+        const std::string bad = unset_boundary(h);
+        if (!bad.empty()) return fail(h, "abl_fv_rhs_vjp_noalloc: " + bad);
+    }
+    const int fl = h->phys.flux_calculator;
+    if (h->form != ABL_FV_FORM_FP64 ||
+        (fl != ABL_FV_FLUX_AUSM && fl != ABL_FV_FLUX_RUSANOV && fl != ABL_FV_FLUX_OFF && fl != ABL_FV_FLUX_NONE))
+        return fail(h, "abl_fv_rhs_vjp_noalloc: supported for the FP64 form and the Ausm / Rusanov / Off flux "
+                       "calculators");
+    if (h->recon || h->visc)
+        return fail(h, "abl_fv_rhs_vjp_noalloc: first-order inviscid handles only (the second-order adjoint needs "
+                       "per-cell workspace: use abl_fv_rhs_vjp)");
+    if (h->n_bfaces > kNoAllocMaxBoundaryFaces)
+        return fail(h, "abl_fv_rhs_vjp_noalloc: " + std::to_string(h->n_bfaces) + " boundary faces exceed the maximum " +
+                           std::to_string(kNoAllocMaxBoundaryFaces));
+    double* sc = nullptr;
+    try {
+        sc = scratch_for_current_device();
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_rhs_vjp_noalloc: ") + e.what());
+    }
+    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, sc, sc + 3 * kNoAllocMaxBoundaryFaces, (hipStream_t)stream,
+                           "abl_fv_rhs_vjp_noalloc");
+}

@@ -241,6 +241,14 @@ int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, 
 int abl_fv_face_interpolate(abl_fv_handle h, const double* U, double* face_sol, double* face_sol_grad,
                             double* face_aux, double* face_aux_grad, abl_fv_stream stream);
 
+/* No counterpart in the reference: abl_fv_rhs_vjp for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * abl_fv_rhs_vjp (bitwise), for first-order inviscid FP64 handles with at most 262144 boundary faces; it allocates
+ * nothing: the theta_bar workspace lives in a static per-GPU device array.  Calls that overlap on one GPU race on
+ * that array: serialise them.  Fails (nonzero, message in abl_fv_last_error) for other handles or larger meshes. */
+int abl_fv_rhs_vjp_noalloc(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
+                           double* theta_bar, abl_fv_stream stream);
+
 /* DMPlex local-vector layout (AoS per cell, fields contiguous) <-> SoA. Device pointers. */
 int abl_fv_pack_aos(abl_fv_handle h, const double* aos, double* soa, abl_fv_stream stream);
 int abl_fv_unpack_aos(abl_fv_handle h, const double* soa, double* aos, abl_fv_stream stream);
