@@ -132,7 +132,7 @@ SIGNATURES = {
     "abl_hbm_probe": (c_int32, [c_int32, c_int64, c_int32, _dp, _dp]),
 }
 PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]
-STRATEGY = {"auto": 0, "recompute": 1, "face_flux": 2, "tile": 3}
+STRATEGY = {"auto": 0, "recompute": 1, "face_flux": 2, "tile": 3, "fused": 4}
 
 _lib = None
 

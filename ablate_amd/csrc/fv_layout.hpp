@@ -45,4 +45,25 @@ struct Tiles {
 };
 void build_tiles(const Layout& L, int KM, int T, Tiles& out);
 
+// Cell tiles of the fused strategy (gradients + face fluxes + gather in one kernel): tile t owns interior cells
+// [t*T, min((t+1)*T, NI)); its halo is the interior cells outside the tile that share a face with it.
+//   fstart / ffaces   the tile's faces (every face with a cell in the tile), ascending face number
+//   fcode             per tile face: left side code | right side code << 16; a code is a tile cell's local index,
+//                     T + the halo face entry of that side, or 0xFFFF for a ghost cell
+//   fslot[k * NI + c] (tile-local index of the k-th face of c) << 1 | side, -1 padding; KM slots
+//   hstart / hcell    the tile's halo cells, ascending
+//   hinfo             per halo cell: bit k set when its k-th face is a tile face, | its first face entry << 8 (one
+//                     entry per such face, in slot order)
+//   nfm / nem         largest face / halo face-entry count of a tile (size the workgroup's LDS)
+struct FusedTiles {
+    int T = 0, nfm = 0, nem = 0;
+    int64_t n = 0;
+    std::vector<int32_t> fstart, ffaces, fcode, fslot, hstart, hcell, hinfo;
+};
+void build_fused_tiles(const Layout& L, int KM, int T, FusedTiles& out);
+
+// True when every interior cell's least-squares stencil (its interior face neighbours, offsets from fdx) has full
+// rank, so that the normal equations the fused kernel solves give the pseudo-inverse solution.
+bool lsq_stencils_full_rank(const Layout& L);
+
 }  // namespace abl

@@ -302,3 +302,139 @@ void build_tiles(const Layout& L, int KM, int T, Tiles& out) {
 }
 
 }  // namespace abl
+
+namespace abl {
+
+void build_fused_tiles(const Layout& L, int KM, int T, FusedTiles& out) {
+    const int64_t NI = L.NI, F = L.F;
+    out.T = T;
+    out.n = (NI + T - 1) / T;
+    out.nfm = 0;
+    out.nem = 0;
+    out.fstart.assign((size_t)out.n + 1, 0);
+    out.hstart.assign((size_t)out.n + 1, 0);
+    out.ffaces.clear();
+    out.fcode.clear();
+    out.hcell.clear();
+    out.hinfo.clear();
+    out.ffaces.reserve((size_t)NI * 4);
+    out.fcode.reserve((size_t)NI * 4);
+    out.fslot.assign((size_t)KM * NI, -1);
+    std::vector<int32_t> list, halo, entry;
+    auto local_face = [&](int32_t f) {
+        return (int32_t)(std::lower_bound(list.begin(), list.end(), f) - list.begin());
+    };
+    for (int64_t t = 0; t < out.n; t++) {
+        const int64_t c0 = t * T, c1 = std::min<int64_t>(c0 + T, NI);
+        auto own = [&](int64_t c) { return c >= c0 && c < c1; };
+        list.clear();
+        for (int k = 0; k < L.K; k++)
+            for (int64_t c = c0; c < c1; c++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e >= 0) list.push_back(e >> 1);
+            }
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        // halo: interior cells across the tile's faces that are not tile cells
+        halo.clear();
+        for (int32_t f : list)
+            for (int s = 0; s < 2; s++) {
+                const int64_t x = L.fcell[(size_t)s * F + f];
+                if (x < NI && !own(x)) halo.push_back((int32_t)x);
+            }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        // one face entry per (halo cell, tile face of that cell), in halo order then slot order
+        entry.assign(list.size(), -1);
+        int32_t ne = 0;
+        for (int32_t h : halo) {
+            int32_t mask = 0;
+            const int32_t first = ne;
+            for (int k = 0; k < L.K; k++) {
+                const int32_t e = L.ell[(size_t)k * NI + h];
+                if (e < 0) continue;
+                const int32_t j = local_face(e >> 1);
+                if (j < (int32_t)list.size() && list[(size_t)j] == (e >> 1)) {
+                    mask |= 1 << k;
+                    entry[(size_t)j] = ne++;
+                }
+            }
+            out.hcell.push_back(h);
+            out.hinfo.push_back(mask | (first << 8));
+        }
+        if ((int64_t)T + ne >= 0xFFFF) throw std::invalid_argument("fused tiles: too many halo faces in a tile");
+        for (size_t j = 0; j < list.size(); j++) {
+            const int32_t f = list[j];
+            const int64_t a = L.fcell[f], b = L.fcell[(size_t)F + f];
+            const bool la = own(a), lb = b < NI && own(b);
+            if ((!la || (b < NI && !lb)) && entry[j] < 0) throw std::logic_error("fused tiles: a halo side without entry");
+            const int32_t lc = la ? (int32_t)(a - c0) : T + entry[j];
+            const int32_t rc = b >= NI ? 0xFFFF : (lb ? (int32_t)(b - c0) : T + entry[j]);
+            out.fcode.push_back((int32_t)((uint32_t)lc | ((uint32_t)rc << 16)));
+        }
+        for (int k = 0; k < L.K; k++)
+            for (int64_t c = c0; c < c1; c++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e < 0) continue;
+                out.fslot[(size_t)k * NI + c] = (local_face(e >> 1) << 1) | (e & 1);
+            }
+        if (out.ffaces.size() + list.size() > (size_t)INT32_MAX || out.hcell.size() > (size_t)INT32_MAX)
+            throw std::invalid_argument("mesh too large for 32-bit fused tile lists");
+        out.ffaces.insert(out.ffaces.end(), list.begin(), list.end());
+        out.fstart[(size_t)t + 1] = (int32_t)out.ffaces.size();
+        out.hstart[(size_t)t + 1] = (int32_t)out.hcell.size();
+        out.nfm = std::max(out.nfm, (int)list.size());
+        out.nem = std::max(out.nem, (int)ne);
+    }
+}
+
+bool lsq_stencils_full_rank(const Layout& L) {
+    const int dim = L.dim;
+    const int64_t NI = L.NI, F = L.F;
+    for (int64_t c = 0; c < NI; c++) {
+        double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int k = 0; k < L.K; k++) {
+            const int32_t e = L.ell[(size_t)k * NI + c];
+            if (e < 0) continue;
+            const int64_t f = e >> 1;
+            const int s = e & 1;
+            if (L.fcell[(size_t)F + f] >= NI) continue;  // ghost neighbours are not in the cell stencil
+            double v[3] = {0, 0, 0};
+            for (int d = 0; d < dim; d++)
+                v[d] = L.fdx[(size_t)2 * dim * f + (size_t)s * dim + d] - L.fdx[(size_t)2 * dim * f + (size_t)(1 - s) * dim + d];
+            for (int r = 0; r < dim; r++)
+                for (int q = 0; q < dim; q++) M[r][q] += v[r] * v[q];
+        }
+        // condition number estimate ||M||_F ||M^-1||_F with M^-1 = adj(M) / det(M): the normal equations lose
+        // cond(M) ulps against the pseudo-inverse, so poorly conditioned stencils keep the precomputed weights
+        double adj[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, det;
+        if (dim == 3) {
+            adj[0][0] = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+            adj[0][1] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+            adj[0][2] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+            adj[1][0] = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+            adj[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+            adj[1][2] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+            adj[2][0] = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+            adj[2][1] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+            adj[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+            det = M[0][0] * adj[0][0] + M[0][1] * adj[1][0] + M[0][2] * adj[2][0];
+        } else {
+            adj[0][0] = M[1][1];
+            adj[0][1] = -M[0][1];
+            adj[1][0] = -M[1][0];
+            adj[1][1] = M[0][0];
+            det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+        }
+        double nm = 0.0, na = 0.0;
+        for (int r = 0; r < dim; r++)
+            for (int q = 0; q < dim; q++) {
+                nm += M[r][q] * M[r][q];
+                na += adj[r][q] * adj[r][q];
+            }
+        if (!(det > 0.0) || !(std::sqrt(nm) * std::sqrt(na) <= 1e4 * det)) return false;
+    }
+    return true;
+}
+
+}  // namespace abl

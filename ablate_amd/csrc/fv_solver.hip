@@ -978,6 +978,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(MeshView<double> m, int ns, c
     }
 }
 
+#include "fv_fused.hpp"
+
 // ============================================================================================ mixed form
 //
 // The residual of a cell is a sum of face fluxes that cancel to O(1 %) of their size (and far more at low Mach),
@@ -1699,9 +1701,10 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
 // VJP of the FP64 RHS with least-squares reconstruction (Barth-Jespersen or no limiter), the viscous flux and
 // species advection / diffusion, in three passes after the forward gradient pass (cell records, species gradient
 // arrays):
-//   1. k_adj_faces  (lane per interior cell): recomputes each face of the cell and reverses the face flux, keeping
-//      its own side's adjoints: state (direct), limited gradients (face state = U + G dx), viscous terms, species
-//      aux gradients.  The adjoint of a ghost state goes to a per-ghost array.
+//   1. k_adj_face_rev (lane per face): recomputes the face flux and reverses it once for both sides into a face
+//      record (face-state, viscous-term and species factors); the adjoint of a ghost state goes to a per-ghost array.
+//      k_adj_face_gather (lane per interior cell) sums its faces' records in slot order into its own adjoints:
+//      state (direct), limited gradients (face state = U + G dx), viscous terms, species aux gradients.
 //   2. k_adj_cells  (lane per interior cell): reverses the limiter, the least-squares gradients and the aux
 //      (temperature / velocity / mass fraction) gradients of the cell; writes its own adjoint and, per face slot,
 //      the contribution for the neighbour across the face.
@@ -1717,25 +1720,282 @@ struct AdjBufs {
     double* contrib;        // [KM][nc][NI]
     const int32_t* mirror;  // [KM][NI] slot of the same face in the neighbour's list, -1 when none / ghost
     int km;                 // slot width of contrib / mirror
-    int lds_species;        // k_adj_faces keeps the species accumulators in dynamic LDS (7 ns doubles per lane)
+    double* W;              // face adjoint records (AdjFaceRec pairs, 64-face blocks)
+    int nwp;                // word pairs per face record
+    double* GL;             // [ns][NG] densityYi adjoint of a boundary face's interior cell from the ghost's gradient
 };
 
-template <int DIM, bool RUS, int KM>
-__global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                      const double* __restrict__ Rb, const double* __restrict__ rec,
-                                                      const double* __restrict__ gS, const double* __restrict__ gY,
-                                                      AdjBufs b) {
+// Face adjoint record of the face-centric reverse pass, read back per cell by k_adj_face_gather.  Words per face:
+//   [0, NE)             adjoint of the left face state (the flux calculator's and AdvectionFlux's reverse)
+//   [NE, 2 NE)          right face state; on a boundary face (ghost right cell, whose adjoint goes to Ughost) the
+//                       left cell's extra aux adjoint [NE, NE + DIM + 1) and density adjoint (NE + DIM + 1) from the
+//                       ghost's one-neighbour viscous gradient
+//   [2 NE, 2 NE + NT)   half the adjoint of the face-averaged viscous terms (each side's share)
+//   + DIM               half the adjoint of the face velocity
+//   + 4                 species coefficients cL, cR (F_sp = cL rhoY_L,f + cR rhoY_R,f), the diffusion coefficient
+//                       cd = -D rho_face A / 2 and the face density adjoint summed over species
+//   + ns                per species g_sp = Rbar_sp(R) / V_R - Rbar_sp(L) / V_L
+// Stored as word pairs in blocks of 64 faces (pair q of face f at double2 index (f / 64) 64 NWP + 64 q + f % 64):
+// each store of a wave (64 consecutive faces) is one coalesced 1 KB access.
+template <int DIM>
+struct AdjFaceRec {
+    static constexpr int NE = DIM + 2;
+    static constexpr int NT = ViscTerms<DIM>::NT;
+    static constexpr int UL = 0, UR = NE, TQ = 2 * NE, VL = TQ + NT, CL = VL + DIM, CR = CL + 1, CD = CL + 2,
+                         RH = CL + 3, GS = (CL + 5) & ~1;  // species factors start on a pair boundary
+    __host__ __device__ static int words(int ns) { return GS + ns; }
+    __host__ __device__ static int pairs(int ns) { return (words(ns) + 1) / 2; }
+};
+__device__ inline int64_t adjw_pair(int64_t f, int nwp) { return (f >> 6) * 64 * (int64_t)nwp + (f & 63); }
+
+// Reverse of one face flux (advection, viscous, species advection and diffusion), both sides at once.  One lane per
+// face (interior and boundary).  The face's ghost-cell adjoints go straight to Ughost (a ghost has one face).
+template <int DIM, bool RUS>
+__global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                         const double* __restrict__ Rb, const double* __restrict__ rec,
+                                                         const double* __restrict__ gS, const double* __restrict__ gY,
+                                                         AdjBufs b) {
     constexpr int NE = DIM + 2;
     constexpr int NT = ViscTerms<DIM>::NT;
     using CR = CellRec<DIM>;
-    const int64_t c = xcd_cell_index();
-    if (c >= m.NI) return;
+    using AW = AdjFaceRec<DIM>;
+    (void)gY;  // the Yi aux gradients live in gS's word pairs
+    const int64_t f = xcd_cell_index();
+    if (f >= m.F) return;
     const int64_t N = m.N, F = m.F, NI = m.NI, NG = N - NI;
-    const int ns = ph.ns, nc = NE + ns;
+    const int ns = ph.ns;
     const bool recon = ph.recon != 0, visc = ph.visc != 0, sdiff = ph.sdiff != 0;
     const bool useRec = recon || visc;
+    const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+    const bool ghost = Rc >= NI;
+    double nh[3], area, x[2][DIM];
+    load_face_na(m.fna, f, nh, area);
+    load_face_pair<DIM>(m.fdx, f, x);
+    const double ivL = m.inv_vol[L], ivR = ghost ? 0.0 : m.inv_vol[Rc];
+    double g[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) g[p] = (ghost ? 0.0 : Rb[p * N + Rc] * ivR) - Rb[p * N + L] * ivL;
+    // both sides: cell state, face state and viscous terms
+    double uL[NE], uR[NE], fL[NE], fR[NE], tqL[NT], tqR[NT];
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+        const int64_t cc = sd == 0 ? L : Rc;
+        double* u = sd == 0 ? uL : uR;
+        double* fu = sd == 0 ? fL : fR;
+        double* tq = sd == 0 ? tqL : tqR;
+        if (useRec && !(sd == 1 && ghost)) {
+            double r[CR::RS];
+            const double2* __restrict__ rb = reinterpret_cast<const double2*>(rec) + rec_pair<DIM>(cc);
+#pragma unroll
+            for (int q = 0; q < CR::NP; q++) {
+                const double2 v = rb[CR::BLOCK * q];
+                r[2 * q] = v.x;
+                r[2 * q + 1] = v.y;
+            }
+#pragma unroll
+            for (int p = 0; p < NE; p++) {
+                u[p] = r[p];
+                double a = 0.0;
+                if (recon)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) a += r[CR::G0 + p * DIM + d] * x[sd][d];
+                fu[p] = u[p] + a;
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) tq[q] = r[CR::T0 + q];
+        } else {
+#pragma unroll
+            for (int p = 0; p < NE; p++) {
+                u[p] = U[p * N + cc];
+                fu[p] = u[p];
+            }
+#pragma unroll
+            for (int q = 0; q < NT; q++) tq[q] = 0.0;
+        }
+    }
+    // species sums of the face values
+    double QL = 0.0, QR = 0.0, QdL = 0.0;
+    for (int sp = 0; sp < ns; sp++) {
+        const int64_t comp = NE + sp;
+        const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+        double yl = U[comp * N + L], yr = U[comp * N + Rc];
+        if (recon)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
+            }
+        QL += gsp * yl;
+        QR += gsp * yr;
+        QdL += gsp * (yr - yl);
+    }
+    double ubL[NE], ubR[NE], cL = 0.0, cR = 0.0;
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        ubL[p] = 0.0;
+        ubR[p] = 0.0;
+    }
+    if (ph.flux == ABL_FV_FLUX_AUSM || ph.flux == ABL_FV_FLUX_RUSANOV)
+        face_vjp<DIM, RUS>(fL, fR, QL, QR, QdL, nh, area, ph, g, ubL, ubR, cL, cR);
+    // viscous flux
+    double tqbf[NT], velbf[DIM], wg[2][DIM];
+#pragma unroll
+    for (int q = 0; q < NT; q++) tqbf[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) velbf[d] = 0.0;
+    if (visc) {
+        double vL[DIM], vR[DIM], vf[DIM], tqf[NT], normal[DIM];
+        const double irL = 1.0 / uL[kRho], irR = 1.0 / uR[kRho];
+        double keL = 0.0, keR = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            vL[d] = uL[kRhoU + d] * irL;
+            vR[d] = uR[kRhoU + d] * irR;
+            keL += vL[d] * vL[d];
+            keR += vR[d] * vR[d];
+            vf[d] = 0.5 * vL[d] + 0.5 * vR[d];
+            normal[d] = area * nh[d];
+        }
+        if (ghost) {
+            double aL[DIM + 1], aG[DIM + 1];
+            load_face_pair<DIM>(m.fwa, f, wg);
+            aL[0] = (uL[kRhoE] * irL - 0.5 * keL) * ph.icv;
+            aG[0] = (uR[kRhoE] * irR - 0.5 * keR) * ph.icv;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                aL[1 + d] = vL[d];
+                aG[1 + d] = vR[d];
+            }
+            double gA[DIM + 1][DIM];
+#pragma unroll
+            for (int q = 0; q <= DIM; q++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gA[q][d] = wg[1][d] * (aL[q] - aG[q]);
+            stress_heat<double, DIM>(ph.mu, ph.k, gA, tqR);
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) tqf[q] = 0.5 * tqL[q] + 0.5 * tqR[q];
+        viscous_flux_vjp<DIM>(normal, vf, tqf, g, tqbf, velbf);
+    }
+    double tqh[NT], velh[DIM];
+#pragma unroll
+    for (int q = 0; q < NT; q++) tqh[q] = 0.5 * tqbf[q];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) velh[d] = 0.5 * velbf[d];
+    // ghost side (a ghost is always the right cell of its boundary face); the left cell's extras replace ubR
+    double ugb[NE], extra[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        ugb[p] = ubR[p];
+        extra[p] = 0.0;
+    }
+    if (ghost && visc) {
+        double gAb[DIM + 1][DIM], aGb[DIM + 1];
+#pragma unroll
+        for (int q = 0; q <= DIM; q++) {
+            aGb[q] = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) gAb[q][d] = 0.0;
+        }
+        stress_heat_vjp<DIM>(ph.mu, ph.k, tqh, gAb);
+#pragma unroll
+        for (int q = 0; q <= DIM; q++) {
+            double t = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) t += wg[1][d] * gAb[q][d];
+            extra[q] = t;  // a_L: the left cell's aux adjoint
+            aGb[q] -= t;
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; d++) aGb[1 + d] += velh[d];
+        aux_vjp<DIM>(uR, ph, aGb, ugb);
+    }
+    // species: diffusion (face density, ghost gradients); the per-species factors go to the record
+    double2* __restrict__ W = reinterpret_cast<double2*>(b.W) + adjw_pair(f, b.nwp);
+    const double cdf = sdiff ? -ph.diff * (0.5 * uL[kRho] + 0.5 * uR[kRho]) * 0.5 * area : 0.0;
+    double rhob = 0.0, rgb = 0.0;
+    for (int sp = 0; sp < ns; sp++) {
+        const int64_t comp = NE + sp;
+        const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+        double ygb = ghost ? cR * gsp : 0.0, yxl = 0.0;
+        if (sdiff) {
+            const double rl = uL[kRho], rr = uR[kRho];
+            const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+            const double YL = yl / rl, YR = yr / rr;
+            double dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const double GL = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
+                const double GR = ghost ? wg[1][d] * (YL - YR) : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
+                dot += nh[d] * (GL + GR);
+            }
+            const double drho = gsp * (-ph.diff * 0.25 * area) * dot;
+            rhob += drho;  // rho_L and rho_R enter the face density alike
+            if (ghost) {
+                // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (left cell) and Y_G
+                double t = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
+                massfrac_vjp(rl, yl, t, yxl, extra[DIM + 1]);
+                double rg = 0.0;
+                massfrac_vjp(rr, yr, -t, ygb, rg);
+                rgb += rg + drho;  // rho_G in the face density
+            }
+        }
+        if (ghost) {
+            b.Ughost[comp * NG + (Rc - NI)] = ygb;
+            b.GL[(int64_t)sp * NG + (Rc - NI)] = yxl;
+        }
+    }
+    if (ghost) {
+        ugb[kRho] += rgb;
+#pragma unroll
+        for (int p = 0; p < NE; p++) b.Ughost[p * NG + (Rc - NI)] = ugb[p];
+    }
+    // the record, pair by pair
+    double w[AW::GS];
+    w[AW::GS - 1] = 0.0;  // padding word (2-D)
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        w[AW::UL + p] = ubL[p];
+        w[AW::UR + p] = ghost ? extra[p] : ubR[p];
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) w[AW::TQ + q] = tqh[q];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[AW::VL + d] = velh[d];
+    w[AW::CL] = cL;
+    w[AW::CR] = cR;
+    w[AW::CD] = cdf;
+    w[AW::RH] = rhob;
+    constexpr int NFIX = AW::GS / 2;  // pairs holding only fixed words
+#pragma unroll
+    for (int q = 0; q < NFIX; q++) W[64 * q] = make_double2(w[2 * q], w[2 * q + 1]);
+    for (int q = 0; 2 * q < ns; q++) {
+        const int sp = 2 * q;
+        const double g0 = (ghost ? 0.0 : Rb[(NE + sp) * N + Rc] * ivR) - Rb[(NE + sp) * N + L] * ivL;
+        const double g1 = sp + 1 < ns ? (ghost ? 0.0 : Rb[(NE + sp + 1) * N + Rc] * ivR) - Rb[(NE + sp + 1) * N + L] * ivL
+                                      : 0.0;
+        W[64 * (NFIX + q)] = make_double2(g0, g1);
+    }
+}
+
+// Per-cell gather of the face adjoint records into the pass-1 record A (state, limited-gradient, viscous-term and
+// species aux-gradient adjoints of the cell), in the cell's slot order.
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_adj_face_gather(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                            AdjBufs b) {
+    constexpr int NE = DIM + 2;
+    constexpr int NT = ViscTerms<DIM>::NT;
+    using AW = AdjFaceRec<DIM>;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI, NG = N - NI;
+    const int ns = ph.ns, nc = NE + ns;
+    const bool recon = ph.recon != 0, visc = ph.visc != 0;
     const int64_t oG = nc, oT = nc + (int64_t)nc * DIM, oY = oT + NT;
     double* __restrict__ A = b.A;
+    const double2* __restrict__ W = reinterpret_cast<const double2*>(b.W);
     double ub[NE], gb[NE][DIM], tqb[NT], velb[DIM], auxb[DIM + 1], rhob = 0.0;
 #pragma unroll
     for (int p = 0; p < NE; p++) {
@@ -1749,238 +2009,103 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(MeshView<double> m, Phys p
     for (int d = 0; d < DIM; d++) velb[d] = 0.0;
 #pragma unroll
     for (int q = 0; q <= DIM; q++) auxb[q] = 0.0;
-    // Species adjoint accumulators of this cell, field q: densityYi (q < ns), its limited-gradient adjoint
-    // (ns + sp * DIM + d) and the Yi aux-gradient adjoint (ns + ns * DIM + sp * DIM + d).  In LDS (one column per
-    // lane) when the launch provides 7 ns doubles per lane, else in place in the A record: the face loop adds to
-    // them once per face and species, which through HBM would be 2 (1 + 2 DIM) ns read-modify-writes per face.
-    extern __shared__ double sacc[];
-    const bool in_lds = b.lds_species != 0;
-    const int nq = ns * (1 + 2 * DIM);
-    auto spa = [&](int q) -> double& {
-        if (in_lds) return sacc[(int64_t)q * blockDim.x + threadIdx.x];
-        int64_t row;
-        if (q < ns) row = NE + q;
-        else if (q < ns + ns * DIM) row = oG + (int64_t)NE * DIM + (q - ns);
-        else row = oY + (q - ns - ns * DIM);
-        return A[row * NI + c];
-    };
-    for (int q = 0; q < nq; q++) spa(q) = 0.0;
-    double uc[NE];
+    // per slot: record base, side, face offset x_f - x_c, unit normal, species coefficient, diffusion coefficient,
+    // ghost index
+    int64_t wb[KM], gi[KM];
+    double xs[KM][DIM], nv[KM][DIM], cs[KM], cdk[KM];
+    bool valid[KM];
 #pragma unroll
-    for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
-
-    for (int k = 0; k < m.K; k++) {
-        const int32_t e = m.ell[k * NI + c];
-        if (e < 0) break;
-        const int64_t f = e >> 1;
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
+        valid[k] = e >= 0;
+        const int64_t f = valid[k] ? (int64_t)(e >> 1) : 0;
         const int s = e & 1;
-        const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
-        const bool ghost = Rc >= NI;
-        double nh[3], area, x[2][DIM];
-        load_face_na(m.fna, f, nh, area);
+        const int32_t nb = m.snbr[k * NI + c];
+        gi[k] = (valid[k] && nb >= NI) ? (int64_t)nb - NI : -1;
+        wb[k] = adjw_pair(f, b.nwp);
+        double x[2][DIM], nh[3], area;
         load_face_pair<DIM>(m.fdx, f, x);
-        const double ivL = m.inv_vol[L], ivR = ghost ? 0.0 : m.inv_vol[Rc];
-        double g[NE];
+        load_face_na(m.fna, f, nh, area);
 #pragma unroll
-        for (int p = 0; p < NE; p++) g[p] = (ghost ? 0.0 : Rb[p * N + Rc] * ivR) - Rb[p * N + L] * ivL;
-        // both sides: cell state, face state and viscous terms
-        double uL[NE], uR[NE], fL[NE], fR[NE], tqL[NT], tqR[NT];
-#pragma unroll
-        for (int sd = 0; sd < 2; sd++) {
-            const int64_t cc = sd == 0 ? L : Rc;
-            double* u = sd == 0 ? uL : uR;
-            double* fu = sd == 0 ? fL : fR;
-            double* tq = sd == 0 ? tqL : tqR;
-            if (useRec && !(sd == 1 && ghost)) {
-#pragma unroll
-                for (int p = 0; p < NE; p++) {
-                    u[p] = rec_word<DIM>(rec, cc, p);
-                    double a = 0.0;
-                    if (recon)
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) a += rec_word<DIM>(rec, cc, CR::G0 + p * DIM + d) * x[sd][d];
-                    fu[p] = u[p] + a;
-                }
-#pragma unroll
-                for (int q = 0; q < NT; q++) tq[q] = rec_word<DIM>(rec, cc, CR::T0 + q);
-            } else {
-#pragma unroll
-                for (int p = 0; p < NE; p++) {
-                    u[p] = U[p * N + cc];
-                    fu[p] = u[p];
-                }
-#pragma unroll
-                for (int q = 0; q < NT; q++) tq[q] = 0.0;
-            }
+        for (int d = 0; d < DIM; d++) {
+            xs[k][d] = s ? x[1][d] : x[0][d];
+            nv[k][d] = nh[d];
         }
-        // species sums of the face values
-        double QL = 0.0, QR = 0.0, QdL = 0.0;
-        for (int sp = 0; sp < ns; sp++) {
-            const int64_t comp = NE + sp;
-            const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
-            double yl = U[comp * N + L], yr = U[comp * N + Rc];
-            if (recon)
+        double w[AW::GS];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                    if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
-                }
-            QL += gsp * yl;
-            QR += gsp * yr;
-            QdL += gsp * (yr - yl);
+        for (int q = 0; q < AW::GS / 2; q++) {
+            const double2 v = W[wb[k] + 64 * q];
+            w[2 * q] = v.x;
+            w[2 * q + 1] = v.y;
         }
-        double ubL[NE], ubR[NE], cL = 0.0, cR = 0.0;
+        cs[k] = s ? w[AW::CR] : w[AW::CL];
+        cdk[k] = w[AW::CD];
+        if (!valid[k]) {
+            cs[k] = 0.0;
+            cdk[k] = 0.0;
+            continue;
+        }
+        const int o = s ? AW::UR : AW::UL;
 #pragma unroll
         for (int p = 0; p < NE; p++) {
-            ubL[p] = 0.0;
-            ubR[p] = 0.0;
+            ub[p] += w[o + p];
+            if (recon)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gb[p][d] += w[o + p] * xs[k][d];
         }
-        if (ph.flux == ABL_FV_FLUX_AUSM || ph.flux == ABL_FV_FLUX_RUSANOV)
-            face_vjp<DIM, RUS>(fL, fR, QL, QR, QdL, nh, area, ph, g, ubL, ubR, cL, cR);
-        // viscous flux
-        double tqbf[NT], velbf[DIM], wg[2][DIM], aL[DIM + 1], aG[DIM + 1];
-#pragma unroll
-        for (int q = 0; q < NT; q++) tqbf[q] = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) velbf[d] = 0.0;
         if (visc) {
-            double vL[DIM], vR[DIM], vf[DIM], tqf[NT], normal[DIM];
-            const double irL = 1.0 / uL[kRho], irR = 1.0 / uR[kRho];
-            double keL = 0.0, keR = 0.0;
+#pragma unroll
+            for (int q = 0; q < NT; q++) tqb[q] += w[AW::TQ + q];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) velb[d] += w[AW::VL + d];
+        }
+        rhob += w[AW::RH];
+        if (gi[k] >= 0) {  // boundary face: the left cell's extras from the ghost's viscous gradient
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) auxb[q] += w[AW::UR + q];
+            rhob += w[AW::UR + DIM + 1];
+        }
+    }
+    // species: densityYi, its limited-gradient and Yi aux-gradient adjoints
+    for (int sp = 0; sp < ns; sp++) {
+        double a = 0.0, gsb[DIM], gyb[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            gsb[d] = 0.0;
+            gyb[d] = 0.0;
+        }
+        const int q = sp >> 1;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const double2 v = W[wb[k] + 64 * (AW::GS / 2 + q)];
+            if (!valid[k]) continue;
+            const double gsp = (sp & 1) ? v.y : v.x;
+            const double yb = cs[k] * gsp;
+            a += yb;
+            if (gi[k] >= 0) a += b.GL[(int64_t)sp * NG + gi[k]];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                vL[d] = uL[kRhoU + d] * irL;
-                vR[d] = uR[kRhoU + d] * irR;
-                keL += vL[d] * vL[d];
-                keR += vR[d] * vR[d];
-                vf[d] = 0.5 * vL[d] + 0.5 * vR[d];
-                normal[d] = area * nh[d];
-            }
-            if (ghost) {
-                load_face_pair<DIM>(m.fwa, f, wg);
-                aL[0] = (uL[kRhoE] * irL - 0.5 * keL) * ph.icv;
-                aG[0] = (uR[kRhoE] * irR - 0.5 * keR) * ph.icv;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    aL[1 + d] = vL[d];
-                    aG[1 + d] = vR[d];
-                }
-                double gA[DIM + 1][DIM];
-#pragma unroll
-                for (int q = 0; q <= DIM; q++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) gA[q][d] = wg[1][d] * (aL[q] - aG[q]);
-                stress_heat<double, DIM>(ph.mu, ph.k, gA, tqR);
-            }
-#pragma unroll
-            for (int q = 0; q < NT; q++) tqf[q] = 0.5 * tqL[q] + 0.5 * tqR[q];
-            viscous_flux_vjp<DIM>(normal, vf, tqf, g, tqbf, velbf);
-        }
-        // own side
-        const double* ubS = s == 0 ? ubL : ubR;
-        const double cS = s == 0 ? cL : cR;
-#pragma unroll
-        for (int p = 0; p < NE; p++) {
-            ub[p] += ubS[p];
-            if (recon)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) gb[p][d] += ubS[p] * x[s][d];
-        }
-        if (visc) {
-#pragma unroll
-            for (int q = 0; q < NT; q++) tqb[q] += 0.5 * tqbf[q];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) velb[d] += 0.5 * velbf[d];
-        }
-        // ghost side (a ghost is always the right cell of its boundary face)
-        double ugb[NE];
-#pragma unroll
-        for (int p = 0; p < NE; p++) ugb[p] = ubR[p];
-        if (ghost && visc) {
-            double gAb[DIM + 1][DIM], tqh[NT], aGb[DIM + 1];
-#pragma unroll
-            for (int q = 0; q <= DIM; q++) {
-                aGb[q] = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) gAb[q][d] = 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < NT; q++) tqh[q] = 0.5 * tqbf[q];
-            stress_heat_vjp<DIM>(ph.mu, ph.k, tqh, gAb);
-#pragma unroll
-            for (int q = 0; q <= DIM; q++) {
-                double t = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) t += wg[1][d] * gAb[q][d];
-                auxb[q] += t;  // a_L: this cell
-                aGb[q] -= t;
-            }
-#pragma unroll
-            for (int d = 0; d < DIM; d++) aGb[1 + d] += 0.5 * velbf[d];
-            aux_vjp<DIM>(uR, ph, aGb, ugb);
-        }
-        // species: face-state adjoints and diffusion
-        for (int sp = 0; sp < ns; sp++) {
-            const int64_t comp = NE + sp;
-            const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
-            const double yb = cS * gsp;
-            spa(sp) += yb;
-            if (recon)
-                for (int d = 0; d < DIM; d++) spa(ns + sp * DIM + d) += yb * x[s][d];
-            double ygb = ghost ? cR * gsp : 0.0, rgb = 0.0;
-            if (sdiff) {
-                const double rl = uL[kRho], rr = uR[kRho];
-                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
-                const double YL = yl / rl, YR = yr / rr;
-                double GL[DIM], GR[DIM], dot = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    GL[d] = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
-                    GR[d] = ghost ? wg[1][d] * (YL - YR) : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
-                    dot += nh[d] * (GL[d] + GR[d]);
-                }
-                const double cdf = -ph.diff * (0.5 * rl + 0.5 * rr) * 0.5 * area;
-                const double drho = gsp * (-ph.diff * 0.25 * area) * dot;
-                rhob += drho;  // rho_L and rho_R enter alike; this cell's side
-                for (int d = 0; d < DIM; d++) spa(ns + ns * DIM + sp * DIM + d) += gsp * cdf * nh[d];
-                if (ghost) {
-                    // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (this cell) and Y_G
-                    double t = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
-                    double yb_c = 0.0, rb_c = 0.0;
-                    massfrac_vjp(rl, yl, t, yb_c, rb_c);
-                    spa(sp) += yb_c;
-                    rhob += rb_c;
-                    massfrac_vjp(rr, yr, -t, ygb, rgb);
-                    rgb += drho;  // rho_G in the face density
-                }
-            }
-            if (ghost) {
-                b.Ughost[comp * NG + (Rc - NI)] = ygb;
-                ugb[kRho] += rgb;
+                gsb[d] += yb * xs[k][d];
+                gyb[d] += gsp * cdk[k] * nv[k][d];
             }
         }
-        if (ghost)
+        A[(int64_t)(NE + sp) * NI + c] = a;
 #pragma unroll
-            for (int p = 0; p < NE; p++) b.Ughost[p * NG + (Rc - NI)] = ugb[p];
+        for (int d = 0; d < DIM; d++) {
+            A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = recon ? gsb[d] : 0.0;
+            A[(oY + (int64_t)sp * DIM + d) * NI + c] = gyb[d];
+        }
     }
     if (visc) {
-        double ab[DIM + 1];
+        double uc[NE], ab[DIM + 1];
+#pragma unroll
+        for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
         ab[0] = auxb[0];
 #pragma unroll
         for (int d = 0; d < DIM; d++) ab[1 + d] = velb[d] + auxb[1 + d];
         aux_vjp<DIM>(uc, ph, ab, ub);
     }
     ub[kRho] += rhob;
-    if (in_lds)
-        for (int sp = 0; sp < ns; sp++) {
-            A[(int64_t)(NE + sp) * NI + c] = spa(sp);
-            for (int d = 0; d < DIM; d++) {
-                A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = spa(ns + sp * DIM + d);
-                A[(oY + (int64_t)sp * DIM + d) * NI + c] = spa(ns + ns * DIM + sp * DIM + d);
-            }
-        }
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         A[(int64_t)p * NI + c] = ub[p];
@@ -2481,13 +2606,15 @@ FaceInterpKernel pick_face_interpolate(int K) {
 }
 
 // ---- compile-time variant selection
+using AdjFaceGatherKernel = void (*)(MeshView<double>, Phys, const double*, AdjBufs);
 using AdjFacesKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
                                 const double*, AdjBufs);
 using AdjCellsKernel = void (*)(MeshView<double>, Phys, const double*, const int32_t*, const int32_t*, AdjBufs);
 using AdjGatherKernel = void (*)(MeshView<double>, Phys, const int32_t*, const int32_t*, AdjBufs, double*);
 template <int DIM, int KM>
-void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjCellsKernel& ck, AdjGatherKernel& gk) {
-    fk = rusanov ? k_adj_faces<DIM, true, KM> : k_adj_faces<DIM, false, KM>;
+void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjFaceGatherKernel& fg, AdjCellsKernel& ck, AdjGatherKernel& gk) {
+    fk = rusanov ? k_adj_face_rev<DIM, true> : k_adj_face_rev<DIM, false>;
+    fg = k_adj_face_gather<DIM, KM>;
     ck = k_adj_cells<DIM, KM>;
     gk = k_adj_gather<DIM, KM>;
 }
@@ -2533,11 +2660,12 @@ GradKernelT<G> pick_gradients_k(bool recon, bool visc, int lim) {
 // Compile-time slot width of the cell -> face table: 4 (quads, tets), 6 (hexes) or 8.
 inline int slot_width(int K) { return K <= 4 ? 4 : (K <= 6 ? 6 : 8); }
 template <int DIM>
-void pick_adj(bool rusanov, int K, AdjFacesKernel& fk, AdjCellsKernel& ck, AdjGatherKernel& gk) {
+void pick_adj(bool rusanov, int K, AdjFacesKernel& fk, AdjFaceGatherKernel& fg, AdjCellsKernel& ck,
+              AdjGatherKernel& gk) {
     switch (slot_width(K)) {
-        case 4: pick_adj_k<DIM, 4>(rusanov, fk, ck, gk); break;
-        case 6: pick_adj_k<DIM, 6>(rusanov, fk, ck, gk); break;
-        default: pick_adj_k<DIM, 8>(rusanov, fk, ck, gk); break;
+        case 4: pick_adj_k<DIM, 4>(rusanov, fk, fg, ck, gk); break;
+        case 6: pick_adj_k<DIM, 6>(rusanov, fk, fg, ck, gk); break;
+        default: pick_adj_k<DIM, 8>(rusanov, fk, fg, ck, gk); break;
     }
 }
 template <typename G, int DIM>
@@ -2671,6 +2799,26 @@ GatherKernel pick_gather(bool species, int K) {
         case 6: return species ? k_gather<DIM, true, 6> : k_gather<DIM, false, 6>;
         default: return species ? k_gather<DIM, true, 8> : k_gather<DIM, false, 8>;
     }
+}
+
+using FusedKernel = void (*)(MeshView<double>, Phys, const double*, FusedView, double*, int);
+template <int DIM, int KM>
+FusedKernel pick_fused_k(bool recon, bool visc, int lim) {
+    if (lim == kLimPetsc) return nullptr;
+    if (recon) {
+        if (visc) return lim == kLimBJ ? k_fused<DIM, true, true, kLimBJ, KM> : k_fused<DIM, true, true, kLimNone, KM>;
+        return lim == kLimBJ ? k_fused<DIM, true, false, kLimBJ, KM> : k_fused<DIM, true, false, kLimNone, KM>;
+    }
+    return visc ? k_fused<DIM, false, true, kLimNone, KM> : nullptr;
+}
+// Fused kernels exist for quads / triangles (2-D, 4 slots), tetrahedra (4) and hexahedra, prisms, pyramids (6).
+template <int DIM>
+FusedKernel pick_fused(bool recon, bool visc, int lim, int K) {
+    const int KM = slot_width(K);
+    if (KM == 4) return pick_fused_k<DIM, 4>(recon, visc, lim);
+    if constexpr (DIM == 3)
+        if (KM == 6) return pick_fused_k<DIM, 6>(recon, visc, lim);
+    return nullptr;
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_rows(double* __restrict__ R, int64_t N, int64_t start, int nc) {
@@ -2928,6 +3076,12 @@ struct abl_fv_solver {
     size_t tile_lds = 0;
     double tile_redundancy = 0.0;  // face evaluations of the tile kernel per face
     abl::TileKernel tile_kernel = nullptr;
+    // fused strategy (built at creation when available)
+    abl::DevBuf<int32_t> fu_start, fu_faces, fu_code, fu_slot, fu_hstart, fu_hcell, fu_hinfo;
+    int fu_T = 0, fu_nfm = 0, fu_nem = 0;
+    int64_t fu_n = 0;
+    size_t fu_lds = 0;
+    abl::FusedKernel fused_kernel = nullptr;
     // mixed form: FP32 geometry and gradients
     abl::DevBuf<float> fna_f, fdx_f, fwa_f;
     abl::DevBuf<float> sw_f, swa_f, slim_f;  // slot-major gradient geometry (mixed form)
@@ -2946,6 +3100,8 @@ struct abl_fv_solver {
     abl::VjpKernel vjp_kernel = nullptr;
     // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
     abl::AdjFacesKernel adj_faces = nullptr;
+    abl::AdjFaceGatherKernel adj_face_gather = nullptr;
+    abl::DevBuf<double> adjW, adjGL;  // face adjoint records, boundary-face species extras
     abl::AdjCellsKernel adj_cells = nullptr;
     abl::AdjGatherKernel adj_gather = nullptr;
     abl::DevBuf<int32_t> mirror;
@@ -3104,6 +3260,55 @@ struct abl_fv_solver {
         tile_redundancy = (double)tl.faces.size() / (double)std::max<int64_t>(host.F, 1);
         return true;
     }
+    // Fused strategy tiles: the largest of 128 / 64 cells whose LDS (tile records, halo face entries, face fluxes)
+    // leaves two workgroups per CU.  Needs the host cell -> face table and face offsets, so it is built at creation.
+    // Returns false when the strategy is unavailable (no kernel for this handle, an ill-conditioned least-squares
+    // stencil, or no tile size fits).
+    bool build_fused(const abl::Layout& lay) {
+        if (!fused_kernel) return false;
+        if ((recon || visc) && !abl::lsq_stencils_full_rank(lay)) return false;
+        const int KM = abl::slot_width(lay.K);
+        const int ne = lay.dim + 2, nt = lay.dim * (lay.dim + 1) / 2 + lay.dim;
+        const int rw = ne + (recon ? ne * lay.dim : 0) + (visc ? nt : 0);
+        const int hw = ne + (visc ? lay.dim + nt : 0);
+        abl::FusedTiles ft;
+        size_t lds = 0;
+        for (int T = 128; T >= 64; T /= 2) {
+            abl::build_fused_tiles(lay, KM, T, ft);
+            lds = sizeof(double) * ((size_t)rw * T + (size_t)hw * ft.nem + (size_t)ne * ft.nfm);
+            if (lds <= (size_t)abl::kCuLdsBytes / 2) break;
+            ft.n = 0;
+        }
+        if (ft.n == 0) return false;
+        HIP_CHECK(hipFuncSetAttribute((const void*)fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        fu_start.upload(ft.fstart);
+        fu_faces.upload(ft.ffaces);
+        fu_code.upload(ft.fcode);
+        fu_slot.upload(ft.fslot);
+        fu_hstart.upload(ft.hstart);
+        fu_hcell.upload(ft.hcell);
+        fu_hinfo.upload(ft.hinfo);
+        fu_T = ft.T;
+        fu_n = ft.n;
+        fu_nfm = ft.nfm;
+        fu_nem = ft.nem;
+        fu_lds = lds;
+        return true;
+    }
+    abl::FusedView fused_view() const {
+        abl::FusedView v;
+        v.fstart = fu_start.ptr;
+        v.ffaces = fu_faces.ptr;
+        v.fcode = fu_code.ptr;
+        v.fslot = fu_slot.ptr;
+        v.hstart = fu_hstart.ptr;
+        v.hcell = fu_hcell.ptr;
+        v.hinfo = fu_hinfo.ptr;
+        v.T = fu_T;
+        v.nfm = fu_nfm;
+        v.nem = fu_nem;
+        return v;
+    }
     // Default strategy.  The tile kernel evaluates faces on tile borders twice and holds its face fluxes in LDS;
     // the face-flux strategy evaluates every face once but writes and re-reads the fluxes through HBM.  Tiles win
     // while the border overhead stays below ~1.8 evaluations per face (rows of a structured box: 1.2-1.7) and the
@@ -3111,6 +3316,7 @@ struct abl_fv_solver {
     // a tile (bandwidth-reducing orders of unstructured meshes: ~2 evaluations per face) and wide species LDS rows
     // go to face flux.
     int auto_strategy() const {
+        if (fu_n > 0) return ABL_FV_STRATEGY_FUSED;
         if (tile_n > 0) {
             const double wgs_per_cu = std::floor((double)abl::kCuLdsBytes / (double)std::max<size_t>(tile_lds, 1));
             const double waves_per_simd = wgs_per_cu * (tile_T / 64.0) / 4.0;
@@ -3119,8 +3325,8 @@ struct abl_fv_solver {
         return ABL_FV_STRATEGY_FACE_FLUX;
     }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + adjA.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+                         gradA.bytes() + rec.bytes() + adjA.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
 };
@@ -3204,6 +3410,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->face_flux_kernel = abl::pick_face_flux<3>(s->recon, s->visc, species);
             s->tile_kernel = abl::pick_tile<3>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<3>(species, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<3>(s->recon, s->visc, lim, L.K);
         } else {
             s->species_kernel = abl::pick_species_gradients<double, 2>(L.K);
             s->species_kernel_f = abl::pick_species_gradients<float, 2>(L.K);
@@ -3212,6 +3419,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->face_flux_kernel = abl::pick_face_flux<2>(s->recon, s->visc, species);
             s->tile_kernel = abl::pick_tile<2>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<2>(species, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<2>(s->recon, s->visc, lim, L.K);
         }
 
         s->ell.upload(L.ell);
@@ -3297,11 +3505,11 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             if (mesh->dim == 3) {
                 s->vjp_kernel = abl::pick_vjp<3>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<3>(rus);
-                abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_cells, s->adj_gather);
+                abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_face_gather, s->adj_cells, s->adj_gather);
             } else {
                 s->vjp_kernel = abl::pick_vjp<2>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<2>(rus);
-                abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_cells, s->adj_gather);
+                abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_face_gather, s->adj_cells, s->adj_gather);
             }
             if (s->recon || s->visc) {
                 // mirror slots: for slot k of cell c, the slot of the same face in the neighbour's list
@@ -3356,6 +3564,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->cfl_out.alloc(1);
         }
         s->build_tiles(L);
+        s->build_fused(L);
         s->strategy = s->auto_strategy();
         abl::resolve_noalloc_scratch();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
@@ -3502,6 +3711,11 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                                        accumulate);
                 });
             }
+        } else if (h->strategy == ABL_FV_STRATEGY_FUSED) {
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->fused_kernel, dim3((unsigned)h->fu_n), dim3(256), h->fu_lds, st, h->view(), h->ph, U,
+                                   h->fused_view(), R, accumulate);
+            });
         } else {
         const auto v = h->view();
         if (h->recon || h->visc) {
@@ -3549,8 +3763,12 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
 extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
     if (!h) return fail(h, "abl_fv_set_strategy: null handle");
     if (strategy != ABL_FV_STRATEGY_AUTO && strategy != ABL_FV_STRATEGY_RECOMPUTE &&
-        strategy != ABL_FV_STRATEGY_FACE_FLUX && strategy != ABL_FV_STRATEGY_TILE)
+        strategy != ABL_FV_STRATEGY_FACE_FLUX && strategy != ABL_FV_STRATEGY_TILE && strategy != ABL_FV_STRATEGY_FUSED)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
+    if (strategy == ABL_FV_STRATEGY_FUSED && h->fu_n == 0)
+        return fail(h, "abl_fv_set_strategy: the fused strategy is unavailable for this handle (it needs the FP64 form "
+                       "without species, no limiter or Barth-Jespersen, 2-D cells or tetrahedra / hexahedra, prisms, "
+                       "pyramids, well-conditioned least-squares stencils and a tile that fits the LDS)");
     if (strategy == ABL_FV_STRATEGY_AUTO) strategy = h->auto_strategy();
     if (strategy == ABL_FV_STRATEGY_TILE && h->tile_n == 0)
         return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (no tile size fits the "
@@ -3642,11 +3860,14 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
         const int nt = L.dim * (L.dim + 1) / 2 + L.dim;
         const int64_t NG = L.N - L.NI;
         const size_t na = (size_t)L.nc + (size_t)L.nc * L.dim + nt + (size_t)L.ns * L.dim;
+        const int nwp = L.dim == 3 ? abl::AdjFaceRec<3>::pairs(L.ns) : abl::AdjFaceRec<2>::pairs(L.ns);
         if (!h->adjA.ptr) {
             h->adjA.alloc(na * L.NI);
             h->adjUghost.alloc((size_t)L.nc * std::max<int64_t>(NG, 1));
             h->adjUself.alloc((size_t)L.nc * L.NI);
             h->adjContrib.alloc((size_t)KM * L.nc * L.NI);
+            h->adjW.alloc((size_t)2 * nwp * ((L.F + 63) / 64 * 64));
+            h->adjGL.alloc((size_t)std::max(L.ns, 1) * std::max<int64_t>(NG, 1));
         }
         const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
         if (h->n_bfaces)
@@ -3664,16 +3885,12 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
             h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
                 hipLaunchKernelGGL(h->species_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
             });
-        const size_t sp_lds = (size_t)L.ns * (1 + 2 * L.dim) * sizeof(double) * abl::kBlock;
-        const bool use_lds = L.ns > 0 && sp_lds <= (size_t)abl::kTileLdsMax;
-        if (use_lds)
-            HIP_CHECK(hipFuncSetAttribute((const void*)h->adj_faces, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)sp_lds));
         abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM,
-                       use_lds ? 1 : 0};
+                       h->adjW.ptr, nwp, h->adjGL.ptr};
         h->timed(ABL_FV_PHASE_VJP, st, [&] {
-            hipLaunchKernelGGL(h->adj_faces, grid, block, use_lds ? sp_lds : 0, st, v, h->ph, U, Rbar, h->rec.ptr,
+            hipLaunchKernelGGL(h->adj_faces, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, Rbar, h->rec.ptr,
                                h->gradC.ptr, h->gradA.ptr, b);
+            hipLaunchKernelGGL(h->adj_face_gather, grid, block, 0, st, v, h->ph, U, b);
             hipLaunchKernelGGL(h->adj_cells, grid, block, 0, st, v, h->ph, U, h->ghost_slot.ptr, h->slot_kind.ptr, b);
             hipLaunchKernelGGL(h->adj_gather, grid, block, 0, st, v, h->ph, h->ghost_slot.ptr, h->slot_kind.ptr, b,
                                Ubar);

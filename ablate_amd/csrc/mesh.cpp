@@ -8,9 +8,8 @@
 //     across the face plane and the interior volume (the DMPlex ghost-cell convention);
 //   - periodic directions produce wrap faces whose right cell is translated into the left cell's frame by
 //     face_shift (the localized-coordinate image).
-// Cell orderings: natural (x fastest), brick (8^3 / 16^2 bricks in Morton order of the bricks), Morton order of the
-// cell centroids (unstructured meshes) and reverse
-// Cuthill-McKee.
+// Cell orderings: natural (x fastest), brick (8^3 / 16^2 bricks in Morton order of the bricks; a 3-D brick walked as
+// four 8x4x4 blocks), Morton order of the cell centroids (unstructured meshes) and reverse Cuthill-McKee.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -67,14 +66,20 @@ std::vector<int64_t> box_order(int dim, const int32_t* n, int32_t ordering) {
         std::sort(bricks.begin(), bricks.end());
         int64_t k = 0;
         const int64_t nz = dim == 3 ? n[2] : 1;
+        // 3-D bricks are walked as four 8x4x4 blocks (y halves inside z halves), so that every 128 consecutive cells
+        // of a whole brick form an 8x4x4 block and every 256 an 8x8x4 slab (compact cell tiles)
+        const int SB = dim == 3 ? B / 2 : B;
         for (auto& br : bricks) {
             int64_t b = br.second;
             int64_t bx = b % nb[0], by = (b / nb[0]) % nb[1], bz = b / (nb[0] * nb[1]);
             int64_t z0 = dim == 3 ? bz * B : 0, z1 = dim == 3 ? std::min<int64_t>(nz, z0 + B) : 1;
-            for (int64_t z = z0; z < z1; z++)
-                for (int64_t y = by * B; y < std::min<int64_t>(n[1], by * B + B); y++)
-                    for (int64_t x = bx * B; x < std::min<int64_t>(n[0], bx * B + B); x++)
-                        order[k++] = x + (int64_t)n[0] * (y + (int64_t)n[1] * z);
+            int64_t y0 = by * B, y1 = std::min<int64_t>(n[1], y0 + B);
+            for (int64_t zs = z0; zs < z1; zs += SB)
+                for (int64_t ys = y0; ys < y1; ys += SB)
+                    for (int64_t z = zs; z < std::min<int64_t>(z1, zs + SB); z++)
+                        for (int64_t y = ys; y < std::min<int64_t>(y1, ys + SB); y++)
+                            for (int64_t x = bx * B; x < std::min<int64_t>(n[0], bx * B + B); x++)
+                                order[k++] = x + (int64_t)n[0] * (y + (int64_t)n[1] * z);
         }
         return order;
     }

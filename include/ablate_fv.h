@@ -304,17 +304,21 @@ int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max
 /* ---- evaluation strategy (all deterministic, no float atomics).  The reference has one CPU loop order; these
  *      have no counterpart there and all give the residual of ComputeRHSFunction. ------------------------------ */
 enum {
-    ABL_FV_STRATEGY_AUTO = 0,      /* pick TILE or FACE_FLUX from the mesh's tile statistics (the default) */
+    ABL_FV_STRATEGY_AUTO = 0,      /* pick FUSED, TILE or FACE_FLUX from the mesh's tile statistics (the default) */
     ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice
                                       (mixed form: no species) */
     ABL_FV_STRATEGY_FACE_FLUX = 2, /* one lane per face stores its flux once; a second pass gathers per cell
                                       (FP64: 8 * (n_euler + n_species rounded up to even) bytes per face; mixed:
                                       8 * (n_euler + n_species)) */
-    ABL_FV_STRATEGY_TILE = 3       /* one workgroup per tile of consecutive cells evaluates the tile's faces once into
+    ABL_FV_STRATEGY_TILE = 3,      /* one workgroup per tile of consecutive cells evaluates the tile's faces once into
                                       LDS and gathers its cells; bitwise equal to FACE_FLUX */
+    ABL_FV_STRATEGY_FUSED = 4      /* one workgroup per tile computes the gradients of its cells and of its halo into
+                                      LDS (least-squares normal equations solved per cell, no stored weights), then
+                                      the tile's face fluxes and the gather: no cell record or face flux in HBM.
+                                      FP64 form without species, no limiter or Barth-Jespersen, full-rank stencils */
 };
-/* AUTO chooses TILE when the tile kernel evaluates at most ~1.8 faces per face (border faces twice) and its LDS
- * footprint leaves two or more waves per SIMD, FACE_FLUX otherwise.  Selecting FACE_FLUX allocates its face-flux
+/* AUTO chooses FUSED when it is available, else TILE when the tile kernel evaluates at most ~1.8 faces per face
+ * (border faces twice) and its LDS footprint leaves two or more waves per SIMD, FACE_FLUX otherwise.  Selecting FACE_FLUX allocates its face-flux
  * storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
 int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);
 /* Cells per tile of the tile strategy: 0 = the default choice (the largest of 256/128/64 whose face fluxes fit

@@ -9,7 +9,14 @@ pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-12
 CASES = standard_cases()
-STRATEGIES = ["recompute", "face_flux", "tile"]
+STRATEGIES = ["recompute", "face_flux", "tile", "fused"]
+
+
+def use_strategy(solver, strategy):
+    """Selects a strategy; the fused one exists only for some handles (FP64, no species, full-rank stencils)."""
+    if strategy == "fused" and not solver.has_strategy("fused"):
+        pytest.skip("fused strategy unavailable for this handle")
+    solver.set_strategy(strategy)
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
@@ -20,7 +27,7 @@ def test_rhs_matches_oracle(case, strategy, torch_cuda):
     U0 = case.initial_state()
     Rref, Ufilled = case.oracle_rhs(U0.copy())
     solver = case.solver()
-    solver.set_strategy(strategy)
+    use_strategy(solver, strategy)
     U = torch.tensor(U0, dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -41,7 +48,7 @@ def test_rhs_matches_oracle(case, strategy, torch_cuda):
 def test_rhs_deterministic_and_accumulates(case, strategy, torch_cuda):
     torch = torch_cuda
     solver = case.solver()
-    solver.set_strategy(strategy)
+    use_strategy(solver, strategy)
     U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
     R1 = torch.zeros_like(U)
     R2 = torch.zeros_like(U)
@@ -63,7 +70,7 @@ def test_conservation_periodic(strategy, torch_cuda):
     torch = torch_cuda
     case = next(c for c in CASES if c.name == "c2_ns_lsq_bj")
     solver = case.solver()
-    solver.set_strategy(strategy)
+    use_strategy(solver, strategy)
     U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
     R = torch.zeros_like(U)
     solver.compute_rhs(0.0, U, R)
@@ -106,10 +113,50 @@ def test_tile_equals_face_flux_bitwise(case, torch_cuda):
     assert torch.equal(Ra, Rb), (Ra - Rb).abs().max().item()
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
+def test_fused_agrees_with_face_flux(case, torch_cuda):
+    """The fused kernel solves each cell's least-squares normal equations instead of applying the stored
+    pseudo-inverse weights, so it agrees with the face-flux residual to rounding (not bitwise); it is deterministic
+    run to run."""
+    torch = torch_cuda
+    solver = case.solver()
+    use_strategy(solver, "fused")
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    Ra, Rb, Rc = torch.zeros_like(U), torch.zeros_like(U), torch.zeros_like(U)
+    solver.compute_rhs(0.0, U, Ra)
+    solver.compute_rhs(0.0, U, Rc)
+    solver.set_strategy("face_flux")
+    solver.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    assert torch.equal(Ra, Rc)
+    NI = case.mesh.n_interior
+    err = ((Ra - Rb)[:, :NI].norm(dim=1) / Rb[:, :NI].norm(dim=1).clamp_min(1e-300)).max().item()
+    assert err <= 1e-13, err
+
+
+def test_fused_availability(torch_cuda):
+    """The fused strategy is offered for FP64 handles without species and refused with a message otherwise."""
+    from ablate_amd.mesh import BoxMesh
+
+    box = BoxMesh([8, 8, 8], [0, 0, 0], [1, 1, 1], [False, True, True])
+    plain = Case("fused_box", box, recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125,
+                 bcs=inflow_x(0))
+    assert plain.solver().has_strategy("fused")
+    assert not plain.solver(form="mixed").has_strategy("fused")
+    species = Case("fused_species", box, recon="leastsquares", limiter="barthjespersen", n_species=3, bcs=inflow_x(3))
+    s = species.solver()
+    assert not s.has_strategy("fused")
+    with pytest.raises(RuntimeError, match="fused strategy is unavailable"):
+        s.set_strategy("fused")
+    petsc = Case("fused_minmod", box, recon="leastsquares", limiter="minmod", bcs=inflow_x(0))
+    assert not petsc.solver().has_strategy("fused")
+
+
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
 def test_auto_strategy_choice(form, torch_cuda):
-    """The default strategy: tiles on a structured box (row tiles evaluate <= ~1.7 faces per face), face flux on a
-    tetrahedral mesh in bandwidth-reducing order (~2 evaluations per face) and where species rows fill the LDS."""
+    """The default strategy: the fused kernel wherever it exists (FP64 without species), else tiles on a structured
+    box (row tiles evaluate <= ~1.7 faces per face), face flux on a tetrahedral mesh in bandwidth-reducing order
+    (~2 evaluations per face) and where species rows fill the LDS."""
     from ablate_amd.mesh import BoxMesh, TetBoxMesh
 
     box = Case("auto_box", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [True] * 3), recon="leastsquares",
@@ -120,14 +167,15 @@ def test_auto_strategy_choice(form, torch_cuda):
                 bcs=all_outflow(6))
     species = Case("auto_species", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [False, True, True]), n_species=10,
                    bcs=inflow_x(10))
-    assert box.solver(form=form).strategy() == "tile"
-    assert tets.solver(form=form).strategy() == "face_flux"
+    best = "fused" if form == "fp64" else "tile"
+    assert box.solver(form=form).strategy() == best
+    assert tets.solver(form=form).strategy() == ("fused" if form == "fp64" else "face_flux")
     assert species.solver(form=form).strategy() == "face_flux"
     s = box.solver(form=form)
     s.set_strategy("face_flux")
     assert s.strategy() == "face_flux"
     s.set_strategy("auto")
-    assert s.strategy() == "tile"
+    assert s.strategy() == best
 
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
