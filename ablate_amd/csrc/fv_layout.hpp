@@ -61,6 +61,9 @@ struct FusedTiles {
     std::vector<int32_t> fstart, ffaces, fcode, fslot, hstart, hcell, hinfo;
 };
 void build_fused_tiles(const Layout& L, int KM, int T, FusedTiles& out);
+// Checks every index the fused kernel takes from the lists against the mesh and the list bounds; throws
+// std::logic_error naming the first violated invariant.
+void check_fused_tiles(const Layout& L, int KM, const FusedTiles& ft);
 
 // True when every interior cell's least-squares stencil (its interior face neighbours, offsets from fdx) has full
 // rank, so that the normal equations the fused kernel solves give the pseudo-inverse solution.

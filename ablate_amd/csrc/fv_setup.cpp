@@ -388,6 +388,97 @@ void build_fused_tiles(const Layout& L, int KM, int T, FusedTiles& out) {
     }
 }
 
+void check_fused_tiles(const Layout& L, int KM, const FusedTiles& ft) {
+    const int64_t NI = L.NI, F = L.F;
+    const int T = ft.T;
+    auto bad = [](const std::string& what) { throw std::logic_error("fused tiles: " + what); };
+    if (T <= 0 || T > 256 || KM < L.K || KM > 8) bad("tile size or slot width out of range");
+    if ((int64_t)ft.fstart.size() != ft.n + 1 || (int64_t)ft.hstart.size() != ft.n + 1 || ft.n * T < NI ||
+        (ft.n - 1) * T >= NI)
+        bad("tile count does not cover the interior cells");
+    if (ft.fcode.size() != ft.ffaces.size() || ft.hinfo.size() != ft.hcell.size() ||
+        ft.fslot.size() != (size_t)KM * (size_t)NI)
+        bad("list sizes disagree");
+    if (ft.fstart[0] != 0 || ft.hstart[0] != 0 || ft.fstart[ft.n] != (int32_t)ft.ffaces.size() ||
+        ft.hstart[ft.n] != (int32_t)ft.hcell.size())
+        bad("list offsets do not span the lists");
+    std::vector<int32_t> side_code;
+    for (int64_t t = 0; t < ft.n; t++) {
+        const int64_t c0 = t * T;
+        const int nown = (int)std::min<int64_t>(T, NI - c0);
+        const int32_t f0 = ft.fstart[t], nf = ft.fstart[t + 1] - f0;
+        const int32_t h0 = ft.hstart[t], nh = ft.hstart[t + 1] - h0;
+        if (nf < 0 || nf > ft.nfm || nh < 0) bad("face or halo count of a tile out of range");
+        // halo entries: consecutive per halo cell in halo order, each bit of the mask a slot of the cell
+        int32_t ne = 0;
+        for (int32_t i = 0; i < nh; i++) {
+            const int64_t h = ft.hcell[(size_t)(h0 + i)];
+            const int32_t info = ft.hinfo[(size_t)(h0 + i)];
+            if (h < 0 || h >= NI || (h >= c0 && h < c0 + nown)) bad("halo cell out of range or inside its tile");
+            const int32_t mask = info & 0xFF;
+            if ((info >> 8) != ne || (mask >> L.K) != 0) bad("halo entry index or slot mask inconsistent");
+            for (int k = 0; k < L.K; k++) {
+                if (!((mask >> k) & 1)) continue;
+                if (L.ell[(size_t)k * NI + h] < 0) bad("halo slot mask names an empty slot");
+                ne++;
+            }
+        }
+        if (ne > ft.nem || (int64_t)T + ne >= 0xFFFF) bad("halo face entries exceed the tile's LDS rows");
+        // side codes of each tile face against the mesh
+        for (int32_t j = 0; j < nf; j++) {
+            const int64_t f = ft.ffaces[(size_t)(f0 + j)];
+            if (f < 0 || f >= F || (j > 0 && ft.ffaces[(size_t)(f0 + j - 1)] >= f)) bad("tile faces not ascending");
+            const uint32_t code = (uint32_t)ft.fcode[(size_t)(f0 + j)];
+            const int lc = (int)(code & 0xFFFF), rc = (int)(code >> 16);
+            const int64_t a = L.fcell[(size_t)f], b = L.fcell[(size_t)F + (size_t)f];
+            auto check_side = [&](int sc, int64_t cell) {
+                if (sc < nown) {
+                    if (c0 + sc != cell) bad("tile-cell side code names another cell");
+                } else if (sc < T || sc >= T + ne) {
+                    bad("side code outside the tile cells and halo entries");
+                }
+            };
+            check_side(lc, a);
+            if (rc == 0xFFFF) {
+                if (b < NI || lc >= nown) bad("ghost face without a tile cell on its left");
+            } else {
+                if (b >= NI) bad("ghost face with a non-ghost side code");
+                check_side(rc, b);
+            }
+            if (lc >= T && rc >= T && rc != 0xFFFF) bad("tile face with no tile cell");
+        }
+        // every halo entry names a tile face of its halo cell on the side where that cell is
+        int32_t e = 0;
+        for (int32_t i = 0; i < nh; i++) {
+            const int64_t h = ft.hcell[(size_t)(h0 + i)];
+            const int32_t mask = ft.hinfo[(size_t)(h0 + i)] & 0xFF;
+            for (int k = 0; k < L.K; k++) {
+                if (!((mask >> k) & 1)) continue;
+                const int32_t s = L.ell[(size_t)k * NI + h];
+                const int32_t f = s >> 1;
+                const auto it = std::lower_bound(ft.ffaces.begin() + f0, ft.ffaces.begin() + f0 + nf, f);
+                if (it == ft.ffaces.begin() + f0 + nf || *it != f) bad("halo entry names a face outside its tile");
+                const uint32_t code = (uint32_t)ft.fcode[(size_t)(it - ft.ffaces.begin())];
+                const int sc = (s & 1) ? (int)(code >> 16) : (int)(code & 0xFFFF);
+                if (sc != T + e) bad("halo entry and face side code disagree");
+                e++;
+            }
+        }
+        // slots of the tile cells name their tile-local faces and sides
+        for (int k = 0; k < KM; k++)
+            for (int64_t c = c0; c < c0 + nown; c++) {
+                const int32_t s = k < L.K ? L.ell[(size_t)k * NI + c] : -1;
+                const int32_t v = ft.fslot[(size_t)k * NI + c];
+                if (s < 0) {
+                    if (v != -1) bad("padding slot not marked");
+                    continue;
+                }
+                if (v < 0 || (v >> 1) >= nf || ft.ffaces[(size_t)(f0 + (v >> 1))] != (s >> 1) || (v & 1) != (s & 1))
+                    bad("tile-cell slot names the wrong face or side");
+            }
+    }
+}
+
 bool lsq_stencils_full_rank(const Layout& L) {
     const int dim = L.dim;
     const int64_t NI = L.NI, F = L.F;

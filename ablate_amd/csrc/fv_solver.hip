@@ -3280,6 +3280,7 @@ struct abl_fv_solver {
             ft.n = 0;
         }
         if (ft.n == 0) return false;
+        abl::check_fused_tiles(lay, KM, ft);
         HIP_CHECK(hipFuncSetAttribute((const void*)fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         fu_start.upload(ft.fstart);
         fu_faces.upload(ft.ffaces);
@@ -3315,8 +3316,9 @@ struct abl_fv_solver {
     // LDS footprint leaves at least two waves per SIMD for latency hiding; cell orders with little locality within
     // a tile (bandwidth-reducing orders of unstructured meshes: ~2 evaluations per face) and wide species LDS rows
     // go to face flux.
+    // The fused strategy is offered on request only: on the 256^3 hex workload it took 8.3 ms per RHS against
+    // 5.3 ms for gradients + tiles (two workgroups per CU by LDS and registers; its phases wait on memory).
     int auto_strategy() const {
-        if (fu_n > 0) return ABL_FV_STRATEGY_FUSED;
         if (tile_n > 0) {
             const double wgs_per_cu = std::floor((double)abl::kCuLdsBytes / (double)std::max<size_t>(tile_lds, 1));
             const double waves_per_simd = wgs_per_cu * (tile_T / 64.0) / 4.0;

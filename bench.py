@@ -115,14 +115,16 @@ def ns_processes(fv, mu, k):
 
 def cpu_baseline(n, seconds, mu, k, mach):
     """Times the CPU oracle (a C restatement of the reference loops) on the host cores: one process per core, each on
-    its own n^3 sample of the same workload (the reference runs one MPI rank per core), plus one process alone.  Runs
-    as a child process (oracle/cpu_baseline.py) so that its workers never share this process's GPU context."""
+    its own n^3 sample of the same workload (the reference runs one MPI rank per core), plus one process alone; then
+    the physics of the other legs (64 x 64 first-order Euler, first-order Euler + 10 species with inflow, tets with
+    NS + 10 species) under `other_workloads`.  Runs as a child process (oracle/cpu_baseline.py) so that its workers
+    never share this process's GPU context."""
     import subprocess
 
     root = os.path.dirname(os.path.abspath(__file__))
-    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--seconds", str(seconds), "--mu", repr(mu),
-           "--k", repr(k), "--mach", repr(mach)]
-    out = subprocess.run(cmd, cwd=root, capture_output=True, text=True, timeout=max(120.0, 6 * seconds + 60))
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--workload", "all", "--n", str(n), "--seconds", str(seconds),
+           "--other-seconds", str(max(1.0, 0.4 * seconds)), "--mu", repr(mu), "--k", repr(k), "--mach", repr(mach)]
+    out = subprocess.run(cmd, cwd=root, capture_output=True, text=True, timeout=max(240.0, 12 * seconds + 120))
     if out.returncode != 0:
         return {"error": f"cpu baseline failed ({out.returncode}): {out.stderr[-400:]}"}
     return json.loads(out.stdout.strip().splitlines()[-1])

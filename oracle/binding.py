@@ -196,29 +196,35 @@ def physics(gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", recon="upwind", lim
                       ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha, compact=int(bool(compact)))
 
 
+def _bc_array(bcs):
+    """Boundary slots {slot: dict(kind, rho, vel, T, Yi, enforce_at_face)} as the oracle's orc_bc array; returns
+    (array or None, slot count, buffers the array points into)."""
+    if not bcs:
+        return None, 0, []
+    n_bcs = max(bcs) + 1
+    bc_arr = (OrcBc * n_bcs)()
+    ykeep = []
+    for slot, b in bcs.items():
+        bc_arr[slot].kind = b["kind"]
+        bc_arr[slot].rho = b.get("rho", 0.0)
+        v = list(b.get("vel", [0.0, 0.0, 0.0])) + [0.0] * 3
+        bc_arr[slot].vel = (c_double * 3)(*v[:3])
+        bc_arr[slot].T = b.get("T", 0.0)
+        bc_arr[slot].enforce_at_face = int(bool(b.get("enforce_at_face", False)))
+        if b.get("Yi") is not None:
+            y = np.ascontiguousarray(b["Yi"], dtype=np.float64)
+            ykeep.append(y)
+            bc_arr[slot].Yi = _ptr(y)
+    return bc_arr, n_bcs, ykeep
+
+
 def rhs(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, bcs=None, time: float = 0.0):
     """Oracle RHS from a zeroed locF. Returns (R_soa, U_soa with ghost rows filled)."""
     nc, N = U_soa.shape
     X = np.ascontiguousarray(U_soa.T)  # AoS [N][nc]
     F = np.zeros_like(X)
     ms, keep = _mesh_struct(mesh_arrays)
-    bc_arr = None
-    n_bcs = 0
-    ykeep = []
-    if bcs:
-        n_bcs = max(bcs) + 1
-        bc_arr = (OrcBc * n_bcs)()
-        for slot, b in bcs.items():
-            bc_arr[slot].kind = b["kind"]
-            bc_arr[slot].rho = b.get("rho", 0.0)
-            v = list(b.get("vel", [0.0, 0.0, 0.0])) + [0.0] * 3
-            bc_arr[slot].vel = (c_double * 3)(*v[:3])
-            bc_arr[slot].T = b.get("T", 0.0)
-            bc_arr[slot].enforce_at_face = int(bool(b.get("enforce_at_face", False)))
-            if b.get("Yi") is not None:
-                y = np.ascontiguousarray(b["Yi"], dtype=np.float64)
-                ykeep.append(y)
-                bc_arr[slot].Yi = _ptr(y)
+    bc_arr, n_bcs, ykeep = _bc_array(bcs)
     rc = lib().orc_rhs(ctypes.byref(ms), ctypes.byref(phys), bc_arr, n_bcs, time, _ptr(X), _ptr(F))
     if rc:
         raise RuntimeError(f"orc_rhs failed with status {rc}")
@@ -236,9 +242,10 @@ class Context:
         if rc:
             raise RuntimeError(f"orc_create failed with status {rc}")
 
-    def rhs_aos(self, X: np.ndarray, F: np.ndarray) -> None:
-        """locF += RHS(locX) on AoS arrays [N][nc] (periodic meshes: no boundary conditions)."""
-        rc = lib().orc_rhs_ctx(self._ctx, None, 0, 0.0, _ptr(X), _ptr(F))
+    def rhs_aos(self, X: np.ndarray, F: np.ndarray, bcs=None) -> None:
+        """locF += RHS(locX) on AoS arrays [N][nc]; ``bcs`` as for rhs() (None on periodic meshes)."""
+        bc_arr, n_bcs, _keep = _bc_array(bcs)
+        rc = lib().orc_rhs_ctx(self._ctx, bc_arr, n_bcs, 0.0, _ptr(X), _ptr(F))
         if rc:
             raise RuntimeError(f"orc_rhs_ctx failed with status {rc}")
 

@@ -154,9 +154,9 @@ def test_fused_availability(torch_cuda):
 
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
 def test_auto_strategy_choice(form, torch_cuda):
-    """The default strategy: the fused kernel wherever it exists (FP64 without species), else tiles on a structured
-    box (row tiles evaluate <= ~1.7 faces per face), face flux on a tetrahedral mesh in bandwidth-reducing order
-    (~2 evaluations per face) and where species rows fill the LDS."""
+    """The default strategy: tiles on a structured box (row tiles evaluate <= ~1.7 faces per face), face flux on a
+    tetrahedral mesh in bandwidth-reducing order (~2 evaluations per face) and where species rows fill the LDS; the
+    fused strategy only on request."""
     from ablate_amd.mesh import BoxMesh, TetBoxMesh
 
     box = Case("auto_box", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [True] * 3), recon="leastsquares",
@@ -167,9 +167,9 @@ def test_auto_strategy_choice(form, torch_cuda):
                 bcs=all_outflow(6))
     species = Case("auto_species", BoxMesh([32, 32, 32], [0, 0, 0], [1, 1, 1], [False, True, True]), n_species=10,
                    bcs=inflow_x(10))
-    best = "fused" if form == "fp64" else "tile"
+    best = "tile"
     assert box.solver(form=form).strategy() == best
-    assert tets.solver(form=form).strategy() == ("fused" if form == "fp64" else "face_flux")
+    assert tets.solver(form=form).strategy() == "face_flux"
     assert species.solver(form=form).strategy() == "face_flux"
     s = box.solver(form=form)
     s.set_strategy("face_flux")
