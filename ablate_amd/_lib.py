@@ -20,7 +20,7 @@ LIMITER = {
     "barthjespersen": 8,
 }
 BC_NONE, BC_INFLOW, BC_OUTFLOW, BC_GHOST = 0, 1, 2, 3
-ORDER = {"natural": 0, "brick": 1, "rcm": 2, "morton": 3}
+ORDER = {"natural": 0, "brick": 1, "rcm": 2, "morton": 3, "column": 4}
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
@@ -132,7 +132,7 @@ SIGNATURES = {
     "abl_hbm_probe": (c_int32, [c_int32, c_int64, c_int32, _dp, _dp]),
 }
 PHASES = ["ghost", "gradients", "species_gradients", "flux", "vjp", "gather"]
-STRATEGY = {"auto": 0, "recompute": 1, "face_flux": 2, "tile": 3, "fused": 4}
+STRATEGY = {"auto": 0, "recompute": 1, "face_flux": 2, "tile": 3, "fused": 4, "march": 5}
 
 _lib = None
 

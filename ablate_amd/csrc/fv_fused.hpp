@@ -398,3 +398,260 @@ __global__ __launch_bounds__(256) void k_fused(MeshView<double> m, Phys ph, cons
     }
 }
 
+
+// ================================================================================================ marching strategy
+//
+// One workgroup per sequence of tiles (MarchPlan).  Step s handles tile t_s ("cur") and holds two record slots in
+// LDS: cur's records (computed by the previous step, or by the prologue) and the records of the tile the next step
+// handles ("next"), which step s computes.  Per step:
+//   A  records of the next tile into the other slot, face entries of the step's halo cells (as k_fused phase 1)
+//   B  every face of cur not shared with the previous tile, from LDS, into the step's flux buffer
+//   C  cur's cells gather their faces in slot order from this step's buffer or the previous step's (the faces
+//      shared with the previous tile), -F/V as left cell, +F/V as right cell (cellInterpolant.cpp:604-629)
+// so faces between consecutive tiles of a sequence are evaluated once and no cell's gradients are computed twice
+// inside a sequence.  Two barriers per step: A -> B and B -> C; the next step's A writes only what B has finished
+// reading, and its barrier orders C before the next B overwrites the previous-step buffer.
+
+struct MarchView {
+    const int32_t* __restrict__ seq;      // [n_seq + 1]
+    const int32_t* __restrict__ pro;      // [n_seq]
+    const int32_t* __restrict__ stile;    // [n_steps]
+    const int32_t* __restrict__ snext;    // [n_steps]
+    const int32_t* __restrict__ fstart;   // [n_steps + 1]
+    const int32_t* __restrict__ ffaces;
+    const int32_t* __restrict__ fcode;
+    const int32_t* __restrict__ pfstart;  // [n_seq + 1]
+    const int32_t* __restrict__ pffaces;
+    const int32_t* __restrict__ pfcode;
+    const int32_t* __restrict__ hstart;   // [n_steps + 1]
+    const int32_t* __restrict__ hcell;
+    const int32_t* __restrict__ hinfo;
+    const int32_t* __restrict__ gslot;    // [KM][NI]
+    int T, nfm, nem;
+};
+constexpr int kMarchBlock = 128;
+
+// Record of a tile cell into a slot ([RW][T], word-major so that a wave's stores are consecutive).
+template <int DIM, bool RECON, bool VISC>
+__device__ inline void march_store_record(double* slot, int T, int i, const double* uc, const double (*g)[DIM],
+                                          const double* tq) {
+    using FL = FusedLayout<DIM, RECON, VISC>;
+    constexpr int NE = DIM + 2;
+#pragma unroll
+    for (int p = 0; p < NE; p++) slot[p * T + i] = uc[p];
+    if (RECON)
+#pragma unroll
+        for (int p = 0; p < NE; p++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) slot[(FL::G0 + p * DIM + d) * T + i] = g[p][d];
+    if (VISC)
+#pragma unroll
+        for (int q = 0; q < FL::NT; q++) slot[(FL::T0 + q) * T + i] = tq[q];
+}
+
+// Side data of a face from LDS: code [0, T) a cell of slot `ra`, [T, 2T) of slot `rb`, [2T, ..) a halo entry.
+template <int DIM, bool RECON, bool VISC>
+__device__ inline void march_side(const double* ra, const double* rb, int T, const double* hal, int nem, int code,
+                                  const double* dx, SideData<DIM, VISC>& sd) {
+    if (code < 2 * T) {
+        const bool second = code >= T;
+        fused_side<DIM, RECON, VISC>(second ? rb : ra, T, hal, nem, second ? code - T : code, dx, sd);
+    } else {
+        fused_side<DIM, RECON, VISC>(ra, T, hal, nem, code - T, dx, sd);  // fused_side: code >= T is entry code - T
+    }
+}
+
+// Face j of a list: flux from the two sides' LDS data (ghost right sides from U) into flx[p * nfm + j].
+template <int DIM, bool RECON, bool VISC>
+__device__ inline void march_face(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U, int64_t f,
+                                  int32_t code, const double* ra, const double* rb, int T, const double* hal, int nem,
+                                  double* flx, int nfm, int j) {
+    using SD = SideData<DIM, VISC>;
+    constexpr int NE = DIM + 2;
+    const int lc = code & 0xFFFF, rc = (int)((uint32_t)code >> 16);
+    double nh[3], area, x[2][DIM];
+    load_face_na(m.fna, f, nh, area);
+    load_face_pair<DIM>(m.fdx, f, x);
+    SD sl, sr;
+    march_side<DIM, RECON, VISC>(ra, rb, T, hal, nem, lc, x[0], sl);
+    if (rc == kFusedGhost) {
+        const int64_t Rc = m.fcell[m.F + f];
+        load_side<DIM, RECON, VISC>(U, nullptr, m.N, Rc, true, x[1], sr);
+        if (VISC) {
+            double w[2][DIM];
+            load_face_pair<DIM>(m.fwa, f, w);
+            ghost_visc<DIM, VISC>(ph, w[1], sl, sr);
+        }
+    } else {
+        march_side<DIM, RECON, VISC>(ra, rb, T, hal, nem, rc, x[1], sr);
+    }
+    double flux[NE], cl, cr, cd;
+    face_flux<DIM, VISC>(ph, nh, area, sl, sr, flux, cl, cr, cd);
+#pragma unroll
+    for (int p = 0; p < NE; p++) flx[p * nfm + j] = flux[p];
+}
+
+template <int DIM, bool RECON, bool VISC, int LIM, int KM>
+__global__ __launch_bounds__(kMarchBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_march(
+    MeshView<double> m, Phys ph, const double* __restrict__ U, MarchView mv, double* __restrict__ R, int accumulate) {
+    using FL = FusedLayout<DIM, RECON, VISC>;
+    constexpr int NE = DIM + 2;
+    extern __shared__ double lds[];
+    const int T = mv.T, nem = mv.nem, nfm = mv.nfm;
+    double* rec = lds;                      // [2][RW][T]
+    double* hal = rec + 2 * FL::RW * T;     // [HW][nem]
+    double* flx = hal + FL::HW * nem;       // [2][NE][nfm]
+    const int64_t w = xcd_block_index();
+    const int64_t N = m.N, NI = m.NI;
+    const int tid = threadIdx.x;
+    const int32_t s0 = mv.seq[w], s1 = mv.seq[w + 1];
+    auto tile_cells = [&](int64_t t) { return (int)(NI - t * T < (int64_t)T ? NI - t * T : (int64_t)T); };
+
+    // prologue: records of the first tile (slot 0) and of the prologue tile (slot 1), then their shared faces into
+    // flux buffer 1 (the buffer step 0 reads as its previous step's)
+    {
+        const int64_t t0 = mv.stile[s0], tp = mv.pro[w];
+        const int n0 = tile_cells(t0), np = tp >= 0 ? tile_cells(tp) : 0;
+        for (int i = tid; i < n0 + np; i += blockDim.x) {
+            const bool first = i < n0;
+            const int li = first ? i : i - n0;
+            const int64_t c = (first ? t0 : tp) * T + li;
+            double uc[NE], g[NE][DIM], tq[ViscTerms<DIM>::NT], xf[KM][DIM];
+            fused_cell<DIM, RECON, VISC, LIM, KM>(m, ph, U, c, uc, g, tq, xf);
+            march_store_record<DIM, RECON, VISC>(rec + (first ? 0 : FL::RW * T), T, li, uc, g, tq);
+        }
+        __syncthreads();
+        const int32_t p0 = mv.pfstart[w], np_f = mv.pfstart[w + 1] - p0;
+        for (int j = tid; j < np_f; j += blockDim.x)
+            march_face<DIM, RECON, VISC>(m, ph, U, mv.pffaces[p0 + j], mv.pfcode[p0 + j], rec, rec + FL::RW * T, T, hal,
+                                         nem, flx + NE * nfm, nfm, j);
+        __syncthreads();
+    }
+
+    for (int32_t s = s0; s < s1; s++) {
+        const int par = (s - s0) & 1;
+        double* rc = rec + par * FL::RW * T;        // cur tile
+        double* ro = rec + (par ^ 1) * FL::RW * T;  // next tile
+        double* fc = flx + par * NE * nfm;          // this step's fluxes
+        double* fo = flx + (par ^ 1) * NE * nfm;    // the previous step's (or the prologue's)
+        // A: next tile's records, halo entries
+        {
+            const int64_t tn = mv.snext[s];
+            const int nn = tn >= 0 ? tile_cells(tn) : 0;
+            const int32_t h0 = mv.hstart[s], nh = mv.hstart[s + 1] - h0;
+            for (int i = tid; i < nn + nh; i += blockDim.x) {
+                const bool own = i < nn;
+                const int64_t c = own ? tn * T + i : (int64_t)mv.hcell[h0 + i - nn];
+                double uc[NE], g[NE][DIM], tq[ViscTerms<DIM>::NT], xf[KM][DIM];
+                fused_cell<DIM, RECON, VISC, LIM, KM>(m, ph, U, c, uc, g, tq, xf);
+                if (own) {
+                    march_store_record<DIM, RECON, VISC>(ro, T, i, uc, g, tq);
+                } else {
+                    const int32_t info = mv.hinfo[h0 + i - nn];
+                    int e = info >> 8;
+                    double vel[DIM];
+                    if (VISC) {
+                        const double irho = 1.0 / uc[kRho];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) vel[d] = uc[kRhoU + d] * irho;
+                    }
+#pragma unroll
+                    for (int k = 0; k < KM; k++) {
+                        if (!((info >> k) & 1)) continue;
+#pragma unroll
+                        for (int p = 0; p < NE; p++) {
+                            double a = 0.0;
+                            if (RECON)
+#pragma unroll
+                                for (int d = 0; d < DIM; d++) a += g[p][d] * xf[k][d];
+                            hal[p * nem + e] = uc[p] + a;
+                        }
+                        if (VISC) {
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) hal[(FL::HV + d) * nem + e] = vel[d];
+#pragma unroll
+                            for (int q = 0; q < FL::NT; q++) hal[(FL::HT + q) * nem + e] = tq[q];
+                        }
+                        e++;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // B: this step's faces
+        {
+            const int32_t f0 = mv.fstart[s], nf = mv.fstart[s + 1] - f0;
+            for (int j = tid; j < nf; j += blockDim.x)
+                march_face<DIM, RECON, VISC>(m, ph, U, mv.ffaces[f0 + j], mv.fcode[f0 + j], rc, ro, T, hal, nem, fc,
+                                             nfm, j);
+        }
+        __syncthreads();
+        // C: gather of the cur tile's cells
+        {
+            const int64_t tc = mv.stile[s];
+            if (tid < tile_cells(tc)) {
+                const int64_t c = tc * T + tid;
+                int li[KM];
+                bool prev[KM];
+                double sg[KM];
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    const int32_t e = mv.gslot[k * NI + c];
+                    li[k] = e >= 0 ? (e >> 2) : 0;
+                    prev[k] = e >= 0 && ((e >> 1) & 1);
+                    sg[k] = e < 0 ? 0.0 : ((e & 1) ? 1.0 : -1.0);
+                }
+                const double iv = m.inv_vol[c];
+#pragma unroll
+                for (int p = 0; p < NE; p++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KM; k++)
+                        if (sg[k] != 0.0) acc += sg[k] * (prev[k] ? fo : fc)[p * nfm + li[k]];
+                    const double r = acc * iv;
+                    R[p * N + c] = accumulate ? R[p * N + c] + r : r;
+                }
+            }
+        }
+    }
+}
+
+// ============================================================================ gradient pass from the face offsets
+//
+// The two-pass strategies' FP64 gradient pass when every least-squares stencil has full rank: each cell solves its
+// normal equations from the face offsets like the fused kernel (fused_cell) instead of streaming the slot-major
+// weights and limiter geometry (2 x KM x DIM doubles per cell), and writes the same cell record as gradients_body.
+template <int DIM, bool RECON, bool VISC, int LIM, int KM>
+__device__ __forceinline__ void gradients_neq_body(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                   void* __restrict__ rec) {
+    using CR = CellRec<DIM>;
+    constexpr int NE = DIM + 2;
+    const int64_t c_lane = xcd_cell_index();
+    const bool active = c_lane < m.NI;
+    const int64_t c = active ? c_lane : m.NI - 1;  // lanes past the end shadow the last cell and store nothing
+    double uc[NE], g[NE][DIM], tq[ViscTerms<DIM>::NT], xf[KM][DIM];
+    fused_cell<DIM, RECON, VISC, LIM, KM>(m, ph, U, c, uc, g, tq, xf);
+    if (!active) return;
+    double w[CR::RS];
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        w[p] = uc[p];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) w[CR::G0 + p * DIM + d] = RECON ? g[p][d] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < ViscTerms<DIM>::NT; q++) w[CR::T0 + q] = VISC ? tq[q] : 0.0;
+#pragma unroll
+    for (int i = CR::END; i < CR::RS; i++) w[i] = 0.0;
+    double2* __restrict__ r = reinterpret_cast<double2*>(rec) + rec_pair<DIM>(c_lane);
+#pragma unroll
+    for (int q = 0; q < CR::NP; q++) r[CR::BLOCK * q] = make_double2(w[2 * q], w[2 * q + 1]);
+}
+template <int DIM, bool RECON, bool VISC, int LIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_gradients_neq(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                          double* __restrict__ gradC, double* __restrict__ gradA,
+                                                          void* __restrict__ rec) {
+    (void)gradC;
+    (void)gradA;
+    gradients_neq_body<DIM, RECON, VISC, LIM, KM>(m, ph, U, rec);
+}

@@ -65,6 +65,33 @@ void build_fused_tiles(const Layout& L, int KM, int T, FusedTiles& out);
 // std::logic_error naming the first violated invariant.
 void check_fused_tiles(const Layout& L, int KM, const FusedTiles& ft);
 
+// Plan of the marching strategy: tiles of T consecutive interior cells (tile t owns [t*T, min((t+1)*T, NI))) are
+// walked in sequences, one workgroup per sequence.  Each step of a sequence handles one tile ("cur") and computes
+// the cell records of the tile the next step handles ("next"), so faces between consecutive tiles of a sequence are
+// evaluated once and no record is computed twice inside a sequence; face neighbours in other tiles ("halo") are
+// recomputed per step as in the fused strategy.  A sequence starts with a prologue: the records of its first tile
+// and of a "prologue tile" sharing faces with it (the tile before it, or its periodic partner), and the fluxes of
+// the faces between the two.
+//   seq[w] .. seq[w+1]          steps of sequence w; pro[w] its prologue tile or -1
+//   stile[s], snext[s]          tile handled by step s, tile whose records step s computes (-1: none)
+//   fstart / ffaces / fcode     per step: faces evaluated (ascending), side codes lc | rc << 16 with
+//                               [0, T) a cur cell, [T, 2T) a next cell, [2T, 2T + nem) a halo entry, 0xFFFF a ghost
+//   pfstart / pffaces / pfcode  per sequence: prologue faces, codes [0, T) first-tile cell, [T, 2T) prologue cell
+//   hstart / hcell / hinfo      per step: halo cells and entries (slot mask | first entry << 8), as FusedTiles
+//   gslot[k * NI + c]           gather code of slot k of cell c: list index << 2 | 2 (if evaluated by the previous
+//                               step or the prologue) | side, -1 padding
+struct MarchPlan {
+    int T = 0, nfm = 0, nem = 0;
+    int64_t n_seq = 0, n_steps = 0;
+    std::vector<int32_t> seq, pro, stile, snext, fstart, ffaces, fcode, pfstart, pffaces, pfcode, hstart, hcell, hinfo,
+        gslot;
+};
+// max_steps bounds the steps of one sequence (chains of face-sharing consecutive tiles are split evenly).
+void build_march_plan(const Layout& L, int KM, int T, int max_steps, MarchPlan& out);
+// Checks every index the marching kernel takes from the plan, and that every gather code names the face of its slot
+// in the list (and buffer) that holds it; throws std::logic_error naming the first violated invariant.
+void check_march_plan(const Layout& L, int KM, const MarchPlan& p);
+
 // True when every interior cell's least-squares stencil (its interior face neighbours, offsets from fdx) has full
 // rank, so that the normal equations the fused kernel solves give the pseudo-inverse solution.
 bool lsq_stencils_full_rank(const Layout& L);

@@ -479,6 +479,316 @@ void check_fused_tiles(const Layout& L, int KM, const FusedTiles& ft) {
     }
 }
 
+void build_march_plan(const Layout& L, int KM, int T, int max_steps, MarchPlan& out) {
+    const int64_t NI = L.NI, F = L.F;
+    const int64_t nt = (NI + T - 1) / T;
+    if (nt >= INT32_MAX / 2) throw std::invalid_argument("march plan: too many tiles");
+    auto tile_of = [&](int64_t c) { return (int64_t)(c / T); };
+    auto tile_end = [&](int64_t t) { return std::min<int64_t>((t + 1) * T, NI); };
+    // faces shared between each tile and the others
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> partners((size_t)nt);
+    for (int64_t t = 0; t < nt; t++) {
+        auto& pl = partners[(size_t)t];
+        for (int64_t c = t * T; c < tile_end(t); c++)
+            for (int k = 0; k < L.K; k++) {
+                const int32_t e = L.ell[(size_t)k * NI + c];
+                if (e < 0) continue;
+                const int64_t o = L.fcell[(size_t)(1 - (e & 1)) * F + (e >> 1)];
+                if (o >= NI || tile_of(o) == t) continue;
+                const int32_t to = (int32_t)tile_of(o);
+                auto it = std::find_if(pl.begin(), pl.end(), [&](const std::pair<int32_t, int32_t>& q) { return q.first == to; });
+                if (it == pl.end())
+                    pl.push_back({to, 1});
+                else
+                    it->second++;
+            }
+    }
+    auto shared = [&](int64_t a, int64_t b) -> int32_t {
+        if (a < 0 || b < 0) return 0;
+        for (const auto& q : partners[(size_t)a])
+            if (q.first == b) return q.second;
+        return 0;
+    };
+    // the tile sharing the most faces with t (at least T / 4 of them), other than x1 and x2
+    auto best_partner = [&](int64_t t, int64_t x1, int64_t x2) -> int32_t {
+        int32_t best = -1, cnt = std::max(1, T / 4) - 1;
+        for (const auto& q : partners[(size_t)t])
+            if (q.first != x1 && q.first != x2 && q.second > cnt) {
+                best = q.first;
+                cnt = q.second;
+            }
+        return best;
+    };
+    // chains of consecutive face-sharing tiles, split into sequences of at most max_steps steps
+    out = MarchPlan();
+    out.T = T;
+    out.seq.push_back(0);
+    std::vector<std::pair<int64_t, int64_t>> seqs;  // [a, b) tiles
+    for (int64_t a = 0; a < nt;) {
+        int64_t b = a + 1;
+        while (b < nt && shared(b - 1, b) > 0) b++;
+        const int64_t len = b - a, parts = (len + max_steps - 1) / max_steps;
+        for (int64_t q = 0; q < parts; q++) seqs.push_back({a + len * q / parts, a + len * (q + 1) / parts});
+        a = b;
+    }
+    out.n_seq = (int64_t)seqs.size();
+    out.gslot.assign((size_t)KM * NI, -1);
+    out.fstart.push_back(0);
+    out.hstart.push_back(0);
+    out.pfstart.push_back(0);
+    std::vector<int32_t> list, prev_list, halo;
+    std::vector<int32_t> entry;
+    auto local = [](const std::vector<int32_t>& v, int32_t f) {
+        const auto it = std::lower_bound(v.begin(), v.end(), f);
+        return (it != v.end() && *it == f) ? (int32_t)(it - v.begin()) : -1;
+    };
+    for (size_t w = 0; w < seqs.size(); w++) {
+        const int64_t a = seqs[w].first, b = seqs[w].second;
+        const bool chain_before = a > 0 && shared(a - 1, a) > 0;
+        // the first step's next tile is a + 1 whenever it shares faces with a (inside the sequence or the chain's
+        // next sequence), so the prologue tile must be another one
+        const int64_t next0 = (b - a > 1 || (a + 1 < nt && shared(a, a + 1) > 0)) ? a + 1 : -1;
+        const int64_t pro = chain_before ? a - 1 : best_partner(a, next0, -1);
+        out.pro.push_back((int32_t)pro);
+        // prologue faces: between the prologue tile and the first tile
+        prev_list.clear();
+        if (pro >= 0) {
+            for (int64_t c = a * T; c < tile_end(a); c++)
+                for (int k = 0; k < L.K; k++) {
+                    const int32_t e = L.ell[(size_t)k * NI + c];
+                    if (e < 0) continue;
+                    const int64_t o = L.fcell[(size_t)(1 - (e & 1)) * F + (e >> 1)];
+                    if (o < NI && tile_of(o) == pro) prev_list.push_back(e >> 1);
+                }
+            std::sort(prev_list.begin(), prev_list.end());
+            prev_list.erase(std::unique(prev_list.begin(), prev_list.end()), prev_list.end());
+            for (int32_t f : prev_list) {
+                auto code = [&](int64_t x) { return tile_of(x) == a ? (int32_t)(x - a * T) : (int32_t)(T + x - pro * T); };
+                out.pffaces.push_back(f);
+                out.pfcode.push_back((int32_t)((uint32_t)code(L.fcell[(size_t)f]) | ((uint32_t)code(L.fcell[(size_t)F + f]) << 16)));
+            }
+        }
+        out.pfstart.push_back((int32_t)out.pffaces.size());
+        out.nfm = std::max(out.nfm, (int)prev_list.size());
+        int64_t pv = pro;
+        for (int64_t t = a; t < b; t++) {
+            int64_t nx;
+            if (t + 1 < b)
+                nx = t + 1;
+            else if (t + 1 < nt && shared(t, t + 1) > 0)
+                nx = t + 1;  // the next sequence of the same chain
+            else
+                nx = best_partner(t, pv, -1);
+            out.stile.push_back((int32_t)t);
+            out.snext.push_back((int32_t)nx);
+            // faces of the tile's cells not shared with the previous tile
+            list.clear();
+            for (int64_t c = t * T; c < tile_end(t); c++)
+                for (int k = 0; k < L.K; k++) {
+                    const int32_t e = L.ell[(size_t)k * NI + c];
+                    if (e < 0) continue;
+                    const int64_t o = L.fcell[(size_t)(1 - (e & 1)) * F + (e >> 1)];
+                    if (o < NI && pv >= 0 && tile_of(o) == pv) continue;
+                    list.push_back(e >> 1);
+                }
+            std::sort(list.begin(), list.end());
+            list.erase(std::unique(list.begin(), list.end()), list.end());
+            // halo cells: interior sides outside the cur and next tiles
+            halo.clear();
+            for (int32_t f : list)
+                for (int s = 0; s < 2; s++) {
+                    const int64_t x = L.fcell[(size_t)s * F + f];
+                    if (x < NI && tile_of(x) != t && tile_of(x) != nx) halo.push_back((int32_t)x);
+                }
+            std::sort(halo.begin(), halo.end());
+            halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+            entry.assign(list.size(), -1);
+            int32_t ne = 0;
+            for (int32_t h : halo) {
+                int32_t mask = 0;
+                const int32_t first = ne;
+                for (int k = 0; k < L.K; k++) {
+                    const int32_t e = L.ell[(size_t)k * NI + h];
+                    if (e < 0) continue;
+                    const int32_t j = local(list, e >> 1);
+                    if (j < 0) continue;
+                    mask |= 1 << k;
+                    entry[(size_t)j] = ne++;
+                }
+                out.hcell.push_back(h);
+                out.hinfo.push_back(mask | (first << 8));
+            }
+            if (2 * (int64_t)T + ne >= 0xFFFF) throw std::invalid_argument("march plan: too many halo faces in a step");
+            for (size_t j = 0; j < list.size(); j++) {
+                const int32_t f = list[j];
+                auto code = [&](int64_t x) -> int32_t {
+                    if (x >= NI) return 0xFFFF;
+                    if (tile_of(x) == t) return (int32_t)(x - t * T);
+                    if (tile_of(x) == nx) return (int32_t)(T + x - nx * T);
+                    return 2 * T + entry[j];
+                };
+                out.ffaces.push_back(f);
+                out.fcode.push_back((int32_t)((uint32_t)code(L.fcell[(size_t)f]) | ((uint32_t)code(L.fcell[(size_t)F + f]) << 16)));
+            }
+            // gather codes of the tile's cells
+            for (int64_t c = t * T; c < tile_end(t); c++)
+                for (int k = 0; k < L.K; k++) {
+                    const int32_t e = L.ell[(size_t)k * NI + c];
+                    if (e < 0) continue;
+                    const int64_t o = L.fcell[(size_t)(1 - (e & 1)) * F + (e >> 1)];
+                    const bool from_prev = o < NI && pv >= 0 && tile_of(o) == pv;
+                    const int32_t j = local(from_prev ? prev_list : list, e >> 1);
+                    if (j < 0) throw std::logic_error("march plan: a face of a tile cell is in no list");
+                    out.gslot[(size_t)k * NI + c] = (j << 2) | (from_prev ? 2 : 0) | (e & 1);
+                }
+            if (out.ffaces.size() > (size_t)INT32_MAX / 2 || out.hcell.size() > (size_t)INT32_MAX / 2)
+                throw std::invalid_argument("mesh too large for 32-bit march plan lists");
+            out.fstart.push_back((int32_t)out.ffaces.size());
+            out.hstart.push_back((int32_t)out.hcell.size());
+            out.nfm = std::max(out.nfm, (int)list.size());
+            out.nem = std::max(out.nem, (int)ne);
+            prev_list.swap(list);
+            pv = t;
+        }
+        out.seq.push_back((int32_t)out.stile.size());
+    }
+    out.n_steps = (int64_t)out.stile.size();
+}
+
+void check_march_plan(const Layout& L, int KM, const MarchPlan& p) {
+    const int64_t NI = L.NI, F = L.F;
+    const int T = p.T;
+    auto bad = [](const std::string& what) { throw std::logic_error("march plan: " + what); };
+    const int64_t nt = (NI + T - 1) / T;
+    auto tile_of = [&](int64_t c) { return (int64_t)(c / T); };
+    auto ncells = [&](int64_t t) { return (int)(std::min<int64_t>((t + 1) * T, NI) - t * T); };
+    if (T <= 0 || T > 128 || KM < L.K || KM > 8) bad("tile size or slot width out of range");
+    if ((int64_t)p.seq.size() != p.n_seq + 1 || (int64_t)p.pro.size() != p.n_seq ||
+        (int64_t)p.pfstart.size() != p.n_seq + 1 || (int64_t)p.stile.size() != p.n_steps ||
+        (int64_t)p.snext.size() != p.n_steps || (int64_t)p.fstart.size() != p.n_steps + 1 ||
+        (int64_t)p.hstart.size() != p.n_steps + 1 || p.gslot.size() != (size_t)KM * (size_t)NI ||
+        p.fcode.size() != p.ffaces.size() || p.pfcode.size() != p.pffaces.size() || p.hinfo.size() != p.hcell.size())
+        bad("list sizes disagree");
+    if (p.seq[0] != 0 || p.seq[p.n_seq] != p.n_steps || p.fstart[0] != 0 || p.fstart[p.n_steps] != (int32_t)p.ffaces.size() ||
+        p.hstart[0] != 0 || p.hstart[p.n_steps] != (int32_t)p.hcell.size() || p.pfstart[0] != 0 ||
+        p.pfstart[p.n_seq] != (int32_t)p.pffaces.size())
+        bad("list offsets do not span the lists");
+    std::vector<char> seen((size_t)nt, 0);
+    // a face list entry's side code for cell x of role offset r (0: cur / first tile, T: next / prologue tile)
+    auto side_code = [](uint32_t code, int side) { return side ? (int)(code >> 16) : (int)(code & 0xFFFF); };
+    for (int64_t w = 0; w < p.n_seq; w++) {
+        const int32_t s0 = p.seq[w], s1 = p.seq[w + 1];
+        if (s1 <= s0) bad("empty sequence");
+        const int64_t pro = p.pro[w];
+        const int64_t first = p.stile[s0];
+        if (pro >= nt || pro == first) bad("prologue tile out of range");
+        // prologue faces
+        for (int32_t j = p.pfstart[w]; j < p.pfstart[w + 1]; j++) {
+            if (pro < 0) bad("prologue faces without a prologue tile");
+            const int64_t f = p.pffaces[(size_t)j];
+            if (f < 0 || f >= F || (j > p.pfstart[w] && p.pffaces[(size_t)j - 1] >= f)) bad("prologue faces not ascending");
+            for (int s = 0; s < 2; s++) {
+                const int64_t x = L.fcell[(size_t)s * F + (size_t)f];
+                const int code = side_code((uint32_t)p.pfcode[(size_t)j], s);
+                if (x >= NI) bad("prologue face with a ghost side");
+                const int64_t want = tile_of(x) == first ? x - first * T : (tile_of(x) == pro ? T + x - pro * T : -1);
+                if (want < 0 || code != want) bad("prologue side code names another cell");
+            }
+        }
+        if (p.pfstart[w + 1] - p.pfstart[w] > p.nfm) bad("prologue face count exceeds the LDS rows");
+        int64_t pv = pro;
+        for (int32_t s = s0; s < s1; s++) {
+            const int64_t t = p.stile[(size_t)s], nx = p.snext[(size_t)s];
+            if (t < 0 || t >= nt || seen[(size_t)t]) bad("a tile is handled by no step or by two");
+            seen[(size_t)t] = 1;
+            if (s > s0 && t != p.snext[(size_t)s - 1]) bad("a step's tile is not the records its previous step computed");
+            if (nx >= nt || nx == t || (nx >= 0 && nx == pv)) bad("next tile out of range or equal to the cur / previous tile");
+            const int32_t f0 = p.fstart[s], nf = p.fstart[s + 1] - f0;
+            const int32_t h0 = p.hstart[s], nh = p.hstart[s + 1] - h0;
+            if (nf < 0 || nf > p.nfm || nh < 0) bad("face or halo count of a step out of range");
+            int32_t ne = 0;
+            for (int32_t i = 0; i < nh; i++) {
+                const int64_t h = p.hcell[(size_t)(h0 + i)];
+                const int32_t info = p.hinfo[(size_t)(h0 + i)];
+                if (h < 0 || h >= NI || tile_of(h) == t || tile_of(h) == nx) bad("halo cell out of range or in the cur / next tile");
+                const int32_t mask = info & 0xFF;
+                if ((info >> 8) != ne || (mask >> L.K) != 0) bad("halo entry index or slot mask inconsistent");
+                for (int k = 0; k < L.K; k++)
+                    if ((mask >> k) & 1) {
+                        const int32_t e = L.ell[(size_t)k * NI + h];
+                        if (e < 0) bad("halo slot mask names an empty slot");
+                        const int64_t f = e >> 1;
+                        const auto it = std::lower_bound(p.ffaces.begin() + f0, p.ffaces.begin() + f0 + nf, (int32_t)f);
+                        if (it == p.ffaces.begin() + f0 + nf || *it != f) bad("halo entry names a face outside its step");
+                        if (side_code((uint32_t)p.fcode[(size_t)(it - p.ffaces.begin())], e & 1) != 2 * T + ne)
+                            bad("halo entry and face side code disagree");
+                        ne++;
+                    }
+            }
+            if (ne > p.nem || 2 * T + ne >= 0xFFFF) bad("halo entries exceed the step's LDS rows");
+            for (int32_t j = 0; j < nf; j++) {
+                const int64_t f = p.ffaces[(size_t)(f0 + j)];
+                if (f < 0 || f >= F || (j > 0 && p.ffaces[(size_t)(f0 + j - 1)] >= f)) bad("step faces not ascending");
+                bool touches = false;
+                for (int s2 = 0; s2 < 2; s2++) {
+                    const int64_t x = L.fcell[(size_t)s2 * F + (size_t)f];
+                    const int code = side_code((uint32_t)p.fcode[(size_t)(f0 + j)], s2);
+                    if (x >= NI) {
+                        if (s2 == 0 || code != 0xFFFF) bad("ghost side code wrong");
+                        continue;
+                    }
+                    if (pv >= 0 && tile_of(x) == pv) bad("a step evaluates a face of the previous tile");
+                    if (tile_of(x) == t) {
+                        touches = true;
+                        if (code != x - t * T) bad("cur side code names another cell");
+                    } else if (nx >= 0 && tile_of(x) == nx) {
+                        if (code != T + x - nx * T) bad("next side code names another cell");
+                    } else if (code < 2 * T || code >= 2 * T + ne) {
+                        bad("halo side code out of range");
+                    }
+                }
+                if (!touches) bad("a step evaluates a face of no cur cell");
+            }
+            // gather codes
+            for (int64_t c = t * T; c < t * T + ncells(t); c++)
+                for (int k = 0; k < KM; k++) {
+                    const int32_t e = k < L.K ? L.ell[(size_t)k * NI + c] : -1;
+                    const int32_t g = p.gslot[(size_t)k * NI + c];
+                    if (e < 0) {
+                        if (g != -1) bad("padding slot not marked");
+                        continue;
+                    }
+                    if (g < 0 || (g & 1) != (e & 1)) bad("gather side wrong");
+                    const bool from_prev = (g >> 1) & 1;
+                    const int32_t j = g >> 2;
+                    int64_t fj;
+                    int code;
+                    if (!from_prev) {
+                        if (j >= nf) bad("gather index out of the step's list");
+                        fj = p.ffaces[(size_t)(f0 + j)];
+                        code = side_code((uint32_t)p.fcode[(size_t)(f0 + j)], e & 1);
+                        if (code != c - t * T) bad("gather names a face entry of another cell");
+                    } else if (s == s0) {
+                        if (pro < 0 || j >= p.pfstart[w + 1] - p.pfstart[w]) bad("gather index out of the prologue list");
+                        fj = p.pffaces[(size_t)(p.pfstart[w] + j)];
+                        code = side_code((uint32_t)p.pfcode[(size_t)(p.pfstart[w] + j)], e & 1);
+                        if (code != c - t * T) bad("gather names a prologue entry of another cell");
+                    } else {
+                        const int32_t pf0 = p.fstart[s - 1], pnf = p.fstart[s] - pf0;
+                        if (j >= pnf) bad("gather index out of the previous step's list");
+                        fj = p.ffaces[(size_t)(pf0 + j)];
+                        code = side_code((uint32_t)p.fcode[(size_t)(pf0 + j)], e & 1);
+                        if (code != T + c - t * T) bad("gather names a previous-step entry of another cell");
+                    }
+                    if (fj != (e >> 1)) bad("gather names another face");
+                }
+            pv = t;
+        }
+    }
+    for (int64_t t = 0; t < nt; t++)
+        if (!seen[(size_t)t]) bad("a tile is handled by no step");
+}
+
 bool lsq_stencils_full_rank(const Layout& L) {
     const int dim = L.dim;
     const int64_t NI = L.NI, F = L.F;

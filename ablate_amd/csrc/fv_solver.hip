@@ -1703,10 +1703,10 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
 // arrays):
 //   1. k_adj_face_rev (lane per face): recomputes the face flux and reverses it once for both sides into a face
 //      record (face-state, viscous-term and species factors); the adjoint of a ghost state goes to a per-ghost array.
-//      k_adj_face_gather (lane per interior cell) sums its faces' records in slot order into its own adjoints:
-//      state (direct), limited gradients (face state = U + G dx), viscous terms, species aux gradients.
-//   2. k_adj_cells  (lane per interior cell): reverses the limiter, the least-squares gradients and the aux
-//      (temperature / velocity / mass fraction) gradients of the cell; writes its own adjoint and, per face slot,
+//   2. k_adj_cells_e, then k_adj_cells_s (lane per interior cell, euler field then species): each sums its faces'
+//      records in slot order into the cell's adjoints (state, limited gradients through the face state U + G dx,
+//      viscous terms, species aux gradients), reverses the limiter, the least-squares gradients and the aux
+//      (temperature / velocity / mass fraction) gradients of the cell, and writes its own adjoint and, per face slot,
 //      the contribution for the neighbour across the face.
 //   3. k_adj_gather (lane per interior cell): own adjoint + the contributions its neighbours wrote for it (mirror
 //      slot of each face) + the adjoint of its zero-gradient ghosts.
@@ -1714,7 +1714,6 @@ __global__ __launch_bounds__(kBlock) void k_vjp_theta(MeshView<double> m, Phys p
 // default 1e10, in a fully limited component) is taken as inactive.
 
 struct AdjBufs {
-    double* A;              // pass-1 record [NA][NI]: ub (nc), gb (nc * DIM), tqb (NT), gyb (ns * DIM)
     double* Ughost;         // [nc][NG] adjoint of ghost states
     double* Uself;          // [nc][NI]
     double* contrib;        // [KM][nc][NI]
@@ -1725,7 +1724,7 @@ struct AdjBufs {
     double* GL;             // [ns][NG] densityYi adjoint of a boundary face's interior cell from the ghost's gradient
 };
 
-// Face adjoint record of the face-centric reverse pass, read back per cell by k_adj_face_gather.  Words per face:
+// Face adjoint record of the face-centric reverse pass, read back per cell by k_adj_cells_e / _s.  Words per face:
 //   [0, NE)             adjoint of the left face state (the flux calculator's and AdvectionFlux's reverse)
 //   [NE, 2 NE)          right face state; on a boundary face (ghost right cell, whose adjoint goes to Ughost) the
 //                       left cell's extra aux adjoint [NE, NE + DIM + 1) and density adjoint (NE + DIM + 1) from the
@@ -1980,142 +1979,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phy
     }
 }
 
-// Per-cell gather of the face adjoint records into the pass-1 record A (state, limited-gradient, viscous-term and
-// species aux-gradient adjoints of the cell), in the cell's slot order.
-template <int DIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_adj_face_gather(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                            AdjBufs b) {
-    constexpr int NE = DIM + 2;
-    constexpr int NT = ViscTerms<DIM>::NT;
-    using AW = AdjFaceRec<DIM>;
-    const int64_t c = xcd_cell_index();
-    if (c >= m.NI) return;
-    const int64_t N = m.N, NI = m.NI, NG = N - NI;
-    const int ns = ph.ns, nc = NE + ns;
-    const bool recon = ph.recon != 0, visc = ph.visc != 0;
-    const int64_t oG = nc, oT = nc + (int64_t)nc * DIM, oY = oT + NT;
-    double* __restrict__ A = b.A;
-    const double2* __restrict__ W = reinterpret_cast<const double2*>(b.W);
-    double ub[NE], gb[NE][DIM], tqb[NT], velb[DIM], auxb[DIM + 1], rhob = 0.0;
-#pragma unroll
-    for (int p = 0; p < NE; p++) {
-        ub[p] = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) gb[p][d] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < NT; q++) tqb[q] = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; d++) velb[d] = 0.0;
-#pragma unroll
-    for (int q = 0; q <= DIM; q++) auxb[q] = 0.0;
-    // per slot: record base, side, face offset x_f - x_c, unit normal, species coefficient, diffusion coefficient,
-    // ghost index
-    int64_t wb[KM], gi[KM];
-    double xs[KM][DIM], nv[KM][DIM], cs[KM], cdk[KM];
-    bool valid[KM];
-#pragma unroll
-    for (int k = 0; k < KM; k++) {
-        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
-        valid[k] = e >= 0;
-        const int64_t f = valid[k] ? (int64_t)(e >> 1) : 0;
-        const int s = e & 1;
-        const int32_t nb = m.snbr[k * NI + c];
-        gi[k] = (valid[k] && nb >= NI) ? (int64_t)nb - NI : -1;
-        wb[k] = adjw_pair(f, b.nwp);
-        double x[2][DIM], nh[3], area;
-        load_face_pair<DIM>(m.fdx, f, x);
-        load_face_na(m.fna, f, nh, area);
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            xs[k][d] = s ? x[1][d] : x[0][d];
-            nv[k][d] = nh[d];
-        }
-        double w[AW::GS];
-#pragma unroll
-        for (int q = 0; q < AW::GS / 2; q++) {
-            const double2 v = W[wb[k] + 64 * q];
-            w[2 * q] = v.x;
-            w[2 * q + 1] = v.y;
-        }
-        cs[k] = s ? w[AW::CR] : w[AW::CL];
-        cdk[k] = w[AW::CD];
-        if (!valid[k]) {
-            cs[k] = 0.0;
-            cdk[k] = 0.0;
-            continue;
-        }
-        const int o = s ? AW::UR : AW::UL;
-#pragma unroll
-        for (int p = 0; p < NE; p++) {
-            ub[p] += w[o + p];
-            if (recon)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) gb[p][d] += w[o + p] * xs[k][d];
-        }
-        if (visc) {
-#pragma unroll
-            for (int q = 0; q < NT; q++) tqb[q] += w[AW::TQ + q];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) velb[d] += w[AW::VL + d];
-        }
-        rhob += w[AW::RH];
-        if (gi[k] >= 0) {  // boundary face: the left cell's extras from the ghost's viscous gradient
-#pragma unroll
-            for (int q = 0; q <= DIM; q++) auxb[q] += w[AW::UR + q];
-            rhob += w[AW::UR + DIM + 1];
-        }
-    }
-    // species: densityYi, its limited-gradient and Yi aux-gradient adjoints
-    for (int sp = 0; sp < ns; sp++) {
-        double a = 0.0, gsb[DIM], gyb[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            gsb[d] = 0.0;
-            gyb[d] = 0.0;
-        }
-        const int q = sp >> 1;
-#pragma unroll
-        for (int k = 0; k < KM; k++) {
-            const double2 v = W[wb[k] + 64 * (AW::GS / 2 + q)];
-            if (!valid[k]) continue;
-            const double gsp = (sp & 1) ? v.y : v.x;
-            const double yb = cs[k] * gsp;
-            a += yb;
-            if (gi[k] >= 0) a += b.GL[(int64_t)sp * NG + gi[k]];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                gsb[d] += yb * xs[k][d];
-                gyb[d] += gsp * cdk[k] * nv[k][d];
-            }
-        }
-        A[(int64_t)(NE + sp) * NI + c] = a;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            A[(oG + (int64_t)(NE + sp) * DIM + d) * NI + c] = recon ? gsb[d] : 0.0;
-            A[(oY + (int64_t)sp * DIM + d) * NI + c] = gyb[d];
-        }
-    }
-    if (visc) {
-        double uc[NE], ab[DIM + 1];
-#pragma unroll
-        for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
-        ab[0] = auxb[0];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) ab[1 + d] = velb[d] + auxb[1 + d];
-        aux_vjp<DIM>(uc, ph, ab, ub);
-    }
-    ub[kRho] += rhob;
-#pragma unroll
-    for (int p = 0; p < NE; p++) {
-        A[(int64_t)p * NI + c] = ub[p];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) A[(oG + (int64_t)p * DIM + d) * NI + c] = gb[p][d];
-    }
-#pragma unroll
-    for (int q = 0; q < NT; q++) A[(oT + q) * NI + c] = tqb[q];
-}
-
 // Barth-Jespersen limiter of one component with its active pieces: phi = min(1, (umax - uc) / dpos, (umin - uc) /
 // dneg); imax / imin = slot of the extreme neighbour (-1: the cell itself), kpos / kneg = slot of the extreme face
 // projection.
@@ -2142,137 +2005,211 @@ __device__ inline void bj_finish(double uc, BJFwd& L) {
     }
 }
 
+// Reverse of one component's limited least-squares gradient g = phi(G) G, G = sum_k w_k (u_n - u_c), given the
+// adjoint gbl of g: adds the adjoint of the cell's own value to ubc and of each neighbour's to cn[k] (ghost
+// neighbours through to_ghost).  x[k] = x_f - x_c of slot k (the Barth-Jespersen projection vector).
+template <int DIM, int KM, typename ToGhost>
+__device__ inline void limited_lsq_vjp(const double* __restrict__ U, int64_t row, double uc, const double (*w)[DIM],
+                                       const double (*x)[DIM], const int64_t* nb, const bool* valid, const bool* bnd,
+                                       bool bj, const double* gbl, double& ubc, double* cn, ToGhost&& to_ghost) {
+    double G[DIM], Gb[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) G[d] = 0.0;
+    BJFwd lim{uc, uc, 0.0, 0.0, 1.0, -1, -1, -1, -1, 0};
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        if (!valid[k]) continue;
+        const double un = U[row + nb[k]];
+        if (!bnd[k])
+#pragma unroll
+            for (int d = 0; d < DIM; d++) G[d] += w[k][d] * (un - uc);
+        if (un > lim.umax) {
+            lim.umax = un;
+            lim.imax = k;
+        }
+        if (un < lim.umin) {
+            lim.umin = un;
+            lim.imin = k;
+        }
+    }
+    if (bj) {
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            if (!valid[k]) continue;
+            double D = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) D += G[d] * x[k][d];
+            if (D > lim.dpos) {
+                lim.dpos = D;
+                lim.kpos = k;
+            }
+            if (D < lim.dneg) {
+                lim.dneg = D;
+                lim.kneg = k;
+            }
+        }
+        bj_finish(uc, lim);
+    }
+    double phib = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        Gb[d] = lim.phi * gbl[d];
+        phib += gbl[d] * G[d];
+    }
+    if (bj && lim.active) {
+        const bool mx = lim.active == 1;
+        const double den = mx ? lim.dpos : lim.dneg;
+        const int ext = mx ? lim.imax : lim.imin;
+        const int kd = mx ? lim.kpos : lim.kneg;
+        const double ub_ext = phib / den;
+        ubc -= ub_ext;
+        if (ext < 0) ubc += ub_ext;
+        const double den_b = -lim.phi * phib / den;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {  // selects, not runtime register indices (which would spill)
+            if (k == ext) {
+                if (bnd[k])
+                    to_ghost(nb[k], ub_ext);
+                else
+                    cn[k] += ub_ext;
+            }
+            if (k == kd)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) Gb[d] += den_b * x[k][d];
+        }
+    }
+    // least-squares gradient: G = sum_k w_k (u_n - u_c)
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        if (!valid[k] || bnd[k]) continue;
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) t += w[k][d] * Gb[d];
+        cn[k] += t;
+        ubc -= t;
+    }
+}
+
+// Pass 2, euler field (lane per interior cell): sums its faces' adjoint records in slot order (state, limited
+// gradients through the face state U + G dx, viscous terms, face velocity and density, boundary-face extras), then
+// reverses the limiter, the least-squares gradients and the aux (temperature / velocity) gradients of the cell;
+// writes its own adjoint and, per face slot, the contribution for the neighbour across the face.
 template <int DIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                      const int32_t* __restrict__ ghost_slot,
-                                                      const int32_t* __restrict__ slot_kind, AdjBufs b) {
+__global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                        AdjBufs b) {
     constexpr int NE = DIM + 2;
     constexpr int NT = ViscTerms<DIM>::NT;
+    using AW = AdjFaceRec<DIM>;
     const int64_t c = xcd_cell_index();
     if (c >= m.NI) return;
     const int64_t N = m.N, NI = m.NI, NG = N - NI;
-    const int ns = ph.ns, nc = NE + ns;
-    const bool recon = ph.recon != 0, visc = ph.visc != 0, sdiff = ph.sdiff != 0;
+    const int nc = NE + ph.ns;
+    const bool recon = ph.recon != 0, visc = ph.visc != 0;
     const bool bj = ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN;
-    const int64_t oG = nc, oT = nc + (int64_t)nc * DIM, oY = oT + NT;
-    const double* __restrict__ A = b.A;
-    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
-    double uc[NE], ubc[NE], ce[KM][NE];
-    int64_t nb[KM];
-    bool valid[KM], bnd[KM];
+    const double2* __restrict__ W = reinterpret_cast<const double2*>(b.W);
+    double ub[NE], gb[NE][DIM], tqb[NT], velb[DIM], auxb[DIM + 1], rhob = 0.0;
 #pragma unroll
     for (int p = 0; p < NE; p++) {
-        uc[p] = U[p * N + c];
-        ubc[p] = A[(int64_t)p * NI + c];
+        ub[p] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) gb[p][d] = 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < NT; q++) tqb[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) velb[d] = 0.0;
+#pragma unroll
+    for (int q = 0; q <= DIM; q++) auxb[q] = 0.0;
+    int64_t nb[KM];
+    bool valid[KM], bnd[KM];
+    double xs[KM][DIM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
-        const int32_t e = m.snbr[k * NI + c];
+        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
         valid[k] = e >= 0;
-        nb[k] = valid[k] ? (int64_t)e : c;
+        const int64_t f = valid[k] ? (int64_t)(e >> 1) : 0;
+        const int s = e & 1;
+        const int32_t n = m.snbr[k * NI + c];
+        nb[k] = valid[k] ? (int64_t)n : c;
         bnd[k] = nb[k] >= NI;
+        double x[2][DIM];
+        load_face_pair<DIM>(m.fdx, f, x);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) xs[k][d] = s ? x[1][d] : x[0][d];
+        double w[AW::GS];
+        const int64_t wb = adjw_pair(f, b.nwp);
+#pragma unroll
+        for (int q = 0; q < AW::GS / 2; q++) {
+            const double2 v = W[wb + 64 * q];
+            w[2 * q] = v.x;
+            w[2 * q + 1] = v.y;
+        }
+        if (!valid[k]) continue;
+        const int o = s ? AW::UR : AW::UL;
+#pragma unroll
+        for (int p = 0; p < NE; p++) {
+            ub[p] += w[o + p];
+            if (recon)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) gb[p][d] += w[o + p] * xs[k][d];
+        }
+        if (visc) {
+#pragma unroll
+            for (int q = 0; q < NT; q++) tqb[q] += w[AW::TQ + q];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) velb[d] += w[AW::VL + d];
+        }
+        rhob += w[AW::RH];
+        if (bnd[k]) {  // boundary face: the left cell's extras from the ghost's viscous gradient
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) auxb[q] += w[AW::UR + q];
+            rhob += w[AW::UR + DIM + 1];
+        }
+    }
+    double uc[NE];
+#pragma unroll
+    for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
+    if (visc) {
+        double ab[DIM + 1];
+        ab[0] = auxb[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ab[1 + d] = velb[d] + auxb[1 + d];
+        aux_vjp<DIM>(uc, ph, ab, ub);
+    }
+    ub[kRho] += rhob;
+    // the cell's reverse: limiter and least-squares gradients, then the aux gradients
+    double ce[KM][NE];
+#pragma unroll
+    for (int k = 0; k < KM; k++)
 #pragma unroll
         for (int p = 0; p < NE; p++) ce[k][p] = 0.0;
-    }
-    // ghost contributions are added to the ghost's adjoint (only this cell touches its ghosts)
-    auto to_ghost = [&](int64_t g, int comp, double v) { b.Ughost[(int64_t)comp * NG + (g - NI)] += v; };
-
     if (recon) {
+        double wl[KM][DIM];
 #pragma unroll
-        for (int p = 0; p < NE; p++) {  // unrolled: ce, ubc and uc stay in registers
-            double G[DIM], Gb[DIM], gbl[DIM];
+        for (int k = 0; k < KM; k++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                G[d] = 0.0;
-                gbl[d] = A[(oG + (int64_t)p * DIM + d) * NI + c];
-            }
-            BJFwd lim{uc[p], uc[p], 0.0, 0.0, 1.0, -1, -1, -1, -1, 0};
+            for (int d = 0; d < DIM; d++) wl[k][d] = m.sw[(k * DIM + d) * NI + c];
 #pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (!valid[k]) continue;
-                const double un = U[p * N + nb[k]];
-                if (!bnd[k])
+        for (int p = 0; p < NE; p++) {
+            double cn[KM];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) G[d] += m.sw[(k * DIM + d) * NI + c] * (un - uc[p]);
-                if (un > lim.umax) {
-                    lim.umax = un;
-                    lim.imax = k;
-                }
-                if (un < lim.umin) {
-                    lim.umin = un;
-                    lim.imin = k;
-                }
-            }
-            if (bj) {
+            for (int k = 0; k < KM; k++) cn[k] = 0.0;
+            limited_lsq_vjp<DIM, KM>(U, (int64_t)p * N, uc[p], wl, xs, nb, valid, bnd, bj, gb[p], ub[p], cn,
+                                     [&](int64_t g, double v) { b.Ughost[(int64_t)p * NG + (g - NI)] += v; });
 #pragma unroll
-                for (int k = 0; k < KM; k++) {
-                    if (!valid[k]) continue;
-                    double D = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) D += G[d] * m.slim[(k * DIM + d) * NI + c];
-                    if (D > lim.dpos) {
-                        lim.dpos = D;
-                        lim.kpos = k;
-                    }
-                    if (D < lim.dneg) {
-                        lim.dneg = D;
-                        lim.kneg = k;
-                    }
-                }
-                bj_finish(uc[p], lim);
-            }
-            const double phi = lim.phi;
-            double phib = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                Gb[d] = phi * gbl[d];
-                phib += gbl[d] * G[d];
-            }
-            if (bj && lim.active) {
-                const bool mx = lim.active == 1;
-                const double den = mx ? lim.dpos : lim.dneg;
-                const int ext = mx ? lim.imax : lim.imin;
-                const int kd = mx ? lim.kpos : lim.kneg;
-                const double ub_ext = phib / den;
-                ubc[p] -= ub_ext;
-                if (ext < 0) ubc[p] += ub_ext;
-#pragma unroll
-                for (int k = 0; k < KM; k++)  // select, not a runtime register index (which would spill ce)
-                    if (k == ext) {
-                        if (bnd[k])
-                            to_ghost(nb[k], p, ub_ext);
-                        else
-                            ce[k][p] += ub_ext;
-                    }
-                const double den_b = -phi * phib / den;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
-            }
-#pragma unroll
-            for (int d = 0; d < DIM; d++) G[d] = Gb[d];
-            // least-squares gradient: G = sum_k w_k (u_n - u_c)
-#pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (!valid[k] || bnd[k]) continue;
-                double t = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) t += m.sw[(k * DIM + d) * NI + c] * G[d];
-                ce[k][p] += t;
-                ubc[p] -= t;
-            }
+            for (int k = 0; k < KM; k++) ce[k][p] += cn[k];
         }
     }
     if (visc) {
-        double tqbl[NT], gAb[DIM + 1][DIM], acb[DIM + 1];
-#pragma unroll
-        for (int q = 0; q < NT; q++) tqbl[q] = A[(oT + q) * NI + c];
+        const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+        double gAb[DIM + 1][DIM], acb[DIM + 1];
 #pragma unroll
         for (int q = 0; q <= DIM; q++) {
             acb[q] = 0.0;
 #pragma unroll
             for (int d = 0; d < DIM; d++) gAb[q][d] = 0.0;
         }
-        stress_heat_vjp<DIM>(ph.mu, ph.k, tqbl, gAb);
+        stress_heat_vjp<DIM>(ph.mu, ph.k, tqb, gAb);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             if (!valid[k]) continue;
@@ -2294,132 +2231,140 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(MeshView<double> m, Phys p
 #pragma unroll
             for (int p = 0; p < NE; p++) {
                 if (bnd[k])
-                    to_ghost(nb[k], p, unb[p]);
+                    b.Ughost[(int64_t)p * NG + (nb[k] - NI)] += unb[p];
                 else
                     ce[k][p] += unb[p];
             }
         }
-        aux_vjp<DIM>(uc, ph, acb, ubc);
+        aux_vjp<DIM>(uc, ph, acb, ub);
     }
-    // species: limited densityYi gradients and Yi aux gradients
+#pragma unroll
+    for (int p = 0; p < NE; p++) b.Uself[(int64_t)p * NI + c] = ub[p];
+#pragma unroll
+    for (int k = 0; k < KM; k++)
+#pragma unroll
+        for (int p = 0; p < NE; p++) b.contrib[((int64_t)k * nc + p) * NI + c] = ce[k][p];
+}
+
+// Pass 2, species (lane per interior cell, after k_adj_cells_e): per species, sums the species factors of its faces'
+// records (densityYi adjoint, limited densityYi gradient adjoint, Yi aux gradient adjoint), then reverses the
+// limiter, the least-squares gradient and the mass-fraction gradient.  The slot geometry stays in registers across
+// the species loop.  The species' density terms (Y = rhoY / rho) are added into the euler kernel's density rows.
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_adj_cells_s(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                        AdjBufs b) {
+    constexpr int NE = DIM + 2;
+    using AW = AdjFaceRec<DIM>;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, NI = m.NI, NG = N - NI;
+    const int ns = ph.ns, nc = NE + ns;
+    const bool recon = ph.recon != 0, sdiff = ph.sdiff != 0;
+    const bool bj = ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN;
+    const double2* __restrict__ W = reinterpret_cast<const double2*>(b.W);
+    const double* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+    int64_t nb[KM], wb[KM], gi[KM];
+    bool valid[KM], bnd[KM];
+    double xs[KM][DIM], cnv[KM][DIM], csk[KM], wl[KM][DIM], wa[KM][DIM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int32_t e = k < m.K ? m.ell[k * NI + c] : -1;
+        valid[k] = e >= 0;
+        const int64_t f = valid[k] ? (int64_t)(e >> 1) : 0;
+        const int s = e & 1;
+        const int32_t n = m.snbr[k * NI + c];
+        nb[k] = valid[k] ? (int64_t)n : c;
+        bnd[k] = nb[k] >= NI;
+        gi[k] = (valid[k] && bnd[k]) ? nb[k] - NI : -1;
+        wb[k] = adjw_pair(f, b.nwp);
+        double x[2][DIM], nh[3], area;
+        load_face_pair<DIM>(m.fdx, f, x);
+        load_face_na(m.fna, f, nh, area);
+        double cw[4];  // words CL, CR, CD of the record (two pairs from CL's pair on)
+        {
+            const double2 v0 = W[wb[k] + 64 * (AW::CL / 2)], v1 = W[wb[k] + 64 * (AW::CL / 2 + 1)];
+            cw[0] = v0.x;
+            cw[1] = v0.y;
+            cw[2] = v1.x;
+            cw[3] = v1.y;
+        }
+        const int o = AW::CL & 1;
+        csk[k] = valid[k] ? (s ? cw[o + 1] : cw[o]) : 0.0;
+        const double cd = valid[k] ? cw[o + 2] : 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            xs[k][d] = s ? x[1][d] : x[0][d];
+            cnv[k][d] = cd * nh[d];
+            wl[k][d] = recon ? m.sw[(k * DIM + d) * NI + c] : 0.0;
+            wa[k][d] = sdiff ? wa_arr[(k * DIM + d) * NI + c] : 0.0;
+        }
+    }
+    const double rho_c = U[c];
+    double rbc = 0.0, rbn[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) rbn[k] = 0.0;
     for (int sp = 0; sp < ns; sp++) {
         const int64_t comp = NE + sp;
         const double ycs = U[comp * N + c];
-        double usb = A[comp * NI + c];
-        double cs[KM];  // this species' contributions to the neighbours, stored once per slot
+        // the faces' species factors
+        double usb = 0.0, gsb[DIM], gyb[DIM];
 #pragma unroll
-        for (int k = 0; k < KM; k++) cs[k] = 0.0;
-        if (recon) {
-            double G[DIM], Gb[DIM];
+        for (int d = 0; d < DIM; d++) {
+            gsb[d] = 0.0;
+            gyb[d] = 0.0;
+        }
+        const int q = sp >> 1;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) G[d] = 0.0;
-            BJFwd lim{ycs, ycs, 0.0, 0.0, 1.0, -1, -1, -1, -1, 0};
-#pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (!valid[k]) continue;
-                const double un = U[comp * N + nb[k]];
-                if (!bnd[k])
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) G[d] += m.sw[(k * DIM + d) * NI + c] * (un - ycs);
-                if (un > lim.umax) {
-                    lim.umax = un;
-                    lim.imax = k;
-                }
-                if (un < lim.umin) {
-                    lim.umin = un;
-                    lim.imin = k;
-                }
-            }
-            if (bj) {
-#pragma unroll
-                for (int k = 0; k < KM; k++) {
-                    if (!valid[k]) continue;
-                    double D = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) D += G[d] * m.slim[(k * DIM + d) * NI + c];
-                    if (D > lim.dpos) {
-                        lim.dpos = D;
-                        lim.kpos = k;
-                    }
-                    if (D < lim.dneg) {
-                        lim.dneg = D;
-                        lim.kneg = k;
-                    }
-                }
-                bj_finish(ycs, lim);
-            }
-            double phib = 0.0;
+        for (int k = 0; k < KM; k++) {
+            const double2 v = W[wb[k] + 64 * (AW::GS / 2 + q)];
+            if (!valid[k]) continue;
+            const double gsp = (sp & 1) ? v.y : v.x;
+            const double yb = csk[k] * gsp;
+            usb += yb;
+            if (gi[k] >= 0) usb += b.GL[(int64_t)sp * NG + gi[k]];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                const double gbl = A[(oG + comp * DIM + d) * NI + c];
-                Gb[d] = lim.phi * gbl;
-                phib += gbl * G[d];
-            }
-            if (bj && lim.active) {
-                const bool mx = lim.active == 1;
-                const double den = mx ? lim.dpos : lim.dneg;
-                const int ext = mx ? lim.imax : lim.imin;
-                const int kd = mx ? lim.kpos : lim.kneg;
-                const double ub_ext = phib / den;
-                usb -= ub_ext;
-                if (ext < 0) usb += ub_ext;
-#pragma unroll
-                for (int k = 0; k < KM; k++)
-                    if (k == ext) {
-                        if (bnd[k])
-                            to_ghost(nb[k], (int)comp, ub_ext);
-                        else
-                            cs[k] += ub_ext;
-                    }
-                const double den_b = -lim.phi * phib / den;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) Gb[d] += den_b * m.slim[(kd * DIM + d) * NI + c];
-            }
-#pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (!valid[k] || bnd[k]) continue;
-                double t = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) t += m.sw[(k * DIM + d) * NI + c] * Gb[d];
-                cs[k] += t;
-                usb -= t;
+                gsb[d] += yb * xs[k][d];
+                gyb[d] += gsp * cnv[k][d];
             }
         }
-        if (sdiff) {
-            double gyb[DIM], ycb = 0.0;
+        double cs[KM];  // this species' contributions to the neighbours
 #pragma unroll
-            for (int d = 0; d < DIM; d++) gyb[d] = A[(oY + (int64_t)sp * DIM + d) * NI + c];
+        for (int k = 0; k < KM; k++) cs[k] = 0.0;
+        if (recon)
+            limited_lsq_vjp<DIM, KM>(U, comp * N, ycs, wl, xs, nb, valid, bnd, bj, gsb, usb, cs,
+                                     [&](int64_t g, double v) { b.Ughost[comp * NG + (g - NI)] += v; });
+        if (sdiff) {
+            double ycb = 0.0;
 #pragma unroll
             for (int k = 0; k < KM; k++) {
                 if (!valid[k]) continue;
                 double t = 0.0;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) t += wa_arr[(k * DIM + d) * NI + c] * gyb[d];
+                for (int d = 0; d < DIM; d++) t += wa[k][d] * gyb[d];
                 ycb -= t;
                 double yb_n = 0.0, rb_n = 0.0;
                 massfrac_vjp(U[nb[k]], U[comp * N + nb[k]], t, yb_n, rb_n);
                 if (bnd[k]) {
-                    to_ghost(nb[k], (int)comp, yb_n);
-                    to_ghost(nb[k], kRho, rb_n);
+                    b.Ughost[comp * NG + (nb[k] - NI)] += yb_n;
+                    b.Ughost[(int64_t)kRho * NG + (nb[k] - NI)] += rb_n;
                 } else {
                     cs[k] += yb_n;
-                    ce[k][kRho] += rb_n;
+                    rbn[k] += rb_n;
                 }
             }
-            double rb_c = 0.0;
-            massfrac_vjp(uc[kRho], ycs, ycb, usb, rb_c);
-            ubc[kRho] += rb_c;
+            massfrac_vjp(rho_c, ycs, ycb, usb, rbc);
         }
         b.Uself[comp * NI + c] = usb;
 #pragma unroll
         for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + comp) * NI + c] = cs[k];
     }
+    if (sdiff) {
+        b.Uself[(int64_t)kRho * NI + c] += rbc;
 #pragma unroll
-    for (int p = 0; p < NE; p++) b.Uself[(int64_t)p * NI + c] = ubc[p];
-#pragma unroll
-    for (int k = 0; k < KM; k++)
-#pragma unroll
-        for (int p = 0; p < NE; p++) b.contrib[((int64_t)k * nc + p) * NI + c] = ce[k][p];
+        for (int k = 0; k < KM; k++)
+            if (valid[k] && !bnd[k]) b.contrib[((int64_t)k * nc + kRho) * NI + c] += rbn[k];
+    }
 }
 
 template <int DIM, int KM>
@@ -2606,16 +2551,15 @@ FaceInterpKernel pick_face_interpolate(int K) {
 }
 
 // ---- compile-time variant selection
-using AdjFaceGatherKernel = void (*)(MeshView<double>, Phys, const double*, AdjBufs);
+using AdjCellsKernel = void (*)(MeshView<double>, Phys, const double*, AdjBufs);
 using AdjFacesKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const double*, const double*,
                                 const double*, AdjBufs);
-using AdjCellsKernel = void (*)(MeshView<double>, Phys, const double*, const int32_t*, const int32_t*, AdjBufs);
 using AdjGatherKernel = void (*)(MeshView<double>, Phys, const int32_t*, const int32_t*, AdjBufs, double*);
 template <int DIM, int KM>
-void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjFaceGatherKernel& fg, AdjCellsKernel& ck, AdjGatherKernel& gk) {
+void pick_adj_k(bool rusanov, AdjFacesKernel& fk, AdjCellsKernel& ce, AdjCellsKernel& cs, AdjGatherKernel& gk) {
     fk = rusanov ? k_adj_face_rev<DIM, true> : k_adj_face_rev<DIM, false>;
-    fg = k_adj_face_gather<DIM, KM>;
-    ck = k_adj_cells<DIM, KM>;
+    ce = k_adj_cells_e<DIM, KM>;
+    cs = k_adj_cells_s<DIM, KM>;
     gk = k_adj_gather<DIM, KM>;
 }
 template <typename G>
@@ -2660,12 +2604,12 @@ GradKernelT<G> pick_gradients_k(bool recon, bool visc, int lim) {
 // Compile-time slot width of the cell -> face table: 4 (quads, tets), 6 (hexes) or 8.
 inline int slot_width(int K) { return K <= 4 ? 4 : (K <= 6 ? 6 : 8); }
 template <int DIM>
-void pick_adj(bool rusanov, int K, AdjFacesKernel& fk, AdjFaceGatherKernel& fg, AdjCellsKernel& ck,
+void pick_adj(bool rusanov, int K, AdjFacesKernel& fk, AdjCellsKernel& ce, AdjCellsKernel& cs,
               AdjGatherKernel& gk) {
     switch (slot_width(K)) {
-        case 4: pick_adj_k<DIM, 4>(rusanov, fk, fg, ck, gk); break;
-        case 6: pick_adj_k<DIM, 6>(rusanov, fk, fg, ck, gk); break;
-        default: pick_adj_k<DIM, 8>(rusanov, fk, fg, ck, gk); break;
+        case 4: pick_adj_k<DIM, 4>(rusanov, fk, ce, cs, gk); break;
+        case 6: pick_adj_k<DIM, 6>(rusanov, fk, ce, cs, gk); break;
+        default: pick_adj_k<DIM, 8>(rusanov, fk, ce, cs, gk); break;
     }
 }
 template <typename G, int DIM>
@@ -2818,6 +2762,46 @@ FusedKernel pick_fused(bool recon, bool visc, int lim, int K) {
     if (KM == 4) return pick_fused_k<DIM, 4>(recon, visc, lim);
     if constexpr (DIM == 3)
         if (KM == 6) return pick_fused_k<DIM, 6>(recon, visc, lim);
+    return nullptr;
+}
+
+// FP64 gradient pass from the face offsets (full-rank stencils, no limiter or Barth-Jespersen).
+template <int DIM, int KM>
+GradKernel pick_gradients_neq_k(bool recon, bool visc, int lim) {
+    if (lim == kLimPetsc) return nullptr;
+    if (recon) {
+        if (visc)
+            return lim == kLimBJ ? k_gradients_neq<DIM, true, true, kLimBJ, KM> : k_gradients_neq<DIM, true, true, kLimNone, KM>;
+        return lim == kLimBJ ? k_gradients_neq<DIM, true, false, kLimBJ, KM> : k_gradients_neq<DIM, true, false, kLimNone, KM>;
+    }
+    return visc ? k_gradients_neq<DIM, false, true, kLimNone, KM> : nullptr;
+}
+template <int DIM>
+GradKernel pick_gradients_neq(bool recon, bool visc, int lim, int K) {
+    const int KM = slot_width(K);
+    if (KM == 4) return pick_gradients_neq_k<DIM, 4>(recon, visc, lim);
+    if constexpr (DIM == 3)
+        if (KM == 6) return pick_gradients_neq_k<DIM, 6>(recon, visc, lim);
+    return nullptr;
+}
+
+using MarchKernel = void (*)(MeshView<double>, Phys, const double*, MarchView, double*, int);
+template <int DIM, int KM>
+MarchKernel pick_march_k(bool recon, bool visc, int lim) {
+    if (lim == kLimPetsc) return nullptr;
+    if (recon) {
+        if (visc) return lim == kLimBJ ? k_march<DIM, true, true, kLimBJ, KM> : k_march<DIM, true, true, kLimNone, KM>;
+        return lim == kLimBJ ? k_march<DIM, true, false, kLimBJ, KM> : k_march<DIM, true, false, kLimNone, KM>;
+    }
+    return visc ? k_march<DIM, false, true, kLimNone, KM> : nullptr;
+}
+// Marching kernels exist where the fused ones do.
+template <int DIM>
+MarchKernel pick_march(bool recon, bool visc, int lim, int K) {
+    const int KM = slot_width(K);
+    if (KM == 4) return pick_march_k<DIM, 4>(recon, visc, lim);
+    if constexpr (DIM == 3)
+        if (KM == 6) return pick_march_k<DIM, 6>(recon, visc, lim);
     return nullptr;
 }
 
@@ -3082,6 +3066,16 @@ struct abl_fv_solver {
     int64_t fu_n = 0;
     size_t fu_lds = 0;
     abl::FusedKernel fused_kernel = nullptr;
+    std::string fused_unavailable;  // why the tiles were not built, when it failed
+    // marching strategy (built at creation when available)
+    abl::DevBuf<int32_t> mr_seq, mr_pro, mr_stile, mr_snext, mr_fstart, mr_ffaces, mr_fcode, mr_pfstart, mr_pffaces,
+        mr_pfcode, mr_hstart, mr_hcell, mr_hinfo, mr_gslot;
+    int mr_T = 0, mr_nfm = 0, mr_nem = 0;
+    int64_t mr_nseq = 0;
+    size_t mr_lds = 0;
+    abl::MarchKernel march_kernel = nullptr;
+    std::string march_unavailable;  // why the plan was not built, when it failed
+    bool grad_from_offsets = false;  // the FP64 gradient pass solves the normal equations from the face offsets
     // mixed form: FP32 geometry and gradients
     abl::DevBuf<float> fna_f, fdx_f, fwa_f;
     abl::DevBuf<float> sw_f, swa_f, slim_f;  // slot-major gradient geometry (mixed form)
@@ -3100,12 +3094,11 @@ struct abl_fv_solver {
     abl::VjpKernel vjp_kernel = nullptr;
     // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
     abl::AdjFacesKernel adj_faces = nullptr;
-    abl::AdjFaceGatherKernel adj_face_gather = nullptr;
     abl::DevBuf<double> adjW, adjGL;  // face adjoint records, boundary-face species extras
-    abl::AdjCellsKernel adj_cells = nullptr;
+    abl::AdjCellsKernel adj_cells_e = nullptr, adj_cells_s = nullptr;
     abl::AdjGatherKernel adj_gather = nullptr;
     abl::DevBuf<int32_t> mirror;
-    abl::DevBuf<double> adjA, adjUghost, adjUself, adjContrib;
+    abl::DevBuf<double> adjUghost, adjUself, adjContrib;
     abl::VjpThetaKernel vjp_theta_kernel = nullptr;
     abl::GradKernel grad_kernel = nullptr;
     abl::SpeciesGradKernelT<double> species_kernel = nullptr;
@@ -3273,14 +3266,19 @@ struct abl_fv_solver {
         const int hw = ne + (visc ? lay.dim + nt : 0);
         abl::FusedTiles ft;
         size_t lds = 0;
-        for (int T = 128; T >= 64; T /= 2) {
-            abl::build_fused_tiles(lay, KM, T, ft);
-            lds = sizeof(double) * ((size_t)rw * T + (size_t)hw * ft.nem + (size_t)ne * ft.nfm);
-            if (lds <= (size_t)abl::kCuLdsBytes / 2) break;
-            ft.n = 0;
+        try {
+            for (int T = 128; T >= 64; T /= 2) {
+                abl::build_fused_tiles(lay, KM, T, ft);
+                lds = sizeof(double) * ((size_t)rw * T + (size_t)hw * ft.nem + (size_t)ne * ft.nfm);
+                if (lds <= (size_t)abl::kCuLdsBytes / 2) break;
+                ft.n = 0;
+            }
+            if (ft.n == 0) return false;
+            abl::check_fused_tiles(lay, KM, ft);
+        } catch (const std::exception& e) {  // the strategy stays unavailable; the others are unaffected
+            fused_unavailable = e.what();
+            return false;
         }
-        if (ft.n == 0) return false;
-        abl::check_fused_tiles(lay, KM, ft);
         HIP_CHECK(hipFuncSetAttribute((const void*)fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         fu_start.upload(ft.fstart);
         fu_faces.upload(ft.ffaces);
@@ -3295,6 +3293,76 @@ struct abl_fv_solver {
         fu_nem = ft.nem;
         fu_lds = lds;
         return true;
+    }
+    // Marching plan: tiles of 64 cells, sequences sized so that the workgroups fill the device in about four rounds
+    // at three workgroups per CU (LDS of ~51 KB per workgroup for the 3-D Navier-Stokes records).  Needs the host
+    // cell -> face table and face offsets, so it is built at creation; false when unavailable.
+    bool build_march(const abl::Layout& lay) {
+        if (!march_kernel) return false;
+        if ((recon || visc) && !abl::lsq_stencils_full_rank(lay)) return false;
+        const int KM = abl::slot_width(lay.K);
+        const int ne = lay.dim + 2, nt = lay.dim * (lay.dim + 1) / 2 + lay.dim;
+        const int rw = ne + (recon ? ne * lay.dim : 0) + (visc ? nt : 0);
+        const int hw = ne + (visc ? lay.dim + nt : 0);
+        const int T = 64;
+        int dev = 0, cus = 256;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t tiles = (lay.NI + T - 1) / T;
+        const int64_t target = std::max<int64_t>(1, (int64_t)cus * 3 * 4);
+        const int max_steps = (int)std::max<int64_t>(2, (tiles + target - 1) / target);
+        abl::MarchPlan mp;
+        try {
+            abl::build_march_plan(lay, KM, T, max_steps, mp);
+            abl::check_march_plan(lay, KM, mp);
+        } catch (const std::exception& e) {  // the strategy stays unavailable; the others are unaffected
+            march_unavailable = e.what();
+            return false;
+        }
+        const size_t lds = sizeof(double) * ((size_t)2 * rw * T + (size_t)hw * mp.nem + (size_t)2 * ne * mp.nfm);
+        if (lds > (size_t)abl::kCuLdsBytes) return false;
+        HIP_CHECK(hipFuncSetAttribute((const void*)march_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        mr_seq.upload(mp.seq);
+        mr_pro.upload(mp.pro);
+        mr_stile.upload(mp.stile);
+        mr_snext.upload(mp.snext);
+        mr_fstart.upload(mp.fstart);
+        mr_ffaces.upload(mp.ffaces);
+        mr_fcode.upload(mp.fcode);
+        mr_pfstart.upload(mp.pfstart);
+        mr_pffaces.upload(mp.pffaces);
+        mr_pfcode.upload(mp.pfcode);
+        mr_hstart.upload(mp.hstart);
+        mr_hcell.upload(mp.hcell);
+        mr_hinfo.upload(mp.hinfo);
+        mr_gslot.upload(mp.gslot);
+        mr_T = mp.T;
+        mr_nfm = std::max(mp.nfm, 1);
+        mr_nem = std::max(mp.nem, 1);
+        mr_nseq = mp.n_seq;
+        mr_lds = sizeof(double) * ((size_t)2 * rw * T + (size_t)hw * mr_nem + (size_t)2 * ne * mr_nfm);
+        return true;
+    }
+    abl::MarchView march_view() const {
+        abl::MarchView v;
+        v.seq = mr_seq.ptr;
+        v.pro = mr_pro.ptr;
+        v.stile = mr_stile.ptr;
+        v.snext = mr_snext.ptr;
+        v.fstart = mr_fstart.ptr;
+        v.ffaces = mr_ffaces.ptr;
+        v.fcode = mr_fcode.ptr;
+        v.pfstart = mr_pfstart.ptr;
+        v.pffaces = mr_pffaces.ptr;
+        v.pfcode = mr_pfcode.ptr;
+        v.hstart = mr_hstart.ptr;
+        v.hcell = mr_hcell.ptr;
+        v.hinfo = mr_hinfo.ptr;
+        v.gslot = mr_gslot.ptr;
+        v.T = mr_T;
+        v.nfm = mr_nfm;
+        v.nem = mr_nem;
+        return v;
     }
     abl::FusedView fused_view() const {
         abl::FusedView v;
@@ -3327,8 +3395,8 @@ struct abl_fv_solver {
         return ABL_FV_STRATEGY_FACE_FLUX;
     }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + adjA.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + mr_seq.bytes() + mr_pro.bytes() + mr_stile.bytes() + mr_snext.bytes() + mr_fstart.bytes() + mr_ffaces.bytes() + mr_fcode.bytes() + mr_pfstart.bytes() + mr_pffaces.bytes() + mr_pfcode.bytes() + mr_hstart.bytes() + mr_hcell.bytes() + mr_hinfo.bytes() + mr_gslot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+                         gradA.bytes() + rec.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
 };
@@ -3413,6 +3481,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->tile_kernel = abl::pick_tile<3>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<3>(species, L.K);
             if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<3>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species) s->march_kernel = abl::pick_march<3>(s->recon, s->visc, lim, L.K);
         } else {
             s->species_kernel = abl::pick_species_gradients<double, 2>(L.K);
             s->species_kernel_f = abl::pick_species_gradients<float, 2>(L.K);
@@ -3422,6 +3491,7 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->tile_kernel = abl::pick_tile<2>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<2>(species, L.K);
             if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<2>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species) s->march_kernel = abl::pick_march<2>(s->recon, s->visc, lim, L.K);
         }
 
         s->ell.upload(L.ell);
@@ -3507,11 +3577,11 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             if (mesh->dim == 3) {
                 s->vjp_kernel = abl::pick_vjp<3>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<3>(rus);
-                abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_face_gather, s->adj_cells, s->adj_gather);
+                abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_cells_e, s->adj_cells_s, s->adj_gather);
             } else {
                 s->vjp_kernel = abl::pick_vjp<2>(rus, L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<2>(rus);
-                abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_face_gather, s->adj_cells, s->adj_gather);
+                abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_cells_e, s->adj_cells_s, s->adj_gather);
             }
             if (s->recon || s->visc) {
                 // mirror slots: for slot k of cell c, the slot of the same face in the neighbour's list
@@ -3567,6 +3637,17 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         }
         s->build_tiles(L);
         s->build_fused(L);
+        s->build_march(L);
+        // FP64 gradient pass from the face offsets where every stencil has full rank (drops the slot-major weight and
+        // limiter geometry reads); the mixed form keeps its FP32 weights
+        if (form == ABL_FV_FORM_FP64 && (s->recon || s->visc) && abl::lsq_stencils_full_rank(L)) {
+            const abl::GradKernel gk = mesh->dim == 3 ? abl::pick_gradients_neq<3>(s->recon, s->visc, lim, L.K)
+                                                      : abl::pick_gradients_neq<2>(s->recon, s->visc, lim, L.K);
+            if (gk) {
+                s->grad_kernel = gk;
+                s->grad_from_offsets = true;
+            }
+        }
         s->strategy = s->auto_strategy();
         abl::resolve_noalloc_scratch();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
@@ -3713,6 +3794,11 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
                                        accumulate);
                 });
             }
+        } else if (h->strategy == ABL_FV_STRATEGY_MARCH) {
+            h->timed(ABL_FV_PHASE_FLUX, st, [&] {
+                hipLaunchKernelGGL(h->march_kernel, dim3((unsigned)h->mr_nseq), dim3(abl::kMarchBlock), h->mr_lds, st,
+                                   h->view(), h->ph, U, h->march_view(), R, accumulate);
+            });
         } else if (h->strategy == ABL_FV_STRATEGY_FUSED) {
             h->timed(ABL_FV_PHASE_FLUX, st, [&] {
                 hipLaunchKernelGGL(h->fused_kernel, dim3((unsigned)h->fu_n), dim3(256), h->fu_lds, st, h->view(), h->ph, U,
@@ -3765,12 +3851,18 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
 extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
     if (!h) return fail(h, "abl_fv_set_strategy: null handle");
     if (strategy != ABL_FV_STRATEGY_AUTO && strategy != ABL_FV_STRATEGY_RECOMPUTE &&
-        strategy != ABL_FV_STRATEGY_FACE_FLUX && strategy != ABL_FV_STRATEGY_TILE && strategy != ABL_FV_STRATEGY_FUSED)
+        strategy != ABL_FV_STRATEGY_FACE_FLUX && strategy != ABL_FV_STRATEGY_TILE && strategy != ABL_FV_STRATEGY_FUSED &&
+        strategy != ABL_FV_STRATEGY_MARCH)
         return fail(h, "abl_fv_set_strategy: unknown strategy");
     if (strategy == ABL_FV_STRATEGY_FUSED && h->fu_n == 0)
         return fail(h, "abl_fv_set_strategy: the fused strategy is unavailable for this handle (it needs the FP64 form "
                        "without species, no limiter or Barth-Jespersen, 2-D cells or tetrahedra / hexahedra, prisms, "
-                       "pyramids, well-conditioned least-squares stencils and a tile that fits the LDS)");
+                       "pyramids, well-conditioned least-squares stencils and a tile that fits the LDS)" +
+                           (h->fused_unavailable.empty() ? std::string() : ": " + h->fused_unavailable));
+    if (strategy == ABL_FV_STRATEGY_MARCH && h->mr_nseq == 0)
+        return fail(h, "abl_fv_set_strategy: the march strategy is unavailable for this handle (it needs what the fused "
+                       "strategy needs and a marching plan whose LDS fits a workgroup)" +
+                           (h->march_unavailable.empty() ? std::string() : ": " + h->march_unavailable));
     if (strategy == ABL_FV_STRATEGY_AUTO) strategy = h->auto_strategy();
     if (strategy == ABL_FV_STRATEGY_TILE && h->tile_n == 0)
         return fail(h, "abl_fv_set_strategy: the tile strategy is unavailable for this handle (no tile size fits the "
@@ -3859,12 +3951,9 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
     try {
         const abl::Layout& L = h->host;
         const int KM = abl::slot_width(L.K);
-        const int nt = L.dim * (L.dim + 1) / 2 + L.dim;
         const int64_t NG = L.N - L.NI;
-        const size_t na = (size_t)L.nc + (size_t)L.nc * L.dim + nt + (size_t)L.ns * L.dim;
         const int nwp = L.dim == 3 ? abl::AdjFaceRec<3>::pairs(L.ns) : abl::AdjFaceRec<2>::pairs(L.ns);
-        if (!h->adjA.ptr) {
-            h->adjA.alloc(na * L.NI);
+        if (!h->adjUself.ptr) {
             h->adjUghost.alloc((size_t)L.nc * std::max<int64_t>(NG, 1));
             h->adjUself.alloc((size_t)L.nc * L.NI);
             h->adjContrib.alloc((size_t)KM * L.nc * L.NI);
@@ -3887,13 +3976,13 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
             h->timed(ABL_FV_PHASE_SPECIES_GRADIENTS, st, [&] {
                 hipLaunchKernelGGL(h->species_kernel, grid, block, 0, st, v, h->ph, U, h->gradC.ptr, h->gradA.ptr);
             });
-        abl::AdjBufs b{h->adjA.ptr, h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM,
+        abl::AdjBufs b{h->adjUghost.ptr, h->adjUself.ptr, h->adjContrib.ptr, h->mirror.ptr, KM,
                        h->adjW.ptr, nwp, h->adjGL.ptr};
         h->timed(ABL_FV_PHASE_VJP, st, [&] {
             hipLaunchKernelGGL(h->adj_faces, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, Rbar, h->rec.ptr,
                                h->gradC.ptr, h->gradA.ptr, b);
-            hipLaunchKernelGGL(h->adj_face_gather, grid, block, 0, st, v, h->ph, U, b);
-            hipLaunchKernelGGL(h->adj_cells, grid, block, 0, st, v, h->ph, U, h->ghost_slot.ptr, h->slot_kind.ptr, b);
+            hipLaunchKernelGGL(h->adj_cells_e, grid, block, 0, st, v, h->ph, U, b);
+            if (L.ns) hipLaunchKernelGGL(h->adj_cells_s, grid, block, 0, st, v, h->ph, U, b);
             hipLaunchKernelGGL(h->adj_gather, grid, block, 0, st, v, h->ph, h->ghost_slot.ptr, h->slot_kind.ptr, b,
                                Ubar);
         });

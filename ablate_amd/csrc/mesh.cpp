@@ -9,7 +9,8 @@
 //   - periodic directions produce wrap faces whose right cell is translated into the left cell's frame by
 //     face_shift (the localized-coordinate image).
 // Cell orderings: natural (x fastest), brick (8^3 / 16^2 bricks in Morton order of the bricks; a 3-D brick walked as
-// four 8x4x4 blocks), Morton order of the cell centroids (unstructured meshes) and reverse Cuthill-McKee.
+// four 8x4x4 blocks), column (3-D boxes: 8 x 8 columns walked layer by layer in z), Morton order of the cell
+// centroids (unstructured meshes) and reverse Cuthill-McKee.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -81,6 +82,19 @@ std::vector<int64_t> box_order(int dim, const int32_t* n, int32_t ordering) {
                             for (int64_t x = bx * B; x < std::min<int64_t>(n[0], bx * B + B); x++)
                                 order[k++] = x + (int64_t)n[0] * (y + (int64_t)n[1] * z);
         }
+        return order;
+    }
+    if (ordering == ABL_FV_ORDER_COLUMN) {
+        // columns of B x B cells in (x, y), x-major; inside a column z-layers, each layer y-major x-minor
+        const int B = 8;
+        const int64_t ncx = (n[0] + B - 1) / B, ncy = (n[1] + B - 1) / B;
+        int64_t k = 0;
+        for (int64_t cy = 0; cy < ncy; cy++)
+            for (int64_t cx = 0; cx < ncx; cx++)
+                for (int64_t z = 0; z < n[2]; z++)
+                    for (int64_t y = cy * B; y < std::min<int64_t>(n[1], cy * B + B); y++)
+                        for (int64_t x = cx * B; x < std::min<int64_t>(n[0], cx * B + B); x++)
+                            order[k++] = x + (int64_t)n[0] * (y + (int64_t)n[1] * z);
         return order;
     }
     // RCM over the box's face graph (periodicity ignored for the ordering)
@@ -206,8 +220,9 @@ extern "C" int abl_fv_mesh_box(int32_t dim, const int32_t* cells, const double* 
     try {
         if (!out || !cells || !lower || !upper) throw std::invalid_argument("abl_fv_mesh_box: null argument");
         if (dim != 2 && dim != 3) throw std::invalid_argument("abl_fv_mesh_box: dim must be 2 or 3");
-        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_BRICK && ordering != ABL_FV_ORDER_RCM)
-            throw std::invalid_argument("abl_fv_mesh_box: ordering must be natural, brick or RCM");
+        if (ordering != ABL_FV_ORDER_NATURAL && ordering != ABL_FV_ORDER_BRICK && ordering != ABL_FV_ORDER_RCM &&
+            !(ordering == ABL_FV_ORDER_COLUMN && dim == 3))
+            throw std::invalid_argument("abl_fv_mesh_box: ordering must be natural, brick, RCM or (3-D) column");
         int32_t n[3] = {1, 1, 1};
         double h[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, len[3] = {1, 1, 1};
         int per[3] = {0, 0, 0};

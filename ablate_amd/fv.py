@@ -426,16 +426,17 @@ class FiniteVolumeSolver:
         return tuple(out)
 
     def set_strategy(self, strategy: str) -> None:
-        """"auto" (the default at creation: fused when available, else tile or face_flux from the mesh's tile
-        statistics), "fused" (a workgroup per tile of cells computes the gradients of the tile and its halo, the
-        tile's face fluxes and its residuals in one kernel), "tile" (a workgroup per tile of cells evaluates the
-        tile's faces into LDS and gathers its cells), "face_flux" (stored once in HBM, gathered per cell) or
-        "recompute" (each face flux evaluated by both cells)."""
+        """"auto" (the default at creation: tile or face_flux from the mesh's tile statistics), "march" (a workgroup
+        walks a sequence of face-sharing tiles, computing the next tile's gradients while it evaluates and gathers
+        the current one: the z-layers of a column-ordered box), "fused" (a workgroup per tile of cells computes the
+        gradients of the tile and its halo, the tile's face fluxes and its residuals in one kernel), "tile" (a
+        workgroup per tile of cells evaluates the tile's faces into LDS and gathers its cells), "face_flux" (stored
+        once in HBM, gathered per cell) or "recompute" (each face flux evaluated by both cells)."""
         _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]), self._h, "abl_fv_set_strategy")
 
     def has_strategy(self, strategy: str) -> bool:
-        """Whether this handle offers a strategy (the fused one needs the FP64 form without species, no limiter or
-        Barth-Jespersen and well-conditioned least-squares stencils).  The current strategy is kept."""
+        """Whether this handle offers a strategy (the fused and march ones need the FP64 form without species, no
+        limiter or Barth-Jespersen and well-conditioned least-squares stencils).  The current strategy is kept."""
         current = self.strategy()
         ok = _lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[strategy]) == 0
         _lib.check(_lib.lib().abl_fv_set_strategy(self._h, _lib.STRATEGY[current]), self._h, "abl_fv_set_strategy")

@@ -34,8 +34,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (default: the workload's 256)")
-    ap.add_argument("--strategy", default="auto", choices=["auto", "recompute", "face_flux", "tile", "fused"],
+    ap.add_argument("--strategy", default="auto", choices=["auto", "recompute", "face_flux", "tile", "fused", "march"],
                     help="evaluation strategy of the headline leg (auto: the library's default choice)")
+    ap.add_argument("--ordering", default="auto", choices=["auto", "brick", "column", "natural"],
+                    help="cell order of the headline mesh (auto: column for the march strategy, else brick)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
@@ -94,7 +96,7 @@ def stream_probe(gib, reps=10):
 
 
 KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recompute": ["k_gradients_w4", "k_flux"],
-           "tile": ["k_gradients_w4", "k_tile_flux"], "fused": ["k_fused"]}
+           "tile": ["k_gradients_w4", "k_tile_flux"], "fused": ["k_fused"], "march": ["k_march"]}
 
 
 def measured_traffic(cfg, n, strategy):
@@ -402,7 +404,8 @@ def main():
     cp = 1.4 * 287.0 / 0.4
     k = mu * cp / 0.72
     t_setup = time.perf_counter()
-    mesh = BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [True, True, True], ordering="brick")
+    ordering = args.ordering if args.ordering != "auto" else ("column" if args.strategy == "march" else "brick")
+    mesh = BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [True, True, True], ordering=ordering)
     solver = fv.FiniteVolumeSolver(mesh, ns_processes(fv, mu, k),
                                    {"petscfv_type": "leastsquares", "petsclimiter_type": "barthjespersen"},
                                    form=cfg["form"])
@@ -456,7 +459,7 @@ def main():
             "workload": cfg["workload"],
             "cells": cells,
             "faces": info.n_faces,
-            "mesh": f"{n}^3 periodic hex, brick-ordered",
+            "mesh": f"{n}^3 periodic hex, {ordering}-ordered",
             "form": cfg["form"],
             "strategy": strategy,
             "physics": "NavierStokesTransport(AUSM) + Constant(mu=1.8e-5, Pr=0.72), LSQ + Barth-Jespersen, Mach "

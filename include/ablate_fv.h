@@ -156,7 +156,9 @@ enum {
     ABL_FV_ORDER_NATURAL = 0,
     ABL_FV_ORDER_BRICK = 1,  /* boxes: 8^3 / 16^2 bricks in Morton order of the bricks */
     ABL_FV_ORDER_RCM = 2,    /* reverse Cuthill-McKee of the cell graph */
-    ABL_FV_ORDER_MORTON = 3  /* Morton (Z-order) curve of the cell centroids (tet and Gmsh meshes) */
+    ABL_FV_ORDER_MORTON = 3, /* Morton (Z-order) curve of the cell centroids (tet and Gmsh meshes) */
+    ABL_FV_ORDER_COLUMN = 4  /* 3-D boxes: columns of 8 x 8 cells in x, y (x-major order of the columns), each column
+                                walked in z one 8 x 8 layer after the other (the MARCH strategy's layout) */
 };
 
 /* Uniform quad (dim 2) / hex (dim 3) box: cells[d] cells per direction on [lower, upper], periodic[d] != 0 for
@@ -304,7 +306,7 @@ int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0, double max
 /* ---- evaluation strategy (all deterministic, no float atomics).  The reference has one CPU loop order; these
  *      have no counterpart there and all give the residual of ComputeRHSFunction. ------------------------------ */
 enum {
-    ABL_FV_STRATEGY_AUTO = 0,      /* pick FUSED, TILE or FACE_FLUX from the mesh's tile statistics (the default) */
+    ABL_FV_STRATEGY_AUTO = 0,      /* pick TILE or FACE_FLUX from the mesh's tile statistics (the default) */
     ABL_FV_STRATEGY_RECOMPUTE = 1, /* one lane per cell evaluates each of its faces: every face flux computed twice
                                       (mixed form: no species) */
     ABL_FV_STRATEGY_FACE_FLUX = 2, /* one lane per face stores its flux once; a second pass gathers per cell
@@ -312,12 +314,16 @@ enum {
                                       8 * (n_euler + n_species)) */
     ABL_FV_STRATEGY_TILE = 3,      /* one workgroup per tile of consecutive cells evaluates the tile's faces once into
                                       LDS and gathers its cells; bitwise equal to FACE_FLUX */
-    ABL_FV_STRATEGY_FUSED = 4      /* one workgroup per tile computes the gradients of its cells and of its halo into
+    ABL_FV_STRATEGY_FUSED = 4,     /* one workgroup per tile computes the gradients of its cells and of its halo into
                                       LDS (least-squares normal equations solved per cell, no stored weights), then
                                       the tile's face fluxes and the gather: no cell record or face flux in HBM.
                                       FP64 form without species, no limiter or Barth-Jespersen, full-rank stencils */
+    ABL_FV_STRATEGY_MARCH = 5      /* one workgroup per sequence of 64-cell tiles that share faces (the z-layers of a
+                                      column in ABL_FV_ORDER_COLUMN): each step computes the records of the next tile
+                                      into LDS, evaluates the current tile's faces and gathers it, so faces between
+                                      consecutive tiles are evaluated once; same availability as FUSED */
 };
-/* AUTO chooses FUSED when it is available, else TILE when the tile kernel evaluates at most ~1.8 faces per face
+/* AUTO chooses TILE when the tile kernel evaluates at most ~1.8 faces per face
  * (border faces twice) and its LDS footprint leaves two or more waves per SIMD, FACE_FLUX otherwise.  Selecting FACE_FLUX allocates its face-flux
  * storage here, never inside abl_fv_rhs; TILE fails when no tile size fits the LDS budget. */
 int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy);

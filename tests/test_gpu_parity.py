@@ -3,19 +3,21 @@ C ABI, plus determinism, accumulate semantics and AoS packing."""
 import numpy as np
 import pytest
 
+from ablate_amd.mesh import BoxMesh
 from tests.fvcases import Case, all_outflow, inflow_x, rel_err_per_field, standard_cases
 
 pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-12
 CASES = standard_cases()
-STRATEGIES = ["recompute", "face_flux", "tile", "fused"]
+STRATEGIES = ["recompute", "face_flux", "tile", "fused", "march"]
 
 
 def use_strategy(solver, strategy):
-    """Selects a strategy; the fused one exists only for some handles (FP64, no species, full-rank stencils)."""
-    if strategy == "fused" and not solver.has_strategy("fused"):
-        pytest.skip("fused strategy unavailable for this handle")
+    """Selects a strategy; the fused and march ones exist only for some handles (FP64, no species, full-rank
+    stencils)."""
+    if strategy in ("fused", "march") and not solver.has_strategy(strategy):
+        pytest.skip(f"{strategy} strategy unavailable for this handle")
     solver.set_strategy(strategy)
 
 
@@ -132,6 +134,72 @@ def test_fused_agrees_with_face_flux(case, torch_cuda):
     NI = case.mesh.n_interior
     err = ((Ra - Rb)[:, :NI].norm(dim=1) / Rb[:, :NI].norm(dim=1).clamp_min(1e-300)).max().item()
     assert err <= 1e-13, err
+
+
+MARCH_CASES = CASES + [
+    Case("march_columns_periodic", BoxMesh([24, 16, 20], [0, 0, 0], [1, 1, 1], [True] * 3, ordering="column"),
+         recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125),
+    Case("march_columns_walls", BoxMesh([20, 12, 16], [0, 0, 0], [1, 1, 1], [False, True, False], ordering="column"),
+         recon="leastsquares", limiter="barthjespersen", mu=0.05, k=0.5, bcs={**inflow_x(0), 1: {"kind": "outflow"},
+                                                                               4: {"kind": "outflow"}, 5: {"kind": "outflow"}}),
+]
+
+
+@pytest.mark.parametrize("case", MARCH_CASES, ids=[c.name for c in MARCH_CASES])
+def test_march_agrees_with_face_flux_and_oracle(case, torch_cuda):
+    """The march kernel solves the least-squares normal equations per cell like the fused one, so it agrees with the
+    face-flux residual to rounding (not bitwise) and with the oracle to 1e-12; deterministic run to run."""
+    torch = torch_cuda
+    solver = case.solver()
+    use_strategy(solver, "march")
+    print(f"[parity] march {case.name}", flush=True)
+    U0 = case.initial_state()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ra, Rb, Rc = torch.zeros_like(U), torch.zeros_like(U), torch.zeros_like(U)
+    solver.compute_rhs(0.0, U, Ra)
+    solver.compute_rhs(0.0, U, Rc)
+    solver.set_strategy("face_flux")
+    solver.compute_rhs(0.0, U, Rb)
+    torch.cuda.synchronize()
+    assert torch.equal(Ra, Rc)
+    NI = case.mesh.n_interior
+    err = ((Ra - Rb)[:, :NI].norm(dim=1) / Rb[:, :NI].norm(dim=1).clamp_min(1e-300)).max().item()
+    assert err <= 1e-13, err
+    Rref, _ = case.oracle_rhs(U0.copy())
+    Rg = Ra.cpu().numpy()
+    for p in range(Rg.shape[0]):
+        ref = Rref[p, :NI]
+        e = np.linalg.norm(Rg[p, :NI] - ref) / max(np.linalg.norm(ref), 1e-300)
+        assert e <= TOL_FP64, (p, e)
+
+
+def test_march_available_on_column_boxes(torch_cuda):
+    """The march strategy is offered on column-ordered boxes (periodic and walled) and refused with a message for
+    species, the mixed form and the PETSc limiter family."""
+    for case in MARCH_CASES[-2:]:
+        assert case.solver().has_strategy("march"), case.name
+    box = MARCH_CASES[-2].mesh
+    assert not Case("m_mixed", box, recon="leastsquares", limiter="barthjespersen").solver(form="mixed").has_strategy("march")
+    species = Case("m_species", box, recon="leastsquares", limiter="barthjespersen", n_species=2).solver()
+    with pytest.raises(RuntimeError, match="march strategy is unavailable"):
+        species.set_strategy("march")
+    assert not Case("m_minmod", box, recon="leastsquares", limiter="minmod").solver().has_strategy("march")
+
+
+def test_march_accumulates(torch_cuda):
+    """accumulate = 1 adds the residual into R (reference locF += semantics), also for the march kernel."""
+    torch = torch_cuda
+    case = MARCH_CASES[-2]
+    solver = case.solver()
+    use_strategy(solver, "march")
+    U = torch.tensor(case.initial_state(), dtype=torch.float64, device="cuda")
+    R1 = torch.zeros_like(U)
+    solver.compute_rhs(0.0, U, R1)
+    R2 = torch.full_like(U, 2.5)
+    solver.compute_rhs(0.0, U, R2, accumulate=True)
+    torch.cuda.synchronize()
+    NI = case.mesh.n_interior
+    assert torch.equal(R2[:, :NI], R1[:, :NI] + 2.5)
 
 
 def test_fused_availability(torch_cuda):

@@ -64,7 +64,7 @@ def test_box_counts_and_orderings_agree():
     assert wall.n_faces == 7 * 5 * 4 + 6 * 6 * 4 + 6 * 5 * 5
     assert wall.n_cells - wall.n_interior == 2 * (5 * 4 + 6 * 4 + 6 * 5)
     # orderings are permutations of the same cell set
-    for order in ("natural", "brick", "rcm"):
+    for order in ("natural", "brick", "rcm", "column"):
         m = BoxMesh([6, 5, 4], [0, 0, 0], [1, 1, 1], [True, True, True], order)
         c = np.sort(np.round(m.arrays.cell_centroid[:120] * 1000).view(np.float64).reshape(-1, 3), axis=0)
         np.testing.assert_allclose(c, np.sort(np.round(per.arrays.cell_centroid * 1000), axis=0))
@@ -79,3 +79,18 @@ def test_tet_mesh_counts_and_jitter():
     np.testing.assert_array_equal(m.arrays.cell_centroid, m2.arrays.cell_centroid)
     with pytest.raises(RuntimeError):
         TetBoxMesh(3, 0.6)
+
+
+def test_column_order_walks_8x8_columns_layer_by_layer():
+    """Column order: 8 x 8 (x, y) columns in x-major order, each walked in z one layer after the other, so every 64
+    consecutive cells of a full column are one z-layer and consecutive layers of a column share 64 faces."""
+    m = BoxMesh([16, 8, 5], [0, 0, 0], [16, 8, 5], [True, True, True], "column")
+    x = m.arrays.cell_centroid[: m.n_interior]
+    ijk = np.floor(x).astype(int)
+    assert sorted(map(tuple, ijk)) == sorted((i, j, k) for i in range(16) for j in range(8) for k in range(5))
+    for c in range(m.n_interior):
+        col, rest = divmod(c, 64 * 5)
+        layer, pos = divmod(rest, 64)
+        assert tuple(ijk[c]) == (8 * col + pos % 8, pos // 8, layer), c
+    with pytest.raises(RuntimeError, match="column"):
+        BoxMesh([8, 8], [0, 0], [1, 1], [True, True], "column")
