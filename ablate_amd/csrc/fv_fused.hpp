@@ -615,43 +615,3 @@ __global__ __launch_bounds__(kMarchBlock) __attribute__((amdgpu_waves_per_eu(2))
         }
     }
 }
-
-// ============================================================================ gradient pass from the face offsets
-//
-// The two-pass strategies' FP64 gradient pass when every least-squares stencil has full rank: each cell solves its
-// normal equations from the face offsets like the fused kernel (fused_cell) instead of streaming the slot-major
-// weights and limiter geometry (2 x KM x DIM doubles per cell), and writes the same cell record as gradients_body.
-template <int DIM, bool RECON, bool VISC, int LIM, int KM>
-__device__ __forceinline__ void gradients_neq_body(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                   void* __restrict__ rec) {
-    using CR = CellRec<DIM>;
-    constexpr int NE = DIM + 2;
-    const int64_t c_lane = xcd_cell_index();
-    const bool active = c_lane < m.NI;
-    const int64_t c = active ? c_lane : m.NI - 1;  // lanes past the end shadow the last cell and store nothing
-    double uc[NE], g[NE][DIM], tq[ViscTerms<DIM>::NT], xf[KM][DIM];
-    fused_cell<DIM, RECON, VISC, LIM, KM>(m, ph, U, c, uc, g, tq, xf);
-    if (!active) return;
-    double w[CR::RS];
-#pragma unroll
-    for (int p = 0; p < NE; p++) {
-        w[p] = uc[p];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) w[CR::G0 + p * DIM + d] = RECON ? g[p][d] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < ViscTerms<DIM>::NT; q++) w[CR::T0 + q] = VISC ? tq[q] : 0.0;
-#pragma unroll
-    for (int i = CR::END; i < CR::RS; i++) w[i] = 0.0;
-    double2* __restrict__ r = reinterpret_cast<double2*>(rec) + rec_pair<DIM>(c_lane);
-#pragma unroll
-    for (int q = 0; q < CR::NP; q++) r[CR::BLOCK * q] = make_double2(w[2 * q], w[2 * q + 1]);
-}
-template <int DIM, bool RECON, bool VISC, int LIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_gradients_neq(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                          double* __restrict__ gradC, double* __restrict__ gradA,
-                                                          void* __restrict__ rec) {
-    (void)gradC;
-    (void)gradA;
-    gradients_neq_body<DIM, RECON, VISC, LIM, KM>(m, ph, U, rec);
-}

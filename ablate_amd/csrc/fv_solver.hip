@@ -420,88 +420,101 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
         irn[k] = aux ? 1.0 / U[kRho * N + nb[k]] : 0.0;
     }
     bool cancel = false;
-    for (int sp = 0; sp < ph.ns; sp++) {
-        const int64_t comp = NE + sp;
-        const double uc = U[comp * N + c];
-        const double yc = uc * irho_c;
-        double un[KM];
+    // species in pairs: both species' loads are issued before either is reduced, so two rounds of neighbour-load
+    // latency overlap
+    for (int sp0 = 0; sp0 < ph.ns; sp0 += 2) {
+        const int nh = sp0 + 1 < ph.ns ? 2 : 1;
+        double ucp[2], unp[2][KM];
 #pragma unroll
-        for (int k = 0; k < KM; k++) un[k] = U[comp * N + nb[k]];
-        double g[DIM], ga[DIM], umin = uc, umax = uc;
+        for (int h = 0; h < 2; h++) {
+            const int64_t comp = NE + sp0 + (h < nh ? h : 0);
+            ucp[h] = U[comp * N + c];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            g[d] = 0.0;
-            ga[d] = 0.0;
+            for (int k = 0; k < KM; k++) unp[h][k] = U[comp * N + nb[k]];
         }
 #pragma unroll
-        for (int k = 0; k < KM; k++) {
-            if (!valid[k]) continue;
-            if (recon) {
-                umin = fmin(umin, un[k]);
-                umax = fmax(umax, un[k]);
-                if (inner[k]) {
-                    const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
-                }
-            }
-            if (aux) {
-                const double delta = un[k] * irn[k] - yc;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
-            }
-        }
-        if (recon) {
-            double phi = DBL_MAX;
-            if (bj) {
-                double dpos = 0.0, dneg = 0.0;
-#pragma unroll
-                for (int k = 0; k < KM; k++) {
-                    if (!valid[k]) continue;
-                    double D = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
-                    dpos = fmax(dpos, D);
-                    dneg = fmin(dneg, D);
-                }
-                phi = 1.0;
-                if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
-                if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
-            } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
-#pragma unroll
-                for (int k = 0; k < KM; k++) {
-                    if (!valid[k]) continue;
-                    double denom = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
-                    const double flim = 0.5 * (un[k] - uc) / denom;
-                    phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
-                }
-            } else {
-                phi = 1.0;
-            }
-            if (phi == 0)
-#pragma unroll
-                for (int d = 0; d < DIM; d++)
-                    if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
-        }
-        {
-            using V2 = typename Vec2T<G>::type;
-            G v[2 * DIM];
+        for (int h = 0; h < 2; h++) {
+            if (h >= nh) break;
+            const int sp = sp0 + h;
+            const double uc = ucp[h];
+            const double yc = uc * irho_c;
+            const double* un = unp[h];
+            double g[DIM], ga[DIM], umin = uc, umax = uc;
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                v[d] = recon ? G(g[d]) : G(0);
-                v[DIM + d] = aux ? G(ga[d]) : G(0);
+                g[d] = 0.0;
+                ga[d] = 0.0;
             }
-            V2* __restrict__ o = reinterpret_cast<V2*>(gradC);
 #pragma unroll
-            for (int j = 0; j < DIM; j++) {
-                V2 t;
-                t.x = v[2 * j];
-                t.y = v[2 * j + 1];
-                o[((int64_t)sp * DIM + j) * N + c] = t;
+            for (int k = 0; k < KM; k++) {
+                if (!valid[k]) continue;
+                if (recon) {
+                    umin = fmin(umin, un[k]);
+                    umax = fmax(umax, un[k]);
+                    if (inner[k]) {
+                        const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
+                    }
+                }
+                if (aux) {
+                    const double delta = un[k] * irn[k] - yc;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
+                }
+            }
+            if (recon) {
+                double phi = DBL_MAX;
+                if (bj) {
+                    double dpos = 0.0, dneg = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KM; k++) {
+                        if (!valid[k]) continue;
+                        double D = 0.0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
+                        dpos = fmax(dpos, D);
+                        dneg = fmin(dneg, D);
+                    }
+                    phi = 1.0;
+                    if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
+                    if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
+                } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
+#pragma unroll
+                    for (int k = 0; k < KM; k++) {
+                        if (!valid[k]) continue;
+                        double denom = 0.0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
+                        const double flim = 0.5 * (un[k] - uc) / denom;
+                        phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
+                    }
+                } else {
+                    phi = 1.0;
+                }
+                if (phi == 0)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
+            }
+            {
+                using V2 = typename Vec2T<G>::type;
+                G v[2 * DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    v[d] = recon ? G(g[d]) : G(0);
+                    v[DIM + d] = aux ? G(ga[d]) : G(0);
+                }
+                V2* __restrict__ o = reinterpret_cast<V2*>(gradC);
+#pragma unroll
+                for (int j = 0; j < DIM; j++) {
+                    V2 t;
+                    t.x = v[2 * j];
+                    t.y = v[2 * j + 1];
+                    o[((int64_t)sp * DIM + j) * N + c] = t;
+                }
             }
         }
     }
@@ -834,10 +847,10 @@ __device__ inline void eval_face(const MeshView<double>& m, const Phys& ph, cons
 // (coalesced 16-byte stores; species inside the record would make every store of a wave touch 64 cache lines,
 // measured 2.5x slower).
 template <int DIM, bool RECON, bool VISC, bool SPECIES>
-__global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
-                                                      const double* __restrict__ gradC, const double* __restrict__ gradA,
-                                                      const double* __restrict__ rec, double* __restrict__ Fe,
-                                                      double* __restrict__ Fs) {
+__device__ __forceinline__ void face_flux_body(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                               const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                               const double* __restrict__ rec, double* __restrict__ Fe,
+                                               double* __restrict__ Fs) {
     constexpr int NE = DIM + 2;
     constexpr int FS = (NE + 1) & ~1;
     const int64_t f = xcd_cell_index();
@@ -849,6 +862,14 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys p
 #pragma unroll
     for (int i = 0; i < FS / 2; i++) out[i] = make_double2(flux[2 * i], flux[2 * i + 1]);
 }
+template <int DIM, bool RECON, bool VISC, bool SPECIES>
+__global__ __launch_bounds__(kBlock) void k_face_flux(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ gradC, const double* __restrict__ gradA,
+                                                      const double* __restrict__ rec, double* __restrict__ Fe,
+                                                      double* __restrict__ Fs) {
+    face_flux_body<DIM, RECON, VISC, SPECIES>(m, ph, U, gradC, gradA, rec, Fe, Fs);
+}
+
 
 // Strategy 3 (tile): one workgroup per tile of T consecutive interior cells.  Its lanes evaluate every face of the
 // tile once into LDS (faces shared with a neighbouring tile are evaluated by both), then each lane gathers its
@@ -1460,24 +1481,34 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
 }
 
 // Mixed form, face-flux strategy, pass 1: one lane per face stores both sides' FP32 perturbations as rows
-// [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.
+// [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.  The
+// species perturbations are formed in LDS (they are assigned, then the diffusion flux is added) and stored once:
+// formed in place in the rows, every value was written back two to three times.
 template <int DIM, bool RECON, bool VISC, bool SPECIES, bool RUS>
 __global__ __launch_bounds__(kBlock) void k_face_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                             const float* __restrict__ rec, const float* __restrict__ gS,
                                                             const float* __restrict__ gY, float* __restrict__ Fm,
                                                             int nrow) {
     constexpr int NE = DIM + 2;
+    extern __shared__ float sbuf[];  // [2 ns][blockDim]: this lane's species perturbations while they are formed
     const int64_t f = xcd_cell_index();
     if (f >= m.F) return;
     const int64_t F = m.F;
+    const int bd = blockDim.x, ns = ph.ns;
+    float* sL = sbuf + threadIdx.x;
+    float* sR = sbuf + (int64_t)ns * bd + threadIdx.x;
     float dFL[NE], dFR[NE];
-    mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, f, dFL, dFR, Fm + NE * F + f,
-                                          Fm + (nrow + NE) * F + f, F);
+    mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, f, dFL, dFR, sL, sR, bd);
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         Fm[p * F + f] = dFL[p];
         Fm[(nrow + p) * F + f] = dFR[p];
     }
+    if (SPECIES)
+        for (int sp = 0; sp < ns; sp++) {
+            Fm[(NE + sp) * F + f] = sL[sp * bd];
+            Fm[(nrow + NE + sp) * F + f] = sR[sp * bd];
+        }
 }
 
 // Mixed form, face-flux strategy, pass 2: each interior cell gathers its own side's perturbations in face-number
@@ -1750,7 +1781,7 @@ __device__ inline int64_t adjw_pair(int64_t f, int nwp) { return (f >> 6) * 64 *
 // Reverse of one face flux (advection, viscous, species advection and diffusion), both sides at once.  One lane per
 // face (interior and boundary).  The face's ghost-cell adjoints go straight to Ughost (a ghost has one face).
 template <int DIM, bool RUS>
-__global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phys ph, const double* __restrict__ U,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_face_rev(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                          const double* __restrict__ Rb, const double* __restrict__ rec,
                                                          const double* __restrict__ gS, const double* __restrict__ gY,
                                                          AdjBufs b) {
@@ -1812,21 +1843,32 @@ __global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phy
             for (int q = 0; q < NT; q++) tq[q] = 0.0;
         }
     }
-    // species sums of the face values
+    // species sums of the face values; each species' g_sp = Rbar_sp(R) / V_R - Rbar_sp(L) / V_L goes straight into
+    // the face record (read back below instead of the scattered Rbar rows)
+    double2* __restrict__ W = reinterpret_cast<double2*>(b.W) + adjw_pair(f, b.nwp);
+    constexpr int NFIX = AW::GS / 2;  // pairs holding only fixed words
     double QL = 0.0, QR = 0.0, QdL = 0.0;
-    for (int sp = 0; sp < ns; sp++) {
-        const int64_t comp = NE + sp;
-        const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
-        double yl = U[comp * N + L], yr = U[comp * N + Rc];
-        if (recon)
+    for (int q = 0; 2 * q < ns; q++) {
+        double gp[2] = {0.0, 0.0};
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
-            }
-        QL += gsp * yl;
-        QR += gsp * yr;
-        QdL += gsp * (yr - yl);
+        for (int h = 0; h < 2; h++) {
+            const int sp = 2 * q + h;
+            if (sp >= ns) break;
+            const int64_t comp = NE + sp;
+            const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+            gp[h] = gsp;
+            double yl = U[comp * N + L], yr = U[comp * N + Rc];
+            if (recon)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
+                    if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
+                }
+            QL += gsp * yl;
+            QR += gsp * yr;
+            QdL += gsp * (yr - yl);
+        }
+        W[64 * (NFIX + q)] = make_double2(gp[0], gp[1]);
     }
     double ubL[NE], ubR[NE], cL = 0.0, cR = 0.0;
 #pragma unroll
@@ -1909,13 +1951,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phy
         for (int d = 0; d < DIM; d++) aGb[1 + d] += velh[d];
         aux_vjp<DIM>(uR, ph, aGb, ugb);
     }
-    // species: diffusion (face density, ghost gradients); the per-species factors go to the record
-    double2* __restrict__ W = reinterpret_cast<double2*>(b.W) + adjw_pair(f, b.nwp);
+    // species: diffusion (face density, ghost gradients)
     const double cdf = sdiff ? -ph.diff * (0.5 * uL[kRho] + 0.5 * uR[kRho]) * 0.5 * area : 0.0;
     double rhob = 0.0, rgb = 0.0;
     for (int sp = 0; sp < ns; sp++) {
         const int64_t comp = NE + sp;
-        const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+        const double2 gv = W[64 * (NFIX + (sp >> 1))];
+        const double gsp = (sp & 1) ? gv.y : gv.x;
         double ygb = ghost ? cR * gsp : 0.0, yxl = 0.0;
         if (sdiff) {
             const double rl = uL[kRho], rr = uR[kRho];
@@ -1967,16 +2009,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_face_rev(MeshView<double> m, Phy
     w[AW::CR] = cR;
     w[AW::CD] = cdf;
     w[AW::RH] = rhob;
-    constexpr int NFIX = AW::GS / 2;  // pairs holding only fixed words
 #pragma unroll
     for (int q = 0; q < NFIX; q++) W[64 * q] = make_double2(w[2 * q], w[2 * q + 1]);
-    for (int q = 0; 2 * q < ns; q++) {
-        const int sp = 2 * q;
-        const double g0 = (ghost ? 0.0 : Rb[(NE + sp) * N + Rc] * ivR) - Rb[(NE + sp) * N + L] * ivL;
-        const double g1 = sp + 1 < ns ? (ghost ? 0.0 : Rb[(NE + sp + 1) * N + Rc] * ivR) - Rb[(NE + sp + 1) * N + L] * ivL
-                                      : 0.0;
-        W[64 * (NFIX + q)] = make_double2(g0, g1);
-    }
 }
 
 // Barth-Jespersen limiter of one component with its active pieces: phi = min(1, (umax - uc) / dpos, (umin - uc) /
@@ -2007,11 +2041,12 @@ __device__ inline void bj_finish(double uc, BJFwd& L) {
 
 // Reverse of one component's limited least-squares gradient g = phi(G) G, G = sum_k w_k (u_n - u_c), given the
 // adjoint gbl of g: adds the adjoint of the cell's own value to ubc and of each neighbour's to cn[k] (ghost
-// neighbours through to_ghost).  x[k] = x_f - x_c of slot k (the Barth-Jespersen projection vector).
+// neighbours through to_ghost).  unv[k] = the component's value in the neighbour across slot k, x[k] = x_f - x_c of
+// slot k (the Barth-Jespersen projection vector).
 template <int DIM, int KM, typename ToGhost>
-__device__ inline void limited_lsq_vjp(const double* __restrict__ U, int64_t row, double uc, const double (*w)[DIM],
-                                       const double (*x)[DIM], const int64_t* nb, const bool* valid, const bool* bnd,
-                                       bool bj, const double* gbl, double& ubc, double* cn, ToGhost&& to_ghost) {
+__device__ inline void limited_lsq_vjp(const double* unv, double uc, const double (*w)[DIM], const double (*x)[DIM],
+                                       const int64_t* nb, const bool* valid, const bool* bnd, bool bj,
+                                       const double* gbl, double& ubc, double* cn, ToGhost&& to_ghost) {
     double G[DIM], Gb[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) G[d] = 0.0;
@@ -2019,7 +2054,7 @@ __device__ inline void limited_lsq_vjp(const double* __restrict__ U, int64_t row
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         if (!valid[k]) continue;
-        const double un = U[row + nb[k]];
+        const double un = unv[k];
         if (!bnd[k])
 #pragma unroll
             for (int d = 0; d < DIM; d++) G[d] += w[k][d] * (un - uc);
@@ -2183,6 +2218,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys
     for (int k = 0; k < KM; k++)
 #pragma unroll
         for (int p = 0; p < NE; p++) ce[k][p] = 0.0;
+    double unb_all[KM][NE];  // neighbour states (the limiter / least-squares and the aux reverses both read them)
+#pragma unroll
+    for (int k = 0; k < KM; k++)
+#pragma unroll
+        for (int p = 0; p < NE; p++) unb_all[k][p] = U[p * N + nb[k]];
     if (recon) {
         double wl[KM][DIM];
 #pragma unroll
@@ -2191,10 +2231,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys
             for (int d = 0; d < DIM; d++) wl[k][d] = m.sw[(k * DIM + d) * NI + c];
 #pragma unroll
         for (int p = 0; p < NE; p++) {
-            double cn[KM];
+            double cn[KM], unv[KM];
 #pragma unroll
-            for (int k = 0; k < KM; k++) cn[k] = 0.0;
-            limited_lsq_vjp<DIM, KM>(U, (int64_t)p * N, uc[p], wl, xs, nb, valid, bnd, bj, gb[p], ub[p], cn,
+            for (int k = 0; k < KM; k++) {
+                cn[k] = 0.0;
+                unv[k] = unb_all[k][p];
+            }
+            limited_lsq_vjp<DIM, KM>(unv, uc[p], wl, xs, nb, valid, bnd, bj, gb[p], ub[p], cn,
                                      [&](int64_t g, double v) { b.Ughost[(int64_t)p * NG + (g - NI)] += v; });
 #pragma unroll
             for (int k = 0; k < KM; k++) ce[k][p] += cn[k];
@@ -2224,7 +2267,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys
             }
 #pragma unroll
             for (int p = 0; p < NE; p++) {
-                un[p] = U[p * N + nb[k]];
+                un[p] = unb_all[k][p];
                 unb[p] = 0.0;
             }
             aux_vjp<DIM>(un, ph, anb, unb);
@@ -2251,7 +2294,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys
 // limiter, the least-squares gradient and the mass-fraction gradient.  The slot geometry stays in registers across
 // the species loop.  The species' density terms (Y = rhoY / rho) are added into the euler kernel's density rows.
 template <int DIM, int KM>
-__global__ __launch_bounds__(kBlock) void k_adj_cells_s(MeshView<double> m, Phys ph, const double* __restrict__ U,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_cells_s(MeshView<double> m, Phys ph, const double* __restrict__ U,
                                                         AdjBufs b) {
     constexpr int NE = DIM + 2;
     using AW = AdjFaceRec<DIM>;
@@ -2300,64 +2343,86 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_s(MeshView<double> m, Phys
         }
     }
     const double rho_c = U[c];
-    double rbc = 0.0, rbn[KM];
+    double rbc = 0.0, rbn[KM], rho_n[KM];
 #pragma unroll
-    for (int k = 0; k < KM; k++) rbn[k] = 0.0;
-    for (int sp = 0; sp < ns; sp++) {
-        const int64_t comp = NE + sp;
-        const double ycs = U[comp * N + c];
-        // the faces' species factors
-        double usb = 0.0, gsb[DIM], gyb[DIM];
+    for (int k = 0; k < KM; k++) {
+        rbn[k] = 0.0;
+        rho_n[k] = sdiff ? U[nb[k]] : 1.0;
+    }
+    const double* __restrict__ GL = b.GL;
+    // species in pairs: both species' loads (own and neighbour densities, the record pair) are issued before either
+    // is reversed, so two dependent rounds of memory latency overlap
+    for (int sp0 = 0; sp0 < ns; sp0 += 2) {
+        const int nh = sp0 + 1 < ns ? 2 : 1;
+        double ycs[2], unv[2][KM], gsp[2][KM];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            gsb[d] = 0.0;
-            gyb[d] = 0.0;
+        for (int h = 0; h < 2; h++) {
+            const int64_t comp = NE + sp0 + (h < nh ? h : 0);
+            ycs[h] = U[comp * N + c];
+#pragma unroll
+            for (int k = 0; k < KM; k++) unv[h][k] = U[comp * N + nb[k]];
         }
-        const int q = sp >> 1;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
-            const double2 v = W[wb[k] + 64 * (AW::GS / 2 + q)];
-            if (!valid[k]) continue;
-            const double gsp = (sp & 1) ? v.y : v.x;
-            const double yb = csk[k] * gsp;
-            usb += yb;
-            if (gi[k] >= 0) usb += b.GL[(int64_t)sp * NG + gi[k]];
+            const double2 v = W[wb[k] + 64 * (AW::GS / 2 + (sp0 >> 1))];
+            gsp[0][k] = v.x;
+            gsp[1][k] = v.y;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h >= nh) break;
+            const int sp = sp0 + h;
+            const int64_t comp = NE + sp;
+            // the faces' species factors
+            double usb = 0.0, gsb[DIM], gyb[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                gsb[d] += yb * xs[k][d];
-                gyb[d] += gsp * cnv[k][d];
+                gsb[d] = 0.0;
+                gyb[d] = 0.0;
             }
-        }
-        double cs[KM];  // this species' contributions to the neighbours
-#pragma unroll
-        for (int k = 0; k < KM; k++) cs[k] = 0.0;
-        if (recon)
-            limited_lsq_vjp<DIM, KM>(U, comp * N, ycs, wl, xs, nb, valid, bnd, bj, gsb, usb, cs,
-                                     [&](int64_t g, double v) { b.Ughost[comp * NG + (g - NI)] += v; });
-        if (sdiff) {
-            double ycb = 0.0;
 #pragma unroll
             for (int k = 0; k < KM; k++) {
                 if (!valid[k]) continue;
-                double t = 0.0;
+                const double yb = csk[k] * gsp[h][k];
+                usb += yb;
+                if (gi[k] >= 0) usb += GL[(int64_t)sp * NG + gi[k]];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) t += wa[k][d] * gyb[d];
-                ycb -= t;
-                double yb_n = 0.0, rb_n = 0.0;
-                massfrac_vjp(U[nb[k]], U[comp * N + nb[k]], t, yb_n, rb_n);
-                if (bnd[k]) {
-                    b.Ughost[comp * NG + (nb[k] - NI)] += yb_n;
-                    b.Ughost[(int64_t)kRho * NG + (nb[k] - NI)] += rb_n;
-                } else {
-                    cs[k] += yb_n;
-                    rbn[k] += rb_n;
+                for (int d = 0; d < DIM; d++) {
+                    gsb[d] += yb * xs[k][d];
+                    gyb[d] += gsp[h][k] * cnv[k][d];
                 }
             }
-            massfrac_vjp(rho_c, ycs, ycb, usb, rbc);
-        }
-        b.Uself[comp * NI + c] = usb;
+            double cs[KM];  // this species' contributions to the neighbours
 #pragma unroll
-        for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + comp) * NI + c] = cs[k];
+            for (int k = 0; k < KM; k++) cs[k] = 0.0;
+            if (recon)
+                limited_lsq_vjp<DIM, KM>(unv[h], ycs[h], wl, xs, nb, valid, bnd, bj, gsb, usb, cs,
+                                         [&](int64_t g, double v) { b.Ughost[comp * NG + (g - NI)] += v; });
+            if (sdiff) {
+                double ycb = 0.0;
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
+                    double t = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) t += wa[k][d] * gyb[d];
+                    ycb -= t;
+                    double yb_n = 0.0, rb_n = 0.0;
+                    massfrac_vjp(rho_n[k], unv[h][k], t, yb_n, rb_n);
+                    if (bnd[k]) {
+                        b.Ughost[comp * NG + (nb[k] - NI)] += yb_n;
+                        b.Ughost[(int64_t)kRho * NG + (nb[k] - NI)] += rb_n;
+                    } else {
+                        cs[k] += yb_n;
+                        rbn[k] += rb_n;
+                    }
+                }
+                massfrac_vjp(rho_c, ycs[h], ycb, usb, rbc);
+            }
+            b.Uself[comp * NI + c] = usb;
+#pragma unroll
+            for (int k = 0; k < KM; k++) b.contrib[((int64_t)k * nc + comp) * NI + c] = cs[k];
+        }
     }
     if (sdiff) {
         b.Uself[(int64_t)kRho * NI + c] += rbc;
@@ -2765,26 +2830,6 @@ FusedKernel pick_fused(bool recon, bool visc, int lim, int K) {
     return nullptr;
 }
 
-// FP64 gradient pass from the face offsets (full-rank stencils, no limiter or Barth-Jespersen).
-template <int DIM, int KM>
-GradKernel pick_gradients_neq_k(bool recon, bool visc, int lim) {
-    if (lim == kLimPetsc) return nullptr;
-    if (recon) {
-        if (visc)
-            return lim == kLimBJ ? k_gradients_neq<DIM, true, true, kLimBJ, KM> : k_gradients_neq<DIM, true, true, kLimNone, KM>;
-        return lim == kLimBJ ? k_gradients_neq<DIM, true, false, kLimBJ, KM> : k_gradients_neq<DIM, true, false, kLimNone, KM>;
-    }
-    return visc ? k_gradients_neq<DIM, false, true, kLimNone, KM> : nullptr;
-}
-template <int DIM>
-GradKernel pick_gradients_neq(bool recon, bool visc, int lim, int K) {
-    const int KM = slot_width(K);
-    if (KM == 4) return pick_gradients_neq_k<DIM, 4>(recon, visc, lim);
-    if constexpr (DIM == 3)
-        if (KM == 6) return pick_gradients_neq_k<DIM, 6>(recon, visc, lim);
-    return nullptr;
-}
-
 using MarchKernel = void (*)(MeshView<double>, Phys, const double*, MarchView, double*, int);
 template <int DIM, int KM>
 MarchKernel pick_march_k(bool recon, bool visc, int lim) {
@@ -3075,7 +3120,6 @@ struct abl_fv_solver {
     size_t mr_lds = 0;
     abl::MarchKernel march_kernel = nullptr;
     std::string march_unavailable;  // why the plan was not built, when it failed
-    bool grad_from_offsets = false;  // the FP64 gradient pass solves the normal equations from the face offsets
     // mixed form: FP32 geometry and gradients
     abl::DevBuf<float> fna_f, fdx_f, fwa_f;
     abl::DevBuf<float> sw_f, swa_f, slim_f;  // slot-major gradient geometry (mixed form)
@@ -3217,6 +3261,11 @@ struct abl_fv_solver {
         } else {
             if (face_flux_m.ptr) return;
             face_flux_m.alloc((size_t)2 * nrow * host.F);
+            // the face kernel forms its species perturbations in LDS: 2 ns floats per lane
+            const size_t lds = sizeof(float) * 2 * (size_t)host.ns * abl::kBlock;
+            if (lds > 0 && face_flux_kernel_mixed)
+                HIP_CHECK(hipFuncSetAttribute((const void*)face_flux_kernel_mixed,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
     }
     // Tiles for strategy 3 (FP64 form): the largest tile (256, 128 or 64 cells) whose face fluxes fit the LDS
@@ -3638,16 +3687,6 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         s->build_tiles(L);
         s->build_fused(L);
         s->build_march(L);
-        // FP64 gradient pass from the face offsets where every stencil has full rank (drops the slot-major weight and
-        // limiter geometry reads); the mixed form keeps its FP32 weights
-        if (form == ABL_FV_FORM_FP64 && (s->recon || s->visc) && abl::lsq_stencils_full_rank(L)) {
-            const abl::GradKernel gk = mesh->dim == 3 ? abl::pick_gradients_neq<3>(s->recon, s->visc, lim, L.K)
-                                                      : abl::pick_gradients_neq<2>(s->recon, s->visc, lim, L.K);
-            if (gk) {
-                s->grad_kernel = gk;
-                s->grad_from_offsets = true;
-            }
-        }
         s->strategy = s->auto_strategy();
         abl::resolve_noalloc_scratch();
         if (s->strategy == ABL_FV_STRATEGY_FACE_FLUX) s->ensure_face_flux();
@@ -3775,8 +3814,9 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
                 const int nrow = L.dim + 2 + L.ns;
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-                    hipLaunchKernelGGL(h->face_flux_kernel_mixed, dim3(grid_for(L.F)), block, 0, st, vf, h->ph, U,
-                                       h->recf.ptr, h->gS_f.ptr, h->gY_f.ptr, h->face_flux_m.ptr, nrow);
+                    hipLaunchKernelGGL(h->face_flux_kernel_mixed, dim3(grid_for(L.F)), block,
+                                       sizeof(float) * 2 * (size_t)L.ns * abl::kBlock, st, vf, h->ph, U, h->recf.ptr,
+                                       h->gS_f.ptr, h->gY_f.ptr, h->face_flux_m.ptr, nrow);
                 });
                 h->timed(ABL_FV_PHASE_GATHER, st, [&] {
                     hipLaunchKernelGGL(h->gather_kernel_mixed, grid, block, 0, st, vf, nrow, h->face_flux_m.ptr, R,
