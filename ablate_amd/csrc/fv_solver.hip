@@ -420,101 +420,88 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
         irn[k] = aux ? 1.0 / U[kRho * N + nb[k]] : 0.0;
     }
     bool cancel = false;
-    // species in pairs: both species' loads are issued before either is reduced, so two rounds of neighbour-load
-    // latency overlap
-    for (int sp0 = 0; sp0 < ph.ns; sp0 += 2) {
-        const int nh = sp0 + 1 < ph.ns ? 2 : 1;
-        double ucp[2], unp[2][KM];
+    for (int sp = 0; sp < ph.ns; sp++) {
+        const int64_t comp = NE + sp;
+        const double uc = U[comp * N + c];
+        const double yc = uc * irho_c;
+        double un[KM];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int64_t comp = NE + sp0 + (h < nh ? h : 0);
-            ucp[h] = U[comp * N + c];
+        for (int k = 0; k < KM; k++) un[k] = U[comp * N + nb[k]];
+        double g[DIM], ga[DIM], umin = uc, umax = uc;
 #pragma unroll
-            for (int k = 0; k < KM; k++) unp[h][k] = U[comp * N + nb[k]];
+        for (int d = 0; d < DIM; d++) {
+            g[d] = 0.0;
+            ga[d] = 0.0;
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            if (h >= nh) break;
-            const int sp = sp0 + h;
-            const double uc = ucp[h];
-            const double yc = uc * irho_c;
-            const double* un = unp[h];
-            double g[DIM], ga[DIM], umin = uc, umax = uc;
+        for (int k = 0; k < KM; k++) {
+            if (!valid[k]) continue;
+            if (recon) {
+                umin = fmin(umin, un[k]);
+                umax = fmax(umax, un[k]);
+                if (inner[k]) {
+                    const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
+                }
+            }
+            if (aux) {
+                const double delta = un[k] * irn[k] - yc;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
+            }
+        }
+        if (recon) {
+            double phi = DBL_MAX;
+            if (bj) {
+                double dpos = 0.0, dneg = 0.0;
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
+                    double D = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
+                    dpos = fmax(dpos, D);
+                    dneg = fmin(dneg, D);
+                }
+                phi = 1.0;
+                if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
+                if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
+            } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    if (!valid[k]) continue;
+                    double denom = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
+                    const double flim = 0.5 * (un[k] - uc) / denom;
+                    phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
+                }
+            } else {
+                phi = 1.0;
+            }
+            if (phi == 0)
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
+        }
+        {
+            using V2 = typename Vec2T<G>::type;
+            G v[2 * DIM];
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                g[d] = 0.0;
-                ga[d] = 0.0;
+                v[d] = recon ? G(g[d]) : G(0);
+                v[DIM + d] = aux ? G(ga[d]) : G(0);
             }
+            V2* __restrict__ o = reinterpret_cast<V2*>(gradC);
 #pragma unroll
-            for (int k = 0; k < KM; k++) {
-                if (!valid[k]) continue;
-                if (recon) {
-                    umin = fmin(umin, un[k]);
-                    umax = fmax(umax, un[k]);
-                    if (inner[k]) {
-                        const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
-                    }
-                }
-                if (aux) {
-                    const double delta = un[k] * irn[k] - yc;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
-                }
-            }
-            if (recon) {
-                double phi = DBL_MAX;
-                if (bj) {
-                    double dpos = 0.0, dneg = 0.0;
-#pragma unroll
-                    for (int k = 0; k < KM; k++) {
-                        if (!valid[k]) continue;
-                        double D = 0.0;
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
-                        dpos = fmax(dpos, D);
-                        dneg = fmin(dneg, D);
-                    }
-                    phi = 1.0;
-                    if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
-                    if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
-                } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
-#pragma unroll
-                    for (int k = 0; k < KM; k++) {
-                        if (!valid[k]) continue;
-                        double denom = 0.0;
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
-                        const double flim = 0.5 * (un[k] - uc) / denom;
-                        phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
-                    }
-                } else {
-                    phi = 1.0;
-                }
-                if (phi == 0)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++)
-                        if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
-            }
-            {
-                using V2 = typename Vec2T<G>::type;
-                G v[2 * DIM];
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    v[d] = recon ? G(g[d]) : G(0);
-                    v[DIM + d] = aux ? G(ga[d]) : G(0);
-                }
-                V2* __restrict__ o = reinterpret_cast<V2*>(gradC);
-#pragma unroll
-                for (int j = 0; j < DIM; j++) {
-                    V2 t;
-                    t.x = v[2 * j];
-                    t.y = v[2 * j + 1];
-                    o[((int64_t)sp * DIM + j) * N + c] = t;
-                }
+            for (int j = 0; j < DIM; j++) {
+                V2 t;
+                t.x = v[2 * j];
+                t.y = v[2 * j + 1];
+                o[((int64_t)sp * DIM + j) * N + c] = t;
             }
         }
     }
