@@ -173,12 +173,14 @@ def mixed_leg(fv, synthetic, mesh, mu, k, R64, U, mach, args, fp64_ms):
     err_low = per_field_rel(R2m, R2d)
     del ref
     cells = mesh.n_interior
+    strategy = solver.strategy()
     return {
         "value": round(cells / (ms * 1e-3) / 1e6, 3),
         "unit": "Mcell/s",
         "ms_per_step": round(ms, 4),
         "speedup_vs_fp64": round(fp64_ms / ms, 3),
-        "strategy": solver.strategy(),
+        "strategy": strategy,
+        "traffic": measured_traffic(CONFIGS["c3"], args.n or CONFIGS["c3"]["n"], strategy),
         "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in phases.items() if c},
         "rel_err_vs_fp64_mach0.3": [float(f"{e:.3e}") for e in err],
         "rel_err_vs_fp64_mach0.01": [float(f"{e:.3e}") for e in err_low],
