@@ -1834,28 +1834,50 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     // the face record (read back below instead of the scattered Rbar rows)
     double2* __restrict__ W = reinterpret_cast<double2*>(b.W) + adjw_pair(f, b.nwp);
     constexpr int NFIX = AW::GS / 2;  // pairs holding only fixed words
-    double QL = 0.0, QR = 0.0, QdL = 0.0;
+    // On interior faces the species diffusion flux's density adjoint (the face density 1/2 (rho_L + rho_R) times
+    // n.(grad Y_L + grad Y_R)) is summed here too, while both sides' gradient words are in registers; boundary faces
+    // take it in the ghost pass below.
+    double QL = 0.0, QR = 0.0, QdL = 0.0, rhob = 0.0;
+    const double2* __restrict__ gS2 = reinterpret_cast<const double2*>(gS);
     for (int q = 0; 2 * q < ns; q++) {
-        double gp[2] = {0.0, 0.0};
-#pragma unroll
+        double gp0 = 0.0, gp1 = 0.0;
+#pragma unroll 1
         for (int h = 0; h < 2; h++) {
             const int sp = 2 * q + h;
             if (sp >= ns) break;
             const int64_t comp = NE + sp;
             const double gsp = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
-            gp[h] = gsp;
-            double yl = U[comp * N + L], yr = U[comp * N + Rc];
-            if (recon)
+            if (h) gp1 = gsp;
+            else gp0 = gsp;
+            double yl = U[comp * N + L], yr = U[comp * N + Rc], dot = 0.0;
+            // each side's gradient words in pairs: [0, DIM) limited densityYi gradient, [DIM, 2 DIM) Yi gradient
 #pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    yl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                    if (!ghost) yr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
+            for (int j = 0; j < DIM; j++) {
+                const bool need = (recon && 2 * j < DIM) || (sdiff && 2 * j + 1 >= DIM);
+                if (!need) continue;
+                const double2 a = gS2[((int64_t)sp * DIM + j) * N + L];
+                const double2 c2 = ghost ? make_double2(0.0, 0.0) : gS2[((int64_t)sp * DIM + j) * N + Rc];
+#pragma unroll
+                for (int h2 = 0; h2 < 2; h2++) {
+                    const int w = 2 * j + h2;
+                    const double wl = h2 ? a.y : a.x, wr = h2 ? c2.y : c2.x;
+                    if (w < DIM) {
+                        if (recon) {
+                            yl += wl * x[0][w];
+                            yr += wr * x[1][w];
+                        }
+                    } else {
+                        dot += nh[w - DIM] * (wl + wr);
+                    }
                 }
+            }
             QL += gsp * yl;
             QR += gsp * yr;
             QdL += gsp * (yr - yl);
+            // rho_L and rho_R enter the face density alike
+            if (sdiff && !ghost) rhob += gsp * (-ph.diff * 0.25 * area) * dot;
         }
-        W[64 * (NFIX + q)] = make_double2(gp[0], gp[1]);
+        W[64 * (NFIX + q)] = make_double2(gp0, gp1);
     }
     double ubL[NE], ubR[NE], cL = 0.0, cR = 0.0;
 #pragma unroll
@@ -1938,14 +1960,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         for (int d = 0; d < DIM; d++) aGb[1 + d] += velh[d];
         aux_vjp<DIM>(uR, ph, aGb, ugb);
     }
-    // species: diffusion (face density, ghost gradients)
+    // species on boundary faces: the ghost's advection and diffusion adjoints (ghost gradients, face density)
     const double cdf = sdiff ? -ph.diff * (0.5 * uL[kRho] + 0.5 * uR[kRho]) * 0.5 * area : 0.0;
-    double rhob = 0.0, rgb = 0.0;
-    for (int sp = 0; sp < ns; sp++) {
+    double rgb = 0.0;
+    for (int sp = 0; ghost && sp < ns; sp++) {
         const int64_t comp = NE + sp;
         const double2 gv = W[64 * (NFIX + (sp >> 1))];
         const double gsp = (sp & 1) ? gv.y : gv.x;
-        double ygb = ghost ? cR * gsp : 0.0, yxl = 0.0;
+        double ygb = cR * gsp, yxl = 0.0;
         if (sdiff) {
             const double rl = uL[kRho], rr = uR[kRho];
             const double yl = U[comp * N + L], yr = U[comp * N + Rc];
@@ -1954,26 +1976,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
                 const double GL = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
-                const double GR = ghost ? wg[1][d] * (YL - YR) : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
+                const double GR = wg[1][d] * (YL - YR);
                 dot += nh[d] * (GL + GR);
             }
             const double drho = gsp * (-ph.diff * 0.25 * area) * dot;
-            rhob += drho;  // rho_L and rho_R enter the face density alike
-            if (ghost) {
-                // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (left cell) and Y_G
-                double t = 0.0;
+            rhob += drho;  // rho_L and rho_G enter the face density alike
+            // GY_ghost = w (Y_L - Y_G): the ghost's gradient adjoint goes to Y_L (left cell) and Y_G
+            double t = 0.0;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
-                massfrac_vjp(rl, yl, t, yxl, extra[DIM + 1]);
-                double rg = 0.0;
-                massfrac_vjp(rr, yr, -t, ygb, rg);
-                rgb += rg + drho;  // rho_G in the face density
-            }
+            for (int d = 0; d < DIM; d++) t += wg[1][d] * gsp * cdf * nh[d];
+            massfrac_vjp(rl, yl, t, yxl, extra[DIM + 1]);
+            double rg = 0.0;
+            massfrac_vjp(rr, yr, -t, ygb, rg);
+            rgb += rg + drho;  // rho_G in the face density
         }
-        if (ghost) {
-            b.Ughost[comp * NG + (Rc - NI)] = ygb;
-            b.GL[(int64_t)sp * NG + (Rc - NI)] = yxl;
-        }
+        b.Ughost[comp * NG + (Rc - NI)] = ygb;
+        b.GL[(int64_t)sp * NG + (Rc - NI)] = yxl;
     }
     if (ghost) {
         ugb[kRho] += rgb;
