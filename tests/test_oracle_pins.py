@@ -13,6 +13,8 @@ mistake (an oracle variant with one formula wrong) that the pin rejects:
     face is -(1/V) sum_f +/- F(U(x_f)).n_f of the physical Euler flux.
   - FaceInterpolant (faceInterpolant.cpp:109-249): face values and gradients of linear fields exact at every face
     with two supports (faceInterpolantTests.cpp:224-265: 5^d boxes, 1e-8).
+  - Periodic wrap faces: translation invariance; a periodic box's second-order residual equals that of the same
+    cells inside a box three times larger, where no stencil reaches a wrap face.
 """
 import numpy as np
 import pytest
@@ -377,3 +379,53 @@ def test_average_mass_flux_rhs_on_tiny_periodic_boxes(faces):
     np.testing.assert_allclose(R[0], want, rtol=1e-9, atol=1e-13 * scale)
     if faces == [2, 2]:  # both faces of a direction reach the same neighbour: the mass fluxes cancel
         assert np.abs(R[0]).max() <= 1e-13 * scale
+
+
+# ------------------------------------------------------------------------------------------- periodic wrap faces
+def _wrap_vs_interior(n, dim, drop_shift=False, seed=11):
+    """Second-order Navier-Stokes residual (least squares + Barth-Jespersen, AUSM, viscous and conduction terms) of a
+    periodic n^dim box, and the residual of the same cells in the middle block of a periodic (3n)^dim box holding the
+    state three times over per direction: the middle block's stencils (two cells deep) never reach a wrap face, so
+    a wrap face must behave exactly like the interior face it stands for.  Returns the largest difference per field
+    relative to that field's largest residual."""
+    small = BoxMesh([n] * dim, [0.0] * dim, [1.0] * dim, [True] * dim, ordering="natural").arrays
+    big = BoxMesh([3 * n] * dim, [-1.0] * dim, [2.0] * dim, [True] * dim, ordering="natural").arrays
+    if drop_shift:  # planted mistake: wrap faces without the periodic image of their right cell
+        small = MeshArrays(**{**small.__dict__, "face_shift": np.zeros_like(small.face_shift)})
+    h = 1.0 / n
+    ijk_s = np.floor(small.cell_centroid[:, :dim] / h).astype(int)
+    ijk_b = np.floor((big.cell_centroid[:, :dim] + 1.0) / h).astype(int)
+    rng = np.random.default_rng(seed)
+    shape = (n,) * dim
+    rho = 1.0 + 0.2 * rng.random(shape)
+    vel = rng.normal(0.0, 30.0, (dim,) + shape)
+    p = 1.0e5 * (1.0 + 0.1 * rng.random(shape))
+
+    def state(ijk):
+        at = tuple((ijk % n).T)
+        r, v, pp = rho[at], vel[(slice(None),) + at], p[at]
+        return np.vstack([r, pp / (G - 1.0) + 0.5 * r * np.sum(v * v, axis=0), r * v])
+
+    phys = orc.physics(flux="ausm", recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.026)
+    Rs, _ = orc.rhs(small, phys, state(ijk_s))
+    Rb, _ = orc.rhs(big, phys, state(ijk_b))
+    key = lambda ijk, m: np.ravel_multi_index(tuple(ijk.T), (m,) * dim)
+    where = np.empty((3 * n) ** dim, dtype=np.int64)
+    where[key(ijk_b, 3 * n)] = np.arange(big.n_cells)
+    Rm = Rb[:, where[key(ijk_s + n, 3 * n)]]  # the middle-block cell at the same place as each small-box cell
+    scale = np.max(np.abs(Rm), axis=1)
+    return np.max(np.abs(Rs - Rm), axis=1) / scale
+
+
+@pytest.mark.parametrize("dim,n", [(2, 6), (3, 5)])
+def test_periodic_wrap_faces_match_interior_faces(dim, n):
+    """Periodic images: the geometry of a face across the wrap (face centroid and right-cell centroid in the left
+    cell's frame) must make the wrap cells' gradients, limiters, face states and viscous fluxes those of an interior
+    cell, as DMPlex's localized periodic coordinates give them."""
+    dev = _wrap_vs_interior(n, dim)
+    assert np.all(dev < 1e-10), dev
+
+
+def test_periodic_wrap_planted_mistake_fails():
+    dev = _wrap_vs_interior(6, 2, drop_shift=True)
+    assert np.max(dev) > 1e-2, dev
