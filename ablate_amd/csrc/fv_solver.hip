@@ -2176,34 +2176,57 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells_e(MeshView<double> m, Phys
         load_face_pair<DIM>(m.fdx, f, x);
 #pragma unroll
         for (int d = 0; d < DIM; d++) xs[k][d] = s ? x[1][d] : x[0][d];
-        double w[AW::GS];
+        // the record pairs this cell needs: its side's face-state adjoint, the viscous terms and face velocity, the
+        // face density adjoint, and on a boundary face the right-side words (the left cell's extras)
+        constexpr int NPS = (NE + 1) / 2;                      // pairs covering NE words from either parity
+        constexpr int V0 = AW::TQ / 2, V1 = (AW::VL + DIM - 1) / 2;
         const int64_t wb = adjw_pair(f, b.nwp);
-#pragma unroll
-        for (int q = 0; q < AW::GS / 2; q++) {
-            const double2 v = W[wb + 64 * q];
-            w[2 * q] = v.x;
-            w[2 * q + 1] = v.y;
-        }
-        if (!valid[k]) continue;
         const int o = s ? AW::UR : AW::UL;
+        const int p0 = o / 2;
+        double sw_[2 * NPS], vw[2 * (V1 - V0 + 1)];
+#pragma unroll
+        for (int i = 0; i < NPS; i++) {
+            const double2 v = W[wb + 64 * (p0 + i)];
+            sw_[2 * i] = v.x;
+            sw_[2 * i + 1] = v.y;
+        }
+        if (visc)
+#pragma unroll
+            for (int q = V0; q <= V1; q++) {
+                const double2 v = W[wb + 64 * q];
+                vw[2 * (q - V0)] = v.x;
+                vw[2 * (q - V0) + 1] = v.y;
+            }
+        const double2 rp = W[wb + 64 * (AW::RH / 2)];
+        const double rh = (AW::RH & 1) ? rp.y : rp.x;
+        if (!valid[k]) continue;
 #pragma unroll
         for (int p = 0; p < NE; p++) {
-            ub[p] += w[o + p];
+            const double wp = sw_[o + p - 2 * p0];
+            ub[p] += wp;
             if (recon)
 #pragma unroll
-                for (int d = 0; d < DIM; d++) gb[p][d] += w[o + p] * xs[k][d];
+                for (int d = 0; d < DIM; d++) gb[p][d] += wp * xs[k][d];
         }
         if (visc) {
 #pragma unroll
-            for (int q = 0; q < NT; q++) tqb[q] += w[AW::TQ + q];
+            for (int q = 0; q < NT; q++) tqb[q] += vw[AW::TQ + q - 2 * V0];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) velb[d] += w[AW::VL + d];
+            for (int d = 0; d < DIM; d++) velb[d] += vw[AW::VL + d - 2 * V0];
         }
-        rhob += w[AW::RH];
+        rhob += rh;
         if (bnd[k]) {  // boundary face: the left cell's extras from the ghost's viscous gradient
+            constexpr int R0 = AW::UR / 2;
+            double rw[2 * NPS];
 #pragma unroll
-            for (int q = 0; q <= DIM; q++) auxb[q] += w[AW::UR + q];
-            rhob += w[AW::UR + DIM + 1];
+            for (int i = 0; i < NPS; i++) {
+                const double2 v = W[wb + 64 * (R0 + i)];
+                rw[2 * i] = v.x;
+                rw[2 * i + 1] = v.y;
+            }
+#pragma unroll
+            for (int q = 0; q <= DIM; q++) auxb[q] += rw[AW::UR + q - 2 * R0];
+            rhob += rw[AW::UR + DIM + 1 - 2 * R0];
         }
     }
     double uc[NE];
