@@ -13,10 +13,11 @@ from .fv import (
     PerfectGas,
     Rusanov,
     SpeciesTransport,
+    Sutherland,
 )
 from .mesh import BoxMesh, TetBoxMesh
 
 __all__ = [
     "Ausm", "AverageFlux", "BoxMesh", "Constant", "FiniteVolumeSolver", "InflowGhost", "NavierStokesTransport",
-    "OffFlux", "OutflowGhost", "PerfectGas", "Rusanov", "SpeciesTransport", "TetBoxMesh",
+    "OffFlux", "OutflowGhost", "PerfectGas", "Rusanov", "SpeciesTransport", "Sutherland", "TetBoxMesh",
 ]

@@ -57,6 +57,8 @@ class PhysicsDesc(ctypes.Structure):
         ("cfl", c_double),
         ("ausmpup_minf", c_double),
         ("compact_species", c_int32),
+        ("transport", c_int32),
+        ("transport_disabled", c_int32),
     ]
 
 

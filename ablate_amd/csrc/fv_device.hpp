@@ -24,7 +24,19 @@ struct Phys {
     int flux, recon, limiter, ns;
     int visc;   // Navier-Stokes diffusion flux active (k or mu nonzero, or any transport active)
     int sdiff;  // species diffusion active
+    // Sutherland transport: the cell records hold the strain-rate and temperature-gradient terms with mu = k = 1
+    // (mu, k, diff above are then 1 or 0 per enabled property) and every face scales them by mu(T_face)
+    int suth;
+    double cp;  // specific heat at constant pressure (PerfectGas: gamma R / (gamma - 1))
 };
+
+// Sutherland's law with the reference's constants (eos/transport/sutherland.hpp: mu_0 = 1.716e-5, T_0 = 273,
+// S = 111, Pr = Sc = 0.707; sutherland.cpp:17-19, 39-40): mu(T) = mu_0 (T / T_0)^(3/2) (T_0 + S) / (T + S).
+constexpr double kSuthMu0 = 1.716e-5, kSuthT0 = 273.0, kSuthS = 111.0, kSuthPr = 0.707, kSuthSc = 0.707;
+__device__ __host__ inline double sutherland_mu(double T) {
+    const double r = T / kSuthT0;
+    return kSuthMu0 * sqrt(r) * r * (kSuthT0 + kSuthS) / (T + kSuthS);
+}
 
 // Geometry view (G = storage type of the face geometry: double for FP64, float for the mixed form).
 template <typename G>

@@ -214,7 +214,12 @@ void build_layout(const abl_fv_mesh_desc& m, const abl_fv_physics_desc& p, Layou
 
     // ---- cell least-squares weights (non-boundary faces only) and aux (face-stencil) weights (all faces)
     const bool recon = p.reconstruction == ABL_FV_RECON_LEASTSQUARES && p.flux_calculator != ABL_FV_FLUX_NONE;
-    const bool diffusion = p.k != 0.0 || p.mu != 0.0 || p.diff != 0.0;
+    const bool diffusion = p.transport == ABL_FV_TRANSPORT_SUTHERLAND
+                               ? (p.transport_disabled & (ABL_FV_TRANSPORT_CONDUCTIVITY | ABL_FV_TRANSPORT_VISCOSITY |
+                                                          ABL_FV_TRANSPORT_DIFFUSIVITY)) !=
+                                     (ABL_FV_TRANSPORT_CONDUCTIVITY | ABL_FV_TRANSPORT_VISCOSITY |
+                                      ABL_FV_TRANSPORT_DIFFUSIVITY)
+                               : (p.k != 0.0 || p.mu != 0.0 || p.diff != 0.0);
     const bool has_bnd = !L.bfaces.empty();
     L.has_boundary = has_bnd;
     if (recon || diffusion) {

@@ -649,9 +649,23 @@ __device__ inline void face_flux(const Phys& ph, const double* nhat, double area
         }
 #pragma unroll
         for (int q = 0; q < ViscTerms<DIM>::NT; q++) tq[q] = 0.5 * L.tq[q] + 0.5 * R.tq[q];
+        const double rho_f = 0.5 * L.rho + 0.5 * R.rho;
+        double diff = ph.diff;
+        if (ph.suth) {
+            // transport properties at the face temperature (the face-interpolated aux T of DiffusionFlux,
+            // navierStokesTransport.cpp:466-468): tau scales with mu, the heat flux with k = mu cp / Pr, and the
+            // species flux with D = mu / (rho Sc) (sutherland.cpp:9-64)
+            const double T_f = (0.5 * L.T + 0.5 * R.T) * ph.icv;
+            const double mu_f = sutherland_mu(T_f);
+#pragma unroll
+            for (int q = 0; q < ViscTerms<DIM>::NTAU; q++) tq[q] *= mu_f;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) tq[ViscTerms<DIM>::NTAU + d] *= mu_f * ph.cp / kSuthPr;
+            diff *= mu_f / rho_f / kSuthSc;
+        }
         viscous_flux_add<double, DIM>(normal, vel, tq, flux);
         // species diffusion: -D rho_face n.(1/2 (gradY_L + gradY_R)), rho_face = 1/2 (rho_L + rho_R)
-        cd = -ph.diff * (0.5 * L.rho + 0.5 * R.rho) * 0.5 * area;
+        cd = -diff * rho_f * 0.5 * area;
     }
 }
 
@@ -3520,6 +3534,14 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         return fail(nullptr, "abl_fv_create: unknown limiter");
     if (form == ABL_FV_FORM_MIXED && p.flux_calculator == ABL_FV_FLUX_AUSMPUP)
         return fail(nullptr, "abl_fv_create: the mixed form has no AusmpUp flux calculator (create the FP64 form)");
+    if (p.transport != ABL_FV_TRANSPORT_CONSTANT && p.transport != ABL_FV_TRANSPORT_SUTHERLAND)
+        return fail(nullptr, "abl_fv_create: unknown transport model");
+    const bool suth = p.transport == ABL_FV_TRANSPORT_SUTHERLAND;
+    if (suth && (p.transport_disabled & ~(ABL_FV_TRANSPORT_CONDUCTIVITY | ABL_FV_TRANSPORT_VISCOSITY |
+                                         ABL_FV_TRANSPORT_DIFFUSIVITY)) != 0)
+        return fail(nullptr, "abl_fv_create: transport_disabled holds bits other than ABL_FV_TRANSPORT_*");
+    if (suth && form == ABL_FV_FORM_MIXED)
+        return fail(nullptr, "abl_fv_create: the mixed form has no Sutherland transport (create the FP64 form)");
     std::unique_ptr<abl_fv_solver> s(new abl_fv_solver());
     try {
         s->phys = p;
@@ -3527,15 +3549,23 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
         abl::build_layout(*mesh, p, s->host);
         abl::Layout& L = s->host;
         s->recon = p.reconstruction == ABL_FV_RECON_LEASTSQUARES && p.flux_calculator != ABL_FV_FLUX_NONE;
-        s->visc = p.k != 0.0 || p.mu != 0.0 || p.diff != 0.0;
         abl::Phys& ph = s->ph;
         ph.gamma = p.gamma;
         ph.R = p.Rgas;
         ph.cv = p.Rgas / (p.gamma - 1.0);
         ph.icv = 1.0 / ph.cv;
-        ph.mu = p.mu;
-        ph.k = p.k;
-        ph.diff = p.diff;
+        ph.cp = p.gamma * p.Rgas / (p.gamma - 1.0);
+        ph.suth = suth;
+        if (suth) {  // record scales 1 or 0 per enabled property; the faces apply mu(T_face)
+            ph.mu = (p.transport_disabled & ABL_FV_TRANSPORT_VISCOSITY) ? 0.0 : 1.0;
+            ph.k = (p.transport_disabled & ABL_FV_TRANSPORT_CONDUCTIVITY) ? 0.0 : 1.0;
+            ph.diff = (p.transport_disabled & ABL_FV_TRANSPORT_DIFFUSIVITY) ? 0.0 : 1.0;
+        } else {
+            ph.mu = p.mu;
+            ph.k = p.k;
+            ph.diff = p.diff;
+        }
+        s->visc = ph.k != 0.0 || ph.mu != 0.0 || ph.diff != 0.0;
         ph.max_lim_grad = p.max_lim_grad;
         ph.flux = p.flux_calculator;
         ph.minf = p.ausmpup_minf;
@@ -3557,8 +3587,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->face_flux_kernel = abl::pick_face_flux<3>(s->recon, s->visc, species);
             s->tile_kernel = abl::pick_tile<3>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<3>(species, L.K);
-            if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<3>(s->recon, s->visc, lim, L.K);
-            if (form == ABL_FV_FORM_FP64 && !species) s->march_kernel = abl::pick_march<3>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species && !suth) s->fused_kernel = abl::pick_fused<3>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species && !suth) s->march_kernel = abl::pick_march<3>(s->recon, s->visc, lim, L.K);
         } else {
             s->species_kernel = abl::pick_species_gradients<double, 2>(L.K);
             s->species_kernel_f = abl::pick_species_gradients<float, 2>(L.K);
@@ -3567,8 +3597,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             s->face_flux_kernel = abl::pick_face_flux<2>(s->recon, s->visc, species);
             s->tile_kernel = abl::pick_tile<2>(s->recon, s->visc, species, L.K);
             s->gather_kernel = abl::pick_gather<2>(species, L.K);
-            if (form == ABL_FV_FORM_FP64 && !species) s->fused_kernel = abl::pick_fused<2>(s->recon, s->visc, lim, L.K);
-            if (form == ABL_FV_FORM_FP64 && !species) s->march_kernel = abl::pick_march<2>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species && !suth) s->fused_kernel = abl::pick_fused<2>(s->recon, s->visc, lim, L.K);
+            if (form == ABL_FV_FORM_FP64 && !species && !suth) s->march_kernel = abl::pick_march<2>(s->recon, s->visc, lim, L.K);
         }
 
         s->ell.upload(L.ell);
@@ -3924,7 +3954,8 @@ extern "C" int abl_fv_set_strategy(abl_fv_handle h, int32_t strategy) {
         return fail(h, "abl_fv_set_strategy: unknown strategy");
     if (strategy == ABL_FV_STRATEGY_FUSED && h->fu_n == 0)
         return fail(h, "abl_fv_set_strategy: the fused strategy is unavailable for this handle (it needs the FP64 form "
-                       "without species, no limiter or Barth-Jespersen, 2-D cells or tetrahedra / hexahedra, prisms, "
+                       "without species, the Constant transport model, no limiter or Barth-Jespersen, 2-D cells or "
+                       "tetrahedra / hexahedra, prisms, "
                        "pyramids, well-conditioned least-squares stencils and a tile that fits the LDS)" +
                            (h->fused_unavailable.empty() ? std::string() : ": " + h->fused_unavailable));
     if (strategy == ABL_FV_STRATEGY_MARCH && h->mr_nseq == 0)
@@ -4095,6 +4126,8 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
         return fail(h, "abl_fv_rhs_vjp: supported for the FP64 form and the Ausm / Rusanov / Off flux calculators");
     if (h->recon && h->phys.limiter != ABL_FV_LIMITER_NONE && h->phys.limiter != ABL_FV_LIMITER_BARTH_JESPERSEN)
         return fail(h, "abl_fv_rhs_vjp: with reconstruction the adjoint supports the Barth-Jespersen limiter or none");
+    if (h->ph.suth && h->visc)
+        return fail(h, "abl_fv_rhs_vjp: the adjoint supports the Constant transport model (Sutherland is forward only)");
     if (h->recon || h->visc) return vjp_second_order(h, U, Rbar, Ubar, theta_bar, (hipStream_t)stream);
     try {
         if (theta_bar && h->n_bfaces && !h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);

@@ -3,6 +3,7 @@
 Class and parameter names follow the reference's YAML surface:
   ablate::eos::PerfectGas{parameters{gamma, Rgas}, species}                 (src/eos/perfectGas.cpp:4-7,328-329)
   ablate::eos::transport::Constant{k, mu, diff}                             (src/eos/transport/constant.cpp:74-75)
+  ablate::eos::transport::Sutherland{eos, enabledProperties}                (src/eos/transport/sutherland.cpp:120-122)
   ablate::finiteVolume::fluxCalculator::{Ausm, AverageFlux, OffFlux, AusmpUp{mInf, pgs}}   (fluxCalculator/*.cpp)
   ablate::finiteVolume::processes::PressureGradientScaling{eos, alphaInit, domainLength, maxAlphaAllowed,
                                                            maxDeltaPressureFac}  (processes/pressureGradientScaling.cpp)
@@ -48,6 +49,28 @@ class Constant:
     diff: float = 0.0
 
 
+@dataclass
+class Sutherland:
+    """transport::Sutherland (sutherland.cpp:9-122): mu(T) = 1.716e-5 (T / 273)^(3/2) 384 / (T + 111),
+    k = mu cp / 0.707, D = mu / (rho 0.707), evaluated at each face from its temperature and density.
+    ``enabledProperties``: any of "Conductivity", "Viscosity", "Diffusivity"; empty = all (the others are zero).
+    FP64 form, forward only."""
+
+    eos: PerfectGas | None = None
+    enabledProperties: list = field(default_factory=list)
+
+    BITS = {"conductivity": 1, "viscosity": 2, "diffusivity": 4}
+
+    def disabled_bits(self) -> int:
+        if not self.enabledProperties:
+            return 0
+        names = {str(x).lower() for x in self.enabledProperties}
+        unknown = names - set(self.BITS)
+        if unknown:
+            raise ValueError(f"unknown transport properties {sorted(unknown)}")
+        return sum(bit for name, bit in self.BITS.items() if name not in names)
+
+
 # ----------------------------------------------------------------------------------------------- flux calculators
 class FluxCalculator:
     kind = "none"
@@ -90,7 +113,7 @@ class NavierStokesTransport:
     parameters: dict = field(default_factory=dict)
     eos: PerfectGas = field(default_factory=PerfectGas)
     fluxCalculator: FluxCalculator | None = None
-    transport: Constant | None = None
+    transport: Constant | Sutherland | None = None
     pgs: object = None
 
 
@@ -148,7 +171,7 @@ class CompactCompressibleNSSpeciesTransport:
     parameters: dict = field(default_factory=dict)
     eos: PerfectGas = field(default_factory=PerfectGas)
     fluxCalculator: FluxCalculator | None = None
-    transport: Constant | None = None
+    transport: Constant | Sutherland | None = None
     pgs: object = None
 
 
@@ -156,7 +179,7 @@ class CompactCompressibleNSSpeciesTransport:
 class SpeciesTransport:
     eos: PerfectGas = field(default_factory=PerfectGas)
     fluxCalculator: FluxCalculator | None = None
-    transport: Constant | None = None
+    transport: Constant | Sutherland | None = None
     parameters: dict = field(default_factory=dict)
 
 
@@ -231,6 +254,9 @@ class FiniteVolumeSolver:
         elif n_species and not compact:
             raise ValueError("species are declared in the EOS but no SpeciesTransport process was given")
         tr = transport or Constant()
+        suth = isinstance(tr, Sutherland)
+        if suth and form == "mixed":
+            raise ValueError("the mixed form has no Sutherland transport: use form='fp64'")
         recon = options.get("petscfv_type", "upwind")
         limiter = options.get("petsclimiter_type", "none").replace("-", "").replace("_", "")
         phys = _lib.PhysicsDesc(
@@ -241,12 +267,14 @@ class FiniteVolumeSolver:
             reconstruction=_lib.RECON[recon],
             limiter=_lib.LIMITER[limiter],
             max_lim_grad=float(maxLimGrad),
-            k=float(tr.k),
-            mu=float(tr.mu),
-            diff=float(tr.diff),
+            k=0.0 if suth else float(tr.k),
+            mu=0.0 if suth else float(tr.mu),
+            diff=0.0 if suth else float(tr.diff),
             cfl=float(nst.parameters.get("cfl", 0.5)),
             ausmpup_minf=float(getattr(nst.fluxCalculator, "mInf", 0.0)),
             compact_species=int(compact),
+            transport=int(suth),
+            transport_disabled=tr.disabled_bits() if suth else 0,
         )
         if form == "mixed" and flux == "ausmpup":
             raise ValueError("the mixed form has no AusmpUp flux calculator: use form='fp64'")

@@ -4,8 +4,9 @@
  * The library replaces the per-face CPU loops behind
  *   ablate::finiteVolume::FiniteVolumeSolver::ComputeRHSFunction(PetscReal time, Vec locX, Vec locF)
  *   (src/finiteVolume/finiteVolumeSolver.cpp:233-291, declared finiteVolumeSolver.hpp:106)
- * for the process set NavierStokesTransport (+ SpeciesTransport) with a PerfectGas EOS, a Constant transport
- * model and the Ausm / AverageFlux / OffFlux flux calculators (plus the build-defined Rusanov flux).
+ * for the process set NavierStokesTransport (+ SpeciesTransport) with a PerfectGas EOS, a Constant or Sutherland
+ * transport model and the Ausm / AverageFlux / OffFlux / AusmpUp flux calculators (plus the build-defined Rusanov
+ * flux).
  *
  * Conventions (mirroring PETSc / ABLATE):
  *   - Every entry point returns an int status: 0 = success (PetscErrorCode semantics), nonzero = error;
@@ -69,6 +70,15 @@ enum {
     ABL_FV_LIMITER_BARTH_JESPERSEN = 8
 };
 
+/* Transport models (eos/transport): Constant{k, mu, diff} (constant.cpp:3-71) or Sutherland (sutherland.cpp:9-122):
+ * mu(T) = 1.716e-5 (T / 273)^(3/2) (273 + 111) / (T + 111), k = mu cp / 0.707, D = mu / (rho 0.707), evaluated at
+ * each face from the face temperature and density as NavierStokesTransport::DiffusionFlux and
+ * SpeciesTransport::DiffusionSpeciesFlux evaluate them (navierStokesTransport.cpp:454-470,
+ * speciesTransport.cpp:196-228).  Sutherland runs in the FP64 form, forward only. */
+enum { ABL_FV_TRANSPORT_CONSTANT = 0, ABL_FV_TRANSPORT_SUTHERLAND = 1 };
+/* Sutherland{enabledProperties}: properties left out return zero (sutherland.cpp:88-91). */
+enum { ABL_FV_TRANSPORT_CONDUCTIVITY = 1, ABL_FV_TRANSPORT_VISCOSITY = 2, ABL_FV_TRANSPORT_DIFFUSIVITY = 4 };
+
 /* Boundary (ghost-cell) conditions.  Every slot that has boundary faces needs one: abl_fv_rhs, abl_fv_rhs_vjp and
  * abl_fv_ts_euler fail while such a slot is still ABL_FV_BC_NONE (the reference would leave those ghost cells
  * unset). */
@@ -123,6 +133,10 @@ typedef struct abl_fv_physics_desc {
                                 instead of NavierStokesTransport + SpeciesTransport: NA flux-calculator directions
                                 advect the averaged species, the CFL step divides a by the PGS alpha; with PerfectGas
                                 (species enthalpies 0) the species-enthalpy diffusion term is zero */
+    int32_t transport;          /* ABL_FV_TRANSPORT_CONSTANT (k, mu, diff above) or ABL_FV_TRANSPORT_SUTHERLAND (k,
+                                   mu, diff ignored) */
+    int32_t transport_disabled; /* Sutherland: ABL_FV_TRANSPORT_* bits of the properties left out of
+                                   enabledProperties; 0 = all enabled (the reference's default) */
 } abl_fv_physics_desc;
 
 /* One boundary-condition slot. */
@@ -317,7 +331,8 @@ enum {
     ABL_FV_STRATEGY_FUSED = 4,     /* one workgroup per tile computes the gradients of its cells and of its halo into
                                       LDS (least-squares normal equations solved per cell, no stored weights), then
                                       the tile's face fluxes and the gather: no cell record or face flux in HBM.
-                                      FP64 form without species, no limiter or Barth-Jespersen, full-rank stencils */
+                                      FP64 form without species, Constant transport, no limiter or Barth-Jespersen,
+                                      full-rank stencils */
     ABL_FV_STRATEGY_MARCH = 5      /* one workgroup per sequence of 64-cell tiles that share faces (the z-layers of a
                                       column in ABL_FV_ORDER_COLUMN): each step computes the records of the next tile
                                       into LDS, evaluates the current tile's faces and gathers it, so faces between

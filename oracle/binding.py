@@ -55,6 +55,8 @@ class OrcPhysics(ctypes.Structure):
         ("ausmpup_minf", c_double),
         ("pgs_alpha", c_double),
         ("compact", c_int),
+        ("transport", c_int),
+        ("transport_disabled", c_int),
     ]
 
 
@@ -68,7 +70,7 @@ _lib = None
 # Planted-mistake variants of the oracle (ORC_PLANT in fv_oracle.c): each changes one formula so that a test can
 # show that its pin rejects the mistake.
 PLANTS = {"project_sign": 1, "rusanov_min_speed": 2, "bj_neighbor": 3, "minmod_as_mc": 4, "face_value_left": 5,
-          "conduction_sign": 6}
+          "conduction_sign": 6, "sutherland_cell_t": 7}
 
 
 def build() -> Path:
@@ -159,6 +161,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_rusanov_flux.restype = None
     L.orc_face_interpolate.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), _dp, _dp, _dp, _dp, _dp]
     L.orc_face_interpolate.restype = c_int
+    L.orc_sutherland.argtypes = [d, d, d, c_int, _dp]
+    L.orc_sutherland.restype = None
 
 
 def _ptr(a):
@@ -189,11 +193,27 @@ def _mesh_struct(m):
     return s, keep
 
 
+TRANSPORT = {"constant": 0, "sutherland": 1}
+TRANSPORT_BITS = {"conductivity": 1, "viscosity": 2, "diffusivity": 4}
+
+
 def physics(gamma=1.4, Rgas=287.0, n_species=0, flux="ausm", recon="upwind", limiter="none", max_lim_grad=1e10,
-            k=0.0, mu=0.0, diff=0.0, ausmpup_minf=0.0, pgs_alpha=1.0, compact=False) -> OrcPhysics:
+            k=0.0, mu=0.0, diff=0.0, ausmpup_minf=0.0, pgs_alpha=1.0, compact=False, transport="constant",
+            disabled=()) -> OrcPhysics:
+    """``transport`` "constant" (k, mu, diff) or "sutherland" (k, mu, diff ignored; ``disabled`` names the
+    properties left out of enabledProperties)."""
     return OrcPhysics(gamma=gamma, Rgas=Rgas, n_species=n_species, flux=FLUX[flux], recon=RECON[recon],
                       limiter=LIMITER[limiter], max_lim_grad=max_lim_grad, k=k, mu=mu, diff=diff,
-                      ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha, compact=int(bool(compact)))
+                      ausmpup_minf=ausmpup_minf, pgs_alpha=pgs_alpha, compact=int(bool(compact)),
+                      transport=TRANSPORT[transport],
+                      transport_disabled=sum(TRANSPORT_BITS[x] for x in disabled))
+
+
+def sutherland(T: float, rho: float, cp: float, disabled=()) -> tuple[float, float, float]:
+    """transport::Sutherland: (conductivity, viscosity, diffusivity) at temperature T, density rho and cp."""
+    out = np.zeros(3)
+    lib().orc_sutherland(T, rho, cp, sum(TRANSPORT_BITS[x] for x in disabled), _ptr(out))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def _bc_array(bcs):

@@ -44,7 +44,8 @@ enum {
     PLANT_BJ_NEIGHBOR = 3,        /* Barth-Jespersen projected to the neighbour centroid instead of the face */
     PLANT_MINMOD_AS_MC = 4,       /* the minmod limiter evaluates the MC formula */
     PLANT_FACE_VALUE_LEFT = 5,    /* face-interpolant value and gradient from the left cell's stencil only */
-    PLANT_CONDUCTION_SIGN = 6     /* heat flux with +k grad T */
+    PLANT_CONDUCTION_SIGN = 6,    /* heat flux with +k grad T */
+    PLANT_SUTHERLAND_CELL_T = 7   /* Sutherland properties at the left cell's temperature instead of the face's */
 };
 /* PETSc's min/max macros: the comparison order decides NaN propagation, keep it. */
 static inline double pmin(double a, double b) { return a < b ? a : b; }
@@ -216,6 +217,16 @@ double orc_pg_speed_of_sound_T(const double* euler, double T, double gamma, doub
 double orc_pg_sensible_enthalpy_T(double T, double gamma, double R) { /* :121-127 */
     double cp = gamma * R / (gamma - 1.0);
     return T * cp;
+}
+
+/* transport::Sutherland, sutherland.cpp:9-64 with the constants of sutherland.hpp:23-27 (Pr, mu_o, T_o, S_o, Sc):
+ * every property starts from the same viscosity mu(T) = mu_o sqrt(T/T_o) (T/T_o) (T_o + S_o) / (T + S_o). */
+void orc_sutherland(double T, double rho, double cp, int disabled, double* out) {
+    const double pr = 0.707, muo = 1.716e-5, to = 273.0, so = 111.0, sc = 0.707;
+    double mu = muo * sqrt(T / to) * (T / to) * (to + so) / (T + so);
+    out[0] = (disabled & ORC_TRANSPORT_CONDUCTIVITY) ? 0.0 : mu * cp / pr;
+    out[1] = (disabled & ORC_TRANSPORT_VISCOSITY) ? 0.0 : mu;
+    out[2] = (disabled & ORC_TRANSPORT_DIFFUSIVITY) ? 0.0 : mu / rho / sc;
 }
 
 /* ============================================================== physics plug-ins */
@@ -895,7 +906,9 @@ int orc_create(const orc_mesh* mesh, const orc_physics* ph, orc_ctx** out) {
             return rc;
         }
     }
-    int transport_active = (ph->k != 0.0) || (ph->mu != 0.0) || (ph->diff != 0.0);
+    int transport_active = ph->transport == ORC_TRANSPORT_SUTHERLAND
+                               ? (ph->transport_disabled & 7) != 7
+                               : ((ph->k != 0.0) || (ph->mu != 0.0) || (ph->diff != 0.0));
     if (transport_active) {
         stencil_t* st = (stencil_t*)malloc(sizeof(stencil_t));
         int64_t cap = 16 * t->F + 16;
@@ -1028,9 +1041,20 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
             double vL = mesh->cell_volume[cl], vR = mesh->cell_volume[cr];
             double* fLp = (cl < t.NI) ? &locF[(size_t)cl * nc] : NULL;
             double* fRp = (cr < t.NI) ? &locF[(size_t)cr * nc] : NULL;
-            /* NavierStokesTransport::DiffusionFlux: aux = {T, vel}, gradAux = {dT, dvel} */
+            /* NavierStokesTransport::DiffusionFlux: aux = {T, vel}, gradAux = {dT, dvel}; mu and k from the
+             * transport model's temperature functions at the face temperature (navierStokesTransport.cpp:464-468) */
+            double mu = ph->mu, kc = ph->k, diff = ph->diff;
+            if (ph->transport == ORC_TRANSPORT_SUTHERLAND) {
+                double tr[3];
+                const double Tf = ORC_PLANT == PLANT_SUTHERLAND_CELL_T ? aux[(size_t)cl * t.naux] : faceAux[0];
+                orc_sutherland(Tf, faceSol[RHO], ph->gamma * ph->Rgas / (ph->gamma - 1.0), ph->transport_disabled,
+                               tr);
+                kc = tr[0];
+                mu = tr[1];
+                diff = tr[2]; /* SpeciesTransport's diffFunction: density from the face field (speciesTransport.cpp:210-213) */
+            }
             for (int k = 0; k < nc; k++) flux[k] = 0.0;
-            ns_diffusion_flux(dim, normal, ph->mu, ph->k, faceAux + 1, faceAuxGrad + 0, faceAuxGrad + dim, flux);
+            ns_diffusion_flux(dim, normal, mu, kc, faceAux + 1, faceAuxGrad + 0, faceAuxGrad + dim, flux);
             for (int d = 0; d < dim + 2; ++d) {
                 if (fLp) fLp[d] -= flux[d] / vL;
                 if (fRp) fRp[d] += flux[d] / vR;
@@ -1044,7 +1068,7 @@ int orc_rhs_ctx(orc_ctx* ctx, const orc_bc* bcs, int n_bcs, double time, double*
                     flux[sp] = 0;
                     for (int d = 0; d < dim; ++d) {
                         const int offset = (1 + dim + sp) * dim + d;
-                        double speciesFlux = -normal[d] * density * ph->diff * faceAuxGrad[offset];
+                        double speciesFlux = -normal[d] * density * diff * faceAuxGrad[offset];
                         flux[sp] += speciesFlux;
                     }
                 }
@@ -1062,6 +1086,7 @@ int orc_face_interpolate(const orc_mesh* mesh, const orc_physics* phys, const do
                          double* faceSolGrad, double* faceAux, double* faceAuxGrad) {
     orc_physics ph = *phys;
     if (ph.k == 0.0 && ph.mu == 0.0 && ph.diff == 0.0) ph.mu = 1.0; /* the stencils are built with transport on */
+    ph.transport = ORC_TRANSPORT_CONSTANT;
     orc_ctx* ctx = NULL;
     int rc = orc_create(mesh, &ph, &ctx);
     if (rc) return rc;

@@ -30,6 +30,8 @@ enum {
 /* ORC_BC_GHOST: ghost values supplied by the caller in locX (a Ghost update function evaluated before the RHS,
  * ghost.cpp:17-28, timeStepper.cpp:347-349): left untouched. */
 enum { ORC_BC_NONE = 0, ORC_BC_INFLOW = 1, ORC_BC_OUTFLOW = 2, ORC_BC_GHOST = 3 };
+enum { ORC_TRANSPORT_CONSTANT = 0, ORC_TRANSPORT_SUTHERLAND = 1 };
+enum { ORC_TRANSPORT_CONDUCTIVITY = 1, ORC_TRANSPORT_VISCOSITY = 2, ORC_TRANSPORT_DIFFUSIVITY = 4 };
 
 typedef struct {
     int dim;
@@ -54,6 +56,8 @@ typedef struct {
     double ausmpup_minf; /* AusmpUp mInf (0 = none) */
     double pgs_alpha;    /* PressureGradientScaling alpha seen by AusmpUp (<= 0: no PGS, alpha 1) */
     int compact;         /* CompactCompressibleNSSpeciesTransport instead of NavierStokes + SpeciesTransport */
+    int transport;       /* ORC_TRANSPORT_CONSTANT (k, mu, diff) or ORC_TRANSPORT_SUTHERLAND */
+    int transport_disabled; /* Sutherland: ORC_TRANSPORT_* bits of the properties left out (0 = all enabled) */
 } orc_physics;
 
 typedef struct {
@@ -83,6 +87,9 @@ double orc_pg_speed_of_sound(const double* euler, int dim, double gamma, double 
 double orc_pg_speed_of_sound_T(const double* euler, double T, double gamma, double R);
 double orc_pg_pressure_T(const double* euler, double T, double R);
 double orc_pg_sensible_enthalpy_T(double T, double gamma, double R);
+/* transport::Sutherland at temperature T, density rho and cp: out = {conductivity, viscosity, diffusivity}
+ * (sutherland.cpp:9-64; a property left out of enabledProperties is zero, :88-91). */
+void orc_sutherland(double T, double rho, double cp, int disabled, double* out);
 double orc_pg_cv(double gamma, double R);
 double orc_pg_cp(double gamma, double R);
 

@@ -31,6 +31,8 @@ class Case:
     minf: float = 0.0  # AusmpUp mInf
     pgs_alpha: float | None = None  # AusmpUp with a PressureGradientScaling held at this alpha
     compact: bool = False  # CompactCompressibleNSSpeciesTransport instead of NavierStokes + SpeciesTransport
+    transport: str = "constant"  # "constant" (k, mu, diff) | "sutherland"
+    disabled: tuple = ()  # Sutherland properties left out of enabledProperties
 
     def processes(self):
         species = [f"S{i}" for i in range(self.n_species)]
@@ -42,6 +44,10 @@ class Case:
             fc = {"ausm": fv.Ausm, "average": fv.AverageFlux, "off": fv.OffFlux, "rusanov": fv.Rusanov}.get(self.flux)
             flux = fc() if fc else None
         tr = fv.Constant(k=self.k, mu=self.mu, diff=self.diff) if (self.k or self.mu or self.diff) else None
+        if self.transport == "sutherland":
+            names = ("Conductivity", "Viscosity", "Diffusivity")
+            enabled = [x for x in names if x.lower() not in self.disabled] if self.disabled else []
+            tr = fv.Sutherland(eos, enabled)
         if self.compact:
             return [fv.CompactCompressibleNSSpeciesTransport({"cfl": 0.5}, eos, flux, tr, pgs)]
         procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, flux, tr)]
@@ -97,7 +103,8 @@ class Case:
     def oracle_physics(self):
         return orc.physics(self.gamma, self.Rgas, self.n_species, self.flux, self.recon, self.limiter,
                            self.max_lim_grad, self.k, self.mu, self.diff, ausmpup_minf=self.minf,
-                           pgs_alpha=self.pgs_alpha or 1.0, compact=self.compact)
+                           pgs_alpha=self.pgs_alpha or 1.0, compact=self.compact, transport=self.transport,
+                           disabled=self.disabled)
 
     def oracle_bcs(self):
         kinds = {"inflow": 1, "outflow": 2, "ghost": 3}
@@ -198,6 +205,16 @@ def standard_cases():
                       mu=1.8e-5, k=0.0251125))
     cases.append(Case("ausmpup_minf_pgs_species", chan(6), flux="ausmpup", minf=0.3, pgs_alpha=3.0, n_species=2,
                       recon="leastsquares", limiter="minmod", mu=0.5, k=5.0, diff=0.2, bcs=inflow_x(2)))
+    # transport::Sutherland (sutherland.cpp): properties at the face temperature and density
+    cases.append(Case("sutherland_ns_lsq_bj", box3(8), recon="leastsquares", limiter="barthjespersen",
+                      transport="sutherland"))
+    cases.append(Case("sutherland_species_walls", chan(6), n_species=3, recon="leastsquares", limiter="barthjespersen",
+                      transport="sutherland", bcs=inflow_x(3)))
+    cases.append(Case("sutherland_diffusion_only", chan(6), flux="none", n_species=2, transport="sutherland",
+                      bcs=inflow_x(2)))
+    cases.append(Case("sutherland_tets_no_diffusivity", TetBoxMesh(3, 0.15), flux="rusanov", n_species=2,
+                      recon="leastsquares", limiter="barthjespersen", transport="sutherland",
+                      disabled=("diffusivity",), bcs=all_outflow(6)))
     cases.append(Case("ausmpup_lowmach_2d", BoxMesh([12, 10], [0, 0], [1, 1], [True, True]), flux="ausmpup",
                       minf=0.05, pgs_alpha=10.0, mach=0.02))
     # CompactCompressibleNSSpeciesTransport: AverageFlux (NA direction: averaged species) and AUSM+up with PGS
