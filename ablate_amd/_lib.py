@@ -59,6 +59,8 @@ class PhysicsDesc(ctypes.Structure):
         ("compact_species", c_int32),
         ("transport", c_int32),
         ("transport_disabled", c_int32),
+        ("conduction_stability_factor", c_double),
+        ("viscous_stability_factor", c_double),
     ]
 
 
@@ -115,6 +117,8 @@ SIGNATURES = {
     "abl_fv_unpack_aos": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abl_fv_check_state": (c_int32, [c_void_p, c_void_p, POINTER(c_int64), c_void_p]),
     "abl_fv_cfl_dt": (c_int32, [c_void_p, c_void_p, _dp, c_void_p]),
+    "abl_fv_time_steps": (c_int32, [c_void_p, c_void_p, _dp, POINTER(c_int32), c_void_p]),
+    "abl_fv_physics_dt": (c_int32, [c_void_p, c_void_p, _dp, c_void_p]),
     "abl_fv_axpy": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "abl_fv_ts_euler": (c_int32, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double, c_int64, _dp,
                                   POINTER(c_int64), c_void_p]),

@@ -2969,6 +2969,43 @@ __global__ __launch_bounds__(kBlock) void k_min_final(const double* __restrict__
     const double m = block_min(v);
     if (threadIdx.x == 0) out[0] = m;
 }
+// NavierStokesTransport::ComputeConductionTimeStep and ComputeViscousDiffusionTimeStep
+// (navierStokesTransport.cpp:287-450): per interior cell |f (2 r_min)^2 / alpha| with alpha = k / (rho cv) and
+// |f (2 r_min)^2 / nu| with nu = mu / rho, k and mu of the transport model at the cell temperature (the aux T);
+// block minima of both into partial[0, nb) and partial[nb, 2 nb).
+__global__ __launch_bounds__(kBlock) void k_diffusion_dt_partial(const double* __restrict__ U,
+                                                                 const double* __restrict__ rmin, int64_t N,
+                                                                 int64_t NI, int dim, Phys ph, double fcond,
+                                                                 double fvisc, double* __restrict__ partial) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dc = 1e10, dv = 1e10;  // utilities::Constants::large
+    if (c < NI) {
+        const double rho = U[c];
+        double ke = 0.0;
+        for (int d = 0; d < dim; d++) {
+            const double v = U[(kRhoU + d) * N + c] / rho;
+            ke += v * v;
+        }
+        ke *= 0.5;
+        const double T = (U[kRhoE * N + c] / rho - ke) / ph.cv;
+        double k = ph.k, mu = ph.mu;
+        if (ph.suth) {  // ph.k, ph.mu are 1 or 0 per enabled property
+            const double m = sutherland_mu(T);
+            k *= m * ph.cp / kSuthPr;
+            mu *= m;
+        }
+        const double dx2 = (2.0 * rmin[c]) * (2.0 * rmin[c]);
+        const double alpha = k / (rho * ph.cv), nu = mu / rho;
+        dc = fmin(dc, fabs(fcond * dx2 / alpha));
+        dv = fmin(dv, fabs(fvisc * dx2 / nu));
+    }
+    const double mc = block_min(dc);
+    const double mv = block_min(dv);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = mc;
+        partial[gridDim.x + blockIdx.x] = mv;
+    }
+}
 // PressureGradientScaling::UpdatePreconditioner reductions over the interior cells (pressureGradientScaling.cpp:55-104,
 // PerfectGas T-route): r = {sum p, min p, max p, max a, max Mach}; a fixed reduction tree (deterministic sums),
 // NaN entries skipped by the min / max.
@@ -3195,6 +3232,7 @@ struct abl_fv_solver {
     int strategy = ABL_FV_STRATEGY_TILE;
     abl::DevBuf<unsigned long long> counter;
     abl::DevBuf<double> rmin, cfl_partial, cfl_out;  // MIN_CELL_RADIUS per interior cell; CFL reduction scratch
+    abl::DevBuf<double> dt_partial;                  // conduction / viscous time-step block minima [2][blocks]
     // PressureGradientScaling attached by abl_fv_set_pgs: its state (alpha in ph.pgs_alpha) and reduction scratch
     bool pgs_on = false;
     abl_fv_pgs_desc pgs{};
@@ -3487,7 +3525,7 @@ struct abl_fv_solver {
     }
     int64_t device_bytes() const {
         return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + mr_seq.bytes() + mr_pro.bytes() + mr_stile.bytes() + mr_snext.bytes() + mr_fstart.bytes() + mr_ffaces.bytes() + mr_fcode.bytes() + mr_pfstart.bytes() + mr_pffaces.bytes() + mr_pfcode.bytes() + mr_hstart.bytes() + mr_hcell.bytes() + mr_hinfo.bytes() + mr_gslot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
-                         gradA.bytes() + rec.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
+                         gradA.bytes() + rec.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + dt_partial.bytes() + cfl_out.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
 };
@@ -3740,7 +3778,8 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             }
             s->rmin.upload(r);
             s->cfl_partial.alloc((size_t)std::max<int64_t>(grid_for(L.NI), 1));
-            s->cfl_out.alloc(1);
+            s->dt_partial.alloc((size_t)2 * std::max<int64_t>(grid_for(L.NI), 1));
+            s->cfl_out.alloc(3);
         }
         s->build_tiles(L);
         s->build_fused(L);
@@ -4241,6 +4280,41 @@ double cfl_dt_impl(abl_fv_handle h, const double* U, hipStream_t st) {
     HIP_CHECK(hipStreamSynchronize(st));
     return dt;
 }
+// ComputePhysicsTimeSteps: {cfl, conduction, viscous} on the host and the mask of registered functions
+// (synchronises the stream).
+int time_steps_impl(abl_fv_handle h, const double* U, double* dt, hipStream_t st) {
+    const abl::Layout& L = h->host;
+    const double fc = h->phys.conduction_stability_factor, fvs = h->phys.viscous_stability_factor;
+    const bool has_transport = h->visc;  // a transport model with an enabled property
+    const int reg = (h->phys.flux_calculator != ABL_FV_FLUX_NONE ? 1 : 0) | (has_transport && fc > 0.0 ? 2 : 0) |
+                    (has_transport && fvs > 0.0 ? 4 : 0);
+    dt[0] = dt[1] = dt[2] = 1e10;
+    if (reg & 1) dt[0] = cfl_dt_impl(h, U, st);
+    if (reg & 6) {
+        const unsigned nb = grid_for(L.NI);
+        hipLaunchKernelGGL(abl::k_diffusion_dt_partial, dim3(nb), dim3(abl::kBlock), 0, st, U, h->rmin.ptr, L.N, L.NI,
+                           L.dim, h->ph, fc, fvs, h->dt_partial.ptr);
+        for (int j = 0; j < 2; j++)
+            hipLaunchKernelGGL(abl::k_min_final, dim3(1), dim3(abl::kBlock), 0, st, h->dt_partial.ptr + (size_t)j * nb,
+                               (int64_t)nb, h->cfl_out.ptr + 1 + j);
+        HIP_CHECK(hipGetLastError());
+        double v[2] = {0.0, 0.0};
+        HIP_CHECK(hipMemcpyAsync(v, h->cfl_out.ptr + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (reg & 2) dt[1] = v[0];
+        if (reg & 4) dt[2] = v[1];
+    }
+    return reg;
+}
+// ComputePhysicsTimeStep: the minimum over the registered functions, Constants::large when none.
+double physics_dt_impl(abl_fv_handle h, const double* U, hipStream_t st) {
+    double dt[3];
+    const int reg = time_steps_impl(h, U, dt, st);
+    double m = 1e10;
+    for (int j = 0; j < 3; j++)
+        if (reg & (1 << j)) m = std::min(m, dt[j]);
+    return m;
+}
 void axpy_impl(abl_fv_handle h, double dt, const double* R, double* U, hipStream_t st) {
     const abl::Layout& L = h->host;
     hipLaunchKernelGGL(abl::k_axpy, dim3(grid_for(L.NI), (unsigned)L.nc), dim3(abl::kBlock), 0, st, dt, R, U, L.N, L.NI);
@@ -4255,6 +4329,27 @@ extern "C" int abl_fv_cfl_dt(abl_fv_handle h, const double* U, double* dt, abl_f
         *dt = cfl_dt_impl(h, U, (hipStream_t)stream);
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_cfl_dt: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_time_steps(abl_fv_handle h, const double* U, double* dt, int32_t* registered,
+                                 abl_fv_stream stream) {
+    if (!h || !U || !dt || !registered) return fail(h, "abl_fv_time_steps: null argument");
+    try {
+        *registered = time_steps_impl(h, U, dt, (hipStream_t)stream);
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_time_steps: ") + e.what());
+    }
+    return 0;
+}
+
+extern "C" int abl_fv_physics_dt(abl_fv_handle h, const double* U, double* dt, abl_fv_stream stream) {
+    if (!h || !U || !dt) return fail(h, "abl_fv_physics_dt: null argument");
+    try {
+        *dt = physics_dt_impl(h, U, (hipStream_t)stream);
+    } catch (const std::exception& e) {
+        return fail(h, std::string("abl_fv_physics_dt: ") + e.what());
     }
     return 0;
 }
@@ -4364,7 +4459,7 @@ extern "C" int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0,
     double t = t0;
     int64_t steps = 0;
     try {
-        double dt = std::min(dt0, safety * cfl_dt_impl(h, U, st));  // AdaptPhysics::AdaptInitializer
+        double dt = std::min(dt0, safety * physics_dt_impl(h, U, st));  // AdaptPhysics::AdaptInitializer
         while (t < max_time && steps < max_steps) {
             if (!(dt > 0.0)) throw std::runtime_error("non-positive or NaN step size at t = " + std::to_string(t));
             // TS pre-step: PressureGradientScaling::UpdatePreconditioner with the step about to be taken
@@ -4373,7 +4468,7 @@ extern "C" int abl_fv_ts_euler(abl_fv_handle h, double* U, double* R, double t0,
                 if (int rc = abl_fv_pgs_update(h, U, dt, nullptr, nullptr, stream)) return rc;
             if (int rc = abl_fv_rhs(h, t, U, R, 0, stream)) return rc;
             // TSStep_Euler asks the adaptor for the next step while the solution still holds U_n
-            double next = safety * cfl_dt_impl(h, U, st);
+            double next = safety * physics_dt_impl(h, U, st);
             axpy_impl(h, dt, R, U, st);
             const double t_end = t + dt, remaining = max_time - t_end;
             // TSAdaptChoose with ts_exact_final_time matchstep (matchstep factors 0.01, 2)

@@ -7,7 +7,8 @@ Class and parameter names follow the reference's YAML surface:
   ablate::finiteVolume::fluxCalculator::{Ausm, AverageFlux, OffFlux, AusmpUp{mInf, pgs}}   (fluxCalculator/*.cpp)
   ablate::finiteVolume::processes::PressureGradientScaling{eos, alphaInit, domainLength, maxAlphaAllowed,
                                                            maxDeltaPressureFac}  (processes/pressureGradientScaling.cpp)
-  ablate::finiteVolume::processes::NavierStokesTransport{parameters{cfl}, eos, fluxCalculator, transport}
+  ablate::finiteVolume::processes::NavierStokesTransport{parameters{cfl, conductionStabilityFactor,
+    viscousStabilityFactor}, eos, fluxCalculator, transport}
                                                                             (processes/navierStokesTransport.cpp:569-574)
   ablate::finiteVolume::processes::SpeciesTransport{eos, fluxCalculator, transport}   (speciesTransport.cpp:474-478)
   ablate::finiteVolume::FiniteVolumeSolver::ComputeRHSFunction(time, locX, locF)     (finiteVolumeSolver.cpp:233-291)
@@ -271,6 +272,8 @@ class FiniteVolumeSolver:
             mu=0.0 if suth else float(tr.mu),
             diff=0.0 if suth else float(tr.diff),
             cfl=float(nst.parameters.get("cfl", 0.5)),
+            conduction_stability_factor=float(nst.parameters.get("conductionStabilityFactor", 0.0)),
+            viscous_stability_factor=float(nst.parameters.get("viscousStabilityFactor", 0.0)),
             ausmpup_minf=float(getattr(nst.fluxCalculator, "mInf", 0.0)),
             compact_species=int(compact),
             transport=int(suth),
@@ -500,6 +503,26 @@ class FiniteVolumeSolver:
                                             self._stream_ptr(stream)), self._h, "abl_fv_cfl_dt")
         return float(dt.value)
 
+    def ComputePhysicsTimeSteps(self, U, stream=None) -> dict:
+        """FiniteVolumeSolver::ComputePhysicsTimeSteps (finiteVolumeSolver.cpp:381-393): the registered time-step
+        functions' minima over the interior cells, by name ("cfl" with a flux calculator; "cond" / "visc" when
+        NavierStokesTransport's conductionStabilityFactor / viscousStabilityFactor are positive)."""
+        self._check_tensor(U, "U")
+        dt = (ctypes.c_double * 3)()
+        reg = ctypes.c_int32()
+        _lib.check(_lib.lib().abl_fv_time_steps(self._h, ctypes.c_void_p(U.data_ptr()), dt, ctypes.byref(reg),
+                                                self._stream_ptr(stream)), self._h, "abl_fv_time_steps")
+        return {name: float(dt[i]) for i, name in enumerate(("cfl", "cond", "visc")) if reg.value >> i & 1}
+
+    def ComputePhysicsTimeStep(self, U, stream=None) -> float:
+        """FiniteVolumeSolver::ComputePhysicsTimeStep (finiteVolumeSolver.cpp:371-379): the smallest registered step
+        (1e10 when none)."""
+        self._check_tensor(U, "U")
+        dt = ctypes.c_double()
+        _lib.check(_lib.lib().abl_fv_physics_dt(self._h, ctypes.c_void_p(U.data_ptr()), ctypes.byref(dt),
+                                                self._stream_ptr(stream)), self._h, "abl_fv_physics_dt")
+        return float(dt.value)
+
     def axpy(self, dt: float, R, U, stream=None) -> None:
         """Forward-Euler stage on the interior rows: U += dt * R (FP64, on the device)."""
         self._check_tensor(U, "U")
@@ -510,8 +533,9 @@ class FiniteVolumeSolver:
 
     def ts_euler(self, U, max_time: float, t0: float = 0.0, dt0: float = 0.1, safety: float = 1.0,
                  max_steps: int = 10**9, R=None, stream=None):
-        """Forward-Euler integration of U in place to max_time under the CFL step-size adaptor with matchstep
-        (the reference TimeStepper with ts_adapt_type physics).  Returns (final time, steps)."""
+        """Forward-Euler integration of U in place to max_time under the physics step-size adaptor (the smallest of
+        the registered CFL / conduction / viscous steps) with matchstep (the reference TimeStepper with ts_adapt_type
+        physics).  Returns (final time, steps)."""
         import torch
 
         if any(g[0].updateFunction is not None for g in self._ghosts):

@@ -137,6 +137,9 @@ typedef struct abl_fv_physics_desc {
                                    mu, diff ignored) */
     int32_t transport_disabled; /* Sutherland: ABL_FV_TRANSPORT_* bits of the properties left out of
                                    enabledProperties; 0 = all enabled (the reference's default) */
+    double conduction_stability_factor; /* NavierStokesTransport{parameters{conductionStabilityFactor}}: > 0
+                                           registers the conduction time step (navierStokesTransport.cpp:31,67-75) */
+    double viscous_stability_factor;    /* {viscousStabilityFactor}: > 0 registers the viscous time step (:32,76-78) */
 } abl_fv_physics_desc;
 
 /* One boundary-condition slot. */
@@ -304,11 +307,22 @@ int abl_fv_set_pgs_alpha(abl_fv_handle h, double alpha);
  * (finiteVolumeSolver.cpp:185-215), cfl from the physics descriptor.  *dt is a host value (synchronises the
  * stream).  Cells with a NaN step are skipped. */
 int abl_fv_cfl_dt(abl_fv_handle h, const double* U, double* dt, abl_fv_stream stream);
+/* FiniteVolumeSolver::ComputePhysicsTimeSteps (finiteVolumeSolver.cpp:381-393) of the process set: dt[0] the CFL
+ * step (registered with a flux calculator, navierStokesTransport.cpp:37-43), dt[1] ComputeConductionTimeStep
+ * (:287-371: f_cond (2 r_min)^2 rho cv / k per cell) and dt[2] ComputeViscousDiffusionTimeStep (:373-450:
+ * f_visc (2 r_min)^2 rho / mu per cell), each the minimum over interior cells starting from Constants::large = 1e10,
+ * with k and mu of the transport model at the cell temperature.  Bit i of *registered is set when function i is
+ * registered (the diffusion steps when their stability factor is > 0).  Host values (synchronises the stream). */
+int abl_fv_time_steps(abl_fv_handle h, const double* U, double* dt, int32_t* registered, abl_fv_stream stream);
+/* FiniteVolumeSolver::ComputePhysicsTimeStep (finiteVolumeSolver.cpp:371-379): the minimum of the registered
+ * time-step functions of abl_fv_time_steps (1e10 when none is registered).  Host value (synchronises). */
+int abl_fv_physics_dt(abl_fv_handle h, const double* U, double* dt, abl_fv_stream stream);
 /* Forward-Euler stage in FP64 on the interior rows: U += dt * R (ghost rows untouched). */
 int abl_fv_axpy(abl_fv_handle h, double dt, const double* R, double* U, abl_fv_stream stream);
 /* Integrates U from t0 to max_time with forward Euler as the reference's TimeStepper runs it under
  * ts_adapt_type physics (safety factor `safety`), ts_exact_final_time matchstep and PETSc's TSEULER step
- * (timeStepper.cpp:80-130, adaptPhysics.cpp:3-59): first step min(dt0, safety * cfl_dt(U0)); each step evaluates
+ * (timeStepper.cpp:80-130, adaptPhysics.cpp:3-59): first step min(dt0, safety * physics_dt(U0)), the physics step
+ * being abl_fv_physics_dt (CFL and, when registered, the diffusion steps); each step evaluates
  * the RHS into the caller's workspace R [n_comp][n_cells], takes the next step size from U_n, updates U; steps
  * that would pass max_time are cut to it.  Stops at max_time or after max_steps; reports the final time and
  * the number of steps taken.  With a PressureGradientScaling attached, its pre-step update (abl_fv_pgs_update with the

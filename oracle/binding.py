@@ -163,6 +163,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_face_interpolate.restype = c_int
     L.orc_sutherland.argtypes = [d, d, d, c_int, _dp]
     L.orc_sutherland.restype = None
+    L.orc_diffusion_dt.argtypes = [POINTER(OrcMesh), POINTER(OrcPhysics), d, d, _dp, _dp, _dp]
+    L.orc_diffusion_dt.restype = None
 
 
 def _ptr(a):
@@ -411,6 +413,18 @@ def cfl_dt(mesh_arrays, phys: OrcPhysics, cfl: float, r: np.ndarray, U_soa: np.n
     X = np.ascontiguousarray(U_soa.T)
     rr = np.ascontiguousarray(r, dtype=np.float64)
     return float(lib().orc_cfl_dt(ctypes.byref(ms), ctypes.byref(phys), cfl, _ptr(rr), _ptr(X)))
+
+
+def diffusion_dt(mesh_arrays, phys: OrcPhysics, fcond: float, fvisc: float, r: np.ndarray,
+                 U_soa: np.ndarray) -> tuple[float, float]:
+    """NavierStokesTransport::ComputeConductionTimeStep / ComputeViscousDiffusionTimeStep
+    (navierStokesTransport.cpp:287-450) on a SoA state: (conduction dt, viscous dt)."""
+    ms, keep = _mesh_struct(mesh_arrays)
+    X = np.ascontiguousarray(U_soa.T)
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(2)
+    lib().orc_diffusion_dt(ctypes.byref(ms), ctypes.byref(phys), fcond, fvisc, _ptr(rr), _ptr(X), _ptr(out))
+    return float(out[0]), float(out[1])
 
 
 def pgs_update(mesh_arrays, phys: OrcPhysics, U_soa: np.ndarray, dt: float, alpha: float, domain_length: float,

@@ -1151,6 +1151,38 @@ double orc_cfl_dt(const orc_mesh* m, const orc_physics* ph, double cfl, const do
     return dtMin;
 }
 
+/* NavierStokesTransport::ComputeConductionTimeStep (navierStokesTransport.cpp:287-371) and
+ * ComputeViscousDiffusionTimeStep (:373-450): over the interior cells, k and mu from the transport model at the aux
+ * temperature, alpha = k / (rho cv), nu = mu / rho, dt = |f (2 r)^2 / alpha| and |f (2 r)^2 / nu|, minima starting
+ * from Constants::large.  out[0] conduction, out[1] viscous. */
+void orc_diffusion_dt(const orc_mesh* m, const orc_physics* ph, double fcond, double fvisc, const double* r,
+                      const double* X, double* out) {
+    const int dim = m->dim, nc = dim + 2 + ph->n_species;
+    const double cv = ph->Rgas / (ph->gamma - 1.0), cp = ph->gamma * ph->Rgas / (ph->gamma - 1.0);
+    double dtCond = 1e10, dtVisc = 1e10;
+    for (int64_t c = 0; c < m->n_interior; ++c) {
+        const double* conserved = X + c * nc;
+        const double T = orc_pg_temperature(conserved, dim, ph->gamma, ph->Rgas);
+        double k = ph->k, mu = ph->mu;
+        if (ph->transport == ORC_TRANSPORT_SUTHERLAND) {
+            double tr[3];
+            orc_sutherland(T, conserved[RHO], cp, ph->transport_disabled, tr);
+            k = tr[0];
+            mu = tr[1];
+        }
+        const double rho = conserved[RHO];
+        const double dx2 = (2.0 * r[c]) * (2.0 * r[c]); /* PetscSqr(2.0 * MIN_CELL_RADIUS) */
+        const double alpha = k / (rho * cv);
+        const double dtc = fabs(fcond * dx2 / alpha);
+        dtCond = dtc < dtCond ? dtc : dtCond;
+        const double nu = mu / rho;
+        const double dtv = fabs(fvisc * dx2 / nu);
+        dtVisc = dtv < dtVisc ? dtv : dtVisc;
+    }
+    out[0] = dtCond;
+    out[1] = dtVisc;
+}
+
 /* PressureGradientScaling::UpdatePreconditioner, pressureGradientScaling.cpp:16-137: pressure mean and extremes,
  * maximum sound speed and Mach number over the interior cells (PerfectGas T route), then the alpha update with the
  * step dt.  Constants of pressureGradientScaling.hpp:32,37; constructor defaults of :10-11 applied by the caller

@@ -90,6 +90,10 @@ double orc_pg_sensible_enthalpy_T(double T, double gamma, double R);
 /* transport::Sutherland at temperature T, density rho and cp: out = {conductivity, viscosity, diffusivity}
  * (sutherland.cpp:9-64; a property left out of enabledProperties is zero, :88-91). */
 void orc_sutherland(double T, double rho, double cp, int disabled, double* out);
+/* ComputeConductionTimeStep / ComputeViscousDiffusionTimeStep (navierStokesTransport.cpp:287-450) over the interior
+ * cells with stability factors fcond, fvisc and per-cell MIN_CELL_RADIUS r: out = {conduction dt, viscous dt}. */
+void orc_diffusion_dt(const orc_mesh* m, const orc_physics* ph, double fcond, double fvisc, const double* r,
+                      const double* X, double* out);
 double orc_pg_cv(double gamma, double R);
 double orc_pg_cp(double gamma, double R);
 

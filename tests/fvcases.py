@@ -33,6 +33,8 @@ class Case:
     compact: bool = False  # CompactCompressibleNSSpeciesTransport instead of NavierStokes + SpeciesTransport
     transport: str = "constant"  # "constant" (k, mu, diff) | "sutherland"
     disabled: tuple = ()  # Sutherland properties left out of enabledProperties
+    cond_factor: float = 0.0  # NavierStokesTransport conductionStabilityFactor
+    visc_factor: float = 0.0  # NavierStokesTransport viscousStabilityFactor
 
     def processes(self):
         species = [f"S{i}" for i in range(self.n_species)]
@@ -48,9 +50,10 @@ class Case:
             names = ("Conductivity", "Viscosity", "Diffusivity")
             enabled = [x for x in names if x.lower() not in self.disabled] if self.disabled else []
             tr = fv.Sutherland(eos, enabled)
+        params = {"cfl": 0.5, "conductionStabilityFactor": self.cond_factor, "viscousStabilityFactor": self.visc_factor}
         if self.compact:
-            return [fv.CompactCompressibleNSSpeciesTransport({"cfl": 0.5}, eos, flux, tr, pgs)]
-        procs = [fv.NavierStokesTransport({"cfl": 0.5}, eos, flux, tr)]
+            return [fv.CompactCompressibleNSSpeciesTransport(params, eos, flux, tr, pgs)]
+        procs = [fv.NavierStokesTransport(params, eos, flux, tr)]
         if self.n_species:
             procs.append(fv.SpeciesTransport(eos, flux, tr))
         return procs

@@ -82,3 +82,62 @@ def test_shock_tube_matches_reference_table(case, torch_cuda):
     NI = a.n_interior
     scale = np.abs(Uo[:, :NI]).max(axis=1, keepdims=True)
     assert np.all(np.abs(U[:, :NI] - Uo[:, :NI]) <= 1e-10 * scale)
+
+
+# ---- diffusion stability time steps (navierStokesTransport.cpp:287-450) and the physics step of their minimum
+
+def _dt_cases():
+    from ablate_amd.mesh import BoxMesh, TetBoxMesh
+    from tests.fvcases import Case, all_outflow, inflow_x
+
+    chan = BoxMesh([6, 5, 4], [0, 0, 0], [1.2, 1, 0.8], [False, True, True])
+    return [
+        Case("dt_constant_walls", chan, recon="leastsquares", limiter="barthjespersen", mu=200.0, k=2.0e5,
+             n_species=2, diff=0.1, bcs=inflow_x(2), cond_factor=0.4, visc_factor=0.3),
+        Case("dt_sutherland_tets", TetBoxMesh(3, 0.2), flux="rusanov", n_species=2, recon="leastsquares",
+             limiter="barthjespersen", transport="sutherland", bcs=all_outflow(6), cond_factor=0.5, visc_factor=0.5),
+        Case("dt_diffusion_only_viscous", chan, flux="none", mu=0.2, k=1.0, bcs=inflow_x(), visc_factor=0.25),
+        Case("dt_none_registered", chan, flux="none", mu=0.2, k=1.0, bcs=inflow_x()),
+    ]
+
+
+DT_CASES = _dt_cases()
+
+
+@pytest.mark.parametrize("case", DT_CASES, ids=[c.name for c in DT_CASES])
+def test_physics_time_steps_match_oracle(case, torch_cuda):
+    torch = torch_cuda
+    U0 = case.initial_state()
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    steps = solver.ComputePhysicsTimeSteps(U)
+    a = case.mesh.arrays
+    r = orc.min_cell_radius(a)
+    ph = case.oracle_physics()
+    want = {}
+    if case.flux != "none":
+        want["cfl"] = orc.cfl_dt(a, ph, 0.5, r, U0)
+    dc, dv = orc.diffusion_dt(a, ph, case.cond_factor, case.visc_factor, r, U0)
+    if case.cond_factor > 0:
+        want["cond"] = dc
+    if case.visc_factor > 0:
+        want["visc"] = dv
+    print(f"[dt] {case.name} gpu={steps} oracle={want}", flush=True)
+    assert set(steps) == set(want)
+    for name, v in want.items():
+        assert steps[name] == pytest.approx(v, rel=1e-14), name
+    assert solver.ComputePhysicsTimeStep(U) == min(want.values(), default=1e10)
+
+
+def test_ts_euler_steps_under_the_diffusion_limit(torch_cuda):
+    """A viscous-limited case: the first step of the loop is safety x the smallest registered step."""
+    torch = torch_cuda
+    case = DT_CASES[0]
+    U0 = case.initial_state()
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    steps = solver.ComputePhysicsTimeSteps(U)
+    assert min(steps["cond"], steps["visc"]) < steps["cfl"]  # the diffusion limit binds
+    t, n = solver.ts_euler(U, max_time=1.0, dt0=1.0, safety=0.9, max_steps=1)
+    assert n == 1
+    assert t == pytest.approx(0.9 * min(steps.values()), rel=1e-15)
