@@ -2,9 +2,10 @@
 FP64 EOS recovery and accumulation).
 
 Tolerance: the residual keeps ~7 digits of the per-face flux perturbations dF = F_f - F(U_c).n.  The 1e-6 target
-(norm-wise relative error per field) is checked where it is specified, on fine Taylor-Green meshes against the
-FP64 form (itself pinned to the oracle at 1e-12); on the coarse oracle meshes, where the state changes by tens of
-percent from cell to cell and |dF| is several times the residual, the bound is 5e-6."""
+(norm-wise relative error per field) is checked where it is specified, at 128^3 on Taylor-Green states, against the
+FP64 oracle itself and against the FP64 form (itself pinned to the oracle at 1e-12); on the coarse
+oracle meshes, where the state changes by tens of percent from cell to cell and |dF| is several times the residual,
+the bound is 5e-6."""
 import numpy as np
 import pytest
 
@@ -120,6 +121,26 @@ def test_mixed_fine_mesh_within_1e6_of_fp64(mach, strategy, torch_cuda):
     torch.cuda.synchronize()
     err = ((Rm - Rd).norm(dim=1) / Rd.norm(dim=1)).cpu().numpy()
     print(f"[mixed] 128^3 mach {mach} {strategy} per-field relative error vs FP64 {err}", flush=True)
+    assert err.max() <= TOL_MIXED, err
+
+
+@pytest.mark.parametrize("mach", [0.3, 0.01])
+@pytest.mark.parametrize("n", [128])
+def test_mixed_within_1e6_of_oracle(n, mach, torch_cuda):
+    """C3 physics on a 128^3 Taylor-Green box: the mixed GPU residual against the FP64 oracle itself, per field
+    <= 1e-6 (on 32^3 and 48^3 boxes the density row, whose residual is small beside its face fluxes for this nearly
+    divergence-free state, sits at 1.8e-6 and 1.3e-6)."""
+    torch = torch_cuda
+    mesh = BoxMesh([n, n, n], [0, 0, 0], [1, 1, 1], [True] * 3)
+    case = Case(f"c3_{n}", mesh, recon="leastsquares", limiter="barthjespersen", mu=1.8e-5, k=0.0251125, mach=mach)
+    U0 = case.initial_state()
+    Rref, _ = case.oracle_rhs(U0.copy())
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Rm = torch.zeros_like(U)
+    case.solver(form="mixed").compute_rhs(0.0, U, Rm)
+    torch.cuda.synchronize()
+    err = rel_err_per_field(Rm.cpu().numpy(), Rref, mesh.n_interior)
+    print(f"[mixed] {n}^3 mach {mach} per-field relative error vs the oracle {err}", flush=True)
     assert err.max() <= TOL_MIXED, err
 
 
