@@ -110,6 +110,15 @@ def measured_traffic(cfg, n, strategy):
     return None if rec is None else rec["traffic_bytes"]
 
 
+def tets_traffic(n, strategy):
+    """HBM bytes of one FP64 RHS of the tets leg from the committed counter record of the same device sources (the
+    ghost fill, gradient, species-gradient, face-flux and gather kernels), or None."""
+    from ablate_amd import rocprof
+
+    rec = rocprof.lookup(f"ns_10species_fp64_tets{n}_lsq_bj_rusanov", n, strategy)
+    return None if rec is None else rec["traffic_bytes"]
+
+
 def ns_processes(fv, mu, k):
     eos = fv.PerfectGas({"gamma": 1.4, "Rgas": 287.0})
     return [fv.NavierStokesTransport({"cfl": 0.5}, eos, fv.Ausm(), fv.Constant(k=k, mu=mu))]
@@ -304,6 +313,7 @@ def tets_leg(fv, synthetic, args):
         "alg_bytes_per_cell": round(info.alg_bytes_per_cell, 1),
         "achieved_GB_per_s": round(achieved, 1),
         "frac_of_8TB_s": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": tets_traffic(n, strategy),
         "setup_s": round(setup_s, 1),
         "finite": bool(torch.isfinite(R).all().item()),
         "mixed": {"value": round(cells / (ms_mixed * 1e-3) / 1e6, 3), "unit": "Mcell/s",
