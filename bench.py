@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (VJP) leg")
     ap.add_argument("--adjoint-n", type=int, default=128, help="cells per direction of the adjoint workload")
     ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral NS + species leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 64 x 64 first-order Euler leg (eager and HIP graph)")
     ap.add_argument("--tets-n", type=int, default=111, help="cubes per direction of the tetrahedral workload")
     ap.add_argument("--cpu-n", type=int, default=32, help="cells per direction of each CPU-baseline process's sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline time per phase")
@@ -329,6 +330,53 @@ def tets_leg(fv, synthetic, args):
     }
 
 
+def c1_leg(fv, synthetic, args):
+    """The reference's CPU-runnable case on the GPU: first-order Euler RHS (AUSM) on a 64 x 64 periodic quad box,
+    rho = 1 +- 10 %, u, v ~ U(-50, 50) m/s, p = 1e5 +- 10 %.  At 4096 cells one RHS is launch-bound: timed eagerly
+    (one call after another on the stream) and as replays of a HIP graph holding 50 RHS calls."""
+    import torch
+
+    from ablate_amd.mesh import BoxMesh
+
+    mesh = BoxMesh([64, 64], [0, 0], [1, 1], [True, True])
+    eos = fv.PerfectGas({"gamma": 1.4, "Rgas": 287.0})
+    solver = fv.FiniteVolumeSolver(mesh, [fv.NavierStokesTransport({"cfl": 0.5}, eos, fv.Ausm())],
+                                   {"petscfv_type": "upwind"})
+    U = torch.tensor(synthetic.random_euler_2d(mesh), dtype=torch.float64, device="cuda")
+    R = torch.zeros_like(U)
+    stream = torch.cuda.current_stream()
+    calls = 200
+    for _ in range(10):
+        solver.compute_rhs(0.0, U, R)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        solver.compute_rhs(0.0, U, R)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eager_us = e0.elapsed_time(e1) * 1e3 / calls
+    per_graph = 50
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(per_graph):
+            solver.compute_rhs(0.0, U, R)
+    g.replay()
+    torch.cuda.synchronize()
+    reps = calls // per_graph
+    e0.record(stream)
+    for _ in range(reps):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph_us = e0.elapsed_time(e1) * 1e3 / (reps * per_graph)
+    cells = mesh.n_interior
+    return {"workload": "euler_fp64_64x64_first_order_ausm", "cells": cells, "strategy": solver.strategy(),
+            "eager_us_per_rhs": round(eager_us, 2), "eager_Mcell_per_s": round(cells / eager_us, 1),
+            "graph_us_per_rhs": round(graph_us, 2), "graph_Mcell_per_s": round(cells / graph_us, 1),
+            "note": f"graph: {per_graph} RHS calls captured once and replayed; launch-bound at this size"}
+
+
 def adjoint_leg(fv, synthetic, args):
     """C4: VJP of the first-order FP64 RHS on a 128^3 hex mesh, Euler + 10 species, inflow (rho 1.2, u 100 m/s,
     T 300 K) on x-, outflow on x+, periodic y/z; base flow the Taylor-Green field at Mach 0.3 (phase-shifted off
@@ -528,6 +576,9 @@ def main():
     if cfg["form"] == "fp64" and not args.no_tets:
         out["tets"] = tets_leg(fv, synthetic, args)
         progress("tets leg done")
+    if cfg["form"] == "fp64" and not args.no_c1:
+        out["c1"] = c1_leg(fv, synthetic, args)
+        progress("64x64 Euler leg done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds, mu, k, cfg["mach"])
     print(json.dumps(out), flush=True)
