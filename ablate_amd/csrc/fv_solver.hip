@@ -402,9 +402,13 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
     const bool bj = ph.limiter == ABL_FV_LIMITER_BARTH_JESPERSEN;
     const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
     const double irho_c = 1.0 / U[kRho * N + c];
+    // arithmetic of the gradient sums and the limiter: FP64 in the FP64 form; FP32 in the mixed form (which stores
+    // these gradients in FP32), on FP64 state differences rounded once
+    using A = G;
     int64_t nb[KM];
     bool valid[KM], inner[KM];
-    double w[KM][DIM], wa[KM][DIM], xl[KM][DIM], irn[KM];
+    G w[KM][DIM], wa[KM][DIM], xl[KM][DIM];
+    double irn[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int32_t e = m.snbr[k * NI + c];
@@ -413,9 +417,9 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
         inner[k] = valid[k] && nb[k] < NI;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            w[k][d] = recon ? double(m.sw[(k * DIM + d) * NI + c]) : 0.0;
-            wa[k][d] = aux ? double(wa_arr[(k * DIM + d) * NI + c]) : 0.0;
-            xl[k][d] = (recon && m.slim) ? double(m.slim[(k * DIM + d) * NI + c]) : 0.0;
+            w[k][d] = recon ? m.sw[(k * DIM + d) * NI + c] : G(0);
+            wa[k][d] = aux ? wa_arr[(k * DIM + d) * NI + c] : G(0);
+            xl[k][d] = (recon && m.slim) ? m.slim[(k * DIM + d) * NI + c] : G(0);
         }
         irn[k] = aux ? 1.0 / U[kRho * N + nb[k]] : 0.0;
     }
@@ -427,11 +431,12 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
         double un[KM];
 #pragma unroll
         for (int k = 0; k < KM; k++) un[k] = U[comp * N + nb[k]];
-        double g[DIM], ga[DIM], umin = uc, umax = uc;
+        A g[DIM], ga[DIM];
+        double umin = uc, umax = uc;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            g[d] = 0.0;
-            ga[d] = 0.0;
+            g[d] = A(0);
+            ga[d] = A(0);
         }
 #pragma unroll
         for (int k = 0; k < KM; k++) {
@@ -440,50 +445,50 @@ __device__ __forceinline__ void species_body(MeshView<G> m, Phys ph, const doubl
                 umin = fmin(umin, un[k]);
                 umax = fmax(umax, un[k]);
                 if (inner[k]) {
-                    const double delta = un[k] - uc;  // both sides: + w.(u_n - u_c), see k_gradients
+                    const A delta = A(un[k] - uc);  // both sides: + w.(u_n - u_c), see k_gradients
 #pragma unroll
                     for (int d = 0; d < DIM; d++) g[d] += w[k][d] * delta;
                 }
             }
             if (aux) {
-                const double delta = un[k] * irn[k] - yc;
+                const A delta = A(un[k] * irn[k] - yc);
 #pragma unroll
                 for (int d = 0; d < DIM; d++) ga[d] += wa[k][d] * delta;
             }
         }
         if (recon) {
-            double phi = DBL_MAX;
+            A phi = sizeof(A) == 8 ? A(DBL_MAX) : A(FLT_MAX);
             if (bj) {
-                double dpos = 0.0, dneg = 0.0;
+                A dpos = A(0), dneg = A(0);
 #pragma unroll
                 for (int k = 0; k < KM; k++) {
                     if (!valid[k]) continue;
-                    double D = 0.0;
+                    A D = A(0);
 #pragma unroll
                     for (int d = 0; d < DIM; d++) D += g[d] * xl[k][d];
                     dpos = fmax(dpos, D);
                     dneg = fmin(dneg, D);
                 }
-                phi = 1.0;
-                if (dpos > 0.0) phi = fmin(phi, (umax - uc) / dpos);
-                if (dneg < 0.0) phi = fmin(phi, (umin - uc) / dneg);
+                phi = A(1);
+                if (dpos > A(0)) phi = fmin(phi, A(umax - uc) / dpos);
+                if (dneg < A(0)) phi = fmin(phi, A(umin - uc) / dneg);
             } else if (ph.limiter != ABL_FV_LIMITER_NONE) {
 #pragma unroll
                 for (int k = 0; k < KM; k++) {
                     if (!valid[k]) continue;
-                    double denom = 0.0;
+                    A denom = A(0);
 #pragma unroll
                     for (int d = 0; d < DIM; d++) denom += g[d] * xl[k][d];
-                    const double flim = 0.5 * (un[k] - uc) / denom;
-                    phi = pmin(phi, limiter_phi<double>(ph.limiter, flim));
+                    const A flim = A(0.5) * A(un[k] - uc) / denom;
+                    phi = pmin(phi, limiter_phi<A>(ph.limiter, flim));
                 }
             } else {
-                phi = 1.0;
+                phi = A(1);
             }
-            if (phi == 0)
+            if (phi == A(0))
 #pragma unroll
                 for (int d = 0; d < DIM; d++)
-                    if (fabs(g[d]) > ph.max_lim_grad) cancel = true;
+                    if (fabs(double(g[d])) > ph.max_lim_grad) cancel = true;
 #pragma unroll
             for (int d = 0; d < DIM; d++) g[d] *= phi;  // stored with the aux gradient below
         }
