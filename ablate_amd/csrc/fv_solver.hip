@@ -1108,11 +1108,6 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
     const int64_t N = m.N, F = m.F, NI = m.NI;
     const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
     const bool ghost = Rc >= NI;
-    if (SPECIES)
-        for (int sp = 0; sp < ph.ns; sp++) {
-            sFL[sp * s_stride] = 0.f;
-            sFR[sp * s_stride] = 0.f;
-        }
     float nh[3], area, x[2][DIM];
     load_face_na(m.fna, f, nh, area);
     load_face_pair<DIM, float>(m.fdx, f, x);
@@ -1173,6 +1168,12 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
         dPhi[p] = 0.f;
     }
     const bool subsonic = fabsf(sL.M) <= 1.f && fabsf(sR.M) <= 1.f && fabsf(Mc) <= 1.f;
+    // species advection coefficients, set by the flux branch and applied per species after the viscous part
+    // (kind 0: no advection, 1: Rusanov, 2: subsonic AUSM, 3: FP64 route)
+    int sp_kind = 0;
+    float sp_a = 0.f, sp_b = 0.f, sp_c = 0.f, sp_d = 0.f, sp_e = 0.f, sp_scdun = 0.f;
+    bool sp_upR = false;
+    double sp_clo = 0.0, sp_cro = 0.0, sp_und = 0.0, sp_unr = 0.0;
     if constexpr (RUS) {
         // Rusanov (build-defined) as a perturbation of the left baseline: F - f(U_L).n A =
         // A [1/2 (df(dL) + df(dR)) - 1/2 lambda (dR - dL)], lambda = max(|u_n| + a) of the face states
@@ -1195,25 +1196,11 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
         if (SPECIES) {
             // F_sp = A [1/2 (rhoY_L u_n,L + rhoY_R u_n,R) - 1/2 lambda (rhoY_R - rhoY_L)] relative to rhoY_c u_n,c:
             // A [rhoY_c 1/2 (du_n,L + du_n,R) + (1/2 u_n,L + 1/2 lambda) dsL + (1/2 u_n,R - 1/2 lambda) dsR]
-            const float cl = 0.5f * unL + 0.5f * lam, cr = 0.5f * unR - 0.5f * lam;
-            const float dun2 = 0.5f * (sL.dun + sR.dun);
-            for (int sp = 0; sp < ph.ns; sp++) {
-                const int64_t comp = NE + sp;
-                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
-                const float dcs = float(yr - yl);
-                float dsl = 0.f, dsr = dcs;
-                if (RECON)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
-                    }
-                const float adv = area * (float(yl) * dun2 + cl * dsl + cr * dsr);
-                float shift = 0.f;
-                if (!ghost) shift = (dcs * unc + (float(yl) + dcs) * sC.dun) * area;
-                sFL[sp * s_stride] = adv;
-                sFR[sp * s_stride] = adv - shift;
-            }
+            sp_kind = 1;
+            sp_a = 0.5f * unL + 0.5f * lam;   // cl
+            sp_b = 0.5f * unR - 0.5f * lam;   // cr
+            sp_c = 0.5f * (sL.dun + sR.dun);  // dun2
+            sp_scdun = ghost ? 0.f : sC.dun;
         }
     } else if (ph.flux == ABL_FV_FLUX_AUSM && subsonic) {
         const float dm = 0.25f * sL.dM * (sL.M + Mc + 2.f) - 0.25f * sR.dM * (sR.M + Mc - 2.f);
@@ -1245,26 +1232,13 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
             // F_sp = mdot Y_up A (speciesTransport.cpp:325-337) as a perturbation of the left baseline rhoY_L u_n
             SidePert<DIM> sC;
             if (!ghost) side_perturbation<DIM>(dcell, bf, nhat, unc, Mc, gR, Rg, icv, sC);
-            for (int sp = 0; sp < ph.ns; sp++) {
-                const int64_t comp = NE + sp;
-                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
-                const float YL = float(yl / rl.u[kRho]);
-                const float dcs = float(yr - yl);
-                float dsl = 0.f, dsr = dcs;
-                if (RECON)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
-                    }
-                const float dsu = mm < 0.f ? dsr : dsl;
-                const float dYu = (dsu - YL * up.drho) / up.rho;
-                const float adv = (dmdot * YL + mdot * dYu) * area;
-                float shift = 0.f;
-                if (!ghost) shift = (dcs * unc + (float(yl) + dcs) * sC.dun) * area;
-                sFL[sp * s_stride] = adv;
-                sFR[sp * s_stride] = adv - shift;
-            }
+            sp_kind = 2;
+            sp_upR = mm < 0.f;
+            sp_a = up.drho;
+            sp_b = up.rho;
+            sp_c = dmdot;
+            sp_d = mdot;
+            sp_scdun = ghost ? 0.f : sC.dun;
         }
     } else if (ph.flux != ABL_FV_FLUX_NONE && ph.flux != ABL_FV_FLUX_OFF && ph.flux != ABL_FV_FLUX_RUSANOV) {
         // FP64 route: full face flux of the reconstructed states minus the FP64 baselines
@@ -1317,20 +1291,12 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
 #pragma unroll
                 for (int d = 0; d < DIM; d++) unr += br.vel[d] * nhd[d];
             }
-            for (int sp = 0; sp < ph.ns; sp++) {
-                const int64_t comp = NE + sp;
-                const double yl = U[comp * N + L], yr = U[comp * N + Rc];
-                float dsl = 0.f, dsr = float(yr - yl);
-                if (RECON)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        dsl += gS[sgp_word<DIM>(N, sp, L, d)] * x[0][d];
-                        if (!ghost) dsr += gS[sgp_word<DIM>(N, sp, Rc, d)] * x[1][d];
-                    }
-                const double fsp = clo * (yl + double(dsl)) + cro * (yl + double(dsr)) - yl * und * ad;
-                sFL[sp * s_stride] = float(fsp);
-                sFR[sp * s_stride] = ghost ? 0.f : float(fsp - (yr * unr - yl * und) * ad);
-            }
+            sp_kind = 3;
+            sp_clo = clo;
+            sp_cro = cro;
+            sp_und = und;
+            sp_unr = unr;
+            sp_e = area;
         }
     }
     if (VISC) {
@@ -1368,29 +1334,77 @@ __device__ inline void mixed_face(const MeshView<float>& m, const Phys& ph, cons
             normal[d] = area * nhat[d];
         }
         viscous_flux_add<float, DIM>(normal, vel, tq, dF);
-        if (SPECIES && ph.sdiff) {
-            // DiffusionSpeciesFlux (speciesTransport.cpp:196-228): -D rho_face n.(1/2 (gradY_L + gradY_R)) A, a full
-            // flux (no baseline), shared by both sides
-            const float cd = -float(ph.diff) * (bf.rho + 0.5f * dcell[kRho]) * 0.5f * area;
-            float w[2][DIM];
-            if (ghost) load_face_pair<DIM, float>(m.fwa, f, w);
-            for (int sp = 0; sp < ph.ns; sp++) {
-                const int64_t comp = NE + sp;
+    }
+    if (SPECIES) {
+        // per species: the advection perturbation of the flux branch's coefficients, plus DiffusionSpeciesFlux
+        // (speciesTransport.cpp:196-228): -D rho_face n.(1/2 (gradY_L + gradY_R)) A, a full flux (no baseline)
+        // shared by both sides.  Both sides' gradient words come in FP32 pairs, each read once.
+        const bool sdiff = VISC && ph.sdiff;
+        const float cd = sdiff ? -float(ph.diff) * (bf.rho + 0.5f * dcell[kRho]) * 0.5f * area : 0.f;
+        float w[2][DIM];
+        if (sdiff && ghost) load_face_pair<DIM, float>(m.fwa, f, w);
+        const float2* __restrict__ gS2 = reinterpret_cast<const float2*>(gS);
+        for (int sp = 0; sp < ph.ns; sp++) {
+            const int64_t comp = NE + sp;
+            const double yl = U[comp * N + L], yr = U[comp * N + Rc];
+            float wl[2 * DIM], wr[2 * DIM];
+#pragma unroll
+            for (int j = 0; j < DIM; j++) {
+                const bool need = (RECON && 2 * j < DIM) || (sdiff && 2 * j + 1 >= DIM);
+                const float2 a = need ? gS2[((int64_t)sp * DIM + j) * N + L] : make_float2(0.f, 0.f);
+                const float2 c2 = (need && !ghost) ? gS2[((int64_t)sp * DIM + j) * N + Rc] : make_float2(0.f, 0.f);
+                wl[2 * j] = a.x;
+                wl[2 * j + 1] = a.y;
+                wr[2 * j] = c2.x;
+                wr[2 * j + 1] = c2.y;
+            }
+            const float dcs = float(yr - yl);
+            float dsl = 0.f, dsr = dcs;
+            if (RECON)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    dsl += wl[d] * x[0][d];
+                    if (!ghost) dsr += wr[d] * x[1][d];
+                }
+            float fl = 0.f, fr = 0.f;
+            if (sp_kind == 1 || sp_kind == 2) {
+                float adv;
+                if (sp_kind == 1) {
+                    adv = area * (float(yl) * sp_c + sp_a * dsl + sp_b * dsr);
+                } else {
+                    const float YL = float(yl / rl.u[kRho]);
+                    const float dsu = sp_upR ? dsr : dsl;
+                    const float dYu = (dsu - YL * sp_a) / sp_b;
+                    adv = (sp_c * YL + sp_d * dYu) * area;
+                }
+                float shift = 0.f;
+                if (!ghost) shift = (dcs * unc + (float(yl) + dcs) * sp_scdun) * area;
+                fl = adv;
+                fr = adv - shift;
+            } else if (sp_kind == 3) {
+                const double ad = double(sp_e);
+                const double fsp = sp_clo * (yl + double(dsl)) + sp_cro * (yl + double(dsr)) - yl * sp_und * ad;
+                fl = float(fsp);
+                fr = ghost ? 0.f : float(fsp - (yr * sp_unr - yl * sp_und) * ad);
+            }
+            if (sdiff) {
                 float dot = 0.f;
                 float gyg = 0.f;
                 if (ghost) {
-                    const double yl = U[comp * N + L] / rl.u[kRho], yg = U[comp * N + Rc] / rr.u[kRho];
-                    gyg = float(yl - yg);
+                    const double ylm = yl / rl.u[kRho], yg = yr / rr.u[kRho];
+                    gyg = float(ylm - yg);
                 }
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    const float gl = gS[sgp_word<DIM>(N, sp, L, DIM + d)];
-                    const float gr = ghost ? w[1][d] * gyg : gS[sgp_word<DIM>(N, sp, Rc, DIM + d)];
+                    const float gl = wl[DIM + d];
+                    const float gr = ghost ? w[1][d] * gyg : wr[DIM + d];
                     dot += nhat[d] * (gl + gr);
                 }
-                sFL[sp * s_stride] += cd * dot;
-                sFR[sp * s_stride] += cd * dot;
+                fl += cd * dot;
+                fr += cd * dot;
             }
+            sFL[sp * s_stride] = fl;
+            sFR[sp * s_stride] = fr;
         }
     }
 #pragma unroll
