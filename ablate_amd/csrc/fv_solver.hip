@@ -194,11 +194,13 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
     if (VISC) aux_of<DIM>(uc, ph, ac);
     G g[NE][DIM], ga[NA][DIM];
-    double umin[NE], umax[NE];
+    // Barth-Jespersen bounds as the extreme neighbour differences u_n - u_c, each rounded to G once: rounding is
+    // monotone, so they equal G(max u_n - u_c) and G(min u_n - u_c) bit for bit, in half the registers of FP64 bounds
+    G dmin[NE], dmax[NE];
 #pragma unroll
     for (int p = 0; p < NE; p++) {
-        umin[p] = uc[p];
-        umax[p] = uc[p];
+        dmin[p] = G(0);
+        dmax[p] = G(0);
 #pragma unroll
         for (int d = 0; d < DIM; d++) g[p][d] = G(0);
     }
@@ -224,22 +226,18 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
         }
         if (!valid) continue;
         if (RECON) {
-            if (LIM == kLimBJ) {
 #pragma unroll
-                for (int p = 0; p < NE; p++) {
-                    umin[p] = fmin(umin[p], un[p]);
-                    umax[p] = fmax(umax[p], un[p]);
+            for (int p = 0; p < NE; p++) {
+                const G delta = G(un[p] - uc[p]);
+                if (LIM == kLimBJ) {
+                    dmin[p] = fmin(dmin[p], delta);
+                    dmax[p] = fmax(dmax[p], delta);
                 }
-            }
-            if (!bnd) {
                 // cellInterpolant.cpp:405-411 with delta = u_R - u_L: the left cell adds w0.delta and the right cell
                 // subtracts w1.delta, i.e. both add w.(u_n - u_c) with their own side's weight (bitwise the same)
-#pragma unroll
-                for (int p = 0; p < NE; p++) {
-                    const G delta = G(un[p] - uc[p]);
+                if (!bnd)
 #pragma unroll
                     for (int d = 0; d < DIM; d++) g[p][d] += w[d] * delta;
-                }
             }
         }
         if (VISC) {
@@ -284,8 +282,8 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
 #pragma unroll
             for (int p = 0; p < NE; p++) {
                 G v = G(1);
-                if (dpos[p] > G(0)) v = fmin(v, G(umax[p] - uc[p]) / dpos[p]);
-                if (dneg[p] < G(0)) v = fmin(v, G(umin[p] - uc[p]) / dneg[p]);
+                if (dpos[p] > G(0)) v = fmin(v, dmax[p] / dpos[p]);
+                if (dneg[p] < G(0)) v = fmin(v, dmin[p] / dneg[p]);
                 phi[p] = v;
             }
         } else {
