@@ -1499,34 +1499,25 @@ __global__ __launch_bounds__(kTileMax) void k_tile_flux_mixed(MeshView<float> m,
 }
 
 // Mixed form, face-flux strategy, pass 1: one lane per face stores both sides' FP32 perturbations as rows
-// [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.  The
-// species perturbations are formed in LDS (they are assigned, then the diffusion flux is added) and stored once:
-// formed in place in the rows, every value was written back two to three times.
+// [2 nrow][F] (left-baseline rows, then right-baseline rows; nrow = n_euler + n_species): coalesced stores.  Each
+// species perturbation is formed once (advection and diffusion together) and stored straight into its rows.
 template <int DIM, bool RECON, bool VISC, bool SPECIES, bool RUS>
 __global__ __launch_bounds__(kBlock) void k_face_flux_mixed(MeshView<float> m, Phys ph, const double* __restrict__ U,
                                                             const float* __restrict__ rec, const float* __restrict__ gS,
                                                             const float* __restrict__ gY, float* __restrict__ Fm,
                                                             int nrow) {
     constexpr int NE = DIM + 2;
-    extern __shared__ float sbuf[];  // [2 ns][blockDim]: this lane's species perturbations while they are formed
     const int64_t f = xcd_cell_index();
     if (f >= m.F) return;
     const int64_t F = m.F;
-    const int bd = blockDim.x, ns = ph.ns;
-    float* sL = sbuf + threadIdx.x;
-    float* sR = sbuf + (int64_t)ns * bd + threadIdx.x;
     float dFL[NE], dFR[NE];
-    mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, f, dFL, dFR, sL, sR, bd);
+    mixed_face<DIM, RECON, VISC, SPECIES, RUS>(m, ph, U, rec, gS, gY, f, dFL, dFR, Fm + NE * F + f,
+                                               Fm + (nrow + NE) * F + f, F);
 #pragma unroll
     for (int p = 0; p < NE; p++) {
         Fm[p * F + f] = dFL[p];
         Fm[(nrow + p) * F + f] = dFR[p];
     }
-    if (SPECIES)
-        for (int sp = 0; sp < ns; sp++) {
-            Fm[(NE + sp) * F + f] = sL[sp * bd];
-            Fm[(nrow + NE + sp) * F + f] = sR[sp * bd];
-        }
 }
 
 // Mixed form, face-flux strategy, pass 2: each interior cell gathers its own side's perturbations in face-number
@@ -2538,17 +2529,33 @@ __global__ __launch_bounds__(kBlock) void k_ghost_adjoint_out(int64_t NG, int64_
 }
 
 // Per-slot sums of the per-boundary-face theta_bar contributions: block 3 s + j sums component j over the faces of
-// slot s, each lane over a fixed stride of faces, then a fixed-order tree (deterministic).
-__global__ __launch_bounds__(kBlock) void k_theta_reduce(int64_t nb, const int32_t* __restrict__ bslot,
-                                                         const double* __restrict__ contrib, double* __restrict__ out) {
+// slot s, each lane over a fixed stride of faces with four loads in flight, then a fixed-order tree
+// (deterministic).  Only inflow slots have nonzero contributions; the blocks of the other slots write zero.
+constexpr int kThetaBlock = 1024;
+__global__ __launch_bounds__(kThetaBlock) void k_theta_reduce(int64_t nb, const int32_t* __restrict__ bslot,
+                                                              const int32_t* __restrict__ slot_kind,
+                                                              const double* __restrict__ contrib,
+                                                              double* __restrict__ out) {
     const int s = blockIdx.x / 3, j = blockIdx.x % 3;
-    double a = 0.0;
-    for (int64_t i = threadIdx.x; i < nb; i += kBlock)
-        if (bslot[i] == s) a += contrib[3 * i + j];
-    __shared__ double red[kBlock];
-    red[threadIdx.x] = a;
+    const int kind = slot_kind[s];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (kind == ABL_FV_BC_INFLOW || kind == kBcInflowAtFace) {
+        int64_t i = threadIdx.x;
+        for (; i + 3 * kThetaBlock < nb; i += 4 * kThetaBlock) {
+            const int32_t b0 = bslot[i], b1 = bslot[i + kThetaBlock], b2 = bslot[i + 2 * kThetaBlock],
+                          b3 = bslot[i + 3 * kThetaBlock];
+            if (b0 == s) a0 += contrib[3 * i + j];
+            if (b1 == s) a1 += contrib[3 * (i + kThetaBlock) + j];
+            if (b2 == s) a2 += contrib[3 * (i + 2 * kThetaBlock) + j];
+            if (b3 == s) a3 += contrib[3 * (i + 3 * kThetaBlock) + j];
+        }
+        for (; i < nb; i += kThetaBlock)
+            if (bslot[i] == s) a0 += contrib[3 * i + j];
+    }
+    __shared__ double red[kThetaBlock];
+    red[threadIdx.x] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
+    for (int w = kThetaBlock / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -3358,11 +3365,6 @@ struct abl_fv_solver {
         } else {
             if (face_flux_m.ptr) return;
             face_flux_m.alloc((size_t)2 * nrow * host.F);
-            // the face kernel forms its species perturbations in LDS: 2 ns floats per lane
-            const size_t lds = sizeof(float) * 2 * (size_t)host.ns * abl::kBlock;
-            if (lds > 0 && face_flux_kernel_mixed)
-                HIP_CHECK(hipFuncSetAttribute((const void*)face_flux_kernel_mixed,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
     }
     // Tiles for strategy 3 (FP64 form): the largest tile (256, 128 or 64 cells) whose face fluxes fit the LDS
@@ -3565,6 +3567,14 @@ std::string unset_boundary(abl_fv_handle h) {
             return "boundary slot " + std::to_string(v) +
                    " has boundary faces but no boundary condition (abl_fv_set_boundary)";
     return std::string();
+}
+
+// theta_bar is the adjoint of the inflow states: with no inflow slot every contribution is zero and the theta_bar
+// passes (and their copy back to the host) are skipped.
+bool has_inflow(abl_fv_handle h) {
+    for (int v : h->bc_slots)
+        if (h->h_slot_kind[v] == ABL_FV_BC_INFLOW) return true;
+    return false;
 }
 
 }  // namespace
@@ -3928,8 +3938,8 @@ extern "C" int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, in
             if (h->strategy == ABL_FV_STRATEGY_FACE_FLUX) {
                 const int nrow = L.dim + 2 + L.ns;
                 h->timed(ABL_FV_PHASE_FLUX, st, [&] {
-                    hipLaunchKernelGGL(h->face_flux_kernel_mixed, dim3(grid_for(L.F)), block,
-                                       sizeof(float) * 2 * (size_t)L.ns * abl::kBlock, st, vf, h->ph, U, h->recf.ptr,
+                    hipLaunchKernelGGL(h->face_flux_kernel_mixed, dim3(grid_for(L.F)), block, 0, st, vf, h->ph, U,
+                                       h->recf.ptr,
                                        h->gS_f.ptr, h->gY_f.ptr, h->face_flux_m.ptr, nrow);
                 });
                 h->timed(ABL_FV_PHASE_GATHER, st, [&] {
@@ -4088,8 +4098,8 @@ namespace {
 // order (deterministic, k_theta_reduce) into sum_dev [3 kMaxBcSlots]; only those sums come back, and are added into
 // the caller's array for the slots that have boundary faces.
 void reduce_theta_into(abl_fv_handle h, hipStream_t st, const double* contrib, double* sum_dev, double* theta_bar) {
-    hipLaunchKernelGGL(abl::k_theta_reduce, dim3(3 * abl::kMaxBcSlots), dim3(abl::kBlock), 0, st, h->n_bfaces,
-                       h->bslot.ptr, contrib, sum_dev);
+    hipLaunchKernelGGL(abl::k_theta_reduce, dim3(3 * abl::kMaxBcSlots), dim3(abl::kThetaBlock), 0, st, h->n_bfaces,
+                       h->bslot.ptr, h->slot_kind.ptr, contrib, sum_dev);
     HIP_CHECK(hipGetLastError());
     double sums[3 * abl::kMaxBcSlots];
     HIP_CHECK(hipMemcpyAsync(sums, sum_dev, sizeof(sums), hipMemcpyDeviceToHost, st));
@@ -4147,7 +4157,7 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
                                h->ghost_slot.ptr, h->slot_kind.ptr, h->adjUghost.ptr, Ubar);
         }
         HIP_CHECK(hipGetLastError());
-        if (theta_bar && h->n_bfaces) {
+        if (theta_bar && h->n_bfaces && has_inflow(h)) {
             if (!h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
             hipLaunchKernelGGL(abl::k_adj_theta, dim3(grid_for(h->n_bfaces)), block, 0, st, h->n_bfaces, L.dim, L.ns,
                                L.N, L.NI, h->bface.ptr, h->bslot.ptr, h->fcell.ptr, L.F, h->slot_kind.ptr,
@@ -4227,7 +4237,7 @@ int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar
             });
         }
         HIP_CHECK(hipGetLastError());
-        if (theta_bar && h->n_bfaces && !noflux) {
+        if (theta_bar && h->n_bfaces && !noflux && has_inflow(h)) {
             const auto v = h->view();
             hipLaunchKernelGGL(h->vjp_theta_kernel, dim3(grid_for(h->n_bfaces)), block, 0, st, v, h->ph, U, Rbar,
                                h->n_bfaces, h->bface.ptr, h->bslot.ptr, h->slot_kind.ptr, h->slot_param.ptr,

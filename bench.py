@@ -100,24 +100,29 @@ KERNELS = {"face_flux": ["k_gradients_w4", "k_face_flux", "k_gather"], "recomput
            "tile": ["k_gradients_w4", "k_tile_flux"], "fused": ["k_fused"], "march": ["k_march"]}
 
 
-def measured_traffic(cfg, n, strategy):
-    """HBM bytes per RHS from hardware counters: the committed record of profiles/traffic.json for this workload,
-    measured on the same device sources (FETCH_SIZE and WRITE_SIZE per dispatch summed over the RHS kernels, each
+def recorded_traffic(workload, n, strategy):
+    """HBM bytes per call from hardware counters: the committed record of profiles/traffic.json for this workload,
+    measured on the same device sources (FETCH_SIZE and WRITE_SIZE per dispatch summed over the call's kernels, each
     scaled by what the counters read for the streaming copy of abl_hbm_probe; see ablate_amd/rocprof.py).  None
     when no record matches the sources that built the library."""
     from ablate_amd import rocprof
 
-    rec = rocprof.lookup(cfg["workload"], n, strategy)
+    rec = rocprof.lookup(workload, n, strategy)
     return None if rec is None else rec["traffic_bytes"]
 
 
-def tets_traffic(n, strategy):
-    """HBM bytes of one FP64 RHS of the tets leg from the committed counter record of the same device sources (the
-    ghost fill, gradient, species-gradient, face-flux and gather kernels), or None."""
-    from ablate_amd import rocprof
+def measured_traffic(cfg, n, strategy):
+    """HBM bytes of one RHS of a headline-mesh workload (gradient and flux kernels), or None."""
+    return recorded_traffic(cfg["workload"], n, strategy)
 
-    rec = rocprof.lookup(f"ns_10species_fp64_tets{n}_lsq_bj_rusanov", n, strategy)
-    return None if rec is None else rec["traffic_bytes"]
+
+def tets_traffic(n, strategy, form="fp64"):
+    """HBM bytes of one RHS of the tets leg (ghost fill, gradient, species-gradient, face-flux and gather kernels;
+    form "fp64" or "mixed"), or of one FP64 VJP call (form "adjoint": ghost fill, the forward gradient passes and
+    the adjoint passes), or None."""
+    if form == "adjoint":
+        return recorded_traffic(f"vjp_fp64_tets{n}_ns_10species_lsq_bj_rusanov", n, "adjoint")
+    return recorded_traffic(f"ns_10species_{form}_tets{n}_lsq_bj_rusanov", n, strategy)
 
 
 def ns_processes(fv, mu, k):
@@ -320,10 +325,12 @@ def tets_leg(fv, synthetic, args):
         "mixed": {"value": round(cells / (ms_mixed * 1e-3) / 1e6, 3), "unit": "Mcell/s",
                   "ms_per_step": round(ms_mixed, 4), "speedup_vs_fp64": round(ms / ms_mixed, 3),
                   "strategy": strategy_mixed,
+                  "traffic": tets_traffic(n, strategy_mixed, "mixed"),
                   "rel_err_vs_fp64_max": float(f"{max(err):.3e}"),
                   "error_norm": "max over the 15 conserved fields of ||R_mixed - R_fp64|| / ||R_fp64||"},
         "adjoint": {"value": round(cells / (ms_adj * 1e-3) / 1e6, 3), "unit": "Mcell/s", "ms_per_step": round(ms_adj, 4),
                     "adjoint_over_forward": round(ms_adj / ms, 3),
+                    "traffic": tets_traffic(n, strategy, "adjoint"),
                     "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in adj_phases.items() if c},
                     "directional_checks": checks,
                     "checks_pass": all(c["pass"] for c in checks.values() if "pass" in c)},
@@ -459,6 +466,7 @@ def adjoint_leg(fv, synthetic, args):
         "phases_ms": {name: round(v / max(c, 1), 4) for name, (v, c) in phases.items() if c},
         "workload": f"vjp_fp64_{n}cube_euler_10species_inflow_ausm_first_order",
         "cells": cells,
+        "traffic": recorded_traffic(f"vjp_fp64_{n}cube_euler_10species_inflow_ausm_first_order", n, "adjoint"),
         "directional_checks": checks,
         "checks_pass": all(c["pass"] for c in checks.values() if "pass" in c),
         "theta_bar_inflow": [float(x) for x in theta_bar],

@@ -243,7 +243,9 @@ int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accum
  *   caller's ghost values, which are inputs of the RHS; other ghost rows receive zero, their dependence on the
  *   interior (OUTFLOW, enforce_at_face) and on theta being folded into Ubar and theta_bar)
  *   theta_bar[3 * slot + {0,1,2}] += Rbar^T dR/d(rho_in, u_in, T_in) for every INFLOW slot (host array sized
- *   3 * number of slots, or NULL).  Fluxes are recomputed, not stored.  Supported for the FP64 form with the
+ *   3 * number of slots, or NULL; with inflow slots the call waits for the stream to copy their sums back, without
+ *   any it leaves theta_bar as it is and returns without waiting).  Fluxes are recomputed, not stored.  Supported
+ *   for the FP64 form with the
  *   Ausm, Rusanov or Off calculators, first order or least squares with the Barth-Jespersen limiter (or none),
  *   viscous and species fluxes; the first second-order call allocates its workspace. */
 int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar, double* theta_bar,
