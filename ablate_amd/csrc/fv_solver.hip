@@ -1559,7 +1559,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_mixed(MeshView<float> m, int 
 template <int DIM>
 __device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U,
                                  const double* __restrict__ Rb, int64_t L, int64_t Rc, bool ghost, double* fb,
-                                 double& QL, double& QR, double& QdL) {
+                                 double& QL, double& QR, double& QdL, double* gout = nullptr, int64_t gstride = 0) {
     constexpr int NE = DIM + 2;
     const int64_t N = m.N;
     const double ivL = m.inv_vol[L], ivR = ghost ? 0.0 : m.inv_vol[Rc];
@@ -1572,6 +1572,7 @@ __device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, cons
     for (int sp = 0; sp < ph.ns; sp++) {  // two species' loads in flight
         const int64_t comp = NE + sp;
         const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
+        if (gout) gout[sp * gstride] = g;
         const double yl = U[comp * N + L], yr = U[comp * N + Rc];
         QL += g * yl;
         QR += g * yr;
@@ -1668,6 +1669,111 @@ __global__ __launch_bounds__(kBlock) void k_vjp(MeshView<double> m, Phys ph, con
         for (int k = 0; k < KM; k++) {
             const bool ghost = sR[k] >= NI;
             const double g = (ghost ? 0.0 : Rb[comp * N + sR[k]] * ivR[k]) - Rb[comp * N + sL[k]] * ivL[k];
+            a += coef[k] * g;
+            if (cghost[k] != 0.0) Ub[comp * N + sR[k]] = cghost[k] * g;
+        }
+        Ub[comp * N + c] = a;
+    }
+}
+
+// First-order VJP, face-centric (abl_fv_rhs_vjp; k_vjp above stays the allocation-free form): one lane per face
+// reverses the face's flux once for both sides and stores the side adjoints, the species multipliers and the
+// species seeds as rows [2 NE + 2 + ns][F] (ubL, ubR, cL, cR, g per species): coalesced stores.  k_vjp_gather then
+// sums them per cell in slot order exactly as k_vjp sums its own evaluations, so both forms give the same Ubar bit
+// for bit; each face is reversed once instead of once per side, and the gather reads each face's species seeds
+// instead of both cells' Rbar rows.
+template <int DIM, bool RUS>
+__global__ __launch_bounds__(kBlock) void k_vjp_faces(MeshView<double> m, Phys ph, const double* __restrict__ U,
+                                                      const double* __restrict__ Rb, double* __restrict__ fr) {
+    constexpr int NE = DIM + 2;
+    const int64_t f = xcd_cell_index();
+    if (f >= m.F) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    const int64_t L = m.fcell[f], Rc = m.fcell[F + f];
+    const bool ghost = Rc >= NI;
+    double nh[3], area, uL[NE], uR[NE], fb[NE], QL, QR, QdL;
+    load_face_na(m.fna, f, nh, area);
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        uL[p] = U[p * N + L];
+        uR[p] = U[p * N + Rc];
+    }
+    face_seed<DIM>(m, ph, U, Rb, L, Rc, ghost, fb, QL, QR, QdL, fr + (2 * NE + 2) * F + f, F);
+    double ubL[NE], ubR[NE], cL, cR;
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        ubL[p] = 0.0;
+        ubR[p] = 0.0;
+    }
+    face_vjp<DIM, RUS>(uL, uR, QL, QR, QdL, nh, area, ph, fb, ubL, ubR, cL, cR);
+#pragma unroll
+    for (int p = 0; p < NE; p++) {
+        fr[p * F + f] = ubL[p];
+        fr[(NE + p) * F + f] = ubR[p];
+    }
+    fr[2 * NE * F + f] = cL;
+    fr[(2 * NE + 1) * F + f] = cR;
+}
+
+// Per-cell sums of the face rows of k_vjp_faces: its own side of each face in slot order, the ghost side folded back
+// for outflow (copy) and enforce-at-face inflow (2 state - cell) ghosts, a caller-filled ghost's own adjoint row;
+// then each species from the faces' multipliers and seeds (the species flux is the multiplier times the species
+// value).  Padding slots add 0 x (face 0's seed), as k_vjp does.
+template <int DIM, int KM>
+__global__ __launch_bounds__(kBlock) void k_vjp_gather(MeshView<double> m, Phys ph, const double* __restrict__ Rb,
+                                                       const double* __restrict__ fr,
+                                                       const int32_t* __restrict__ ghost_slot,
+                                                       const int32_t* __restrict__ slot_kind, double* __restrict__ Ub) {
+    constexpr int NE = DIM + 2;
+    const int64_t c = xcd_cell_index();
+    if (c >= m.NI) return;
+    const int64_t N = m.N, F = m.F, NI = m.NI;
+    double ub[NE], coef[KM], cghost[KM];
+    int64_t sf[KM], sR[KM];
+#pragma unroll
+    for (int p = 0; p < NE; p++) ub[p] = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const Slot sl = face_slot(m, k, c);
+        const int64_t f = sl.f;
+        sf[k] = f;
+        sR[k] = m.fcell[F + f];
+        cghost[k] = 0.0;
+        coef[k] = 0.0;
+        if (!sl.valid) continue;
+        const int64_t Rc = sR[k];
+        double cf = fr[(2 * NE + sl.s) * F + f];
+#pragma unroll
+        for (int p = 0; p < NE; p++) ub[p] += fr[(sl.s * NE + p) * F + f];
+        if (Rc >= NI) {
+            const int gk = slot_kind[ghost_slot[Rc - NI]];
+            if (gk == ABL_FV_BC_OUTFLOW) {
+#pragma unroll
+                for (int p = 0; p < NE; p++) ub[p] += fr[(NE + p) * F + f];
+                cf += fr[(2 * NE + 1) * F + f];
+            } else if (gk == kBcInflowAtFace) {
+#pragma unroll
+                for (int p = 0; p < NE; p++) ub[p] -= fr[(NE + p) * F + f];
+                cf -= fr[(2 * NE + 1) * F + f];
+            } else if (gk == ABL_FV_BC_GHOST) {
+#pragma unroll
+                for (int p = 0; p < NE; p++) Ub[p * N + Rc] = fr[(NE + p) * F + f];
+                cghost[k] = fr[(2 * NE + 1) * F + f];
+            }
+        }
+        coef[k] = cf;
+    }
+#pragma unroll
+    for (int p = 0; p < NE; p++) Ub[p * N + c] = ub[p];
+    if (ph.ns == 0) return;
+    const double* __restrict__ gs = fr + (2 * NE + 2) * F;
+#pragma unroll 2
+    for (int sp = 0; sp < ph.ns; sp++) {
+        const int64_t comp = NE + sp;
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const double g = gs[(int64_t)sp * F + sf[k]];
             a += coef[k] * g;
             if (cghost[k] != 0.0) Ub[comp * N + sR[k]] = cghost[k] * g;
         }
@@ -2844,6 +2950,21 @@ VjpKernel pick_vjp(bool rusanov, int K) {
         default: return rusanov ? k_vjp<DIM, true, 8> : k_vjp<DIM, false, 8>;
     }
 }
+using VjpFacesKernel = void (*)(MeshView<double>, Phys, const double*, const double*, double*);
+using VjpGatherKernel = void (*)(MeshView<double>, Phys, const double*, const double*, const int32_t*,
+                                 const int32_t*, double*);
+template <int DIM>
+VjpFacesKernel pick_vjp_faces(bool rusanov) {
+    return rusanov ? k_vjp_faces<DIM, true> : k_vjp_faces<DIM, false>;
+}
+template <int DIM>
+VjpGatherKernel pick_vjp_gather(int K) {
+    switch (slot_width(K)) {
+        case 4: return k_vjp_gather<DIM, 4>;
+        case 6: return k_vjp_gather<DIM, 6>;
+        default: return k_vjp_gather<DIM, 8>;
+    }
+}
 template <int DIM>
 VjpThetaKernel pick_vjp_theta(bool rusanov) {
     return rusanov ? k_vjp_theta<DIM, true> : k_vjp_theta<DIM, false>;
@@ -3238,7 +3359,10 @@ struct abl_fv_solver {
     abl::DevBuf<double> slot_param, slot_Y, theta_contrib;
     abl::DevBuf<double> theta_sum;    // [3 kMaxBcSlots] per-slot sums (k_theta_reduce)
     std::vector<double> h_slot_param, h_slot_Y;
-    abl::VjpKernel vjp_kernel = nullptr;
+    abl::VjpKernel vjp_kernel = nullptr;         // cell-centric first-order VJP (allocation-free entry)
+    abl::VjpFacesKernel vjp_faces = nullptr;     // face-centric first-order VJP: face rows, then per-cell sums
+    abl::VjpGatherKernel vjp_gather = nullptr;
+    abl::DevBuf<double> vjp_rows;                // [2 NE + 2 + ns][F] face rows of vjp_faces (first call allocates)
     // second-order adjoint: kernels, mirror-slot table, workspace (allocated by the first such call)
     abl::AdjFacesKernel adj_faces = nullptr;
     abl::DevBuf<double> adjW, adjGL;  // face adjoint records, boundary-face species extras
@@ -3543,7 +3667,7 @@ struct abl_fv_solver {
         return ABL_FV_STRATEGY_FACE_FLUX;
     }
     int64_t device_bytes() const {
-        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + mr_seq.bytes() + mr_pro.bytes() + mr_stile.bytes() + mr_snext.bytes() + mr_fstart.bytes() + mr_ffaces.bytes() + mr_fcode.bytes() + mr_pfstart.bytes() + mr_pffaces.bytes() + mr_pfcode.bytes() + mr_hstart.bytes() + mr_hcell.bytes() + mr_hinfo.bytes() + mr_gslot.bytes() + snbr.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
+        return (int64_t)(ell.bytes() + fcell.bytes() + bface.bytes() + bslot.bytes() + slot_kind.bytes() + fna.bytes() + fdx.bytes() + face_flux_e.bytes() + face_flux_s.bytes() + face_flux_m.bytes() + tile_start.bytes() + tile_faces.bytes() + tile_slot.bytes() + fu_start.bytes() + fu_faces.bytes() + fu_code.bytes() + fu_slot.bytes() + fu_hstart.bytes() + fu_hcell.bytes() + fu_hinfo.bytes() + mr_seq.bytes() + mr_pro.bytes() + mr_stile.bytes() + mr_snext.bytes() + mr_fstart.bytes() + mr_ffaces.bytes() + mr_fcode.bytes() + mr_pfstart.bytes() + mr_pffaces.bytes() + mr_pfcode.bytes() + mr_hstart.bytes() + mr_hcell.bytes() + mr_hinfo.bytes() + mr_gslot.bytes() + snbr.bytes() + vjp_rows.bytes() + sw.bytes() + swa.bytes() + slim.bytes() + sw_f.bytes() + swa_f.bytes() + slim_f.bytes() + fwa.bytes() + inv_vol.bytes() + gradC.bytes() +
                          gradA.bytes() + rec.bytes() + adjUghost.bytes() + adjUself.bytes() + adjContrib.bytes() + adjW.bytes() + adjGL.bytes() + mirror.bytes() + recf.bytes() + gS_f.bytes() + gY_f.bytes() + rmin.bytes() + cfl_partial.bytes() + dt_partial.bytes() + cfl_out.bytes() + slot_state.bytes() + counter.bytes() + fna_f.bytes() + fdx_f.bytes() +
                          fwa_f.bytes());
     }
@@ -3748,10 +3872,14 @@ extern "C" int abl_fv_create(const abl_fv_mesh_desc* mesh, const abl_fv_physics_
             const bool rus = p.flux_calculator == ABL_FV_FLUX_RUSANOV;
             if (mesh->dim == 3) {
                 s->vjp_kernel = abl::pick_vjp<3>(rus, L.K);
+                s->vjp_faces = abl::pick_vjp_faces<3>(rus);
+                s->vjp_gather = abl::pick_vjp_gather<3>(L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<3>(rus);
                 abl::pick_adj<3>(rus, L.K, s->adj_faces, s->adj_cells_e, s->adj_cells_s, s->adj_gather);
             } else {
                 s->vjp_kernel = abl::pick_vjp<2>(rus, L.K);
+                s->vjp_faces = abl::pick_vjp_faces<2>(rus);
+                s->vjp_gather = abl::pick_vjp_gather<2>(L.K);
                 s->vjp_theta_kernel = abl::pick_vjp_theta<2>(rus);
                 abl::pick_adj<2>(rus, L.K, s->adj_faces, s->adj_cells_e, s->adj_cells_s, s->adj_gather);
             }
@@ -4174,7 +4302,7 @@ int vjp_second_order(abl_fv_handle h, double* U, const double* Rbar, double* Uba
 
 namespace {
 int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, double* contrib,
-                    double* sums, hipStream_t st, const char* who);
+                    double* sums, double* rows, hipStream_t st, const char* who);
 }  // namespace
 
 extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar,
@@ -4198,19 +4326,22 @@ extern "C" int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const dou
     try {
         if (theta_bar && h->n_bfaces && !h->theta_contrib.ptr) h->theta_contrib.alloc((size_t)3 * h->n_bfaces);
         if (theta_bar && h->n_bfaces && !h->theta_sum.ptr) h->theta_sum.alloc((size_t)3 * abl::kMaxBcSlots);
+        if (fl != ABL_FV_FLUX_OFF && fl != ABL_FV_FLUX_NONE && L.F > 0 && !h->vjp_rows.ptr)
+            h->vjp_rows.alloc((size_t)(2 * (L.dim + 2) + 2 + L.ns) * L.F);
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_rhs_vjp: ") + e.what());
     }
-    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, h->theta_contrib.ptr, h->theta_sum.ptr,
+    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, h->theta_contrib.ptr, h->theta_sum.ptr, h->vjp_rows.ptr,
                            (hipStream_t)stream, "abl_fv_rhs_vjp");
 }
 
 namespace {
-// First-order VJP (ghost fill, ghost rows of Ubar, the cell-centric VJP kernel, theta_bar) with the theta_bar
-// workspace supplied by the caller: contrib [3 n_bfaces] and sums [3 kMaxBcSlots] device doubles (written in
-// full before they are read).
+// First-order VJP (ghost fill, ghost rows of Ubar, the VJP kernels, theta_bar) with the theta_bar workspace
+// supplied by the caller: contrib [3 n_bfaces] and sums [3 kMaxBcSlots] device doubles (written in full before they
+// are read).  rows [2 NE + 2 + ns][F]: the face-centric kernels (each face reversed once); null: the cell-centric kernel
+// (each face reversed by both of its cells, no workspace).  Both give the same Ubar bit for bit.
 int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar, double* theta_bar, double* contrib,
-                    double* sums, hipStream_t st, const char* who) {
+                    double* sums, double* rows, hipStream_t st, const char* who) {
     const abl::Layout& L = h->host;
     const int fl = h->phys.flux_calculator;
     try {
@@ -4223,8 +4354,8 @@ int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar
         }
         const dim3 grid(grid_for(L.NI)), block(abl::kBlock);
         const bool noflux = fl == ABL_FV_FLUX_OFF || fl == ABL_FV_FLUX_NONE;
-        if (noflux) {
-            HIP_CHECK(hipMemsetAsync(Ubar, 0, sizeof(double) * (size_t)L.nc * L.N, st));
+        if (noflux) {  // a fill kernel, not a memset node, so that a captured call replays correctly
+            hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for(L.N * L.nc)), block, 0, st, Ubar, L.N, (int64_t)0, L.nc);
         } else {
             const auto v = h->view();
             // ghost rows first: the VJP kernel then writes those of caller-filled ghosts
@@ -4232,8 +4363,14 @@ int vjp_first_order(abl_fv_handle h, double* U, const double* Rbar, double* Ubar
                 hipLaunchKernelGGL(abl::k_zero_rows, dim3(grid_for((L.N - L.NI) * L.nc)), block, 0, st, Ubar, L.N, L.NI,
                                    L.nc);
             h->timed(ABL_FV_PHASE_VJP, st, [&] {
-                hipLaunchKernelGGL(h->vjp_kernel, grid, block, 0, st, v, h->ph, U, Rbar, h->ghost_slot.ptr,
-                                   h->slot_kind.ptr, Ubar);
+                if (rows) {
+                    hipLaunchKernelGGL(h->vjp_faces, dim3(grid_for(L.F)), block, 0, st, v, h->ph, U, Rbar, rows);
+                    hipLaunchKernelGGL(h->vjp_gather, grid, block, 0, st, v, h->ph, Rbar, rows, h->ghost_slot.ptr,
+                                       h->slot_kind.ptr, Ubar);
+                } else {
+                    hipLaunchKernelGGL(h->vjp_kernel, grid, block, 0, st, v, h->ph, U, Rbar, h->ghost_slot.ptr,
+                                       h->slot_kind.ptr, Ubar);
+                }
             });
         }
         HIP_CHECK(hipGetLastError());
@@ -4596,6 +4733,6 @@ extern "C" int abl_fv_rhs_vjp_noalloc(abl_fv_handle h, double time, double* U, c
     } catch (const std::exception& e) {
         return fail(h, std::string("abl_fv_rhs_vjp_noalloc: ") + e.what());
     }
-    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, sc, sc + 3 * kNoAllocMaxBoundaryFaces, (hipStream_t)stream,
-                           "abl_fv_rhs_vjp_noalloc");
+    return vjp_first_order(h, U, Rbar, Ubar, theta_bar, sc, sc + 3 * kNoAllocMaxBoundaryFaces, nullptr,
+                           (hipStream_t)stream, "abl_fv_rhs_vjp_noalloc");
 }

@@ -247,7 +247,8 @@ int abl_fv_rhs(abl_fv_handle h, double time, double* U, double* R, int32_t accum
  *   any it leaves theta_bar as it is and returns without waiting).  Fluxes are recomputed, not stored.  Supported
  *   for the FP64 form with the
  *   Ausm, Rusanov or Off calculators, first order or least squares with the Barth-Jespersen limiter (or none),
- *   viscous and species fluxes; the first second-order call allocates its workspace. */
+ *   viscous and species fluxes; the first call allocates its workspace (first order: 2 n_euler + 2 + n_species
+ *   doubles per face). */
 int abl_fv_rhs_vjp(abl_fv_handle h, double time, double* U, const double* Rbar, double* Ubar, double* theta_bar,
                    abl_fv_stream stream);
 

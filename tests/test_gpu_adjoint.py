@@ -248,3 +248,19 @@ def test_second_order_vjp_c5_physics_at_55k_tets(torch_cuda):
     for p in range(U0.shape[0]):
         ep = np.linalg.norm(Ug[p, :NI] - ref[p, :NI]) / np.linalg.norm(ref[p, :NI])
         assert ep <= 1e-10, (p, ep)
+
+
+def test_vjp_off_flux_is_zero_everywhere(torch_cuda):
+    """OffFlux without diffusion terms: R = 0 for every state, so Ubar = 0 in every row (ghost rows included), written
+    by a fill kernel over a NaN-filled output."""
+    torch = torch_cuda
+    chan = BoxMesh([6, 5, 4], [0, 0, 0], [1, 1, 1], [False, True, True])
+    case = Case("off_species_inflow", chan, flux="off", n_species=2, bcs=inflow_x(2), phase=PHASE)
+    U0 = case.initial_state()
+    solver = case.solver()
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    Ub = torch.full_like(U, float("nan"))
+    theta_bar = solver.rhs_vjp(0.0, U, torch.tensor(_seed(case, U0), device="cuda"), Ub)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(Ub).item() == 0 and not torch.isnan(Ub).any().item()
+    assert not np.any(np.asarray(theta_bar))
