@@ -1556,7 +1556,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_mixed(MeshView<float> m, int 
 
 // Adjoint seeds of face f: g[p] for euler components and the species sums QL = sum g_sp rhoY_L,
 // QR = sum g_sp rhoY_R, QdL = sum g_sp (rhoY_R - rhoY_L).
-template <int DIM>
+template <int DIM, int UNR = 2>
 __device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, const double* __restrict__ U,
                                  const double* __restrict__ Rb, int64_t L, int64_t Rc, bool ghost, double* fb,
                                  double& QL, double& QR, double& QdL, double* gout = nullptr, int64_t gstride = 0) {
@@ -1568,8 +1568,8 @@ __device__ inline void face_seed(const MeshView<double>& m, const Phys& ph, cons
     QL = 0.0;
     QR = 0.0;
     QdL = 0.0;
-#pragma unroll 2
-    for (int sp = 0; sp < ph.ns; sp++) {  // two species' loads in flight
+#pragma unroll UNR
+    for (int sp = 0; sp < ph.ns; sp++) {  // UNR species' loads in flight
         const int64_t comp = NE + sp;
         const double g = (ghost ? 0.0 : Rb[comp * N + Rc] * ivR) - Rb[comp * N + L] * ivL;
         if (gout) gout[sp * gstride] = g;
@@ -1698,7 +1698,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_faces(MeshView<double> m, Phys p
         uL[p] = U[p * N + L];
         uR[p] = U[p * N + Rc];
     }
-    face_seed<DIM>(m, ph, U, Rb, L, Rc, ghost, fb, QL, QR, QdL, fr + (2 * NE + 2) * F + f, F);
+    face_seed<DIM, 5>(m, ph, U, Rb, L, Rc, ghost, fb, QL, QR, QdL, fr + (2 * NE + 2) * F + f, F);
     double ubL[NE], ubR[NE], cL, cR;
 #pragma unroll
     for (int p = 0; p < NE; p++) {
