@@ -563,6 +563,9 @@ def main():
             "traffic_source": ("rocprofv3 FETCH_SIZE/WRITE_SIZE record in profiles/traffic.json for these device "
                                "sources" if measured_traffic(cfg, n, strategy) is not None
                                else "unmeasured: no counter record of these device sources"),
+            # the counter bytes over the same kernel time: how busy the memory system is on the bytes it moves
+            "traffic_GB_per_s": (None if measured_traffic(cfg, n, strategy) is None
+                                 else round(measured_traffic(cfg, n, strategy) / (kernel_ms * 1e-3) / 1e9, 1)),
             "kernel": "one RHS launch sequence (" + " + ".join(KERNELS[strategy]) + "), event-timed per "
                       f"launch on its stream; algorithmic bytes {info.alg_bytes_per_cell:.1f} B/cell x {cells} cells",
         },
