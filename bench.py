@@ -529,6 +529,16 @@ def main():
     alg_bytes = info.alg_bytes_per_cell * cells
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     finite = bool(torch.isfinite(R).all().item())
+    # per-call event timing after the timed region (the measurement plan's median of >= 20 calls): every call is
+    # bracketed by its own event pair on the launch stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(stream)
+        solver.compute_rhs(0.0, U, R)
+        b.record(stream)
+    torch.cuda.synchronize()
+    per_call = sorted(a.elapsed_time(b) for a, b in evs)
+    median_ms = per_call[len(per_call) // 2] if per_call else None
 
     out = {
         "metric": "RHS Mcell/s + %HBM peak, FP32-mixed vs FP64 at <=1e-6 rel err; adjoint Mcell/s",
@@ -571,6 +581,7 @@ def main():
         },
         "phases_ms": {name: round(ms / max(cnt, 1), 4) for name, (ms, cnt) in phases.items() if cnt},
         "event_ms_per_step": round(event_ms, 4),
+        "event_ms_median_per_call": None if median_ms is None else round(median_ms, 4),
         "setup_s": round(setup_s, 2),
         "hbm_copy_probe": copy,
         "hbm_stream_probe": sprobe,
