@@ -26,6 +26,9 @@ def adjoint_cases():
         Case("tets_rusanov_outflow", TetBoxMesh(3, 0.2), flux="rusanov", n_species=2, bcs=all_outflow(6),
              phase=PHASE),
         Case("caller_ghost_at_face_species", chan(6), n_species=3, bcs=ghost_and_at_face(3), phase=PHASE),
+        # 4608 boundary faces: the theta_bar reduction's four-loads-in-flight loop (1024-lane blocks) and its tail
+        Case("inflow_4608_boundary_faces", BoxMesh([3, 48, 48], [0, 0, 0], [1, 1, 1], [False, True, True]),
+             n_species=2, bcs=inflow_x(2), phase=PHASE),
     ]
 
 
