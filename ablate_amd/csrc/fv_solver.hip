@@ -189,6 +189,7 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
     const int64_t c = active ? c_lane : m.NI - 1;  // lanes past the end shadow the last cell and store nothing
     const int64_t N = m.N, NI = m.NI;
     const G* __restrict__ wa_arr = m.swa ? m.swa : m.sw;
+    const bool own_wa = m.swa != nullptr;  // meshes without boundary faces share one weight set
     double uc[NE], ac[NA];
 #pragma unroll
     for (int p = 0; p < NE; p++) uc[p] = U[p * N + c];
@@ -220,9 +221,16 @@ __device__ __forceinline__ void gradients_body(MeshView<G> m, Phys ph, const dou
         for (int p = 0; p < NE; p++) un[p] = U[p * N + n];
         G w[DIM], wa[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) {
+        for (int d = 0; d < DIM; d++)
             if (RECON) w[d] = m.sw[(k * DIM + d) * NI + c];
-            if (VISC) wa[d] = wa_arr[(k * DIM + d) * NI + c];
+        if (VISC) {
+            if (RECON && !own_wa) {  // the aux set is the cell set: reuse the words just loaded
+#pragma unroll
+                for (int d = 0; d < DIM; d++) wa[d] = w[d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < DIM; d++) wa[d] = wa_arr[(k * DIM + d) * NI + c];
+            }
         }
         if (!valid) continue;
         if (RECON) {
