@@ -320,6 +320,8 @@ def tets_leg(fv, synthetic, args):
         "achieved_GB_per_s": round(achieved, 1),
         "frac_of_8TB_s": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": tets_traffic(n, strategy),
+        "traffic_GB_per_s": (None if tets_traffic(n, strategy) is None
+                             else round(tets_traffic(n, strategy) / (kernel_ms * 1e-3) / 1e9, 1)),
         "setup_s": round(setup_s, 1),
         "finite": bool(torch.isfinite(R).all().item()),
         "mixed": {"value": round(cells / (ms_mixed * 1e-3) / 1e6, 3), "unit": "Mcell/s",
